@@ -1,0 +1,177 @@
+"""ctypes binding of libfastwigner.so (C ABI: include/fastwigner.h).
+
+Follows the reference's own FFI pattern for its native crossing, cufft.py:16-29
+(ctypes library handle, argtypes/restype per entry point) and cufft.py:31-102
+(non-zero status codes raised as typed exceptions).
+
+The library is built in-tree by ``python -m fastwigner_amd.build``. There is no
+fallback: if the library is missing every propagator raises ``FastWignerError``.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfastwigner.so")
+KCACHE = os.path.join(HERE, "_kcache")
+
+FW_NSER = 9
+
+
+class FastWignerError(RuntimeError):
+    """Base class of the library errors (cf. cuFFT's cufftError, cufft.py:31)."""
+
+
+class InvalidArgumentError(FastWignerError):
+    pass
+
+
+class CompileError(FastWignerError):
+    """hipRTC rejected the generated source, e.g. a malformed V/K/mask string."""
+
+
+class HipError(FastWignerError):
+    pass
+
+
+class NoDeviceError(FastWignerError):
+    pass
+
+
+class UnsupportedError(FastWignerError):
+    pass
+
+
+_ERRORS = {-1: InvalidArgumentError, -2: CompileError, -3: HipError, -4: NoDeviceError, -5: UnsupportedError}
+
+
+class WignerDesc(ctypes.Structure):
+    _fields_ = [
+        ("x_grid_dim", ctypes.c_int), ("p_grid_dim", ctypes.c_int),
+        ("x_amplitude", ctypes.c_double), ("p_amplitude", ctypes.c_double),
+        ("dt", ctypes.c_double), ("phase_time_shift", ctypes.c_double),
+        ("precision", ctypes.c_int), ("device", ctypes.c_int),
+        ("col_e", ctypes.c_int), ("col_p", ctypes.c_int),
+        ("row_e", ctypes.c_int), ("row_p", ctypes.c_int),
+    ]
+
+
+class WignerExprs(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_char_p) for n in (
+        "consts", "V", "K", "diff_V", "diff_K",
+        "abs_boundary_x_theta", "abs_boundary_lambda_p", "abs_boundary_x_p")]
+
+
+class RhoDesc(ctypes.Structure):
+    _fields_ = [
+        ("x_grid_dim", ctypes.c_int), ("pad_", ctypes.c_int),
+        ("x_amplitude", ctypes.c_double), ("dt", ctypes.c_double),
+        ("phase_time_shift", ctypes.c_double),
+        ("precision", ctypes.c_int), ("device", ctypes.c_int),
+        ("row_e", ctypes.c_int), ("col_e", ctypes.c_int), ("col_p", ctypes.c_int), ("pad2_", ctypes.c_int),
+    ]
+
+
+class RhoExprs(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_char_p) for n in (
+        "consts", "V", "K", "diff_V", "diff_K", "abs_boundary_x", "abs_boundary_p")]
+
+
+def _b(s):
+    return None if s is None else s.encode()
+
+
+def wigner_desc(nx, xamp, npp, pamp, dt, shift=0.5, precision=0, device=0, tiles=None):
+    d = WignerDesc()
+    d.x_grid_dim, d.p_grid_dim = int(nx), int(npp)
+    d.x_amplitude, d.p_amplitude = float(xamp), float(pamp)
+    d.dt, d.phase_time_shift = float(dt), float(shift)
+    d.precision, d.device = int(precision), int(device)
+    if tiles:
+        d.col_e, d.col_p, d.row_e, d.row_p = [int(v) for v in tiles]
+    return d
+
+
+def wigner_exprs(V, K, consts=None, diff_V=None, diff_K=None, abs_boundary_x_theta=None,
+                 abs_boundary_lambda_p=None, abs_boundary_x_p=None):
+    e = WignerExprs()
+    e.consts, e.V, e.K = _b(consts), _b(V), _b(K)
+    e.diff_V, e.diff_K = _b(diff_V), _b(diff_K)
+    e.abs_boundary_x_theta = _b(abs_boundary_x_theta)
+    e.abs_boundary_lambda_p = _b(abs_boundary_lambda_p)
+    e.abs_boundary_x_p = _b(abs_boundary_x_p)
+    return e
+
+
+_P = ctypes.c_void_p
+_D = ctypes.c_double
+_I = ctypes.c_int
+_PD = ctypes.POINTER(ctypes.c_double)
+_PI = ctypes.POINTER(ctypes.c_int)
+_PP = ctypes.POINTER(ctypes.c_void_p)
+
+# name -> (argtypes); every entry point returns int
+PROTOTYPES = {
+    "fw_version": [],
+    "fw_device_count": [_PI],
+    "fw_wigner_grid_constants": [ctypes.POINTER(WignerDesc), _PD],
+    "fw_wigner_check": [ctypes.POINTER(WignerDesc), ctypes.POINTER(WignerExprs)],
+    "fw_wigner_create": [ctypes.POINTER(WignerDesc), ctypes.POINTER(WignerExprs), _PP],
+    "fw_wigner_destroy": [_P],
+    "fw_wigner_set_state": [_P, _PD],
+    "fw_wigner_set_state_expr": [_P, ctypes.c_char_p],
+    "fw_wigner_fill": [_P, _D],
+    "fw_wigner_normalize": [_P, _PD],
+    "fw_wigner_get_state": [_P, _PD],
+    "fw_wigner_state_ptr": [_P, _PP, _PI],
+    "fw_wigner_propagate": [_P, _PD, _I, _PD],
+    "fw_wigner_moments": [_P, _D, _PD],
+    "fw_wigner_get_average": [_P, ctypes.c_char_p, _D, _PD],
+    "fw_wigner_bloch_setup": [_P, _I, _D, _PD],
+    "fw_wigner_bloch_propagate": [_P, _D, _I, _D, _PD, _PI],
+    "fw_wigner_set_dissipator": [_P, _PD, _D],
+    "fw_wigner_snapshot": [_P, _I],
+    "fw_wigner_set_timing": [_P, _I],
+    "fw_wigner_kernel_times": [_P, _PD],
+}
+
+_lib = None
+
+
+def load(force_reload=False):
+    """Load libfastwigner.so (no fallback) and attach prototypes."""
+    global _lib
+    if _lib is not None and not force_reload:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FastWignerError(
+            "libfastwigner.so not found at %s; build it with `python -m fastwigner_amd.build`" % LIB_PATH)
+    if "FASTWIGNER_CACHE_DIR" not in os.environ and os.access(HERE, os.W_OK):
+        os.environ["FASTWIGNER_CACHE_DIR"] = KCACHE
+    lib = ctypes.CDLL(LIB_PATH)
+    lib.fw_last_error.restype = ctypes.c_char_p
+    lib.fw_last_error.argtypes = []
+    for name, args in PROTOTYPES.items():
+        fn = getattr(lib, name, None)
+        if fn is None:
+            continue
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def check(rc, what=""):
+    """Raise the typed exception of a non-zero status (cufft.py:82-102)."""
+    if rc != 0:
+        msg = load().fw_last_error().decode(errors="replace")
+        raise _ERRORS.get(rc, FastWignerError)("%s: %s" % (what, msg) if what else msg)
+    return rc
+
+
+def call(name, *args):
+    return check(getattr(load(), name)(*args), name)
+
+
+def dptr(a):
+    """ctypes double* to a contiguous float64 numpy array."""
+    return a.ctypes.data_as(_PD)

@@ -1,0 +1,718 @@
+// fw_wigner.cpp -- Wigner-function propagator handles (Moyal, Bloch, random
+// collision) of libfastwigner.so. See include/fastwigner.h for the contract.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "fastwigner.h"
+#include "fw_internal.h"
+
+using fw::KArgs;
+
+namespace {
+
+const int kChunk = 64;   // steps per captured graph / per times-series upload
+
+struct Tiles {
+  int col_e, col_p, row_e, row_p;
+};
+
+int default_e(int n) {
+  if (n >= 256) return 16;
+  if (n >= 8) return 8;
+  return n;
+}
+
+// Default tiles: E elements per thread; column pairs per block so a wave reads
+// >= 32 contiguous bytes per row while the grid keeps >= 256 blocks where the
+// grid allows; rows fill one wave.
+Tiles default_tiles(int nx, int np, bool fp32) {
+  Tiles t;
+  t.col_e = default_e(np);
+  t.row_e = default_e(nx);
+  const int pairs_col = nx / 2, pairs_row = np / 2;
+  const int esz = fp32 ? 8 : 16;
+  int p = 1;
+  while (p < 4 && pairs_col % (2 * p) == 0 && pairs_col / (2 * p) >= 256 &&
+         (size_t)(2 * p) * np * esz * 17 / 16 <= 48 * 1024 && (2 * p) * (np / t.col_e) <= 512)
+    p *= 2;
+  t.col_p = p;
+  const int tpl = nx / t.row_e;
+  int pr = tpl >= 64 ? 1 : 64 / tpl;
+  while (pr > 1 && pairs_row % pr) pr /= 2;
+  t.row_p = pr;
+  return t;
+}
+
+struct ExprMod {
+  hipModule_t mod = nullptr;
+  hipFunction_t fn = nullptr;
+};
+
+}  // namespace
+
+struct fw_wigner {
+  fw_wigner_desc d{};
+  Tiles tl{};
+  int NX = 0, NP = 0;
+  bool fp32 = false;
+  size_t esz = 8;
+  double dX = 0, dP = 0, dLambda = 0, dTheta = 0;
+  std::string prelude;  // constants + spliced expressions (shared with expression modules)
+  std::string arch;
+  bool has_mask_xp = false;
+
+  hipModule_t mod = nullptr, mod_bloch = nullptr, mod_dissip = nullptr;
+  hipFunction_t f_v_first = nullptr, f_v_last = nullptr, f_v_first_dissip = nullptr, f_v_last_dissip = nullptr,
+                f_k = nullptr, f_bv_first = nullptr, f_bv_last = nullptr, f_bk = nullptr, f_moments = nullptr,
+                f_apply_norm = nullptr, f_set_scalar = nullptr, f_vk_min = nullptr;
+  hipStream_t stream = nullptr;
+
+  void* W = nullptr;
+  void* W0s = nullptr;
+  void* snap = nullptr;
+  void* tw_col = nullptr;
+  void* tw_row = nullptr;
+  double* times = nullptr;
+  double* scal = nullptr;
+  double* partials = nullptr;
+  unsigned int* counter = nullptr;
+  double* series = nullptr;
+  int col_nb = 0, row_nb = 0, col_nt = 0, row_nt = 0;
+
+  bool dissip = false;
+  double w_coeff = 1.0;
+  int bloch_conv = FW_BLOCH_CUDA;
+  double t_initial = 0.0, vmin = 0.0, kmin = 0.0;
+  bool bloch_ready = false;
+
+  hipGraphExec_t gexec = nullptr;
+  int gkind = -1;
+
+  std::map<std::string, ExprMod> init_mods, avg_mods;
+
+  bool timing = false;
+  double ktimes[3] = {0, 0, 0};
+};
+
+namespace {
+
+int validate(const fw_wigner_desc* d, const fw_wigner_exprs* x) {
+  if (!d || !x) return fw::fail(FW_ERR_INVALID_ARG, "descriptor or expressions are NULL");
+  if (d->x_grid_dim % 2 || d->p_grid_dim % 2)
+    return fw::fail(FW_ERR_INVALID_ARG, "X_gridDIM and P_gridDIM must be even");
+  if (!fw::is_pow2(d->x_grid_dim) || !fw::is_pow2(d->p_grid_dim) || d->x_grid_dim < 8 || d->p_grid_dim < 8 ||
+      d->x_grid_dim > 4096 || d->p_grid_dim > 4096)
+    return fw::fail(FW_ERR_INVALID_ARG, "grid dimensions must be powers of two in [8, 4096]");
+  if (!(d->x_amplitude > 0) || !(d->p_amplitude > 0))
+    return fw::fail(FW_ERR_INVALID_ARG, "X_amplitude and P_amplitude must be positive");
+  if (!std::isfinite(d->dt)) return fw::fail(FW_ERR_INVALID_ARG, "dt must be finite");
+  if (d->precision != FW_FP64 && d->precision != FW_FP32)
+    return fw::fail(FW_ERR_INVALID_ARG, "precision must be FW_FP64 or FW_FP32");
+  if (!x->V || !x->K) return fw::fail(FW_ERR_INVALID_ARG, "V and K expressions are required");
+  return FW_OK;
+}
+
+void grid_constants(const fw_wigner_desc* d, double* dX, double* dP, double* dL, double* dT) {
+  // as the reference: dX = 2 X_amplitude / X_gridDIM; Lambda = fftfreq(N, dX/(2 pi))
+  // whose spacing numpy computes as 1.0/(N * (dX/(2 pi))) (wigner_moyal_cuda_1d.py:113-135)
+  const double two_pi = 2.0 * 3.141592653589793;
+  *dX = 2.0 * d->x_amplitude / d->x_grid_dim;
+  *dP = 2.0 * d->p_amplitude / d->p_grid_dim;
+  *dL = 1.0 / ((double)d->x_grid_dim * (*dX / two_pi));
+  *dT = 1.0 / ((double)d->p_grid_dim * (*dP / two_pi));
+}
+
+int resolve_tiles(const fw_wigner_desc* d, Tiles& t) {
+  const bool fp32 = d->precision == FW_FP32;
+  t = default_tiles(d->x_grid_dim, d->p_grid_dim, fp32);
+  if (d->col_e > 0) t.col_e = d->col_e;
+  if (d->col_p > 0) t.col_p = d->col_p;
+  if (d->row_e > 0) t.row_e = d->row_e;
+  if (d->row_p > 0) t.row_p = d->row_p;
+  auto ok_e = [](int e, int n) { return fw::is_pow2(e) && e >= 2 && e <= n && e <= 32; };
+  if (!ok_e(t.col_e, d->p_grid_dim) || !ok_e(t.row_e, d->x_grid_dim))
+    return fw::fail(FW_ERR_INVALID_ARG, "elements per thread must be a power of two in [2, min(N, 32)]");
+  if (!fw::is_pow2(t.col_p) || (d->x_grid_dim / 2) % t.col_p || !fw::is_pow2(t.row_p) ||
+      (d->p_grid_dim / 2) % t.row_p)
+    return fw::fail(FW_ERR_INVALID_ARG, "lines per block must be a power of two dividing the line count");
+  const int col_nt = t.col_p * (d->p_grid_dim / t.col_e);
+  const int row_nt = t.row_p * (d->x_grid_dim / t.row_e);
+  if (col_nt > 1024 || row_nt > 1024) return fw::fail(FW_ERR_INVALID_ARG, "tile needs more than 1024 threads");
+  const size_t esz = fp32 ? 8 : 16;
+  const size_t lds_c = (size_t)t.col_p * d->p_grid_dim, lds_r = (size_t)t.row_p * d->x_grid_dim;
+  if ((lds_c + lds_c / 16 + 1) * esz > 64 * 1024 || (lds_r + lds_r / 16 + 1) * esz > 64 * 1024)
+    return fw::fail(FW_ERR_INVALID_ARG, "tile needs more than 64 KiB of LDS");
+  return FW_OK;
+}
+
+std::string or_default(const char* s, const char* d) { return (s && *s) ? std::string(s) : std::string(d); }
+
+// Constants and spliced expressions, wigner_moyal_cuda_1d.py:151-166,244-277.
+std::string make_prelude(const fw_wigner_desc* d, const fw_wigner_exprs* x, const Tiles& t) {
+  double dX, dP, dL, dT;
+  grid_constants(d, &dX, &dP, &dL, &dT);
+  const bool fp32 = d->precision == FW_FP32;
+  std::ostringstream s;
+  s << "#define FW_REAL " << (fp32 ? "float" : "double") << "\n";
+  s << "#define FW_FP32 " << (fp32 ? 1 : 0) << "\n";
+  s << "#define FW_NX " << d->x_grid_dim << "\n#define FW_NP " << d->p_grid_dim << "\n";
+  s << "#define FW_COL_E " << t.col_e << "\n#define FW_COL_P " << t.col_p << "\n";
+  s << "#define FW_ROW_E " << t.row_e << "\n#define FW_ROW_P " << t.row_p << "\n";
+  s << "#define FW_PHASE_SHIFT (" << fw::fmt_double(d->phase_time_shift) << ")\n";
+  s << "#ifndef M_PI\n#define M_PI 3.14159265358979323846\n#endif\n";
+  s << "const int X_gridDIM = " << d->x_grid_dim << ";\n";
+  s << "const int P_gridDIM = " << d->p_grid_dim << ";\n";
+  s << "const double X_amplitude = " << fw::fmt_double(d->x_amplitude) << ";\n";
+  s << "const double P_amplitude = " << fw::fmt_double(d->p_amplitude) << ";\n";
+  s << "const double dt = " << fw::fmt_double(d->dt) << ";\n";
+  s << "const double dX = " << fw::fmt_double(dX) << ";\n";
+  s << "const double dP = " << fw::fmt_double(dP) << ";\n";
+  s << "const double dLambda = " << fw::fmt_double(dL) << ";\n";
+  s << "const double dTheta = " << fw::fmt_double(dT) << ";\n";
+  s << "// ---- caller constants and functions ----\n";
+  if (x->consts) s << x->consts << "\n";
+  s << "// ---- spliced expressions ----\n";
+  s << "__device__ double V(double X, double t) { return (" << x->V << "); }\n";
+  s << "__device__ double K(double P, double t) { return (" << x->K << "); }\n";
+  s << "__device__ double fw_diff_V(double X, double t) { return (" << or_default(x->diff_V, "0.") << "); }\n";
+  s << "__device__ double fw_diff_K(double P, double t) { return (" << or_default(x->diff_K, "0.") << "); }\n";
+  s << "__device__ double fw_mask_x_theta(double X, double Theta, double t) {\n"
+       "  const double X_minus = X - 0.5 * Theta; const double X_plus = X + 0.5 * Theta;\n"
+       "  (void)X_minus; (void)X_plus;\n  return ("
+    << or_default(x->abs_boundary_x_theta, "1.") << ");\n}\n";
+  s << "__device__ double fw_mask_lambda_p(double Lambda, double P, double t) { return ("
+    << or_default(x->abs_boundary_lambda_p, "1.") << "); }\n";
+  s << "__device__ double fw_mask_x_p(double X, double P, double t) { return ("
+    << or_default(x->abs_boundary_x_p, "1.") << "); }\n";
+  return s.str();
+}
+
+std::string kset_source(const std::string& prelude, int kset) {
+  return prelude + "#define FW_KSET " + std::to_string(kset) + "\n#include \"fw_device.h\"\n";
+}
+
+int load_module(const std::vector<char>& code, hipModule_t* mod) {
+  HIPCHK(hipModuleLoadData(mod, code.data()));
+  return FW_OK;
+}
+
+int getfn(hipModule_t m, const char* name, hipFunction_t* f) {
+  HIPCHK(hipModuleGetFunction(f, m, name));
+  return FW_OK;
+}
+
+int launch(fw_wigner* h, hipFunction_t f, int grid, int block, KArgs& a) {
+  void* params[] = {&a};
+  HIPCHK(hipModuleLaunchKernel(f, grid, 1, 1, block, 1, 1, 0, h->stream, params, nullptr));
+  return FW_OK;
+}
+
+KArgs base_args(fw_wigner* h) {
+  KArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.W = h->W;
+  a.W0s = h->W0s;
+  a.tw_col = h->tw_col;
+  a.tw_row = h->tw_row;
+  a.times = h->times;
+  a.scal = h->scal;
+  a.partials = h->partials;
+  a.counter = h->counter;
+  a.series = h->series;
+  a.w_coeff = h->w_coeff;
+  a.flags = h->has_mask_xp ? fw::F_MASK_XP : 0;
+  a.dbeta = 0;
+  a.cv = a.ck = 0;
+  a.vmin = h->vmin;
+  a.kmin = h->kmin;
+  a.t_initial = h->t_initial;
+  a.guard = 0;
+  return a;
+}
+
+int set_scalar(fw_wigner* h, int slot, double v) {
+  double* p = h->scal + slot;
+  void* params[] = {&p, &v};
+  HIPCHK(hipModuleLaunchKernel(h->f_set_scalar, 1, 1, 1, 1, 1, 1, 0, h->stream, params, nullptr));
+  return FW_OK;
+}
+
+int helper_blocks(fw_wigner* h) { return h->NP < 1024 ? h->NP : 1024; }
+
+// Step kernels of one step s (chunk-local). kind 0 = Moyal, 1 = Bloch.
+int enqueue_step(fw_wigner* h, int kind, int s, const KArgs& base, hipEvent_t* ev) {
+  KArgs a = base;
+  a.step = s;
+  hipFunction_t f1, f2, f3;
+  if (kind == 1) {
+    f1 = h->f_bv_first; f2 = h->f_bk; f3 = h->f_bv_last;
+  } else {
+    f1 = h->dissip ? h->f_v_first_dissip : h->f_v_first;
+    f2 = h->f_k;
+    f3 = h->dissip ? h->f_v_last_dissip : h->f_v_last;
+  }
+  if (ev) HIPCHK(hipEventRecord(ev[0], h->stream));
+  FWCHK(launch(h, f1, h->col_nb, h->col_nt, a));
+  if (ev) HIPCHK(hipEventRecord(ev[1], h->stream));
+  FWCHK(launch(h, f2, h->row_nb, h->row_nt, a));
+  if (ev) HIPCHK(hipEventRecord(ev[2], h->stream));
+  FWCHK(launch(h, f3, h->col_nb, h->col_nt, a));
+  if (ev) HIPCHK(hipEventRecord(ev[3], h->stream));
+  return FW_OK;
+}
+
+int ensure_graph(fw_wigner* h, int kind, const KArgs& base) {
+  if (h->gexec && h->gkind == kind) return FW_OK;
+  if (h->gexec) {
+    (void)hipGraphExecDestroy(h->gexec);
+    h->gexec = nullptr;
+  }
+  hipGraph_t g;
+  HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  int rc = FW_OK;
+  for (int s = 0; s < kChunk && rc == FW_OK; ++s) rc = enqueue_step(h, kind, s, base, nullptr);
+  hipError_t e = hipStreamEndCapture(h->stream, &g);
+  if (rc != FW_OK) return rc;
+  if (e != hipSuccess) return fw::fail(FW_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  e = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) return fw::fail(FW_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  h->gkind = kind;
+  return FW_OK;
+}
+
+// Run `steps` steps of `kind`; times[s] per step; series (optional) gets NSER
+// sums per step. Returns the number of steps applied before a guard stop.
+int run_steps(fw_wigner* h, int kind, const KArgs& base, const double* times, int steps, double* series,
+              int* done) {
+  FWCHK(set_scalar(h, fw::SC_STOP, 0.0));
+  const bool use_graph = !h->timing && kind == 0;
+  if (use_graph && steps >= kChunk) FWCHK(ensure_graph(h, kind, base));
+  std::vector<hipEvent_t> evs;
+  double acc[3] = {0, 0, 0};
+  int nacc = 0;
+  int applied = steps;
+  for (int off = 0; off < steps; off += kChunk) {
+    const int n = std::min(kChunk, steps - off);
+    HIPCHK(hipMemcpyAsync(h->times, times + off, sizeof(double) * n, hipMemcpyHostToDevice, h->stream));
+    if (use_graph && n == kChunk) {
+      HIPCHK(hipGraphLaunch(h->gexec, h->stream));
+    } else if (h->timing) {
+      evs.resize(4 * n);
+      for (auto& e : evs) HIPCHK(hipEventCreate(&e));
+      for (int s = 0; s < n; ++s) FWCHK(enqueue_step(h, kind, s, base, &evs[4 * s]));
+      HIPCHK(hipStreamSynchronize(h->stream));
+      for (int s = 0; s < n; ++s) {
+        for (int k = 0; k < 3; ++k) {
+          float ms = 0;
+          HIPCHK(hipEventElapsedTime(&ms, evs[4 * s + k], evs[4 * s + k + 1]));
+          acc[k] += ms * 1000.0;
+        }
+        ++nacc;
+      }
+      for (auto& e : evs) (void)hipEventDestroy(e);
+    } else {
+      for (int s = 0; s < n; ++s) FWCHK(enqueue_step(h, kind, s, base, nullptr));
+    }
+    if (series)
+      HIPCHK(hipMemcpyAsync(series + (size_t)off * FW_NSER, h->series, sizeof(double) * FW_NSER * n,
+                            hipMemcpyDeviceToHost, h->stream));
+    if (base.guard > 0) {
+      double stop = 0;
+      HIPCHK(hipMemcpyAsync(&stop, h->scal + fw::SC_STOP, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+      HIPCHK(hipStreamSynchronize(h->stream));
+      if (stop != 0) {
+        applied = off + (int)stop;
+        break;
+      }
+    }
+  }
+  if (nacc)
+    for (int k = 0; k < 3; ++k) h->ktimes[k] = acc[k] / nacc;
+  // the deferred normalisation of the last step
+  KArgs a = base;
+  FWCHK(launch(h, h->f_apply_norm, 1024, 256, a));
+  FWCHK(set_scalar(h, fw::SC_NORM, 1.0));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (done) *done = applied;
+  return FW_OK;
+}
+
+int ensure_kset(fw_wigner* h, int kset) {
+  hipModule_t* m = kset == 2 ? &h->mod_bloch : &h->mod_dissip;
+  if (*m) return FW_OK;
+  std::vector<char> code;
+  FWCHK(fw::compile(kset_source(h->prelude, kset), h->arch, code));
+  FWCHK(load_module(code, m));
+  if (kset == 2) {
+    FWCHK(getfn(*m, "fw_bv_first", &h->f_bv_first));
+    FWCHK(getfn(*m, "fw_bv_last", &h->f_bv_last));
+    FWCHK(getfn(*m, "fw_bk", &h->f_bk));
+  } else {
+    FWCHK(getfn(*m, "fw_v_first_dissip", &h->f_v_first_dissip));
+    FWCHK(getfn(*m, "fw_v_last_dissip", &h->f_v_last_dissip));
+  }
+  return FW_OK;
+}
+
+int build_expr_module(fw_wigner* h, const std::string& body, const char* fname, ExprMod& em) {
+  std::string src = kset_source(h->prelude, 0) + body;
+  std::vector<char> code;
+  FWCHK(fw::compile(src, h->arch, code));
+  FWCHK(load_module(code, &em.mod));
+  FWCHK(getfn(em.mod, fname, &em.fn));
+  return FW_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" int fw_wigner_grid_constants(const fw_wigner_desc* desc, double out[4]) {
+  if (!desc || !out) return fw::fail(FW_ERR_INVALID_ARG, "NULL argument");
+  grid_constants(desc, &out[0], &out[1], &out[2], &out[3]);
+  return FW_OK;
+}
+
+extern "C" int fw_wigner_check(const fw_wigner_desc* desc, const fw_wigner_exprs* exprs) {
+  fw::clear_error();
+  FWCHK(validate(desc, exprs));
+  Tiles t;
+  FWCHK(resolve_tiles(desc, t));
+  std::vector<char> code;
+  const std::string pre = make_prelude(desc, exprs, t);
+  for (int kset = 1; kset <= 3; ++kset) FWCHK(fw::compile(kset_source(pre, kset), "gfx950", code));
+  return FW_OK;
+}
+
+extern "C" int fw_wigner_create(const fw_wigner_desc* desc, const fw_wigner_exprs* exprs, fw_wigner** out) {
+  fw::clear_error();
+  if (!out) return fw::fail(FW_ERR_INVALID_ARG, "out is NULL");
+  *out = nullptr;
+  FWCHK(validate(desc, exprs));
+  Tiles t;
+  FWCHK(resolve_tiles(desc, t));
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fw::fail(FW_ERR_NO_DEVICE, "no GPU visible");
+  if (desc->device < 0 || desc->device >= ndev) return fw::fail(FW_ERR_INVALID_ARG, "device ordinal out of range");
+  HIPCHK(hipSetDevice(desc->device));
+
+  fw_wigner* h = new fw_wigner();
+  h->d = *desc;
+  h->tl = t;
+  h->NX = desc->x_grid_dim;
+  h->NP = desc->p_grid_dim;
+  h->fp32 = desc->precision == FW_FP32;
+  h->esz = h->fp32 ? 4 : 8;
+  grid_constants(desc, &h->dX, &h->dP, &h->dLambda, &h->dTheta);
+  h->has_mask_xp = exprs->abs_boundary_x_p && *exprs->abs_boundary_x_p;
+  h->prelude = make_prelude(desc, exprs, t);
+  h->arch = fw::target_arch(desc->device);
+  h->col_nt = t.col_p * (h->NP / t.col_e);
+  h->row_nt = t.row_p * (h->NX / t.row_e);
+  h->col_nb = (h->NX / 2) / t.col_p;
+  h->row_nb = (h->NP / 2) / t.row_p;
+
+  auto bail = [&](int rc) {
+    fw_wigner_destroy(h);
+    return rc;
+  };
+  std::vector<char> code;
+  int rc = fw::compile(kset_source(h->prelude, 1), h->arch, code);
+  if (rc != FW_OK) return bail(rc);
+  if ((rc = load_module(code, &h->mod)) != FW_OK) return bail(rc);
+  struct {
+    const char* n;
+    hipFunction_t* f;
+  } fns[] = {{"fw_v_first", &h->f_v_first},   {"fw_v_last", &h->f_v_last},
+             {"fw_k", &h->f_k},               {"fw_moments", &h->f_moments},
+             {"fw_apply_norm", &h->f_apply_norm}, {"fw_set_scalar", &h->f_set_scalar},
+             {"fw_vk_min", &h->f_vk_min}};
+  for (auto& f : fns)
+    if ((rc = getfn(h->mod, f.n, f.f)) != FW_OK) return bail(rc);
+
+  auto alloc = [&](void** p, size_t bytes) -> int {
+    HIPCHK(hipMalloc(p, bytes));
+    HIPCHK(hipMemset(*p, 0, bytes));
+    return FW_OK;
+  };
+  const size_t npts = (size_t)h->NX * h->NP;
+  const int maxnb = std::max(std::max(h->col_nb, h->row_nb), 1024);
+  if ((rc = alloc(&h->W, npts * h->esz)) != FW_OK) return bail(rc);
+  if ((rc = alloc((void**)&h->times, sizeof(double) * kChunk)) != FW_OK) return bail(rc);
+  if ((rc = alloc((void**)&h->scal, sizeof(double) * fw::NSCAL)) != FW_OK) return bail(rc);
+  if ((rc = alloc((void**)&h->partials, sizeof(double) * FW_NSER * maxnb)) != FW_OK) return bail(rc);
+  if ((rc = alloc((void**)&h->counter, sizeof(unsigned int) * 4)) != FW_OK) return bail(rc);
+  if ((rc = alloc((void**)&h->series, sizeof(double) * FW_NSER * kChunk)) != FW_OK) return bail(rc);
+  if ((rc = fw::upload_twiddles(h->NP, h->fp32, &h->tw_col)) != FW_OK) return bail(rc);
+  if ((rc = fw::upload_twiddles(h->NX, h->fp32, &h->tw_row)) != FW_OK) return bail(rc);
+  if (hipStreamCreate(&h->stream) != hipSuccess) return bail(fw::fail(FW_ERR_HIP, "hipStreamCreate failed"));
+  if ((rc = set_scalar(h, fw::SC_NORM, 1.0)) != FW_OK) return bail(rc);
+  if (hipStreamSynchronize(h->stream) != hipSuccess) return bail(fw::fail(FW_ERR_HIP, "initial sync failed"));
+  *out = h;
+  return FW_OK;
+}
+
+extern "C" int fw_wigner_destroy(fw_wigner* h) {
+  if (!h) return FW_OK;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+  for (auto* m : {&h->init_mods, &h->avg_mods})
+    for (auto& kv : *m)
+      if (kv.second.mod) (void)hipModuleUnload(kv.second.mod);
+  for (hipModule_t m : {h->mod, h->mod_bloch, h->mod_dissip})
+    if (m) (void)hipModuleUnload(m);
+  for (void* p : {h->W, h->W0s, h->snap, h->tw_col, h->tw_row, (void*)h->times, (void*)h->scal,
+                  (void*)h->partials, (void*)h->counter, (void*)h->series})
+    if (p) (void)hipFree(p);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return FW_OK;
+}
+
+static int upload_real(fw_wigner* h, void* dst, const double* src) {
+  const size_t n = (size_t)h->NX * h->NP;
+  if (h->fp32) {
+    std::vector<float> tmp(n);
+    for (size_t i = 0; i < n; ++i) tmp[i] = (float)src[i];
+    HIPCHK(hipMemcpy(dst, tmp.data(), n * 4, hipMemcpyHostToDevice));
+  } else {
+    HIPCHK(hipMemcpy(dst, src, n * 8, hipMemcpyHostToDevice));
+  }
+  return FW_OK;
+}
+
+extern "C" int fw_wigner_set_state(fw_wigner* h, const double* host_w) {
+  if (!h || !host_w) return fw::fail(FW_ERR_INVALID_ARG, "NULL argument");
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return upload_real(h, h->W, host_w);
+}
+
+extern "C" int fw_wigner_set_state_expr(fw_wigner* h, const char* expr) {
+  if (!h || !expr) return fw::fail(FW_ERR_INVALID_ARG, "NULL argument");
+  auto it = h->init_mods.find(expr);
+  if (it == h->init_mods.end()) {
+    // init_wigner, wigner_moyal_cuda_1d.py:666-685
+    std::string body =
+        "extern \"C\" __global__ void fw_init_expr(FW_REAL* W) {\n"
+        "  const int i = blockIdx.y;\n"
+        "  const int j = threadIdx.x + blockDim.x * blockIdx.x;\n"
+        "  if (j >= X_gridDIM) return;\n"
+        "  const size_t indexTotal = (size_t)j + (size_t)i * X_gridDIM; (void)indexTotal;\n"
+        "  const double X = dX * (j - 0.5 * X_gridDIM);\n"
+        "  const double P = dP * (i - 0.5 * P_gridDIM);\n"
+        "  W[indexTotal] = (FW_REAL)(" +
+        std::string(expr) + ");\n}\n";
+    ExprMod em;
+    FWCHK(build_expr_module(h, body, "fw_init_expr", em));
+    it = h->init_mods.emplace(expr, em).first;
+  }
+  void* W = h->W;
+  void* params[] = {&W};
+  const int bx = std::min(h->NX, 256);
+  HIPCHK(hipModuleLaunchKernel(it->second.fn, (h->NX + bx - 1) / bx, h->NP, 1, bx, 1, 1, 0, h->stream, params,
+                               nullptr));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return FW_OK;
+}
+
+extern "C" int fw_wigner_fill(fw_wigner* h, double value) {
+  if (!h) return fw::fail(FW_ERR_INVALID_ARG, "NULL handle");
+  std::vector<double> v((size_t)h->NX * h->NP, value);
+  return fw_wigner_set_state(h, v.data());
+}
+
+static int moments_at(fw_wigner* h, double t) {
+  HIPCHK(hipMemcpyAsync(h->times, &t, sizeof(double), hipMemcpyHostToDevice, h->stream));
+  KArgs a = base_args(h);
+  a.step = 0;
+  FWCHK(launch(h, h->f_moments, helper_blocks(h), 256, a));
+  return FW_OK;
+}
+
+extern "C" int fw_wigner_normalize(fw_wigner* h, double* norm_before) {
+  if (!h) return fw::fail(FW_ERR_INVALID_ARG, "NULL handle");
+  FWCHK(moments_at(h, 0.0));
+  KArgs a = base_args(h);
+  FWCHK(launch(h, h->f_apply_norm, 1024, 256, a));
+  FWCHK(set_scalar(h, fw::SC_NORM, 1.0));
+  double s0 = 0;
+  HIPCHK(hipMemcpyAsync(&s0, h->scal + fw::SC_OUT, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (norm_before) *norm_before = s0 * h->dX * h->dP;
+  return FW_OK;
+}
+
+extern "C" int fw_wigner_get_state(fw_wigner* h, double* host_w) {
+  if (!h || !host_w) return fw::fail(FW_ERR_INVALID_ARG, "NULL argument");
+  HIPCHK(hipStreamSynchronize(h->stream));
+  const size_t n = (size_t)h->NX * h->NP;
+  if (h->fp32) {
+    std::vector<float> tmp(n);
+    HIPCHK(hipMemcpy(tmp.data(), h->W, n * 4, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i) host_w[i] = tmp[i];
+  } else {
+    HIPCHK(hipMemcpy(host_w, h->W, n * 8, hipMemcpyDeviceToHost));
+  }
+  return FW_OK;
+}
+
+extern "C" int fw_wigner_state_ptr(fw_wigner* h, void** dev_ptr, int* elem_size) {
+  if (!h || !dev_ptr || !elem_size) return fw::fail(FW_ERR_INVALID_ARG, "NULL argument");
+  *dev_ptr = h->W;
+  *elem_size = (int)h->esz;
+  return FW_OK;
+}
+
+extern "C" int fw_wigner_propagate(fw_wigner* h, const double* times, int steps, double* series) {
+  if (!h || (!times && steps > 0)) return fw::fail(FW_ERR_INVALID_ARG, "NULL argument");
+  if (steps < 0) return fw::fail(FW_ERR_INVALID_ARG, "steps must be >= 0");
+  if (steps == 0) return FW_OK;
+  KArgs a = base_args(h);
+  return run_steps(h, 0, a, times, steps, series, nullptr);
+}
+
+extern "C" int fw_wigner_moments(fw_wigner* h, double t, double sums[FW_NSER]) {
+  if (!h || !sums) return fw::fail(FW_ERR_INVALID_ARG, "NULL argument");
+  FWCHK(moments_at(h, t));
+  HIPCHK(hipMemcpyAsync(sums, h->scal + fw::SC_OUT, sizeof(double) * FW_NSER, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return FW_OK;
+}
+
+extern "C" int fw_wigner_get_average(fw_wigner* h, const char* expr, double t, double* out) {
+  if (!h || !expr || !out) return fw::fail(FW_ERR_INVALID_ARG, "NULL argument");
+  auto it = h->avg_mods.find(expr);
+  if (it == h->avg_mods.end()) {
+    // weighted_func_cuda_code + gpuarray.sum, wigner_moyal_cuda_1d.py:378-396,687-709
+    std::string body =
+        "extern \"C\" __global__ void __launch_bounds__(256) fw_avg_expr(fwk::Args a) {\n"
+        "  __shared__ double red[fwk::NSER * 4];\n"
+        "  const double t = a.times[a.step];\n"
+        "  double s[fwk::NSER];\n"
+        "  for (int k = 0; k < fwk::NSER; ++k) s[k] = 0.0;\n"
+        "  for (int i = blockIdx.x; i < P_gridDIM; i += gridDim.x) {\n"
+        "    for (int j = threadIdx.x; j < X_gridDIM; j += 256) {\n"
+        "      const size_t indexTotal = (size_t)j + (size_t)i * X_gridDIM;\n"
+        "      const double X = dX * (j - 0.5 * X_gridDIM);\n"
+        "      const double P = dP * (i - 0.5 * P_gridDIM);\n"
+        "      s[0] += (double)a.W[indexTotal] * (" +
+        std::string(expr) +
+        ");\n"
+        "    }\n"
+        "  }\n"
+        "  fwk::block_reduce_publish<256>(s, red, a.partials, a.counter, blockIdx.x, gridDim.x, a.scal + "
+        "fwk::SC_OUT);\n"
+        "}\n";
+    ExprMod em;
+    FWCHK(build_expr_module(h, body, "fw_avg_expr", em));
+    it = h->avg_mods.emplace(expr, em).first;
+  }
+  HIPCHK(hipMemcpyAsync(h->times, &t, sizeof(double), hipMemcpyHostToDevice, h->stream));
+  KArgs a = base_args(h);
+  FWCHK(launch(h, it->second.fn, helper_blocks(h), 256, a));
+  double s = 0;
+  HIPCHK(hipMemcpyAsync(&s, h->scal + fw::SC_OUT, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  *out = s * h->dX * h->dP;
+  return FW_OK;
+}
+
+extern "C" int fw_wigner_bloch_setup(fw_wigner* h, int convention, double t_initial, double mins_out[2]) {
+  if (!h) return fw::fail(FW_ERR_INVALID_ARG, "NULL handle");
+  if (convention != FW_BLOCH_CUDA && convention != FW_BLOCH_FFTW)
+    return fw::fail(FW_ERR_INVALID_ARG, "unknown Bloch convention");
+  FWCHK(ensure_kset(h, 2));
+  h->bloch_conv = convention;
+  h->t_initial = t_initial;
+  KArgs a = base_args(h);
+  a.t_initial = t_initial;
+  FWCHK(launch(h, h->f_vk_min, 1, 1024, a));
+  double m[2];
+  HIPCHK(hipMemcpyAsync(m, h->scal + fw::SC_OUT, sizeof m, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->vmin = m[0];
+  h->kmin = m[1];
+  h->bloch_ready = true;
+  if (mins_out) {
+    mins_out[0] = m[0];
+    mins_out[1] = m[1];
+  }
+  return FW_OK;
+}
+
+extern "C" int fw_wigner_bloch_propagate(fw_wigner* h, double dbeta, int steps, double purity_guard,
+                                         double* series, int* steps_done) {
+  if (!h) return fw::fail(FW_ERR_INVALID_ARG, "NULL handle");
+  if (!h->bloch_ready) return fw::fail(FW_ERR_INVALID_ARG, "call fw_wigner_bloch_setup first");
+  if (steps < 0) return fw::fail(FW_ERR_INVALID_ARG, "steps must be >= 0");
+  if (steps_done) *steps_done = 0;
+  if (steps == 0) return FW_OK;
+  KArgs a = base_args(h);
+  a.dbeta = dbeta;
+  const bool fftw = h->bloch_conv == FW_BLOCH_FFTW;
+  a.cv = fftw ? 0.5 : 0.25;
+  a.ck = fftw ? 1.0 : 0.5;
+  if (fftw) a.flags |= fw::F_BLOCH_MASK;
+  a.flags &= ~fw::F_MASK_XP;
+  a.t_initial = h->t_initial;
+  a.vmin = h->vmin;
+  a.kmin = h->kmin;
+  a.guard = purity_guard;
+  std::vector<double> times(steps, h->t_initial);
+  return run_steps(h, 1, a, times.data(), steps, series, steps_done);
+}
+
+extern "C" int fw_wigner_set_dissipator(fw_wigner* h, const double* host_w0s, double w_coeff) {
+  if (!h) return fw::fail(FW_ERR_INVALID_ARG, "NULL handle");
+  if (!host_w0s) {
+    h->dissip = false;
+    h->w_coeff = 1.0;
+    return FW_OK;
+  }
+  FWCHK(ensure_kset(h, 3));
+  if (!h->W0s) HIPCHK(hipMalloc(&h->W0s, (size_t)h->NX * h->NP * h->esz));
+  FWCHK(upload_real(h, h->W0s, host_w0s));
+  h->dissip = true;
+  h->w_coeff = w_coeff;
+  if (h->gexec) {
+    (void)hipGraphExecDestroy(h->gexec);
+    h->gexec = nullptr;
+  }
+  return FW_OK;
+}
+
+extern "C" int fw_wigner_snapshot(fw_wigner* h, int which) {
+  if (!h) return fw::fail(FW_ERR_INVALID_ARG, "NULL handle");
+  const size_t bytes = (size_t)h->NX * h->NP * h->esz;
+  if (which == 0) {
+    if (!h->snap) HIPCHK(hipMalloc(&h->snap, bytes));
+    HIPCHK(hipMemcpyAsync(h->snap, h->W, bytes, hipMemcpyDeviceToDevice, h->stream));
+  } else if (which == 1) {
+    if (!h->snap) return fw::fail(FW_ERR_INVALID_ARG, "no snapshot saved");
+    HIPCHK(hipMemcpyAsync(h->W, h->snap, bytes, hipMemcpyDeviceToDevice, h->stream));
+  } else {
+    return fw::fail(FW_ERR_INVALID_ARG, "which must be 0 (save) or 1 (restore)");
+  }
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return FW_OK;
+}
+
+extern "C" int fw_wigner_set_timing(fw_wigner* h, int enable) {
+  if (!h) return fw::fail(FW_ERR_INVALID_ARG, "NULL handle");
+  h->timing = enable != 0;
+  return FW_OK;
+}
+
+extern "C" int fw_wigner_kernel_times(fw_wigner* h, double us_out[3]) {
+  if (!h || !us_out) return fw::fail(FW_ERR_INVALID_ARG, "NULL argument");
+  for (int k = 0; k < 3; ++k) us_out[k] = h->ktimes[k];
+  return FW_OK;
+}

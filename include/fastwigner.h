@@ -1,0 +1,158 @@
+/* fastwigner.h -- C ABI of libfastwigner.so, the MI355X (gfx950) engine of the
+ * split-operator phase-space propagators.
+ *
+ * The reference crosses into native code in two ways, both replaced here:
+ *   1. runtime-compiled CUDA kernels with the user's V/K/mask strings spliced in
+ *      (pycuda SourceModule, wigner_moyal_cuda_1d.py:244-277, templates :564-731),
+ *   2. the ctypes FFI over cuFFT (cufft.py:16-343: cufftPlanMany, cufftExec*, cufftDestroy).
+ * Here one handle owns the hipRTC-compiled module (same spliced strings), the
+ * device buffers, the stream and the captured graphs; the FFTs are the library's
+ * own LDS Stockham kernels fused with the phase multiplies.
+ *
+ * Conventions
+ *   - Every call returns FW_OK (0) or a negative status; fw_last_error() gives the
+ *     message of the last failure on the calling thread (cf. the typed cuFFT
+ *     exceptions of cufft.py:31-102).
+ *   - Host pointers are borrowed for the duration of the call only.
+ *   - A handle is not re-entrant; distinct handles are independent.
+ *   - Arrays are row-major: the Wigner function is (p_grid_dim, x_grid_dim) real
+ *     with x contiguous (wigner_moyal_cuda_1d.py:302); the density matrix is
+ *     (N, N) complex with rho[i][j] = <x_j|rho|x'_i> (rho_vneumann_cuda_1d.py:222).
+ */
+#ifndef FASTWIGNER_H
+#define FASTWIGNER_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FW_OK 0
+#define FW_ERR_INVALID_ARG (-1) /* bad sizes, null handle, unsupported grid */
+#define FW_ERR_COMPILE (-2)     /* hipRTC rejected the generated source (bad V/K/mask string) */
+#define FW_ERR_HIP (-3)         /* a HIP runtime call failed */
+#define FW_ERR_NO_DEVICE (-4)   /* no usable GPU */
+#define FW_ERR_UNSUPPORTED (-5) /* configuration not implemented */
+
+#define FW_NSER 9 /* per-step sums: S0 Sx Sp Sxx Spp SdK SdV SH SWW */
+
+#define FW_FP64 0
+#define FW_FP32 1 /* W and FFTs in FP32; phase argument, its mod-2pi reduction and all sums in FP64 */
+
+#define FW_BLOCH_CUDA 0 /* exp(-db/4 [V+V-Vmin]), exp(-db/2 [K+K-Kmin]) (wigner_bloch_cuda_1d.py:299-340) */
+#define FW_BLOCH_FFTW 1 /* exp(-db/2 [..]) * mask, exp(-db [..]) (wigner_bloch_fftw_1d.py:74-97) */
+
+/* ------------------------------------------------------------------------ */
+/* Library                                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* Library version (major*10000 + minor*100 + patch). */
+int fw_version(void);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* fw_last_error(void);
+
+/* Number of visible GPUs (0 on a machine without one; never an error). */
+int fw_device_count(int* count);
+
+/* ------------------------------------------------------------------------ */
+/* Wigner-function propagators (Moyal, Bloch, random collision)             */
+/* replaces WignerMoyalCUDA1D (wigner_moyal_cuda_1d.py:10-731),             */
+/* WignerBlochCUDA1D (wigner_bloch_cuda_1d.py:8-445) and the random-        */
+/* collision step of wigner_random_collision_model_fftw_1d.py:56-76          */
+/* ------------------------------------------------------------------------ */
+
+typedef struct fw_wigner fw_wigner;
+
+typedef struct fw_wigner_desc {
+  int x_grid_dim;          /* X_gridDIM: even power of two, 8..4096 */
+  int p_grid_dim;          /* P_gridDIM: even power of two, 8..4096 */
+  double x_amplitude;      /* X_amplitude: x in [-X_amplitude, X_amplitude) */
+  double p_amplitude;      /* P_amplitude */
+  double dt;               /* time step */
+  double phase_time_shift; /* V, K evaluated at t_{n+1} + shift*dt; 0.5 = the CUDA kernels (wigner_moyal_cuda_1d.py:602,643) */
+  int precision;           /* FW_FP64 or FW_FP32 */
+  int device;              /* HIP device ordinal */
+  int col_e, col_p;        /* V pass: elements per thread, column pairs per block (0 = default) */
+  int row_e, row_p;        /* K pass: elements per thread, row pairs per block (0 = default) */
+} fw_wigner_desc;
+
+typedef struct fw_wigner_exprs {
+  const char* consts;                /* C declarations (constants, __device__ functions), verbatim; may be NULL */
+  const char* V;                     /* V(X, t) */
+  const char* K;                     /* K(P, t) */
+  const char* diff_V;                /* dV/dX (X, t) for the Ehrenfest sums; NULL = "0." */
+  const char* diff_K;                /* dK/dP (P, t); NULL = "0." */
+  const char* abs_boundary_x_theta;  /* mask of (X, Theta, t) in the V factor; NULL = "1." */
+  const char* abs_boundary_lambda_p; /* mask of (Lambda, P, t) in the K factor; NULL = "1." */
+  const char* abs_boundary_x_p;      /* mask of (X, P, t) applied after each step; NULL = none */
+} fw_wigner_exprs;
+
+/* The grid constants spliced into the kernels: dX, dP, dLambda, dTheta
+ * (wigner_moyal_cuda_1d.py:113-135), computed as numpy does. */
+int fw_wigner_grid_constants(const fw_wigner_desc* desc, double out[4]);
+
+/* Generate and compile the kernel source with hipRTC without touching a device.
+ * On FW_ERR_COMPILE the compiler log is in fw_last_error(). */
+int fw_wigner_check(const fw_wigner_desc* desc, const fw_wigner_exprs* exprs);
+
+/* Compile (cached per source), allocate W and scratch, upload twiddles. */
+int fw_wigner_create(const fw_wigner_desc* desc, const fw_wigner_exprs* exprs, fw_wigner** out);
+int fw_wigner_destroy(fw_wigner* h);
+
+/* W <- host array (p_grid_dim*x_grid_dim doubles), no normalisation. */
+int fw_wigner_set_state(fw_wigner* h, const double* host_w);
+/* W <- expr(X, P) evaluated on the grid (init_wigner, wigner_moyal_cuda_1d.py:666-685); i, j, X, P in scope. */
+int fw_wigner_set_state_expr(fw_wigner* h, const char* expr);
+/* W <- value everywhere (set_wignerfunction(float), wigner_moyal_cuda_1d.py:485-487). */
+int fw_wigner_fill(fw_wigner* h, double value);
+/* W /= sum(W) dXdP (wigner_moyal_cuda_1d.py:491-492); returns sum(W) dXdP before scaling. */
+int fw_wigner_normalize(fw_wigner* h, double* norm_before);
+/* Copy W to a host array of doubles. */
+int fw_wigner_get_state(fw_wigner* h, double* host_w);
+/* Device address of W and its element size (8 or 4 bytes). */
+int fw_wigner_state_ptr(fw_wigner* h, void** dev_ptr, int* elem_size);
+
+/* Advance `steps` Moyal steps (wigner_moyal_cuda_1d.py:521-540). times[s] is t
+ * after the s-th increment (host-accumulated, as the reference's t += dt).
+ * After each step W is renormalised and the FW_NSER sums over the step's
+ * unnormalised W are written to series[s*FW_NSER..] when series != NULL.
+ * No host synchronisation inside; steps are replayed from a captured hipGraph. */
+int fw_wigner_propagate(fw_wigner* h, const double* times, int steps, double* series);
+
+/* The FW_NSER sums over the current W at time t (get_average building block). */
+int fw_wigner_moments(fw_wigner* h, double t, double sums[FW_NSER]);
+
+/* sum(W * expr(X, P, t)) dXdP (get_average, wigner_moyal_cuda_1d.py:378-396),
+ * compiled once per expression string and cached on the handle. */
+int fw_wigner_get_average(fw_wigner* h, const char* expr, double t, double* out);
+
+/* Imaginary-time (Bloch) setup: compute V_min and K_min over the theta-x and
+ * p-lambda grids at t_initial (wigner_bloch_cuda_1d.py:72-102). */
+int fw_wigner_bloch_setup(fw_wigner* h, int convention, double t_initial, double mins_out[2]);
+
+/* `steps` Bloch steps of size dbeta with renormalisation each step
+ * (wigner_bloch_cuda_1d.py:229-254). If purity_guard > 0 the run stops after the
+ * first step whose purity is not < purity_guard (wigner_bloch_cuda_1d.py:149-163);
+ * *steps_done reports how many steps were applied. */
+int fw_wigner_bloch_propagate(fw_wigner* h, double dbeta, int steps, double purity_guard, double* series,
+                              int* steps_done);
+
+/* Random-collision dissipator W <- w W + W0s around every step
+ * (wigner_random_collision_model_fftw_1d.py:63-74). host_w0s = (1-w) W_Gibbs;
+ * NULL disables it. */
+int fw_wigner_set_dissipator(fw_wigner* h, const double* host_w0s, double w_coeff);
+
+/* Snapshot / restore of W for the adaptive roll-back of get_ground_state
+ * (wigner_bloch_cuda_1d.py:186,209,218): which = 0 save, 1 restore. */
+int fw_wigner_snapshot(fw_wigner* h, int which);
+
+/* Mean device time of the three step kernels over the last propagate call, in
+ * microseconds (v_first, k, v_last), measured with HIP events on the handle's
+ * stream when timing is enabled. */
+int fw_wigner_set_timing(fw_wigner* h, int enable);
+int fw_wigner_kernel_times(fw_wigner* h, double us_out[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FASTWIGNER_H */

@@ -1,0 +1,162 @@
+"""GPU parity of the plain Moyal propagator against the CPU oracle.
+
+Tolerances: FP64 relative L2 <= 1e-12 against the oracle (the
+FFT-to-FFT noise floor measured between pocketfft and torch on config 1 is
+1.6e-13). FP32 form: relative L2 <= 1e-4 against the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from oracle import cexpr
+from oracle import fw_oracle as fo
+
+pytestmark = pytest.mark.gpu
+
+V_Q = "0.5*X*X+0.1*X*X*X*X"
+K_Q = "0.5*P*P"
+DV_Q = "X+0.4*X*X*X"
+DK_Q = "P"
+
+
+def rel_l2(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def make_pair(nx, xa, npp, pa, dt, V, K, dV=None, dK=None, consts=None, masks=None, shift=0.5,
+              precision="fp64", tiles=None, **kw):
+    from fastwigner_amd.wigner_moyal import WignerMoyalHIP1D
+    masks = masks or {}
+    cvals = consts or {}
+    args = dict(X_gridDIM=nx, X_amplitude=xa, P_gridDIM=npp, P_amplitude=pa, dt=dt, t=0.0, V=V, K=K,
+                precision=precision, phase_time_shift=shift, **cvals, **masks, **kw)
+    if dV:
+        args.update(diff_V=dV, diff_K=dK)
+    if tiles:
+        args["tiles"] = tiles
+    g = WignerMoyalHIP1D(**args)
+    o = fo.MoyalOracle(
+        nx, xa, npp, pa, dt, V=cexpr.make(V, cvals), K=cexpr.make(K, cvals, ("P", "t")),
+        diff_V=cexpr.make(dV, cvals) if dV else None, diff_K=cexpr.make(dK, cvals, ("P", "t")) if dK else None,
+        phase_time_shift=shift,
+        mask_x_theta=cexpr.make(masks.get("abs_boundary_x_theta"), cvals, ("X", "Theta", "t")),
+        mask_lambda_p=cexpr.make(masks.get("abs_boundary_lambda_p"), cvals, ("Lambda", "P", "t")),
+        mask_x_p=cexpr.make(masks.get("abs_boundary_x_p"), cvals, ("X", "P", "t")))
+    return g, o
+
+
+def gaussian(o, x0=1.0, p0=0.5):
+    return np.exp(-(o.X - x0) ** 2 - (o.P - p0) ** 2) / np.pi
+
+
+def test_config1_parity_fp64(gpu):
+    """Config 1: 256^2, x,p in [-8,8), V=0.5x^2+0.1x^4, dt=0.01, 1000 steps,
+    <x>,<p>,<H>, purity logged each step."""
+    g, o = make_pair(256, 8., 256, 8., 0.01, V_Q, K_Q, DV_Q, DK_Q)
+    W0 = gaussian(o)
+    g.set_wignerfunction(W0)
+    o.set_wignerfunction(W0)
+    Wg = g.propagate(1000).get()
+    o.propagate(1000)
+    err = rel_l2(Wg, o.W)
+    print("config1 fp64 rel L2 vs oracle:", err)
+    assert err <= 1e-12
+    ox = np.array([s["X_average"] for s in o.series])
+    op = np.array([s["P_average"] for s in o.series])
+    oh = np.array([s["hamiltonian_average"] for s in o.series])
+    opu = np.array([s["purity"] for s in o.series])
+    assert np.max(np.abs(np.array(g.X_average) - ox)) <= 1e-11
+    assert np.max(np.abs(np.array(g.P_average) - op)) <= 1e-11
+    assert np.max(np.abs(np.array(g.hamiltonian_average) - oh)) <= 1e-11 * np.max(np.abs(oh))
+    assert np.max(np.abs(np.array(g.purity_series) - opu)) <= 1e-11
+    assert abs(g.t - o.t) == 0.0
+
+
+def test_harmonic_closed_form_gpu(gpu):
+    """The Strang step is the leapfrog map for V=x^2/2: a Gaussian stays Gaussian
+    with mean and covariance propagated by the 2x2 leapfrog matrix."""
+    g, o = make_pair(256, 8., 256, 8., 0.01, "0.5*X*X", K_Q)
+    g.set_wignerfunction(gaussian(o))
+    n = 500
+    W = g.propagate(n).get()
+    a = 0.5 * 0.01
+    M = np.array([[1, 0], [-a, 1]]) @ np.array([[1, 0.01], [0, 1]]) @ np.array([[1, 0], [-a, 1]])
+    Mn = np.linalg.matrix_power(M, n)
+    mu = Mn @ np.array([1., 0.5])
+    S = Mn @ (0.5 * np.eye(2)) @ Mn.T
+    Si = np.linalg.inv(S)
+    dx, dp = o.X - mu[0], o.P - mu[1]
+    Wa = np.exp(-0.5 * (Si[0, 0] * dx * dx + 2 * Si[0, 1] * dx * dp + Si[1, 1] * dp * dp))
+    Wa /= Wa.sum() * o.dXdP
+    assert rel_l2(W, Wa) <= 1e-12
+
+
+@pytest.mark.parametrize("nx,npp", [(128, 64), (64, 256), (512, 512)])
+def test_rectangular_masks_time_dependent(gpu, nx, npp):
+    """Non-square grids, all three absorbing-boundary strings and a
+    time-dependent potential V(x, t) = x^2/2 + x E(t) (strong-field pattern)."""
+    consts = {"E0": 0.05, "om": 0.3}
+    masks = dict(abs_boundary_x_theta="pow(fabs(cos(0.5*M_PI*X_minus/X_amplitude)*cos(0.5*M_PI*X_plus/X_amplitude)), 0.01)",
+                 abs_boundary_lambda_p="exp(-1e-4*Lambda*Lambda)",
+                 abs_boundary_x_p="pow(fabs(cos(0.5*M_PI*X/X_amplitude)), 0.125)")
+    g, o = make_pair(nx, 10., npp, 6., 0.02, "0.5*X*X + X*E0*sin(om*t)", "0.5*P*P",
+                     "X + E0*sin(om*t)", "P", consts=consts, masks=masks)
+    W0 = np.exp(-(o.X + 0.5) ** 2 - (o.P - 0.3) ** 2)
+    g.set_wignerfunction(W0)
+    o.set_wignerfunction(W0)
+    Wg = g.propagate(200).get()
+    o.propagate(200)
+    assert rel_l2(Wg, o.W) <= 1e-12
+    dip = np.array([s["P_average_RHS"] for s in o.series])
+    assert np.max(np.abs(np.array(g.P_average_RHS) - dip)) <= 1e-11
+
+
+@pytest.mark.parametrize("tiles", [(16, 1, 16, 1), (8, 2, 8, 2), (16, 4, 4, 4), (32, 1, 32, 1), (4, 8, 2, 8)])
+def test_tile_schedules_agree(gpu, tiles):
+    """Every FFT tile/radix schedule gives the oracle's answer."""
+    g, o = make_pair(256, 8., 256, 8., 0.01, V_Q, K_Q, DV_Q, DK_Q, tiles=tiles)
+    W0 = gaussian(o)
+    g.set_wignerfunction(W0)
+    o.set_wignerfunction(W0)
+    Wg = g.propagate(50).get()
+    o.propagate(50)
+    assert rel_l2(Wg, o.W) <= 1e-12
+
+
+def test_fp32_form_error(gpu):
+    """FP32 W and FFTs, FP64 phase argument and sums: L2 <= 1e-4 vs the FP64 oracle."""
+    g, o = make_pair(256, 8., 256, 8., 0.01, V_Q, K_Q, DV_Q, DK_Q, precision="fp32")
+    W0 = gaussian(o)
+    g.set_wignerfunction(W0)
+    o.set_wignerfunction(W0)
+    Wg = g.propagate(1000).get()
+    o.propagate(1000)
+    err = rel_l2(Wg, o.W)
+    print("config1 fp32 rel L2 vs fp64 oracle:", err)
+    assert err <= 1e-4
+    assert np.max(np.abs(np.array(g.X_average) - np.array([s["X_average"] for s in o.series]))) <= 1e-4
+
+
+def test_observable_getters(gpu):
+    g, o = make_pair(128, 8., 128, 8., 0.01, V_Q, K_Q, DV_Q, DK_Q)
+    W0 = gaussian(o)
+    g.set_wignerfunction(W0)
+    o.set_wignerfunction(W0)
+    g.propagate(37)
+    o.propagate(37)
+    f = lambda X, P, t: X * P + 0.25 * X * X
+    assert abs(g.get_average("X * P + 0.25 * X * X") - o.get_average(f)) <= 1e-12
+    assert abs(g.get_purity() - o.get_purity()) <= 1e-12
+    assert abs(g.get_sigma_x_sigma_p() - o.series[-1]["sigma_x_sigma_p"]) <= 1e-11
+
+
+def test_set_wignerfunction_forms(gpu):
+    """ndarray, C string, python function and float initial states agree."""
+    from fastwigner_amd.wigner_moyal import WignerMoyalHIP1D
+    kw = dict(X_gridDIM=64, X_amplitude=8., P_gridDIM=128, P_amplitude=6., dt=0.01, t=0., V="0.5*X*X", K="0.5*P*P")
+    a = WignerMoyalHIP1D(**kw).set_wignerfunction("exp(-(X-1.)*(X-1.) - (P-0.5)*(P-0.5))").wignerfunction.get()
+    sysb = WignerMoyalHIP1D(**kw)
+    b = sysb.set_wignerfunction(lambda self, x, p: np.exp(-(x - 1.) ** 2 - (p - 0.5) ** 2)).wignerfunction.get()
+    assert np.max(np.abs(a - b)) <= 1e-14 * np.max(np.abs(b))
+    c = WignerMoyalHIP1D(**kw).set_wignerfunction(1.).wignerfunction.get()
+    assert np.allclose(c, 1.0 / (64 * 128 * sysb.dXdP), rtol=1e-14)
+    assert abs(b.sum() * sysb.dXdP - 1.0) <= 1e-13
