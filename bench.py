@@ -1,0 +1,181 @@
+"""Benchmark: Wigner grid-point*steps/s of the plain FP64 Moyal step at 1024^2.
+
+Workload (one "step" = one Strang step V/2-K-V/2 over the whole grid, plus the
+per-step renormalisation and the logged observables <x>, <p>, <H>, purity):
+config-1 physics (x, p in [-8, 8), V = 0.5x^2 + 0.1x^4, K = p^2/2, dt = 0.01,
+W0 = exp(-(x-1)^2 - (p-0.5)^2)/pi) on a 1024 x 1024 grid, FP64. The state is
+resident in HBM before the timed region; nothing crosses PCIe inside it except
+the per-chunk time/observable series (64 steps per chunk).
+
+Prints ONE JSON line. Extra fields:
+  roofline      dominant kernel's algorithmic HBM bytes per launch / its mean
+                duration (HIP events on the propagator's stream) vs 8 TB/s
+  cpu_baseline  the in-repo oracle (scipy.fft, pocketfft) on the host cores,
+                timed on a bounded sample of the same workload
+  parity        relative L2 of the GPU state against the oracle after the
+                sample's steps (FP64 bar 1e-12)
+  variants      FP32 form at 1024^2 and FP64 at 2048^2 with their own rates,
+                roofline fractions and errors against the FP64 oracle
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_PASS = {"fp64": 16, "fp32": 8}   # one read + one write of W per pass, per grid point
+
+CFG = dict(X_amplitude=8., P_amplitude=8., dt=0.01, t=0.,
+           V="0.5*X*X+0.1*X*X*X*X", K="0.5*P*P", diff_V="X+0.4*X*X*X", diff_K="P")
+
+
+def w0(sys_):
+    return np.exp(-(sys_.X - 1.) ** 2 - (sys_.P - 0.5) ** 2) / np.pi
+
+
+def make(n, precision):
+    from fastwigner_amd.wigner_moyal import WignerMoyalHIP1D
+    s = WignerMoyalHIP1D(X_gridDIM=n, P_gridDIM=n, precision=precision, **CFG)
+    s.set_wignerfunction(w0(s))
+    return s
+
+
+def kernel_times(s, steps=64):
+    """Mean device time (us) of v_first, k, v_last with HIP events on the
+    propagator's stream."""
+    import ctypes
+    from fastwigner_amd import _lib
+    _lib.call("fw_wigner_set_timing", s._handle, 1)
+    s.propagate(steps)
+    out = (ctypes.c_double * 3)()
+    _lib.call("fw_wigner_kernel_times", s._handle, out)
+    _lib.call("fw_wigner_set_timing", s._handle, 0)
+    return list(out)
+
+
+def roofline(n, precision, ktimes):
+    names = ["v_first", "k", "v_last"]
+    i = int(np.argmax(ktimes))
+    bytes_per_launch = BYTES_PER_PASS[precision] * n * n
+    achieved = bytes_per_launch / (ktimes[i] * 1e-6) / 1e9
+    return {"bound": "hbm", "kernel": "fw_" + names[i], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "bytes_per_launch": bytes_per_launch, "mean_us": [round(x, 3) for x in ktimes]}
+
+
+def timed_rate(s, n, steps, warmup):
+    s.propagate(warmup)
+    t0 = time.perf_counter()
+    s.propagate(steps)          # returns after the stream is synchronised
+    t1 = time.perf_counter()
+    dt = t1 - t0
+    return n * n * steps / dt, dt * 1e3 / steps
+
+
+def oracle_sample(n, max_seconds, max_steps, workers):
+    """The oracle on the host cores: steps until max_seconds (bounded sample)."""
+    import scipy  # noqa: F401
+    from oracle import fw_oracle as fo
+    o = fo.MoyalOracle(n, CFG["X_amplitude"], n, CFG["P_amplitude"], CFG["dt"],
+                       V=lambda X, t: 0.5 * X * X + 0.1 * X ** 4, K=lambda P, t: 0.5 * P * P,
+                       diff_V=lambda X, t: X + 0.4 * X ** 3, diff_K=lambda P, t: P, workers=workers)
+    o.set_wignerfunction(w0(o))
+    t0 = time.perf_counter()
+    steps = 0
+    while steps < max_steps and time.perf_counter() - t0 < max_seconds:
+        o.propagate(1)
+        steps += 1
+    el = time.perf_counter() - t0
+    return o, steps, el
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--n", type=int, default=1024)
+    ap.add_argument("--no-baseline", action="store_true")
+    ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run device-to-device copies of one 2048^2 FP64 state (32 MiB), the known-size "
+                         "reference that scales the FETCH_SIZE / WRITE_SIZE counters")
+    args = ap.parse_args()
+    if args.copy_probe:
+        from fastwigner_amd import _lib
+        s = make(2048, "fp64")
+        for _ in range(10):
+            _lib.call("fw_wigner_snapshot", s._handle, 0)
+        print(json.dumps({"copy_probe_bytes": 2048 * 2048 * 8, "copies": 10}))
+        return
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: a single Wigner grid is kept on one GPU (sharding it would be "
+              "transpose-bound over xGMI) and no multi-GPU run is part of this build; use --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+    n = args.n
+    print("[bench] compiling and initialising %dx%d fp64" % (n, n), file=sys.stderr, flush=True)
+    s = make(n, "fp64")
+    rate, ms = timed_rate(s, n, args.steps, args.warmup)
+    print("[bench] fp64 %dx%d: %.3e pt*steps/s, %.2f us/step" % (n, n, rate, ms * 1e3), file=sys.stderr, flush=True)
+    kt = kernel_times(s)
+    rl = roofline(n, "fp64", kt)
+
+    result = {
+        "metric": "Wigner grid-point*steps/s at 1024^2 (plain FP64 Moyal step)",
+        "value": rate, "unit": "grid-point*steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (config-1 Gaussian initial state)",
+        "config": {"workload": "plain FP64 Moyal split-operator step, %dx%d, x,p in [-8,8), "
+                               "V=0.5x^2+0.1x^4, K=p^2/2, dt=0.01, renormalisation and <x>,<p>,<H>,purity "
+                               "every step" % (n, n), "grid": [n, n], "parallelism": "single GPU"},
+        "roofline": rl,
+    }
+
+    if not args.no_baseline:
+        workers = min(os.cpu_count() or 1, 16)
+        print("[bench] oracle sample on %d host threads" % workers, file=sys.stderr, flush=True)
+        o, osteps, oel = oracle_sample(n, 15.0, 400, workers)
+        result["cpu_baseline"] = {"value": n * n * osteps / oel, "unit": "grid-point*steps/s", "cores": workers,
+                                  "kind": "port",
+                                  "sample": "%d steps of the same %dx%d FP64 workload with the scipy.fft oracle "
+                                            "(oracle/fw_oracle.py), %.1f s" % (osteps, n, n, oel)}
+        g = make(n, "fp64")
+        g.propagate(osteps)
+        err = float(np.linalg.norm(g.wignerfunction.get() - o.W) / np.linalg.norm(o.W))
+        result["parity"] = {"rel_l2_vs_oracle": err, "steps": osteps, "tolerance": 1e-12}
+        print("[bench] parity fp64 after %d steps: %.2e" % (osteps, err), file=sys.stderr, flush=True)
+        del g
+    else:
+        o = None
+
+    if not args.no_variants:
+        var = {}
+        s32 = make(n, "fp32")
+        r32, ms32 = timed_rate(s32, n, args.steps, args.warmup)
+        v = {"value": r32, "ms_per_step": ms32, "roofline": roofline(n, "fp32", kernel_times(s32))}
+        if o is not None:
+            g32 = make(n, "fp32")
+            g32.propagate(osteps)
+            v["rel_l2_vs_fp64_oracle"] = float(np.linalg.norm(g32.wignerfunction.get() - o.W) / np.linalg.norm(o.W))
+            v["steps_compared"] = osteps
+        var["fp32_%d" % n] = v
+        print("[bench] fp32 %dx%d: %.3e pt*steps/s" % (n, n, r32), file=sys.stderr, flush=True)
+        del s32
+        s2 = make(2048, "fp64")
+        r2, ms2 = timed_rate(s2, 2048, max(args.steps // 4, 100), 50)
+        var["fp64_2048"] = {"value": r2, "ms_per_step": ms2, "roofline": roofline(2048, "fp64", kernel_times(s2))}
+        print("[bench] fp64 2048^2: %.3e pt*steps/s" % r2, file=sys.stderr, flush=True)
+        result["variants"] = var
+
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -12,7 +12,8 @@ GRID_CONST_NAMES = frozenset(
     ["X_gridDIM", "P_gridDIM", "X_amplitude", "P_amplitude", "dt", "dX", "dP", "dLambda", "dTheta"])
 
 # Keyword arguments of this implementation that are not physics constants.
-CONTROL_KWARGS = frozenset(["precision", "phase_time_shift", "device", "tiles", "bloch_convention"])
+CONTROL_KWARGS = frozenset(["precision", "phase_time_shift", "device", "tiles", "bloch_convention",
+                            "phase_tables"])
 
 
 def bind_kwargs(obj, kwargs):

@@ -52,6 +52,7 @@ class WignerDesc(ctypes.Structure):
         ("precision", ctypes.c_int), ("device", ctypes.c_int),
         ("col_e", ctypes.c_int), ("col_p", ctypes.c_int),
         ("row_e", ctypes.c_int), ("row_p", ctypes.c_int),
+        ("phase_tables", ctypes.c_int),
     ]
 
 
@@ -80,7 +81,7 @@ def _b(s):
     return None if s is None else s.encode()
 
 
-def wigner_desc(nx, xamp, npp, pamp, dt, shift=0.5, precision=0, device=0, tiles=None):
+def wigner_desc(nx, xamp, npp, pamp, dt, shift=0.5, precision=0, device=0, tiles=None, phase_tables=0):
     d = WignerDesc()
     d.x_grid_dim, d.p_grid_dim = int(nx), int(npp)
     d.x_amplitude, d.p_amplitude = float(xamp), float(pamp)
@@ -88,6 +89,7 @@ def wigner_desc(nx, xamp, npp, pamp, dt, shift=0.5, precision=0, device=0, tiles
     d.precision, d.device = int(precision), int(device)
     if tiles:
         d.col_e, d.col_p, d.row_e, d.row_p = [int(v) for v in tiles]
+    d.phase_tables = int(phase_tables)
     return d
 
 

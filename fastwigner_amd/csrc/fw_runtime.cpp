@@ -114,6 +114,12 @@ int compile(const std::string& src, const std::string& arch, std::vector<char>& 
       }
     }
   }
+  if (const char* dd = getenv("FASTWIGNER_DUMP_DIR")) {
+    // debugging aid: the generated source, compilable with hipcc -I csrc
+    mkdirs(dd);
+    std::ofstream f(std::string(dd) + "/" + key + ".hip");
+    f << src;
+  }
   hiprtcProgram prog;
   const char* hdr_names[] = {"fw_device.h"};
   const char* hdrs[] = {fw_device_h_src};
@@ -184,8 +190,15 @@ int upload_twiddles(int n, bool fp32, void** dptr) {
   std::vector<double> re, im;
   twiddles(n, re, im);
   if (fp32) {
-    std::vector<float> t(2 * n);
-    for (int j = 0; j < n; ++j) { t[2 * j] = (float)re[j]; t[2 * j + 1] = (float)im[j]; }
+    // hi + lo float pairs: (hi.x, hi.y, lo.x, lo.y) per entry
+    std::vector<float> t(4 * n);
+    for (int j = 0; j < n; ++j) {
+      const float hr = (float)re[j], hi = (float)im[j];
+      t[4 * j] = hr;
+      t[4 * j + 1] = hi;
+      t[4 * j + 2] = (float)(re[j] - (double)hr);
+      t[4 * j + 3] = (float)(im[j] - (double)hi);
+    }
     HIPCHK(hipMalloc(dptr, t.size() * sizeof(float)));
     HIPCHK(hipMemcpy(*dptr, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
   } else {

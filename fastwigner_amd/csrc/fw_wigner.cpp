@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cctype>
 #include <cstring>
 #include <map>
 #include <sstream>
@@ -91,6 +92,13 @@ struct fw_wigner {
   double t_initial = 0.0, vmin = 0.0, kmin = 0.0;
   bool bloch_ready = false;
 
+  bool vtab_ok = false, ktab_ok = false;   // V (K) factors are time independent
+  void* vtab = nullptr;
+  void* ktab = nullptr;
+  int tab_kind = -1;          // what the tables hold: 0 Moyal, 1 Bloch
+  double tab_dbeta = 0.0;
+  hipFunction_t f_build_vtab = nullptr, f_build_ktab = nullptr;
+
   hipGraphExec_t gexec = nullptr;
   int gkind = -1;
 
@@ -146,14 +154,35 @@ int resolve_tiles(const fw_wigner_desc* d, Tiles& t) {
   if (col_nt > 1024 || row_nt > 1024) return fw::fail(FW_ERR_INVALID_ARG, "tile needs more than 1024 threads");
   const size_t esz = fp32 ? 8 : 16;
   const size_t lds_c = (size_t)t.col_p * d->p_grid_dim, lds_r = (size_t)t.row_p * d->x_grid_dim;
-  if ((lds_c + lds_c / 16 + 1) * esz > 64 * 1024 || (lds_r + lds_r / 16 + 1) * esz > 64 * 1024)
-    return fw::fail(FW_ERR_INVALID_ARG, "tile needs more than 64 KiB of LDS");
+  // gfx950: 160 KiB of LDS per workgroup (exchange buffer + twiddle table + reduction scratch)
+  const size_t tws = 16;
+  const size_t twc = (size_t)d->p_grid_dim * 17 / 16 + 1, twr = (size_t)d->x_grid_dim * 17 / 16 + 1;
+  if ((lds_c + lds_c / 16 + 1) * esz + twc * tws + 1152 > 160 * 1024 ||
+      (lds_r + lds_r / 16 + 1) * esz + twr * tws > 160 * 1024)
+    return fw::fail(FW_ERR_INVALID_ARG, "tile needs more than 160 KiB of LDS");
   return FW_OK;
+}
+
+// Does the expression reference the time argument (identifier token "t")?
+bool uses_t(const char* s) {
+  if (!s) return false;
+  const size_t n = strlen(s);
+  auto ident = [](char c) { return isalnum((unsigned char)c) || c == '_'; };
+  for (size_t i = 0; i < n; ++i)
+    if (s[i] == 't' && (i == 0 || !ident(s[i - 1])) && (i + 1 == n || !ident(s[i + 1]))) return true;
+  return false;
 }
 
 std::string or_default(const char* s, const char* d) { return (s && *s) ? std::string(s) : std::string(d); }
 
 // Constants and spliced expressions, wigner_moyal_cuda_1d.py:151-166,244-277.
+bool tables_v(const fw_wigner_desc* d, const fw_wigner_exprs* x) {
+  return d->phase_tables > 0 || (d->phase_tables == 0 && !uses_t(x->V) && !uses_t(x->abs_boundary_x_theta));
+}
+bool tables_k(const fw_wigner_desc* d, const fw_wigner_exprs* x) {
+  return d->phase_tables > 0 || (d->phase_tables == 0 && !uses_t(x->K) && !uses_t(x->abs_boundary_lambda_p));
+}
+
 std::string make_prelude(const fw_wigner_desc* d, const fw_wigner_exprs* x, const Tiles& t) {
   double dX, dP, dL, dT;
   grid_constants(d, &dX, &dP, &dL, &dT);
@@ -165,6 +194,7 @@ std::string make_prelude(const fw_wigner_desc* d, const fw_wigner_exprs* x, cons
   s << "#define FW_COL_E " << t.col_e << "\n#define FW_COL_P " << t.col_p << "\n";
   s << "#define FW_ROW_E " << t.row_e << "\n#define FW_ROW_P " << t.row_p << "\n";
   s << "#define FW_PHASE_SHIFT (" << fw::fmt_double(d->phase_time_shift) << ")\n";
+  s << "#define FW_VTAB " << (tables_v(d, x) ? 1 : 0) << "\n#define FW_KTAB " << (tables_k(d, x) ? 1 : 0) << "\n";
   s << "#ifndef M_PI\n#define M_PI 3.14159265358979323846\n#endif\n";
   s << "const int X_gridDIM = " << d->x_grid_dim << ";\n";
   s << "const int P_gridDIM = " << d->p_grid_dim << ";\n";
@@ -225,8 +255,11 @@ KArgs base_args(fw_wigner* h) {
   a.partials = h->partials;
   a.counter = h->counter;
   a.series = h->series;
+  a.vtab = h->vtab;
+  a.ktab = h->ktab;
   a.w_coeff = h->w_coeff;
   a.flags = h->has_mask_xp ? fw::F_MASK_XP : 0;
+  a.nbp = h->col_nb;
   a.dbeta = 0;
   a.cv = a.ck = 0;
   a.vmin = h->vmin;
@@ -243,7 +276,40 @@ int set_scalar(fw_wigner* h, int slot, double v) {
   return FW_OK;
 }
 
-int helper_blocks(fw_wigner* h) { return h->NP < 1024 ? h->NP : 1024; }
+int helper_blocks(fw_wigner* h) { return h->NP < 128 ? h->NP : 128; }
+
+// Fill the factor tables for a run of `kind` (0 Moyal, 1 Bloch with a.dbeta)
+// and set the table flags in a. Tables are allocated once; contents follow kind.
+int ensure_tables(fw_wigner* h, int kind, KArgs& a) {
+  const bool need_v = h->vtab_ok || kind == 1, need_k = h->ktab_ok || kind == 1;
+  if (!need_v && !need_k) return FW_OK;
+  const size_t cs = h->fp32 ? 8 : 16;
+  if (need_v && !h->vtab) HIPCHK(hipMalloc(&h->vtab, (size_t)(h->NP / 2 + 1) * h->NX * cs));
+  if (need_k && !h->ktab) HIPCHK(hipMalloc(&h->ktab, (size_t)h->NP * (h->NX / 2 + 1) * cs));
+  a.vtab = h->vtab;
+  a.ktab = h->ktab;
+  if (h->tab_kind != kind || (kind == 1 && h->tab_dbeta != a.dbeta)) {
+    KArgs b = a;
+    if (kind == 1) b.flags |= fw::F_BLOCH;
+    const double t0 = kind == 1 ? h->t_initial : 0.0;
+    HIPCHK(hipMemcpyAsync(h->times, &t0, sizeof(double), hipMemcpyHostToDevice, h->stream));
+    if (need_v) {
+      void* tab = h->vtab;
+      void* params[] = {&b, &tab};
+      HIPCHK(hipModuleLaunchKernel(h->f_build_vtab, 1024, 1, 1, 256, 1, 1, 0, h->stream, params, nullptr));
+    }
+    if (need_k) {
+      void* tab = h->ktab;
+      void* params[] = {&b, &tab};
+      HIPCHK(hipModuleLaunchKernel(h->f_build_ktab, 1024, 1, 1, 256, 1, 1, 0, h->stream, params, nullptr));
+    }
+    h->tab_kind = kind;
+    h->tab_dbeta = a.dbeta;
+  }
+  if (need_v) a.flags |= fw::F_VTAB;
+  if (need_k) a.flags |= fw::F_KTAB;
+  return FW_OK;
+}
 
 // Step kernels of one step s (chunk-local). kind 0 = Moyal, 1 = Bloch.
 int enqueue_step(fw_wigner* h, int kind, int s, const KArgs& base, hipEvent_t* ev) {
@@ -288,13 +354,20 @@ int ensure_graph(fw_wigner* h, int kind, const KArgs& base) {
 }
 
 // Run `steps` steps of `kind`; times[s] per step; series (optional) gets NSER
-// sums per step. Returns the number of steps applied before a guard stop.
-int run_steps(fw_wigner* h, int kind, const KArgs& base, const double* times, int steps, double* series,
+// sums per step. The sums of step s are published by the first pass of step
+// s+1 into slot (s+1) mod chunk of the device series buffer, and those of the
+// last step by the closing normalisation into slot kChunk. *done = steps
+// applied before a purity-guard stop.
+int run_steps(fw_wigner* h, int kind, const KArgs& base0, const double* times, int steps, double* series,
               int* done) {
+  KArgs base = base0;
+  FWCHK(ensure_tables(h, kind, base));
   FWCHK(set_scalar(h, fw::SC_STOP, 0.0));
+  FWCHK(set_scalar(h, fw::SC_FRESH, 1.0));
   const bool use_graph = !h->timing && kind == 0;
   if (use_graph && steps >= kChunk) FWCHK(ensure_graph(h, kind, base));
   std::vector<hipEvent_t> evs;
+  std::vector<double> slots((size_t)FW_NSER * (kChunk + 1));
   double acc[3] = {0, 0, 0};
   int nacc = 0;
   int applied = steps;
@@ -320,25 +393,35 @@ int run_steps(fw_wigner* h, int kind, const KArgs& base, const double* times, in
     } else {
       for (int s = 0; s < n; ++s) FWCHK(enqueue_step(h, kind, s, base, nullptr));
     }
-    if (series)
-      HIPCHK(hipMemcpyAsync(series + (size_t)off * FW_NSER, h->series, sizeof(double) * FW_NSER * n,
-                            hipMemcpyDeviceToHost, h->stream));
-    if (base.guard > 0) {
-      double stop = 0;
+    double stop = 0;
+    if (series || base.guard > 0) {
+      HIPCHK(hipMemcpyAsync(slots.data(), h->series, sizeof(double) * FW_NSER * n, hipMemcpyDeviceToHost,
+                            h->stream));
       HIPCHK(hipMemcpyAsync(&stop, h->scal + fw::SC_STOP, sizeof(double), hipMemcpyDeviceToHost, h->stream));
       HIPCHK(hipStreamSynchronize(h->stream));
-      if (stop != 0) {
-        applied = off + (int)stop;
-        break;
-      }
+      const int ns = stop != 0 ? (int)stop : n;   // slots written in this chunk
+      if (series)
+        for (int s = 0; s < ns; ++s) {
+          const int g = off + s - 1;
+          if (g >= 0) std::memcpy(series + (size_t)g * FW_NSER, &slots[(size_t)s * FW_NSER], sizeof(double) * FW_NSER);
+        }
+    }
+    if (stop != 0) {
+      applied = off + (int)stop - 1;
+      break;
     }
   }
   if (nacc)
     for (int k = 0; k < 3; ++k) h->ktimes[k] = acc[k] / nacc;
-  // the deferred normalisation of the last step
+  // the deferred normalisation of the last applied step, which also publishes its sums
   KArgs a = base;
+  a.flags |= fw::F_FROM_PARTIALS;
+  a.step = kChunk;
   FWCHK(launch(h, h->f_apply_norm, 1024, 256, a));
   FWCHK(set_scalar(h, fw::SC_NORM, 1.0));
+  if (series && applied > 0)
+    HIPCHK(hipMemcpyAsync(series + (size_t)(applied - 1) * FW_NSER, h->series + (size_t)kChunk * FW_NSER,
+                          sizeof(double) * FW_NSER, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
   if (done) *done = applied;
   return FW_OK;
@@ -413,6 +496,8 @@ extern "C" int fw_wigner_create(const fw_wigner_desc* desc, const fw_wigner_expr
   h->esz = h->fp32 ? 4 : 8;
   grid_constants(desc, &h->dX, &h->dP, &h->dLambda, &h->dTheta);
   h->has_mask_xp = exprs->abs_boundary_x_p && *exprs->abs_boundary_x_p;
+  h->vtab_ok = tables_v(desc, exprs);
+  h->ktab_ok = tables_k(desc, exprs);
   h->prelude = make_prelude(desc, exprs, t);
   h->arch = fw::target_arch(desc->device);
   h->col_nt = t.col_p * (h->NP / t.col_e);
@@ -434,7 +519,8 @@ extern "C" int fw_wigner_create(const fw_wigner_desc* desc, const fw_wigner_expr
   } fns[] = {{"fw_v_first", &h->f_v_first},   {"fw_v_last", &h->f_v_last},
              {"fw_k", &h->f_k},               {"fw_moments", &h->f_moments},
              {"fw_apply_norm", &h->f_apply_norm}, {"fw_set_scalar", &h->f_set_scalar},
-             {"fw_vk_min", &h->f_vk_min}};
+             {"fw_vk_min", &h->f_vk_min},   {"fw_build_vtab", &h->f_build_vtab},
+             {"fw_build_ktab", &h->f_build_ktab}};
   for (auto& f : fns)
     if ((rc = getfn(h->mod, f.n, f.f)) != FW_OK) return bail(rc);
 
@@ -450,7 +536,7 @@ extern "C" int fw_wigner_create(const fw_wigner_desc* desc, const fw_wigner_expr
   if ((rc = alloc((void**)&h->scal, sizeof(double) * fw::NSCAL)) != FW_OK) return bail(rc);
   if ((rc = alloc((void**)&h->partials, sizeof(double) * FW_NSER * maxnb)) != FW_OK) return bail(rc);
   if ((rc = alloc((void**)&h->counter, sizeof(unsigned int) * 4)) != FW_OK) return bail(rc);
-  if ((rc = alloc((void**)&h->series, sizeof(double) * FW_NSER * kChunk)) != FW_OK) return bail(rc);
+  if ((rc = alloc((void**)&h->series, sizeof(double) * FW_NSER * (kChunk + 1))) != FW_OK) return bail(rc);
   if ((rc = fw::upload_twiddles(h->NP, h->fp32, &h->tw_col)) != FW_OK) return bail(rc);
   if ((rc = fw::upload_twiddles(h->NX, h->fp32, &h->tw_row)) != FW_OK) return bail(rc);
   if (hipStreamCreate(&h->stream) != hipSuccess) return bail(fw::fail(FW_ERR_HIP, "hipStreamCreate failed"));
@@ -469,7 +555,7 @@ extern "C" int fw_wigner_destroy(fw_wigner* h) {
       if (kv.second.mod) (void)hipModuleUnload(kv.second.mod);
   for (hipModule_t m : {h->mod, h->mod_bloch, h->mod_dissip})
     if (m) (void)hipModuleUnload(m);
-  for (void* p : {h->W, h->W0s, h->snap, h->tw_col, h->tw_row, (void*)h->times, (void*)h->scal,
+  for (void* p : {h->W, h->W0s, h->snap, h->tw_col, h->tw_row, h->vtab, h->ktab, (void*)h->times, (void*)h->scal,
                   (void*)h->partials, (void*)h->counter, (void*)h->series})
     if (p) (void)hipFree(p);
   if (h->stream) (void)hipStreamDestroy(h->stream);

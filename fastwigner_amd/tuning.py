@@ -1,0 +1,45 @@
+"""FFT tile schedules picked by the autotuner (``fastwigner_amd.autotune``).
+
+The tuned entries live in ``tuning.json`` next to this file, keyed by
+propagator kind, grid and precision, the way FFTW wisdom persists plans across
+runs (strong_field_physics_fftw_1d.py:21-28,249-255). A missing entry falls back
+to the library's built-in defaults.
+"""
+import json
+import os
+
+PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
+_cache = None
+
+
+def _load():
+    global _cache
+    if _cache is None:
+        try:
+            with open(PATH) as f:
+                _cache = json.load(f)
+        except (OSError, ValueError):
+            _cache = {}
+    return _cache
+
+
+def key(kind, nx, npp, precision):
+    return "%s:%dx%d:%s" % (kind, nx, npp, "fp32" if precision else "fp64")
+
+
+def lookup(kind, nx, npp, precision):
+    """(col_e, col_p, row_e, row_p) or None."""
+    e = _load().get(key(kind, nx, npp, precision))
+    return tuple(e["tiles"]) if e else None
+
+
+def store(kind, nx, npp, precision, tiles, info=None, path=PATH):
+    """Add or replace one entry of the tuning file at ``path`` (merging with what it holds)."""
+    try:
+        with open(path) as f:
+            data = json.load(f)
+    except (OSError, ValueError):
+        data = {}
+    data[key(kind, nx, npp, precision)] = {"tiles": list(tiles), **(info or {})}
+    with open(path, "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)

@@ -18,13 +18,16 @@ Additional keyword arguments:
                       0.0 the FFTW class, -0.5 the midpoint rule
     alpha             the FFTW class's absorbing boundary (wigner_moyal_fftw_1d.py:135-156),
                       translated into an abs_boundary_x_theta expression
+    phase_tables      1 / 0 / -1: cache the V and K factors in tables always /
+                      when the strings do not reference t (default) / never
     device, tiles     HIP device ordinal; (col_e, col_p, row_e, row_p) FFT tiles
+                      (default: the autotuned entry of fastwigner_amd/tuning.json)
 """
 import ctypes
 
 import numpy as np
 
-from . import _lib
+from . import _lib, tuning
 from ._common import (DeviceArray, bind_kwargs, c_double, consts_source, precision_code)
 
 
@@ -114,8 +117,10 @@ class WignerMoyalHIP1D:
         self.hamiltonian = self.K + ' + ' + self.V
 
         self._handle = None
+        tiles = getattr(self, "tiles", None) or tuning.lookup("wigner", self.X_gridDIM, self.P_gridDIM, self._prec)
         desc = _lib.wigner_desc(self.X_gridDIM, self.X_amplitude, self.P_gridDIM, self.P_amplitude, self.dt,
-                                self.phase_time_shift, self._prec, self.device, getattr(self, "tiles", None))
+                                self.phase_time_shift, self._prec, self.device, tiles,
+                                int(getattr(self, "phase_tables", 0)))
         ex = _lib.wigner_exprs(
             V=self.V, K=self.K, consts=self.hip_consts,
             diff_V=getattr(self, "diff_V", None) if isinstance(getattr(self, "diff_V", None), str) else None,

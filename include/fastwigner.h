@@ -74,6 +74,9 @@ typedef struct fw_wigner_desc {
   int device;              /* HIP device ordinal */
   int col_e, col_p;        /* V pass: elements per thread, column pairs per block (0 = default) */
   int row_e, row_p;        /* K pass: elements per thread, row pairs per block (0 = default) */
+  int phase_tables;        /* cache the V/K factors in tables (as the FFTW class caches _expV/_expK,
+                              wigner_moyal_fftw_1d.py:164-213): 0 = when the V (K) string and its
+                              mask do not contain the token t, 1 = always, -1 = never */
 } fw_wigner_desc;
 
 typedef struct fw_wigner_exprs {

@@ -19,12 +19,14 @@ _NS = {
 }
 
 
-def make(expr, consts=None, args=("X", "t")):
+def make(expr, consts=None, args=("X", "t"), derived=None):
     """Return a numpy callable f(*args) evaluating the C expression ``expr``.
 
     ``consts`` maps extra names (kwargs the reference turns into C constants,
     wigner_moyal_cuda_1d.py:151-166) to values or to python callables standing
-    in for ``functions`` (e.g. ``E(t)``).
+    in for ``functions`` (e.g. ``E(t)``). ``derived(*vals)`` returns further names
+    the kernel defines from its arguments (e.g. X_minus, X_plus of the x-theta
+    mask, wigner_moyal_cuda_1d.py:640-641).
     """
     if expr is None:
         return None
@@ -36,9 +38,15 @@ def make(expr, consts=None, args=("X", "t")):
     def f(*vals):
         ns = dict(base)
         ns.update(zip(args, vals))
+        if derived is not None:
+            ns.update(derived(*vals))
         out = eval(code, {"__builtins__": {}}, ns)
         # broadcast scalars (e.g. "1.") to the shape of the inputs
         shape = np.broadcast(*[np.asarray(v) for v in vals]).shape
         return np.broadcast_to(np.asarray(out, dtype=np.result_type(out, np.float64)), shape).copy()
 
     return f
+
+
+def x_theta_names(X, Theta, t):
+    return {"X_minus": X - 0.5 * Theta, "X_plus": X + 0.5 * Theta}

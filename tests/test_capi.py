@@ -1,0 +1,111 @@
+"""Host-only checks of the C ABI (no GPU needed): the library loads, exports every
+entry point include/*.h declares, validates arguments, maps status codes to typed
+exceptions, and hipRTC compiles the device code for gfx950 (or reports the
+compiler log for a malformed expression)."""
+import ctypes
+import glob
+import os
+import re
+
+import pytest
+
+from fastwigner_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names.update(re.findall(r"^\s*(?:const\s+)?\w+\*?\s+\**(fw_\w+)\s*\(", text, flags=re.M))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = declared_symbols()
+    assert len(names) > 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_version_and_device_count():
+    lib = _lib.load()
+    assert lib.fw_version() >= 100
+    n = ctypes.c_int(-1)
+    assert lib.fw_device_count(ctypes.byref(n)) == 0
+    assert n.value >= 0
+
+
+@pytest.mark.parametrize("nx,npp,msg", [(63, 64, "even"), (96, 64, "powers of two"), (4, 64, "powers of two"),
+                                        (8192, 64, "powers of two")])
+def test_invalid_grids(nx, npp, msg):
+    lib = _lib.load()
+    d = _lib.wigner_desc(nx, 8., npp, 8., 0.01)
+    e = _lib.wigner_exprs(V="0.5*X*X", K="0.5*P*P")
+    rc = lib.fw_wigner_check(ctypes.byref(d), ctypes.byref(e))
+    assert rc == -1
+    assert msg in lib.fw_last_error().decode()
+    with pytest.raises(_lib.InvalidArgumentError):
+        _lib.check(rc)
+
+
+def test_invalid_tiles_and_null_args():
+    lib = _lib.load()
+    d = _lib.wigner_desc(64, 8., 64, 8., 0.01, tiles=(3, 1, 8, 1))
+    e = _lib.wigner_exprs(V="0.5*X*X", K="0.5*P*P")
+    assert lib.fw_wigner_check(ctypes.byref(d), ctypes.byref(e)) == -1
+    assert lib.fw_wigner_check(None, ctypes.byref(e)) == -1
+    assert lib.fw_wigner_destroy(None) == 0
+    assert lib.fw_wigner_propagate(None, None, 1, None) == -1
+
+
+def test_malformed_expression_is_a_compile_error():
+    lib = _lib.load()
+    d = _lib.wigner_desc(16, 8., 16, 8., 0.01)
+    e = _lib.wigner_exprs(V="0.5*X*X +* undefined_thing(", K="0.5*P*P")
+    rc = lib.fw_wigner_check(ctypes.byref(d), ctypes.byref(e))
+    assert rc == -2
+    log = lib.fw_last_error().decode()
+    assert "undefined_thing" in log
+    with pytest.raises(_lib.CompileError):
+        _lib.check(rc)
+
+
+def test_device_code_compiles_for_gfx950():
+    """The fused step kernels, Bloch and dissipator sets compile for gfx950 with
+    constants and functions in the reference's style (strong-field E(t))."""
+    lib = _lib.load()
+    consts = ("    const double F = 0.04;\n    const double omega = 0.05698;\n"
+              "__device__ double E(double t)\n{{\n  return -F * sin(omega * t) * pow(sin(omega * t / 16.), 2);\n}}\n")
+    d = _lib.wigner_desc(32, 60., 16, 6., 0.05, precision=1)
+    e = _lib.wigner_exprs(V="-1. / sqrt(X * X + 1.) + X * E(t)", K="0.5 * P * P", consts=consts,
+                          diff_V="X / pow(X * X + 1., 1.5) + E(t)", diff_K="P",
+                          abs_boundary_x_p="pow(fabs(cos(0.5 * M_PI * X / X_amplitude)), 0.125)")
+    rc = lib.fw_wigner_check(ctypes.byref(d), ctypes.byref(e))
+    assert rc == 0, lib.fw_last_error().decode()
+
+
+def test_grid_constants_match_numpy():
+    import numpy as np
+    d = _lib.wigner_desc(1024, 60., 512, 6., 0.05)
+    out = (ctypes.c_double * 4)()
+    assert _lib.load().fw_wigner_grid_constants(ctypes.byref(d), out) == 0
+    dX = 2. * 60. / 1024
+    dP = 2. * 6. / 512
+    lam = np.fft.fftfreq(1024, dX / (2 * np.pi))
+    th = np.fft.fftfreq(512, dP / (2 * np.pi))
+    assert list(out) == [dX, dP, lam[1] - lam[0], th[1] - th[0]]
+
+
+def test_product_fails_loudly_without_gpu():
+    n = ctypes.c_int(0)
+    _lib.load().fw_device_count(ctypes.byref(n))
+    if n.value:
+        pytest.skip("a GPU is visible")
+    from fastwigner_amd.wigner_moyal import WignerMoyalHIP1D
+    with pytest.raises(_lib.NoDeviceError):
+        WignerMoyalHIP1D(X_gridDIM=16, X_amplitude=8., P_gridDIM=16, P_amplitude=8., dt=0.01, t=0.,
+                         V="0.5*X*X", K="0.5*P*P")
