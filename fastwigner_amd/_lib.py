@@ -64,11 +64,10 @@ class WignerExprs(ctypes.Structure):
 
 class RhoDesc(ctypes.Structure):
     _fields_ = [
-        ("x_grid_dim", ctypes.c_int), ("pad_", ctypes.c_int),
+        ("x_grid_dim", ctypes.c_int), ("precision", ctypes.c_int),
         ("x_amplitude", ctypes.c_double), ("dt", ctypes.c_double),
-        ("phase_time_shift", ctypes.c_double),
-        ("precision", ctypes.c_int), ("device", ctypes.c_int),
-        ("row_e", ctypes.c_int), ("col_e", ctypes.c_int), ("col_p", ctypes.c_int), ("pad2_", ctypes.c_int),
+        ("phase_time_shift", ctypes.c_double), ("device", ctypes.c_int),
+        ("row_e", ctypes.c_int), ("row_p", ctypes.c_int), ("col_e", ctypes.c_int), ("col_p", ctypes.c_int),
     ]
 
 
@@ -104,6 +103,25 @@ def wigner_exprs(V, K, consts=None, diff_V=None, diff_K=None, abs_boundary_x_the
     return e
 
 
+FW_RSER = 20
+
+
+def rho_desc(n, xamp, dt, shift=0.0, precision=0, device=0, tiles=None):
+    d = RhoDesc()
+    d.x_grid_dim, d.x_amplitude, d.dt, d.phase_time_shift = int(n), float(xamp), float(dt), float(shift)
+    d.precision, d.device = int(precision), int(device)
+    if tiles:
+        d.row_e, d.row_p, d.col_e, d.col_p = [int(v) for v in tiles]
+    return d
+
+
+def rho_exprs(V, K, consts=None, diff_V=None, diff_K=None, abs_boundary_x=None, abs_boundary_p=None):
+    e = RhoExprs()
+    e.consts, e.V, e.K, e.diff_V, e.diff_K = _b(consts), _b(V), _b(K), _b(diff_V), _b(diff_K)
+    e.abs_boundary_x, e.abs_boundary_p = _b(abs_boundary_x), _b(abs_boundary_p)
+    return e
+
+
 _P = ctypes.c_void_p
 _D = ctypes.c_double
 _I = ctypes.c_int
@@ -134,6 +152,25 @@ PROTOTYPES = {
     "fw_wigner_snapshot": [_P, _I],
     "fw_wigner_set_timing": [_P, _I],
     "fw_wigner_kernel_times": [_P, _PD],
+    "fw_rho_grid_constants": [ctypes.POINTER(RhoDesc), _PD],
+    "fw_rho_check": [ctypes.POINTER(RhoDesc), ctypes.POINTER(RhoExprs)],
+    "fw_rho_create": [ctypes.POINTER(RhoDesc), ctypes.POINTER(RhoExprs), _PP],
+    "fw_rho_destroy": [_P],
+    "fw_rho_set_state": [_P, _PD],
+    "fw_rho_set_state_expr": [_P, ctypes.c_char_p],
+    "fw_rho_normalize": [_P, _PD],
+    "fw_rho_get_state": [_P, _PD],
+    "fw_rho_state_ptr": [_P, _PP, _PI],
+    "fw_rho_propagate": [_P, _PD, _I, _I, _PD],
+    "fw_rho_moments": [_P, _D, _PD],
+    "fw_rho_get_average": [_P, ctypes.POINTER(ctypes.c_char_p), _I, _D, _PD],
+    "fw_rho_bloch_setup": [_P, _D, _PD],
+    "fw_rho_bloch_propagate": [_P, _D, _I, _D, _PD, _PI],
+    "fw_rho_snapshot": [_P, _I],
+    "fw_rho_wigner": [_P, _PD],
+    "fw_rho_wigner_time": [_P, _PD, _D, _PD],
+    "fw_rho_set_timing": [_P, _I],
+    "fw_rho_kernel_times": [_P, _PD],
 }
 
 _lib = None

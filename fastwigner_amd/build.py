@@ -21,7 +21,8 @@ LIB = os.path.join(HERE, "libfastwigner.so")
 KCACHE = os.path.join(HERE, "_kcache")
 
 SOURCES = ["fw_runtime.cpp", "fw_wigner.cpp", "fw_rho.cpp", "fw_batch.cpp", "fw_fno.cpp"]
-EMBED = [("fw_device.h", "fw_device_h_src")]
+EMBED = [("fw_common.h", "fw_common_h_src"), ("fw_device.h", "fw_device_h_src"), ("fw_rho.h", "fw_rho_h_src"),
+         ("fw_batch.h", "fw_batch_h_src"), ("fw_fno.h", "fw_fno_h_src")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -41,7 +42,7 @@ def _embed(header, symbol):
 def build_library(verbose=True):
     os.makedirs(BUILD, exist_ok=True)
     srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    srcs += [_embed(h, sym) for h, sym in EMBED]
+    srcs += [_embed(h, sym) for h, sym in EMBED if os.path.exists(os.path.join(CSRC, h))]
     cmd = [HIPCC, "--offload-arch=gfx950", "-O2", "-std=c++17", "-fPIC", "-shared",
            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
            "-I", os.path.join(ROOT, "include"), "-I", CSRC,

@@ -1,0 +1,454 @@
+// fw_common.h -- shared CDNA4 (gfx950) device code of the propagators: complex
+// arithmetic, radix-2/4/8/16 DFT kernels, the LDS Stockham line-FFT engine,
+// phase factors and the block reductions. Embedded in libfastwigner.so and
+// compiled by hipRTC; included by fw_device.h (Wigner function) and fw_rho.h
+// (density matrix). Needs FW_REAL (double|float) and FW_FP32 (0|1).
+#pragma once
+
+#ifndef FW_REAL
+#error "FW_REAL must be defined"
+#endif
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+// Complex scalar for the caller's expressions (the reference's density-matrix
+// strings use pycuda's complex type under the name cuda_complex, e.g.
+// "exp(-sigma*(...) + cuda_complex(0., p0*(X - X_prime)))",
+// rho_vneumann_cuda_1d.py:962-966).
+struct fw_complex {
+  double re, im;
+  __device__ fw_complex(double r = 0., double i = 0.) : re(r), im(i) {}
+};
+typedef fw_complex cuda_complex;
+__device__ inline fw_complex operator+(fw_complex a, fw_complex b) { return fw_complex(a.re + b.re, a.im + b.im); }
+__device__ inline fw_complex operator-(fw_complex a, fw_complex b) { return fw_complex(a.re - b.re, a.im - b.im); }
+__device__ inline fw_complex operator-(fw_complex a) { return fw_complex(-a.re, -a.im); }
+__device__ inline fw_complex operator*(fw_complex a, fw_complex b) {
+  return fw_complex(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+__device__ inline fw_complex operator/(fw_complex a, fw_complex b) {
+  const double d = b.re * b.re + b.im * b.im;
+  return fw_complex((a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d);
+}
+__device__ inline fw_complex operator+(fw_complex a, double b) { return fw_complex(a.re + b, a.im); }
+__device__ inline fw_complex operator+(double a, fw_complex b) { return fw_complex(a + b.re, b.im); }
+__device__ inline fw_complex operator-(fw_complex a, double b) { return fw_complex(a.re - b, a.im); }
+__device__ inline fw_complex operator-(double a, fw_complex b) { return fw_complex(a - b.re, -b.im); }
+__device__ inline fw_complex operator*(fw_complex a, double b) { return fw_complex(a.re * b, a.im * b); }
+__device__ inline fw_complex operator*(double a, fw_complex b) { return fw_complex(a * b.re, a * b.im); }
+__device__ inline fw_complex operator/(fw_complex a, double b) { return fw_complex(a.re / b, a.im / b); }
+__device__ inline fw_complex operator/(double a, fw_complex b) { return fw_complex(a, 0.) / b; }
+__device__ inline fw_complex exp(fw_complex z) {
+  double s, c;
+  sincos(z.im, &s, &c);
+  const double e = exp(z.re);
+  return fw_complex(e * c, e * s);
+}
+__device__ inline fw_complex conj(fw_complex z) { return fw_complex(z.re, -z.im); }
+__device__ inline double real(fw_complex z) { return z.re; }
+__device__ inline double imag(fw_complex z) { return z.im; }
+__device__ inline double real(double x) { return x; }
+__device__ inline double imag(double) { return 0.; }
+__device__ inline double abs(fw_complex z) { return hypot(z.re, z.im); }
+__device__ inline double norm(fw_complex z) { return z.re * z.re + z.im * z.im; }
+
+namespace fwk {
+
+template <bool B, class T, class F> struct pick { using type = T; };
+template <class T, class F> struct pick<false, T, F> { using type = F; };
+
+typedef FW_REAL real_t;
+
+struct __attribute__((aligned(2 * sizeof(real_t)))) cpx {
+  real_t x, y;
+};
+
+#define FWD __device__ __forceinline__
+
+FWD cpx mk(real_t a, real_t b) { cpx r; r.x = a; r.y = b; return r; }
+FWD cpx cadd(cpx a, cpx b) { return mk(a.x + b.x, a.y + b.y); }
+FWD cpx csub(cpx a, cpx b) { return mk(a.x - b.x, a.y - b.y); }
+FWD cpx cmul(cpx a, cpx b) { return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+FWD cpx cmul_mi(cpx a) { return mk(a.y, -a.x); }   // a * (-i)
+FWD cpx cconj(cpx a) { return mk(a.x, -a.y); }
+FWD cpx cscale(cpx a, real_t s) { return mk(a.x * s, a.y * s); }
+
+// Multiplication by a constant rotation wr + i wi given in double precision.
+// In the FP32 form the constant is split into float hi + lo parts and the
+// product compensated: the fixed rounding of an FP32 constant would otherwise
+// perturb the step operator identically every step and the error would grow
+// linearly with the number of steps.
+FWD cpx cmulk(cpx a, double wr, double wi) {
+#if FW_FP32
+  const float hr = (float)wr, hi = (float)wi;
+  const float lr = (float)(wr - (double)hr), li = (float)(wi - (double)hi);
+  return cadd(cmul(a, mk(hr, hi)), cmul(a, mk(lr, li)));
+#else
+  return cmul(a, mk(wr, wi));
+#endif
+}
+
+// FFT twiddle table entry exp(-2 pi i j / N): FP64 value, or FP32 hi + lo pair.
+#if FW_FP32
+struct tw_t {
+  cpx hi, lo;
+};
+FWD cpx twmul(cpx a, const tw_t& w) { return cadd(cmul(a, w.hi), cmul(a, w.lo)); }
+#else
+typedef cpx tw_t;
+FWD cpx twmul(cpx a, const tw_t& w) { return cmul(a, w); }
+#endif
+
+// ---------------------------------------------------------------------------
+// Forward DFT kernels of radix 2, 4, 8, 16 on registers, natural order in/out.
+// X_q = sum_m a_m exp(-2 pi i m q / R).
+// ---------------------------------------------------------------------------
+FWD void dft4(cpx& a0, cpx& a1, cpx& a2, cpx& a3) {
+  const cpx t0 = cadd(a0, a2), t1 = csub(a0, a2);
+  const cpx t2 = cadd(a1, a3), t3 = cmul_mi(csub(a1, a3));
+  a0 = cadd(t0, t2);
+  a2 = csub(t0, t2);
+  a1 = cadd(t1, t3);
+  a3 = csub(t1, t3);
+}
+
+template <int R> struct Dft;
+
+template <> struct Dft<1> {
+  FWD static void run(cpx*) {}
+};
+
+template <> struct Dft<2> {
+  FWD static void run(cpx* a) {
+    const cpx t = a[0];
+    a[0] = cadd(t, a[1]);
+    a[1] = csub(t, a[1]);
+  }
+};
+
+template <> struct Dft<4> {
+  FWD static void run(cpx* a) { dft4(a[0], a[1], a[2], a[3]); }
+};
+
+template <> struct Dft<8> {
+  FWD static void run(cpx* a) {
+    const real_t c = (real_t)0.70710678118654752440;
+    cpx e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6];
+    cpx o0 = a[1], o1 = a[3], o2 = a[5], o3 = a[7];
+    dft4(e0, e1, e2, e3);
+    dft4(o0, o1, o2, o3);
+#if FW_FP32
+    o1 = cmulk(o1, 0.70710678118654752440, -0.70710678118654752440);
+    o2 = cmul_mi(o2);
+    o3 = cmulk(o3, -0.70710678118654752440, -0.70710678118654752440);
+#else
+    o1 = mk(c * (o1.x + o1.y), c * (o1.y - o1.x));     // * exp(-i pi/4)
+    o2 = cmul_mi(o2);                                  // * exp(-i pi/2)
+    o3 = mk(c * (o3.y - o3.x), -c * (o3.x + o3.y));    // * exp(-3 i pi/4)
+#endif
+    a[0] = cadd(e0, o0); a[4] = csub(e0, o0);
+    a[1] = cadd(e1, o1); a[5] = csub(e1, o1);
+    a[2] = cadd(e2, o2); a[6] = csub(e2, o2);
+    a[3] = cadd(e3, o3); a[7] = csub(e3, o3);
+  }
+};
+
+template <> struct Dft<16> {
+  // 4 x 4 decomposition: X_{k1 + 4 k2} = sum_n2 W16^{n2 k1} W4^{n2 k2} sum_n1 a_{4 n1 + n2} W4^{n1 k1}
+  FWD static void run(cpx* a) {
+    const double c1 = 0.92387953251128675613;   // cos(pi/8)
+    const double s1 = 0.38268343236508977173;   // sin(pi/8)
+    const double h = 0.70710678118654752440;    // cos(pi/4)
+    cpx b[16];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+      cpx x0 = a[n2], x1 = a[4 + n2], x2 = a[8 + n2], x3 = a[12 + n2];
+      dft4(x0, x1, x2, x3);
+      b[4 * n2 + 0] = x0; b[4 * n2 + 1] = x1; b[4 * n2 + 2] = x2; b[4 * n2 + 3] = x3;
+    }
+    // twiddles W16^{n2 k1}
+    b[5] = cmulk(b[5], c1, -s1);        // n2=1,k1=1 : W^1
+    b[6] = cmulk(b[6], h, -h);          // n2=1,k1=2 : W^2
+    b[7] = cmulk(b[7], s1, -c1);        // n2=1,k1=3 : W^3
+    b[9] = cmulk(b[9], h, -h);          // n2=2,k1=1 : W^2
+    b[10] = cmul_mi(b[10]);             // n2=2,k1=2 : W^4
+    b[11] = cmulk(b[11], -h, -h);       // n2=2,k1=3 : W^6
+    b[13] = cmulk(b[13], s1, -c1);      // n2=3,k1=1 : W^3
+    b[14] = cmulk(b[14], -h, -h);       // n2=3,k1=2 : W^6
+    b[15] = cmulk(b[15], -c1, s1);      // n2=3,k1=3 : W^9
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      cpx y0 = b[k1], y1 = b[4 + k1], y2 = b[8 + k1], y3 = b[12 + k1];
+      dft4(y0, y1, y2, y3);
+      a[k1] = y0; a[k1 + 4] = y1; a[k1 + 8] = y2; a[k1 + 12] = y3;
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Synchronisation
+// ---------------------------------------------------------------------------
+FWD void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8).
+// Remap so each XCD processes a contiguous range of tiles: neighbouring column
+// tiles share 128-byte lines and then share the XCD's L2.
+FWD int xcd_remap(int b, int nb) {
+  if (nb & 7) return b;
+  return (b & 7) * (nb >> 3) + (b >> 3);
+}
+
+// Twiddles are staged in LDS with one pad entry per 16: a stage reads entries
+// m*k*N/(NS*R) for consecutive k, whose power-of-two stride would otherwise put
+// the lanes of a wave on the same banks.
+FWD int twpad(int i) { return i + (i >> 4); }
+template <int N> struct TwLds { static constexpr int SIZE = N + N / 16 + 1; };
+
+// ---------------------------------------------------------------------------
+// Line FFT engine: P lines of length N per block, E complex elements per thread.
+// LM (line-major) lane mapping: t = tid % TPL, q = tid / TPL (rows, contiguous);
+// interleaved mapping: q = tid % P, t = tid / P (columns: a wave reads 2P
+// adjacent columns of several rows per instruction).
+// ---------------------------------------------------------------------------
+template <int N, int E, int P, bool LM>
+struct Line {
+  static_assert((N & (N - 1)) == 0 && N >= 2, "line length must be a power of two");
+  static_assert(E <= N && (E & (E - 1)) == 0, "elements per thread must be a power of two <= N");
+  static constexpr int TPL = N / E;
+  static constexpr int NT = P * TPL;
+  static constexpr int RMAX = E < 16 ? E : 16;
+  static constexpr int RAW = P * N;
+  static constexpr int LDS = RAW + (RAW >> 4) + 1;
+  static constexpr bool WAVE_LOCAL = (NT <= 64) || (LM && TPL <= 64 && (64 % TPL) == 0);
+
+  FWD static int lidx(int q, int i) {
+    const int r = LM ? q * N + i : i * P + q;
+    return r + (r >> 4);
+  }
+  FWD static void sync() {
+    if constexpr (WAVE_LOCAL) wave_sync();
+    else __syncthreads();
+  }
+  FWD static void load_natural(cpx (&v)[E], const cpx* lds, int q, int t) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = lds[lidx(q, t + TPL * e)];
+  }
+
+  // One Stockham radix-R stage; NS = product of the radices already applied.
+  // Butterfly j = t + TPL b takes x[j + m N/R] = v[b + m B] (B = E/R), applies
+  // the twiddles w^{m k}, k = j mod NS, w = exp(-2 pi i/(NS R)), and writes
+  // output q to index (j/NS) NS R + k + q NS. TOREG: the last stage, whose
+  // outputs land on the thread's own natural positions, stays in registers.
+  template <int R, int NS, bool TOREG>
+  FWD static void stage(cpx (&v)[E], cpx* lds, int q, int t, const tw_t* __restrict__ tw) {
+    constexpr int B = E / R;
+    constexpr int TWS = N / (NS * R);
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const int j = t + TPL * b;
+      cpx a[R];
+#pragma unroll
+      for (int m = 0; m < R; ++m) a[m] = v[b + m * B];
+      if constexpr (NS > 1) {
+        const int k = j & (NS - 1);
+#pragma unroll
+        for (int m = 1; m < R; ++m) a[m] = twmul(a[m], tw[twpad(m * k * TWS)]);
+      }
+      Dft<R>::run(a);
+      if constexpr (TOREG) {
+#pragma unroll
+        for (int m = 0; m < R; ++m) v[b + m * B] = a[m];
+      } else {
+        const int base = (j & ~(NS - 1)) * R + (j & (NS - 1));
+#pragma unroll
+        for (int m = 0; m < R; ++m) lds[lidx(q, base + m * NS)] = a[m];
+      }
+    }
+  }
+
+  // Forward FFT. TOLDS: leave the spectrum in LDS in natural order (synced);
+  // otherwise leave it in registers at the natural positions t + TPL e.
+  template <bool TOLDS, int NS = 1>
+  FWD static void fwd(cpx (&v)[E], cpx* lds, int q, int t, const tw_t* __restrict__ tw) {
+    constexpr int REM = N / NS;
+    constexpr int R = REM < RMAX ? REM : RMAX;
+    constexpr bool LAST = (NS * R == N);
+    if constexpr (LAST && !TOLDS) {
+      stage<R, NS, true>(v, lds, q, t, tw);
+    } else {
+      stage<R, NS, false>(v, lds, q, t, tw);
+      sync();
+      if constexpr (!LAST) {
+        load_natural(v, lds, q, t);
+        sync();
+        fwd<TOLDS, NS * R>(v, lds, q, t, tw);
+      }
+    }
+  }
+
+  // Unnormalised inverse FFT, registers to registers (conjugation identity).
+  FWD static void inv(cpx (&v)[E], cpx* lds, int q, int t, const tw_t* __restrict__ tw) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e].y = -v[e].y;
+    fwd<false>(v, lds, q, t, tw);
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e].y = -v[e].y;
+  }
+};
+
+enum { NSER = 9 };   // sums per step published by the propagators
+
+FWD cpx unit_phase(double ph, double m) {
+#if FW_FP32
+  // FP32 form: argument and its mod-2pi reduction in FP64, sincos in FP32.
+  const double two_pi = 6.283185307179586476925;
+  const double r = ph - two_pi * rint(ph * 0.15915494309189533577);
+  const float rf = (float)r;
+  const float rl = (float)(r - (double)rf);   // argument rounding, corrected to first order
+  float s, c;
+  sincosf(rf, &s, &c);
+  const float cc = c - s * rl, ss = s + c * rl;
+  const float mf = (float)m;
+  return mk(cc * mf, ss * mf);
+#else
+  double s, c;
+  sincos(ph, &s, &c);
+  return mk(c * m, s * m);
+#endif
+}
+
+// Table entries are built once, so they take the FP64 sincos in both forms and
+// are rounded once to the storage precision.
+FWD cpx unit_phase_exact(double ph, double m) {
+  double s, c;
+  sincos(ph, &s, &c);
+  return mk((real_t)(c * m), (real_t)(s * m));
+}
+
+// ---------------------------------------------------------------------------
+// Reductions
+// ---------------------------------------------------------------------------
+FWD double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Block-reduce NSER sums into partials[blk], then the last block to finish adds
+// the partials of all blocks (every thread takes a strided share, Neumaier-
+// compensated FP64, fixed order: deterministic), publishes the totals to out[]
+// and resets the ticket. red: >= NSER * 16 doubles of LDS.
+template <int NT>
+FWD double block_sum(double v, double* red, int i) {
+  constexpr int NW = (NT + 63) / 64;
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) red[(threadIdx.x >> 6) * NSER + i] = v;
+  (void)NW;
+  return v;
+}
+
+template <int NT>
+FWD bool block_reduce_publish(double (&s)[NSER], double* red, double* partials, unsigned int* counter,
+                              int blk, int nb, double* out) {
+  const int tid = threadIdx.x;
+  constexpr int NW = (NT + 63) / 64;
+  __shared__ int is_last;
+#pragma unroll
+  for (int i = 0; i < NSER; ++i) block_sum<NT>(s[i], red, i);
+  __syncthreads();
+  if (tid < NSER) {
+    double acc = 0.0;
+    for (int w = 0; w < NW; ++w) acc += red[w * NSER + tid];
+    __hip_atomic_store(partials + (size_t)blk * NSER + tid, acc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) {
+    const unsigned int tk = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    is_last = (tk == (unsigned int)(nb - 1));
+  }
+  __syncthreads();
+  if (!is_last) return false;
+  __threadfence();
+  double sum[NSER], comp[NSER];
+#pragma unroll
+  for (int i = 0; i < NSER; ++i) { sum[i] = 0.0; comp[i] = 0.0; }
+  for (int b = tid; b < nb; b += NT) {
+#pragma unroll
+    for (int i = 0; i < NSER; ++i) {
+      const double x = __hip_atomic_load(partials + (size_t)b * NSER + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const double tt = sum[i] + x;
+      comp[i] += (fabs(sum[i]) >= fabs(x)) ? (sum[i] - tt) + x : (x - tt) + sum[i];
+      sum[i] = tt;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NSER; ++i) block_sum<NT>(sum[i] + comp[i], red, i);
+  __syncthreads();
+  if (tid < NSER) {
+    double acc = 0.0;
+    for (int w = 0; w < NW; ++w) acc += red[w * NSER + tid];
+    out[tid] = acc;
+  }
+  __syncthreads();
+  if (tid == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// Deferred normalisation. The last V pass of a step writes one row of NSER
+// partial sums per block (plain stores, no atomics: an agent-scope fence or
+// atomic per block serialises the grid). The next kernel that needs the sums
+// adds the partials itself: every block takes the same strided order and the
+// same tree, so every block obtains the same bits.
+// ---------------------------------------------------------------------------
+template <int NT, int NS = NSER>
+FWD double sum_partials(const double* __restrict__ partials, int nb, int i, double* red) {
+  constexpr int NW = (NT + 63) / 64;
+  double sum = 0.0, comp = 0.0;
+  for (int b = threadIdx.x; b < nb; b += NT) {
+    const double x = partials[(size_t)b * NS + i];
+    const double tt = sum + x;
+    comp += (fabs(sum) >= fabs(x)) ? (sum - tt) + x : (x - tt) + sum;
+    sum = tt;
+  }
+  double v = wave_sum(sum + comp);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double tot = 0.0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) tot += red[w];
+  return tot;
+}
+
+// Block sums of the last V pass -> partials[blk] (plain stores).
+template <int NT, int NS = NSER>
+FWD void write_partials(double (&s)[NS], double* red, double* partials, int blk) {
+  constexpr int NW = (NT + 63) / 64;
+#pragma unroll
+  for (int i = 0; i < NS; ++i) {
+    const double v = wave_sum(s[i]);
+    if ((threadIdx.x & 63) == 0) red[(threadIdx.x >> 6) * NS + i] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < NS) {
+    double acc = 0.0;
+    for (int w = 0; w < NW; ++w) acc += red[w * NS + threadIdx.x];
+    partials[(size_t)blk * NS + threadIdx.x] = acc;
+  }
+}
+
+FWD double wave_min(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+}  // namespace fwk

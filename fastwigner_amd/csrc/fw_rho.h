@@ -1,0 +1,499 @@
+// fw_rho.h -- gfx950 device code of the density-matrix propagators
+// (von Neumann and Bloch), RhoVNeumannCUDA1D / RhoBlochCUDA1D of the reference.
+//
+// Generated prelude (before this header): FW_REAL, FW_FP32, FW_N, FW_ROW_E,
+// FW_ROW_P, FW_COL_E, FW_COL_P, FW_PHASE_SHIFT, FW_HAS_MASK_X, FW_HAS_MASK_P,
+// #include "fw_common.h", the constants (X_gridDIM, X_amplitude, dt, dX, dP) and
+// the spliced device functions V(X,t), K(P,t), fw_diff_V, fw_diff_K,
+// fw_mask_x(X, X_prime, t), fw_mask_p(P, P_prime, t).
+//
+// State: rho[i][j] = <X_j| rho |X'_i>, complex N x N, row-major
+// (rho_vneumann_cuda_1d.py:222,672-684). One step
+//   rho *= expV; FFT2; rho *= expK; IFFT2; rho *= expV; rho /= Tr(rho) dX
+// (rho_vneumann_cuda_1d.py:340-374) is three fused passes, each one read and one
+// write of rho:
+//   A (rows):    deferred norm, expV, forward FFT along j
+//   B (columns): forward FFT along i, expK, inverse FFT along i
+//   C (rows):    inverse FFT along j, expV, per-block partial sums (trace,
+//                <x>, <x^2>, <V'>, <V>, sum |rho|^2)
+// Both factors are separable: expV[i][j] = u_j conj(u_i) with
+// u_j = exp(-i dt/2 V(X_j, tau)) and expK[k'][k] = w_k conj(w_k') with
+// w_k = exp(-i dt K(P_k, tau)) (FFT-ordered momenta, :657-658), so only 2N phases
+// per step are evaluated, into per-step rows computed once per chunk of steps.
+// The Bloch factors exp(-db/4 [V(X)+V(X')-Vmin]) and exp(-db/2 [K(P)+K(P')-Kmin])
+// (rho_bloch_cuda_1d.py:271-309) factor the same way with real rows.
+#pragma once
+
+#include "fw_common.h"
+
+namespace fwk {
+
+// published sums (slots of the series): Re/Im of Tr rho, sum rho_ii X_i,
+// sum rho_ii X_i^2, sum rho_ii V'(X_i), sum rho_ii V(X_i); sum |rho|^2;
+// then (diagonal pass) Re/Im of Tr(rho P), Tr(rho P^2), Tr(rho K'(P)), Tr(rho K(P))
+enum { RSER = 20, R_TR = 0, R_X = 2, R_X2 = 4, R_DV = 6, R_V = 8, R_PUR = 10, R_P = 11, R_P2 = 13, R_DK = 15,
+       R_K = 17 };
+enum { RC_FRESH = 2, RC_STOP = 1 };
+enum { RF_FIRST = 1, RF_BLOCH = 4 };
+
+struct RArgs {
+  cpx* rho;            // state (or the work buffer of the Wigner transform)
+  cpx* aux;            // second buffer (Wigner transform target, diagonal partials)
+  const tw_t* tw;      // exp(-2 pi i j / N)
+  const double* times; // t after the increment, per step
+  double* scal;        // device scalars
+  double* partials;    // per-block partial sums (RSER per block)
+  double* series;      // RSER sums per step
+  const cpx* uv;       // per-step rows u_j (unitary) or real Bloch rows
+  const cpx* uk;       // per-step rows w_k
+  int step;
+  int flags;
+  int nbp;             // number of partial rows written by pass C
+  int nrows;           // phase rows available (chunk)
+  double dbeta, vmin, kmin, t_initial, guard;
+};
+
+FWD double xcoord(int j) { return ::dX * (j - 0.5 * ::X_gridDIM); }
+FWD double pcoord(int k) { return ::dP * ((k + FW_N / 2) % FW_N - 0.5 * FW_N); }
+
+// Complex norm of the previous step from the partials of pass C (same bits in
+// every block); block 0 publishes the previous step's sums to series[step] and
+// raises the purity guard. Returns false when the step must not run.
+template <int NT>
+FWD bool rho_previous(const RArgs& a, int blk, double* red, cpx& nrm) {
+  const double tr_re = sum_partials<NT, RSER>(a.partials, a.nbp, R_TR, red);
+  const double tr_im = sum_partials<NT, RSER>(a.partials, a.nbp, R_TR + 1, red);
+  // c = 1 / (Tr rho * dX), complex, as rho /= trace(rho) * dX (rho_vneumann_cuda_1d.py:369)
+  const double d = (tr_re * tr_re + tr_im * tr_im) * ::dX;
+  nrm = mk((real_t)(tr_re / d), (real_t)(-tr_im / d));
+  bool stop = false;
+  if (a.guard > 0.0) {
+    const double pur = sum_partials<NT, RSER>(a.partials, a.nbp, R_PUR, red);
+    const double purity = pur * ::dX * ::dX / ((tr_re * tr_re + tr_im * tr_im) * ::dX * ::dX);
+    stop = !(purity < a.guard);
+  }
+  if (blk == 0) {
+    double tot[R_PUR + 1];
+#pragma unroll
+    for (int i = 0; i <= R_PUR; ++i)
+      tot[i] = (i == R_TR) ? tr_re : (i == R_TR + 1) ? tr_im : sum_partials<NT, RSER>(a.partials, a.nbp, i, red);
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int i = 0; i <= R_PUR; ++i) a.series[(size_t)a.step * RSER + i] = tot[i];
+      if (stop) a.scal[RC_STOP] = (double)(a.step + 1);
+    }
+  }
+  return !stop;
+}
+
+FWD cpx rowfac(const cpx* __restrict__ u, int s, int j) { return u[(size_t)s * FW_N + j]; }
+
+// ---------------------------------------------------------------------------
+// Pass A / C: rows (along j). PASS_A: [norm], expV, forward FFT.
+// PASS_C: inverse FFT, expV, sums.
+// ---------------------------------------------------------------------------
+using RL = Line<FW_N, FW_ROW_E, FW_ROW_P, true>;
+using CL = Line<FW_N, FW_COL_E, FW_COL_P, false>;
+
+template <bool PASS_A, int FL>
+FWD void rho_row(const RArgs& a) {
+  using L = RL;
+  constexpr int N = FW_N, E = FW_ROW_E, P = FW_ROW_P, TPL = L::TPL, NT = L::NT;
+  constexpr int TWN = (N + NT - 1) / NT;
+  __shared__ cpx lds[L::LDS];
+  __shared__ tw_t twl[TwLds<N>::SIZE];
+  __shared__ double red[RSER * 16];
+  if (a.scal[RC_STOP] != 0.0) return;
+  const int tid = threadIdx.x;
+  const int t = tid % TPL, q = tid / TPL;
+  const int nb = gridDim.x;
+  const int blk = xcd_remap(blockIdx.x, nb);
+  const int i = blk * P + q;                 // row = x'
+  cpx* __restrict__ row = a.rho + (size_t)i * N;
+  const double tnow = a.times[a.step];
+
+  tw_t twr[TWN];
+#pragma unroll
+  for (int r = 0; r < TWN; ++r) twr[r] = a.tw[(tid + r * NT) & (N - 1)];
+  cpx v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = row[t + TPL * e];
+  cpx fj[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) fj[e] = rowfac(a.uv, a.step, t + TPL * e);
+  const cpx fi = cconj(rowfac(a.uv, a.step, i));
+#pragma unroll
+  for (int r = 0; r < TWN; ++r)
+    if ((N % NT) == 0 || tid + r * NT < N) twl[twpad(tid + r * NT)] = twr[r];
+  const double Xi = xcoord(i);
+
+  if constexpr (PASS_A) {
+    cpx nrm = mk((real_t)1, (real_t)0);
+    if constexpr (FL & RF_FIRST) {
+      if (a.scal[RC_FRESH] == 0.0) {
+        if (!rho_previous<NT>(a, blk, red, nrm)) return;
+      } else {
+        __syncthreads();
+      }
+    } else {
+      __syncthreads();
+    }
+    const cpx g = cmul(nrm, fi);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      cpx f = cmul(fj[e], g);
+#if FW_HAS_MASK_X
+      if (!(FL & RF_BLOCH)) f = cscale(f, (real_t)::fw_mask_x(xcoord(t + TPL * e), Xi, tnow));
+#endif
+      v[e] = cmul(v[e], f);
+    }
+    L::template fwd<false>(v, lds, q, t, twl);
+#pragma unroll
+    for (int e = 0; e < E; ++e) row[t + TPL * e] = v[e];
+  } else {
+    __syncthreads();
+    L::inv(v, lds, q, t, twl);
+    const real_t sN = (real_t)1 / (real_t)N;
+    double s[RSER];
+#pragma unroll
+    for (int r = 0; r < RSER; ++r) s[r] = 0.0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int j = t + TPL * e;
+      cpx f = cmul(fj[e], fi);
+#if FW_HAS_MASK_X
+      if (!(FL & RF_BLOCH)) f = cscale(f, (real_t)::fw_mask_x(xcoord(j), Xi, tnow));
+#endif
+      cpx w = cmul(cscale(v[e], sN), f);
+      row[j] = w;
+      const double wr = (double)w.x, wi = (double)w.y;
+      s[R_PUR] += wr * wr + wi * wi;
+      if (j == i) {
+        const double dv = ::fw_diff_V(Xi, tnow), vv = ::V(Xi, tnow);
+        s[R_TR] += wr; s[R_TR + 1] += wi;
+        s[R_X] += wr * Xi; s[R_X + 1] += wi * Xi;
+        s[R_X2] += wr * Xi * Xi; s[R_X2 + 1] += wi * Xi * Xi;
+        s[R_DV] += wr * dv; s[R_DV + 1] += wi * dv;
+        s[R_V] += wr * vv; s[R_V + 1] += wi * vv;
+      }
+    }
+    write_partials<NT, RSER>(s, red, a.partials, blk);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pass B: columns (along i): forward FFT, expK, inverse FFT.
+// ---------------------------------------------------------------------------
+template <int FL>
+FWD void rho_col(const RArgs& a) {
+  using L = CL;
+  constexpr int N = FW_N, E = FW_COL_E, P = FW_COL_P, TPL = L::TPL, NT = L::NT;
+  constexpr int TWN = (N + NT - 1) / NT;
+  __shared__ cpx lds[L::LDS];
+  __shared__ tw_t twl[TwLds<N>::SIZE];
+  if (a.scal[RC_STOP] != 0.0) return;
+  const int tid = threadIdx.x;
+  const int q = tid % P, t = tid / P;
+  const int nb = gridDim.x;
+  const int blk = xcd_remap(blockIdx.x, nb);
+  const int j = blk * P + q;                 // column = momentum index of x
+  if (blk == 0 && tid == 0 && a.scal[RC_FRESH] != 0.0) a.scal[RC_FRESH] = 0.0;
+  const double tnow = a.times[a.step];
+
+  tw_t twr[TWN];
+#pragma unroll
+  for (int r = 0; r < TWN; ++r) twr[r] = a.tw[(tid + r * NT) & (N - 1)];
+  cpx v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = a.rho[(size_t)(t + TPL * e) * N + j];
+  cpx wk[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) wk[e] = cconj(rowfac(a.uk, a.step, t + TPL * e));
+  const cpx wj = rowfac(a.uk, a.step, j);
+#pragma unroll
+  for (int r = 0; r < TWN; ++r)
+    if ((N % NT) == 0 || tid + r * NT < N) twl[twpad(tid + r * NT)] = twr[r];
+  __syncthreads();
+  L::template fwd<false>(v, lds, q, t, twl);
+  const real_t sN = (real_t)1 / (real_t)N;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    cpx f = cscale(cmul(wj, wk[e]), sN);
+#if FW_HAS_MASK_P
+    if (!(FL & RF_BLOCH)) f = cscale(f, (real_t)::fw_mask_p(pcoord(j), pcoord(t + TPL * e), tnow));
+#endif
+    v[e] = cmul(v[e], f);
+  }
+  L::sync();
+  L::inv(v, lds, q, t, twl);
+#pragma unroll
+  for (int e = 0; e < E; ++e) a.rho[(size_t)(t + TPL * e) * N + j] = v[e];
+}
+
+// ---------------------------------------------------------------------------
+// Per-step phase rows for `nrows` steps: times[s], s < nrows.
+// Unitary: uv = exp(-i dt/2 V(X_j, tau)), uk = exp(-i dt K(P_k, tau)).
+// Bloch:   uv = exp(-db/4 (V(X_j) - Vmin/2)), uk = exp(-db/2 (K(P_k) - Kmin/2)) at t_initial.
+// ---------------------------------------------------------------------------
+FWD void rho_rows(const RArgs& a, cpx* uv, cpx* uk) {
+  const size_t n = (size_t)a.nrows * FW_N;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
+    const int s = (int)(idx / FW_N), j = (int)(idx % FW_N);
+    const double X = xcoord(j), P = pcoord(j);
+    if (a.flags & RF_BLOCH) {
+      const double ti = a.t_initial;
+      uv[idx] = mk((real_t)exp(-0.25 * a.dbeta * (::V(X, ti) - 0.5 * a.vmin)), (real_t)0);
+      uk[idx] = mk((real_t)exp(-0.5 * a.dbeta * (::K(P, ti) - 0.5 * a.kmin)), (real_t)0);
+    } else {
+      const double tau = a.times[s] + FW_PHASE_SHIFT * ::dt;
+      uv[idx] = unit_phase_exact(-0.5 * ::dt * ::V(X, tau), 1.0);
+      uk[idx] = unit_phase_exact(-::dt * ::K(P, tau), 1.0);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Momentum observables: c_d = sum_i rho[i][(i - d) mod N] over row tiles, then
+// n_k = (1/N) sum_d c_d exp(2 pi i k d / N) and Tr(rho g(P)) = sum_k g(P_k) n_k,
+// which is the reference's fft(axis 1)/N -> g(P) -> ifft -> trace
+// (rho_vneumann_cuda_1d.py:452-472).
+// ---------------------------------------------------------------------------
+constexpr int DT = 256;   // threads of the diagonal kernels
+FWD void rho_diag_partial(const RArgs& a, int rows_per_block) {
+  const int i0 = blockIdx.x * rows_per_block;
+  for (int d = threadIdx.x; d < FW_N; d += DT) {
+    double cr = 0.0, ci = 0.0;
+    for (int r = 0; r < rows_per_block; ++r) {
+      const int i = i0 + r;
+      const cpx z = a.rho[(size_t)i * FW_N + ((i - d) & (FW_N - 1))];
+      cr += (double)z.x;
+      ci += (double)z.y;
+    }
+    a.aux[(size_t)blockIdx.x * FW_N + d] = mk((real_t)cr, (real_t)ci);
+  }
+}
+
+template <int E>
+FWD void rho_diag_finish(const RArgs& a, int nblk) {
+  // one block of N/E threads (a single line of the FFT engine)
+  using L = Line<FW_N, E, 1, true>;
+  constexpr int TPL = L::TPL;
+  __shared__ cpx lds[L::LDS];
+  __shared__ tw_t twl[TwLds<FW_N>::SIZE];
+  __shared__ double red[RSER * 16];
+  const int t = threadIdx.x;
+  for (int r = t; r < FW_N; r += TPL) twl[twpad(r)] = a.tw[r];
+  cpx v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int d = t + TPL * e;
+    double cr = 0.0, ci = 0.0;
+    for (int b = 0; b < nblk; ++b) {
+      const cpx z = a.aux[(size_t)b * FW_N + d];
+      cr += (double)z.x;
+      ci += (double)z.y;
+    }
+    v[e] = mk((real_t)cr, (real_t)ci);
+  }
+  __syncthreads();
+  L::inv(v, lds, 0, t, twl);   // sum_d c_d exp(+2 pi i k d / N)
+  const double tnow = a.times[a.step];
+  double s[RSER];
+#pragma unroll
+  for (int r = 0; r < RSER; ++r) s[r] = 0.0;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int k = t + TPL * e;
+    const double P = pcoord(k);
+    const double nr = (double)v[e].x / FW_N, ni = (double)v[e].y / FW_N;
+    const double g[4] = {P, P * P, ::fw_diff_K(P, tnow), ::K(P, tnow)};
+#pragma unroll
+    for (int o = 0; o < 4; ++o) {
+      s[R_P + 2 * o] += g[o] * nr;
+      s[R_P + 2 * o + 1] += g[o] * ni;
+    }
+  }
+  write_partials<TPL, RSER>(s, red, a.partials + (size_t)a.nbp * RSER, 0);
+  __syncthreads();
+  if (t < 8) a.series[(size_t)a.step * RSER + R_P + t] = a.partials[(size_t)a.nbp * RSER + R_P + t];
+}
+
+// ---------------------------------------------------------------------------
+// rho *= c (complex): c from the partials of the last pass C (end of a run;
+// block 0 also publishes that step's sums), or c = scal[0] + i scal[1].
+// ---------------------------------------------------------------------------
+FWD void rho_apply_norm(const RArgs& a, int from_partials) {
+  __shared__ double red[RSER * 4];
+  cpx c;
+  if (from_partials) {
+    (void)rho_previous<256>(a, blockIdx.x, red, c);   // called with guard = 0
+  } else {
+    c = mk((real_t)a.scal[0], (real_t)a.scal[3]);
+  }
+  const size_t n = (size_t)FW_N * FW_N;
+  for (size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x; idx < n; idx += (size_t)gridDim.x * 256)
+    a.rho[idx] = cmul(a.rho[idx], c);
+}
+
+// Sums over the current rho (trace family + purity) into scal[8..8+R_PUR];
+// scal[0] + i scal[3] = 1/(Tr rho dX). One block.
+FWD void rho_moments(const RArgs& a) {
+  __shared__ double red[RSER * 16];
+  const int tid = threadIdx.x;
+  const double tnow = a.times[a.step];
+  double s[RSER];
+#pragma unroll
+  for (int r = 0; r < RSER; ++r) s[r] = 0.0;
+  for (int i = tid; i < FW_N; i += 1024) {
+    const cpx z = a.rho[(size_t)i * FW_N + i];
+    const double Xi = xcoord(i), dv = ::fw_diff_V(Xi, tnow), vv = ::V(Xi, tnow);
+    const double wr = (double)z.x, wi = (double)z.y;
+    s[R_TR] += wr; s[R_TR + 1] += wi;
+    s[R_X] += wr * Xi; s[R_X + 1] += wi * Xi;
+    s[R_X2] += wr * Xi * Xi; s[R_X2 + 1] += wi * Xi * Xi;
+    s[R_DV] += wr * dv; s[R_DV + 1] += wi * dv;
+    s[R_V] += wr * vv; s[R_V + 1] += wi * vv;
+  }
+  const size_t n = (size_t)FW_N * FW_N;
+  for (size_t idx = tid; idx < n; idx += 1024) {
+    const cpx z = a.rho[idx];
+    s[R_PUR] += (double)z.x * z.x + (double)z.y * z.y;
+  }
+  write_partials<1024, RSER>(s, red, a.scal + 8, 0);
+  __syncthreads();
+  if (tid == 0) {
+    const double tr = a.scal[8 + R_TR], ti = a.scal[8 + R_TR + 1];
+    const double d = (tr * tr + ti * ti) * ::dX;
+    a.scal[0] = tr / d;
+    a.scal[3] = -ti / d;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Wigner function of rho by the Larkin three-shear 45 degree rotation and the
+// sign-flipped FFT (rho_vneumann_cuda_1d.py:502-592, kernels :753-800).
+// MODE 0: shear along x (rows):    FFT_j, * exp(-i tan(pi/8) P_k X'_i), IFFT_j / N
+// MODE 1: shear along x' (columns): FFT_i, * exp(+i sin(pi/4) P'_k X_j), IFFT_i / N
+// MODE 2: columns: * (-1)^i, IFFT_i (unnormalised), * (-1)^k, block sums of the result
+// ---------------------------------------------------------------------------
+template <int MODE>
+FWD void rho_wigner_pass(const RArgs& a, const cpx* __restrict__ src, cpx* __restrict__ dst) {
+  using L = typename pick<MODE == 0, RL, CL>::type;
+  constexpr int N = FW_N, E = (MODE == 0) ? FW_ROW_E : FW_COL_E, P = (MODE == 0) ? FW_ROW_P : FW_COL_P;
+  constexpr int TPL = L::TPL, NT = L::NT;
+  __shared__ cpx lds[L::LDS];
+  __shared__ tw_t twl[TwLds<N>::SIZE];
+  __shared__ double red[RSER * 16];
+  const int tid = threadIdx.x;
+  const int t = (MODE == 0) ? tid % TPL : tid / P;
+  const int q = (MODE == 0) ? tid / TPL : tid % P;
+  const int line = blockIdx.x * P + q;
+  for (int r = tid; r < N; r += NT) twl[twpad(r)] = a.tw[r];
+  auto at = [&](int e) -> size_t {
+    const int n = t + TPL * e;
+    return (MODE == 0) ? (size_t)line * N + n : (size_t)n * N + line;
+  };
+  cpx v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = src[at(e)];
+  __syncthreads();
+  if constexpr (MODE == 2) {
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+      if ((t + TPL * e) & 1) v[e] = mk(-v[e].x, -v[e].y);
+    L::inv(v, lds, q, t, twl);
+    double s[RSER];
+#pragma unroll
+    for (int r = 0; r < RSER; ++r) s[r] = 0.0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      if ((t + TPL * e) & 1) v[e] = mk(-v[e].x, -v[e].y);
+      s[0] += (double)v[e].x;
+      s[1] += (double)v[e].y;
+      dst[at(e)] = v[e];
+    }
+    write_partials<NT, RSER>(s, red, a.partials, blockIdx.x);
+  } else {
+    L::template fwd<false>(v, lds, q, t, twl);
+    const real_t sN = (real_t)1 / (real_t)N;
+    const double coord = xcoord(line);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const double pk = pcoord(t + TPL * e);
+      const double ph = (MODE == 0) ? -0.41421356237309504880 * pk * coord   // -tan(pi/8) P X'
+                                    : 0.70710678118654752440 * pk * coord;   // +sin(pi/4) P' X
+      v[e] = cmul(v[e], unit_phase_exact(ph, (double)sN));
+    }
+    L::sync();
+    L::inv(v, lds, q, t, twl);
+#pragma unroll
+    for (int e = 0; e < E; ++e) dst[at(e)] = v[e];
+  }
+}
+
+// W /= sum(W) * wigner_dxdp (complex), sums from the MODE 2 partials
+FWD void rho_wigner_scale(const RArgs& a, cpx* w, double wigner_dxdp) {
+  __shared__ double red[RSER * 4];
+  const double sr = sum_partials<256, RSER>(a.partials, a.nbp, 0, red);
+  const double si = sum_partials<256, RSER>(a.partials, a.nbp, 1, red);
+  const double d = (sr * sr + si * si) * wigner_dxdp;
+  const cpx c = mk((real_t)(sr / d), (real_t)(-si / d));
+  const size_t n = (size_t)FW_N * FW_N;
+  for (size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x; idx < n; idx += (size_t)gridDim.x * 256)
+    w[idx] = cmul(w[idx], c);
+}
+
+}  // namespace fwk
+
+#ifndef FW_KSET
+#define FW_KSET 1
+#endif
+#if FW_KSET == 1
+extern "C" __global__ void __launch_bounds__(fwk::RL::NT) fw_rho_a(fwk::RArgs a) { fwk::rho_row<true, fwk::RF_FIRST>(a); }
+extern "C" __global__ void __launch_bounds__(fwk::CL::NT) fw_rho_b(fwk::RArgs a) { fwk::rho_col<0>(a); }
+extern "C" __global__ void __launch_bounds__(fwk::RL::NT) fw_rho_c(fwk::RArgs a) { fwk::rho_row<false, 0>(a); }
+extern "C" __global__ void __launch_bounds__(fwk::RL::NT) fw_rho_ba(fwk::RArgs a) {
+  fwk::rho_row<true, fwk::RF_FIRST | fwk::RF_BLOCH>(a);
+}
+extern "C" __global__ void __launch_bounds__(fwk::CL::NT) fw_rho_bb(fwk::RArgs a) { fwk::rho_col<fwk::RF_BLOCH>(a); }
+extern "C" __global__ void __launch_bounds__(fwk::RL::NT) fw_rho_bc(fwk::RArgs a) {
+  fwk::rho_row<false, fwk::RF_BLOCH>(a);
+}
+extern "C" __global__ void __launch_bounds__(256) fw_rho_rows(fwk::RArgs a, fwk::cpx* uv, fwk::cpx* uk) {
+  fwk::rho_rows(a, uv, uk);
+}
+extern "C" __global__ void __launch_bounds__(256) fw_rho_diag_partial(fwk::RArgs a, int rows) {
+  fwk::rho_diag_partial(a, rows);
+}
+extern "C" __global__ void __launch_bounds__(FW_N / FW_DIAG_E) fw_rho_diag_finish(fwk::RArgs a, int nblk) {
+  fwk::rho_diag_finish<FW_DIAG_E>(a, nblk);
+}
+extern "C" __global__ void __launch_bounds__(256) fw_rho_apply_norm(fwk::RArgs a, int from_partials) {
+  fwk::rho_apply_norm(a, from_partials);
+}
+extern "C" __global__ void __launch_bounds__(1024) fw_rho_moments(fwk::RArgs a) { fwk::rho_moments(a); }
+extern "C" __global__ void __launch_bounds__(fwk::RL::NT) fw_rho_shear_x(fwk::RArgs a, const fwk::cpx* s, fwk::cpx* d) {
+  fwk::rho_wigner_pass<0>(a, s, d);
+}
+extern "C" __global__ void __launch_bounds__(fwk::CL::NT) fw_rho_shear_y(fwk::RArgs a, const fwk::cpx* s, fwk::cpx* d) {
+  fwk::rho_wigner_pass<1>(a, s, d);
+}
+extern "C" __global__ void __launch_bounds__(fwk::CL::NT) fw_rho_flip_ifft(fwk::RArgs a, const fwk::cpx* s, fwk::cpx* d) {
+  fwk::rho_wigner_pass<2>(a, s, d);
+}
+extern "C" __global__ void __launch_bounds__(256) fw_rho_wigner_scale(fwk::RArgs a, fwk::cpx* w, double dxdp) {
+  fwk::rho_wigner_scale(a, w, dxdp);
+}
+extern "C" __global__ void fw_rho_set_scalar(double* p, double v) { *p = v; }
+// min over j of V(X_j, t_initial): V_min of the Bloch factor is twice it
+// (fill_V + gpuarray.min over the X, X' grid, rho_bloch_cuda_1d.py:69-82,336-346)
+extern "C" __global__ void __launch_bounds__(256) fw_rho_vmin(fwk::RArgs a) {
+  __shared__ double r[4];
+  double m = __builtin_inf();
+  for (int j = threadIdx.x; j < FW_N; j += 256) m = fmin(m, ::V(fwk::xcoord(j), a.t_initial));
+  m = fwk::wave_min(m);
+  if ((threadIdx.x & 63) == 0) r[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) a.scal[8] = 2.0 * fmin(fmin(r[0], r[1]), fmin(r[2], r[3]));
+}
+#endif

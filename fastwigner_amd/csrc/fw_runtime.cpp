@@ -25,7 +25,11 @@
 #include "fastwigner.h"
 #include "fw_internal.h"
 
+extern const char fw_common_h_src[];
 extern const char fw_device_h_src[];
+extern const char fw_rho_h_src[] __attribute__((weak));
+extern const char fw_batch_h_src[] __attribute__((weak));
+extern const char fw_fno_h_src[] __attribute__((weak));
 
 namespace fw {
 
@@ -89,7 +93,18 @@ int compile(const std::string& src, const std::string& arch, std::vector<char>& 
   std::vector<std::string> opts = {"--offload-arch=" + arch, "-O3", "-std=c++17", "-Xclang", "-target-feature",
                                    "-Xclang", "-packed-fp32-ops", "-Wno-unused-variable",
                                    "-Wno-unused-but-set-variable"};
-  std::string key_src = src + "\n//" + arch + "\n//" + fw_device_h_src;
+  // the embedded headers hipRTC may include
+  std::vector<const char*> hdr_names, hdrs;
+  const char* names_all[] = {"fw_common.h", "fw_device.h", "fw_rho.h", "fw_batch.h", "fw_fno.h"};
+  const char* srcs_all[] = {fw_common_h_src, fw_device_h_src, &fw_rho_h_src ? fw_rho_h_src : nullptr,
+                            &fw_batch_h_src ? fw_batch_h_src : nullptr, &fw_fno_h_src ? fw_fno_h_src : nullptr};
+  std::string key_src = src + "\n//" + arch;
+  for (int i = 0; i < 5; ++i)
+    if (srcs_all[i]) {
+      hdr_names.push_back(names_all[i]);
+      hdrs.push_back(srcs_all[i]);
+      key_src += std::string("\n//") + srcs_all[i];
+    }
   for (auto& o : opts) key_src += "\n//" + o;
   const std::string key = hex128(key_src);
   {
@@ -121,9 +136,8 @@ int compile(const std::string& src, const std::string& arch, std::vector<char>& 
     f << src;
   }
   hiprtcProgram prog;
-  const char* hdr_names[] = {"fw_device.h"};
-  const char* hdrs[] = {fw_device_h_src};
-  if (hiprtcCreateProgram(&prog, src.c_str(), "fastwigner_kernels.hip", 1, hdrs, hdr_names) != HIPRTC_SUCCESS)
+  if (hiprtcCreateProgram(&prog, src.c_str(), "fastwigner_kernels.hip", (int)hdrs.size(), hdrs.data(),
+                          hdr_names.data()) != HIPRTC_SUCCESS)
     return fail(FW_ERR_COMPILE, "hiprtcCreateProgram failed");
   std::vector<const char*> copts;
   for (auto& o : opts) copts.push_back(o.c_str());

@@ -1,0 +1,278 @@
+"""von Neumann propagator of the density matrix on MI355X.
+
+``RhoVNeumannHIP1D`` is the drop-in for ``RhoVNeumannCUDA1D``
+(rho_vneumann_cuda_1d.py:15-858): same keyword arguments (``X_gridDIM``,
+``X_amplitude``, ``dt``, ``t``, ``V``/``K``/``diff_V``/``diff_K`` strings,
+``consts``, ``functions``, ``abs_boundary_x``, ``abs_boundary_p``,
+``max_thread_block``), methods (``set_rho``, ``propagate``,
+``single_step_propagation``, ``get_Ehrenfest``, ``get_average`` with its tuple of
+alternating x / p observables, ``get_purity``, ``get_sigma_x_sigma_p``,
+``get_wignerfunction``, ``get_wigner_time``) and attributes (``X``, ``P``,
+``dX``, ``dP``, ``X_wigner``, ``P_wigner``, ``wigner_dxdp``, ``rho``, ``t`` and
+the Ehrenfest lists).
+
+Per-step Ehrenfest observables come from sums fused into the step kernels
+(trace, <x>, <V'>, <V>, purity) and one diagonal-sum pass for the momentum
+family (<p>, <K'>, <K>). The reference also records the Wigner time each step,
+which needs the user attributes ``wigner_initial`` and ``Ip``
+(rho_vneumann_cuda_1d.py:408-412); here it is recorded when both are set.
+
+Additional keyword arguments: ``precision`` ("fp64" or "fp32": complex64
+storage and FFTs, FP64 phases and sums), ``phase_time_shift`` (V, K at
+t_{n+1} + shift dt; default 0 as the reference's density-matrix kernels),
+``device``, ``tiles`` = (row_e, row_p, col_e, col_p).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._common import DeviceArray, bind_kwargs, consts_source, precision_code
+
+_GRID_NAMES = ("X_gridDIM", "X_amplitude", "dt", "dX", "dP")
+
+
+class RhoVNeumannHIP1D:
+    """Second-order split-operator propagator for the von Neumann equation for
+    rho(x, x', t) with H = K(p, t) + V(x, t), on hand-written gfx950 kernels."""
+
+    _not_consts = ("t",)
+
+    def __init__(self, **kwargs):
+        bind_kwargs(self, kwargs)
+        for name, msg in (("X_gridDIM", "Coordinate grid size (X_gridDIM) was not specified"),
+                          ("X_amplitude", "Coordinate grid range (X_amplitude) was not specified"),
+                          ("V", "Potential energy (V) was not specified"),
+                          ("K", "Momentum dependence (K) was not specified"),
+                          ("dt", "Time-step (dt) was not specified")):
+            if not hasattr(self, name):
+                raise AttributeError(msg)
+        if not hasattr(self, "t"):
+            print("Warning: Initial time (t) was not specified, thus it is set to zero.")
+            self.t = 0.
+        self.t = np.float64(self.t)
+
+        # grids, rho_vneumann_cuda_1d.py:93-110
+        self.dX = 2. * self.X_amplitude / self.X_gridDIM
+        self.X = np.linspace(-self.X_amplitude, self.X_amplitude - self.dX, self.X_gridDIM)
+        self.P = np.fft.fftfreq(self.X_gridDIM, self.dX / (2 * np.pi))
+        self.dP = self.P[1] - self.P[0]
+        self.P_wigner = self.P / np.sqrt(2.)
+        self.X_wigner = self.X / np.sqrt(2.)
+        self.wigner_dxdp = (self.X_wigner[1] - self.X_wigner[0]) * (self.P_wigner[1] - self.P_wigner[0])
+        self.P_wigner = np.fft.fftshift(self.P_wigner)[:, np.newaxis]
+        self.X_wigner = self.X_wigner[np.newaxis, :]
+
+        self.precision = getattr(self, "precision", "fp64")
+        self._prec = precision_code(self.precision)
+        self.phase_time_shift = float(getattr(self, "phase_time_shift", 0.0))
+        self.device = int(getattr(self, "device", 0))
+        self.max_thread_block = getattr(self, "max_thread_block", 512)
+
+        # C constants, rho_vneumann_cuda_1d.py:118-133 (+ consts, functions :130-133,188-192)
+        self.hip_consts = consts_source(kwargs, exclude=self._not_consts + self._extra_not_consts() + _GRID_NAMES)
+        self.hip_consts += getattr(self, "consts", "") or ""
+        self.hip_consts += getattr(self, "functions", "") or ""
+        self.cuda_consts = self.hip_consts
+        mx = getattr(self, "abs_boundary_x", None)
+        mp = getattr(self, "abs_boundary_p", None)
+        self.abs_boundary_x = mx if isinstance(mx, str) else "1."
+        self.abs_boundary_p = mp if isinstance(mp, str) else "1."
+
+        self._handle = None
+        desc = _lib.rho_desc(self.X_gridDIM, self.X_amplitude, self.dt, self.phase_time_shift, self._prec,
+                             self.device, getattr(self, "tiles", None))
+        dv = getattr(self, "diff_V", None)
+        dk = getattr(self, "diff_K", None)
+        ex = _lib.rho_exprs(V=self.V, K=self.K, consts=self.hip_consts,
+                            diff_V=dv if isinstance(dv, str) else None, diff_K=dk if isinstance(dk, str) else None,
+                            abs_boundary_x=mx if isinstance(mx, str) else None,
+                            abs_boundary_p=mp if isinstance(mp, str) else None)
+        self._desc, self._exprs = desc, ex
+        h = ctypes.c_void_p()
+        _lib.call("fw_rho_create", ctypes.byref(desc), ctypes.byref(ex), ctypes.byref(h))
+        self._handle = h
+        n = self.X_gridDIM
+        self.rho = DeviceArray(self, (n, n), np.complex64 if self._prec else np.complex128,
+                               self._get_host_state, self._state_ptr)
+        self._wigner = DeviceArray(self, (n, n), np.complex64 if self._prec else np.complex128,
+                                   self._get_wigner, lambda: None)
+
+        try:
+            self.diff_K
+            self.diff_V
+            self.X_average = []
+            self.P_average = []
+            self.X_average_RHS = []
+            self.P_average_RHS = []
+            self.hamiltonian_average = []
+            self.wigner_time = []
+            self.purity_series = []
+            self.isEhrenfest = True
+        except AttributeError:
+            self.isEhrenfest = False
+
+    def _extra_not_consts(self):
+        return ()
+
+    def __del__(self):
+        h = getattr(self, "_handle", None)
+        if h is not None and h.value:
+            try:
+                _lib.load().fw_rho_destroy(h)
+            except Exception:
+                pass
+            self._handle = None
+
+    # -- device state ----------------------------------------------------------------
+    def _get_host_state(self):
+        n = self.X_gridDIM
+        out = np.empty((n, n), np.complex128)
+        _lib.call("fw_rho_get_state", self._handle, out.ctypes.data_as(_lib._PD))
+        return out
+
+    def _state_ptr(self):
+        p, es = ctypes.c_void_p(), ctypes.c_int()
+        _lib.call("fw_rho_state_ptr", self._handle, ctypes.byref(p), ctypes.byref(es))
+        return p.value
+
+    def _get_wigner(self):
+        n = self.X_gridDIM
+        out = np.empty((n, n), np.complex128)
+        _lib.call("fw_rho_wigner", self._handle, out.ctypes.data_as(_lib._PD))
+        return out
+
+    # -- API ----------------------------------------------------------------------------
+    def set_rho(self, new_rho):
+        """rho_vneumann_cuda_1d.py:301-338: ndarray / DeviceArray, C string of
+        (X, X_prime, i, j), python function F(self, X, X_prime) or a float;
+        then rho /= Tr(rho) dX."""
+        n = self.X_gridDIM
+        if isinstance(new_rho, (np.ndarray, DeviceArray)):
+            assert new_rho.shape == (n, n), "The grid sizes does not match with the density matrix"
+            r = new_rho.get() if isinstance(new_rho, DeviceArray) else new_rho
+            r = np.ascontiguousarray(r, dtype=np.complex128)
+            _lib.call("fw_rho_set_state", self._handle, r.ctypes.data_as(_lib._PD))
+        elif isinstance(new_rho, str):
+            _lib.call("fw_rho_set_state_expr", self._handle, new_rho.encode())
+        elif isinstance(new_rho, float):
+            r = np.full((n, n), new_rho, dtype=np.complex128)
+            _lib.call("fw_rho_set_state", self._handle, r.ctypes.data_as(_lib._PD))
+        elif callable(new_rho):
+            r = np.ascontiguousarray(np.broadcast_to(new_rho(self, self.X[:, np.newaxis], self.X[np.newaxis, :]),
+                                                     (n, n)), dtype=np.complex128)
+            _lib.call("fw_rho_set_state", self._handle, r.ctypes.data_as(_lib._PD))
+        else:
+            raise NotImplementedError("new_rho must be either function or numpy.array")
+        tr = (ctypes.c_double * 2)()
+        _lib.call("fw_rho_normalize", self._handle, tr)
+        return self
+
+    def _times(self, steps):
+        times = np.empty(steps)
+        for s in range(steps):
+            self.t += self.dt
+            times[s] = self.t
+        return times
+
+    def _observables(self, s):
+        """Normalised Ehrenfest observables from the raw sums of one step."""
+        tr = complex(s[0], s[1])
+        c = 1.0 / (tr * self.dX)
+
+        def tr_of(k):
+            return (c * complex(s[k], s[k + 1])).real * self.dX
+
+        return dict(X_average=tr_of(2), P_average=tr_of(11), X_average_RHS=tr_of(15), P_average_RHS=-tr_of(6),
+                    hamiltonian_average=tr_of(17) + tr_of(8), purity=s[10] * abs(c) ** 2 * self.dX ** 2)
+
+    def _record(self, series):
+        for s in series:
+            o = self._observables(s)
+            self.X_average.append(o["X_average"])
+            self.X_average_RHS.append(o["X_average_RHS"])
+            self.P_average.append(o["P_average"])
+            self.P_average_RHS.append(o["P_average_RHS"])
+            self.hamiltonian_average.append(o["hamiltonian_average"])
+            self.purity_series.append(o["purity"])
+
+    def _wigner_time_enabled(self):
+        return self.isEhrenfest and hasattr(self, "wigner_initial") and hasattr(self, "Ip")
+
+    def propagate(self, steps=1):
+        """rho_vneumann_cuda_1d.py:355-374: steps x (t += dt, step, rho /= Tr(rho) dX,
+        Ehrenfest observables); returns ``self.rho``."""
+        steps = int(steps)
+        if steps <= 0:
+            return self.rho
+        if self._wigner_time_enabled():
+            for _ in range(steps):       # the Wigner time needs the Wigner function every step
+                self._propagate_chunk(1)
+                self.wigner_current = self.get_wignerfunction()
+                self.wigner_time.append(self.get_wigner_time(self.wigner_current, self.wigner_initial, self.t))
+            return self.rho
+        self._propagate_chunk(steps)
+        return self.rho
+
+    def _propagate_chunk(self, steps):
+        times = self._times(steps)
+        series = np.empty((steps, _lib.FW_RSER))
+        _lib.call("fw_rho_propagate", self._handle, _lib.dptr(times), steps, 1 if self.isEhrenfest else 0,
+                  _lib.dptr(series))
+        self.last_series = series
+        if self.isEhrenfest:
+            self._record(series)
+
+    def single_step_propagation(self):
+        """One step at the current t (renormalised, unlike the reference)."""
+        times = np.array([self.t])
+        _lib.call("fw_rho_propagate", self._handle, _lib.dptr(times), 1, 0, None)
+        return self.rho
+
+    def get_Ehrenfest(self, t):
+        if self.isEhrenfest:
+            s = np.empty(_lib.FW_RSER)
+            _lib.call("fw_rho_moments", self._handle, float(t), _lib.dptr(s))
+            self._record([s])
+
+    def get_average(self, observable):
+        """Tr[rho F1(x) g1(p) F2(x) ...] dX for observable = ("F1(x)", "g1(p)", ...),
+        None skipping a slot (rho_vneumann_cuda_1d.py:431-472)."""
+        obs = (ctypes.c_char_p * len(observable))(*[o.encode() if o else None for o in observable])
+        out = ctypes.c_double()
+        _lib.call("fw_rho_get_average", self._handle, obs, len(observable), float(self.t), ctypes.byref(out))
+        return out.value
+
+    def get_purity(self):
+        """Tr rho^2 = sum |rho|^2 dX^2 (rho_vneumann_cuda_1d.py:474-488)."""
+        s = np.empty(_lib.FW_RSER)
+        _lib.call("fw_rho_moments", self._handle, float(self.t), _lib.dptr(s))
+        return self._observables(s)["purity"]
+
+    def get_sigma_x_sigma_p(self):
+        """rho_vneumann_cuda_1d.py:490-500."""
+        return np.sqrt((self.get_average(("X * X",)) - self.get_average(("X",)) ** 2) *
+                       (self.get_average((None, "P * P")) - self.get_average((None, "P")) ** 2))
+
+    def get_wignerfunction(self):
+        """Wigner function on the (X/sqrt2, P/sqrt2) grid by the Larkin three-shear
+        rotation (rho_vneumann_cuda_1d.py:502-592); a complex DeviceArray."""
+        _lib.call("fw_rho_wigner", self._handle, None)
+        return self._wigner
+
+    def get_wigner_time(self, wigner_current, wigner_init, t):
+        """int_{H > -Ip} (W - W_init) dxdp with the reference's indicator
+        (rho_vneumann_cuda_1d.py:594-620,830-858); wigner_current must be the
+        Wigner function last computed by get_wignerfunction."""
+        w0 = wigner_init.get() if isinstance(wigner_init, DeviceArray) else np.asarray(wigner_init)
+        w0 = np.ascontiguousarray(np.real(w0), dtype=np.float64)
+        out = ctypes.c_double()
+        _lib.call("fw_rho_wigner_time", self._handle, _lib.dptr(w0), float(t), ctypes.byref(out))
+        return out.value
+
+    @classmethod
+    def print_memory_info(cls):
+        pass
+
+
+RhoVNeumannCUDA1D = RhoVNeumannHIP1D

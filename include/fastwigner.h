@@ -155,6 +155,76 @@ int fw_wigner_snapshot(fw_wigner* h, int which);
 int fw_wigner_set_timing(fw_wigner* h, int enable);
 int fw_wigner_kernel_times(fw_wigner* h, double us_out[3]);
 
+/* ------------------------------------------------------------------------ */
+/* Density-matrix propagators (von Neumann, Bloch)                           */
+/* replaces RhoVNeumannCUDA1D (rho_vneumann_cuda_1d.py:15-858) and           */
+/* RhoBlochCUDA1D (rho_bloch_cuda_1d.py:9-366)                              */
+/* ------------------------------------------------------------------------ */
+
+#define FW_RSER 20 /* per-step sums of the density-matrix propagators:
+   [0,1] Re/Im Tr rho, [2,3] sum rho_ii x_i, [4,5] sum rho_ii x_i^2, [6,7] sum rho_ii V'(x_i),
+   [8,9] sum rho_ii V(x_i), [10] sum |rho|^2, [11,12] Tr(rho P), [13,14] Tr(rho P^2),
+   [15,16] Tr(rho K'(P)), [17,18] Tr(rho K(P)) -- over the unnormalised rho of the step */
+
+typedef struct fw_rho fw_rho;
+
+typedef struct fw_rho_desc {
+  int x_grid_dim;          /* X_gridDIM: power of two, 8..4096 */
+  int precision;           /* FW_FP64 (complex128) or FW_FP32 (complex64, FP64 phases and sums) */
+  double x_amplitude;      /* X_amplitude */
+  double dt;
+  double phase_time_shift; /* V, K at t_{n+1} + shift*dt; 0 = the reference kernels (rho_vneumann_cuda_1d.py:660,681) */
+  int device;
+  int row_e, row_p;        /* row passes: elements per thread, rows per block (0 = default) */
+  int col_e, col_p;        /* column pass: elements per thread, columns per block (0 = default) */
+} fw_rho_desc;
+
+typedef struct fw_rho_exprs {
+  const char* consts;         /* C declarations, verbatim (may be NULL) */
+  const char* V;              /* V(X, t) */
+  const char* K;              /* K(P, t) */
+  const char* diff_V;         /* NULL = "0." */
+  const char* diff_K;         /* NULL = "0." */
+  const char* abs_boundary_x; /* mask of (X, X_prime, t) in the V factor (rho_vneumann_cuda_1d.py:683); NULL = none */
+  const char* abs_boundary_p; /* mask of (P, P_prime, t) in the K factor (:662); NULL = none */
+} fw_rho_exprs;
+
+/* dX and dP (P = fftfreq(N, dX/2pi), rho_vneumann_cuda_1d.py:94-101) */
+int fw_rho_grid_constants(const fw_rho_desc* desc, double out[2]);
+int fw_rho_check(const fw_rho_desc* desc, const fw_rho_exprs* exprs);
+int fw_rho_create(const fw_rho_desc* desc, const fw_rho_exprs* exprs, fw_rho** out);
+int fw_rho_destroy(fw_rho* h);
+/* rho <- host array of N*N complex values stored as (re, im) pairs */
+int fw_rho_set_state(fw_rho* h, const double* host_rho);
+/* rho <- expr(X, X_prime, i, j) (init_rho, rho_vneumann_cuda_1d.py:687-708); cuda_complex is available */
+int fw_rho_set_state_expr(fw_rho* h, const char* expr);
+/* rho /= Tr(rho) dX (complex, rho_vneumann_cuda_1d.py:336); trace_out = Tr(rho) before */
+int fw_rho_normalize(fw_rho* h, double trace_out[2]);
+int fw_rho_get_state(fw_rho* h, double* host_rho);
+int fw_rho_state_ptr(fw_rho* h, void** dev_ptr, int* elem_size);
+/* `steps` von Neumann steps (rho_vneumann_cuda_1d.py:355-374): t = times[s], rho /= Tr(rho) dX after
+ * each; series (optional) gets FW_RSER sums per step; momentum_sums != 0 adds the Tr(rho g(P)) family
+ * (one extra read of rho per step). */
+int fw_rho_propagate(fw_rho* h, const double* times, int steps, int momentum_sums, double* series);
+/* FW_RSER sums over the current rho at time t (the momentum family included) */
+int fw_rho_moments(fw_rho* h, double t, double sums[FW_RSER]);
+/* Tr[rho F1(x) g1(p) F2(x) ...] dX, real part (get_average, rho_vneumann_cuda_1d.py:431-472); obs[k] NULL
+ * skips a slot; each string is compiled once */
+int fw_rho_get_average(fw_rho* h, const char* const* obs, int nobs, double t, double* out);
+/* Bloch: V_min = min of V(X)+V(X') at t_initial, K_min = 0 as in the reference (rho_bloch_cuda_1d.py:354-365) */
+int fw_rho_bloch_setup(fw_rho* h, double t_initial, double mins_out[2]);
+int fw_rho_bloch_propagate(fw_rho* h, double dbeta, int steps, double purity_guard, double* series,
+                           int* steps_done);
+int fw_rho_snapshot(fw_rho* h, int which);
+/* Wigner function of rho on the (X/sqrt2, P/sqrt2) grid (get_wignerfunction, rho_vneumann_cuda_1d.py:502-592),
+ * complex N*N (re, im) pairs, rows in fftshift(P) order; kept on the device for fw_rho_wigner_time */
+int fw_rho_wigner(fw_rho* h, double* host_w);
+/* sum over the last Wigner function of 0.5(1 + signbit(H + Ip)) (Re W - W_init) dxdp
+ * (get_wigner_time, rho_vneumann_cuda_1d.py:594-620,830-858); needs a constant Ip */
+int fw_rho_wigner_time(fw_rho* h, const double* host_w_init, double t, double* out);
+int fw_rho_set_timing(fw_rho* h, int enable);
+int fw_rho_kernel_times(fw_rho* h, double us_out[3]);
+
 #ifdef __cplusplus
 }
 #endif
