@@ -59,13 +59,28 @@ def kernel_times(s, steps=64):
     return list(out)
 
 
+def measured_traffic(kernel, n, precision):
+    """HBM-side bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload (scaled by a copy kernel of
+    known size), or None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles", "moyal_%s_%d_traffic.md" % (precision, n))
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+            return int(f[2])
+    return None
+
+
 def roofline(n, precision, ktimes):
     names = ["v_first", "k", "v_last"]
     i = int(np.argmax(ktimes))
     bytes_per_launch = BYTES_PER_PASS[precision] * n * n
     achieved = bytes_per_launch / (ktimes[i] * 1e-6) / 1e9
     return {"bound": "hbm", "kernel": "fw_" + names[i], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic("fw_" + names[i], n, precision),
             "bytes_per_launch": bytes_per_launch, "mean_us": [round(x, 3) for x in ktimes]}
 
 
@@ -104,15 +119,18 @@ def main():
     ap.add_argument("--no-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
-                    help="only run device-to-device copies of one 2048^2 FP64 state (32 MiB), the known-size "
-                         "reference that scales the FETCH_SIZE / WRITE_SIZE counters")
+                    help="only run 10 launches of a plain elementwise copy (y = 1.0 * x) of 32 MiB, the "
+                         "known-size reference that scales the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.copy_probe:
-        from fastwigner_amd import _lib
-        s = make(2048, "fp64")
+        import torch
+        x = torch.rand(2048 * 2048, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
         for _ in range(10):
-            _lib.call("fw_wigner_snapshot", s._handle, 0)
-        print(json.dumps({"copy_probe_bytes": 2048 * 2048 * 8, "copies": 10}))
+            torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": x.numel() * 8, "copy_probe_bytes_written": y.numel() * 8,
+                          "launches": 10}))
         return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: a single Wigner grid is kept on one GPU (sharding it would be "
