@@ -110,6 +110,68 @@ def oracle_sample(n, max_seconds, max_steps, workers):
     return o, steps, el
 
 
+def batched_variant(steps=500, o_workers=1, with_oracle=True):
+    """Batched random-collision workload: 256 instances, grids from
+    {128,256,512,1024}^2 (rng seed 4), quartic potentials, Gaussian initial
+    states, gamma ~ U[0, 0.1], kT ~ U[0.1, 1] (Gibbs states from the batched
+    Bloch run), FP32 with FP64 phases and sums, dt = 0.01, `steps` steps with
+    one persistent launch per step replayed from a hipGraph. Rate in total
+    grid-point*steps/s; error of the three smallest instances against the FP64
+    oracle."""
+    from fastwigner_amd.batch import WignerBatchHIP1D, gaussian_state, random_quartic_instances, QUARTIC_V, \
+        QUARTIC_DV
+    inst, _ = random_quartic_instances(256, seed=4)
+    b = WignerBatchHIP1D(inst, V=QUARTIC_V, K="0.5*P*P", dt=0.01, param_names=("a2", "a3", "a4"),
+                         diff_V=QUARTIC_DV, diff_K="P", precision="fp32")
+    t0 = time.perf_counter()
+    b.prepare_random_collision(dbeta=0.01)
+    prep = time.perf_counter() - t0
+    for i, d in enumerate(inst):
+        b.set_wignerfunction(i, gaussian_state(d["X_gridDIM"], 8., d["x0"], d["p0"]))
+    b.propagate(64, record=False)                     # warm-up (graph capture)
+    for i, d in enumerate(inst):
+        b.set_wignerfunction(i, gaussian_state(d["X_gridDIM"], 8., d["x0"], d["p0"]))
+    b.t = 0.0
+    t0 = time.perf_counter()
+    b.propagate(steps)                                # observables of every instance, every step
+    el = time.perf_counter() - t0
+    rate = b.total_points * steps / el
+    out = {"value": rate, "unit": "grid-point*steps/s", "ms_per_step": el * 1e3 / steps, "steps": steps,
+           "instances": 256, "total_points": b.total_points, "gibbs_prep_s": round(prep, 2),
+           "tasks_per_step": b.tasks_per_step, "persistent_grid": b.grid}
+    # one step moves W in three passes plus the dissipator's two reads of W0s: 32 B per point (FP32)
+    us = b.step_time_us(640)
+    bytes_step = 32 * b.total_points
+    out["roofline"] = {"bound": "hbm", "kernel": "fw_batch_step", "achieved": round(bytes_step / (us * 1e-6) / 1e9, 1),
+                       "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": round(bytes_step / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                       "bytes_per_launch": bytes_step, "mean_us": round(us, 3)}
+    print("[bench] batched fp32: %.3e pt*steps/s, %.1f us/step (device %.1f us)" % (rate, el * 1e6 / steps, us),
+          file=sys.stderr, flush=True)
+    if with_oracle:
+        from oracle import fw_oracle as fo
+        errs = []
+        b2 = WignerBatchHIP1D(inst, V=QUARTIC_V, K="0.5*P*P", dt=0.01, param_names=("a2", "a3", "a4"),
+                              diff_V=QUARTIC_DV, diff_K="P", precision="fp32")
+        b2.prepare_random_collision(dbeta=0.01)
+        for i, d in enumerate(inst):
+            b2.set_wignerfunction(i, gaussian_state(d["X_gridDIM"], 8., d["x0"], d["p0"]))
+        b2.propagate(steps, record=False)
+        for i in sorted(range(256), key=lambda k: inst[k]["X_gridDIM"])[:3]:
+            d = inst[i]
+            a2, a3, a4 = d["params"]
+            n = d["X_gridDIM"]
+            o = fo.RandomCollisionOracle(n, 8., n, 8., 0.01, gamma=d["gamma"], kT=d["kT"], dbeta=0.01,
+                                         V=lambda X, t: a2 * X * X + a3 * X ** 3 + a4 * X ** 4,
+                                         K=lambda P, t: 0.5 * P * P, phase_time_shift=0.0, workers=o_workers)
+            o.set_wignerfunction(gaussian_state(n, 8., d["x0"], d["p0"]))
+            o.propagate(steps)
+            W = b2.get_wignerfunction(i)
+            errs.append(float(np.linalg.norm(W - o.W) / np.linalg.norm(o.W)))
+        out["rel_l2_vs_fp64_oracle"] = {"instances": "three smallest", "max": max(errs), "values": errs}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,7 +183,12 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run 10 launches of a plain elementwise copy (y = 1.0 * x) of 32 MiB, the "
                          "known-size reference that scales the FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only the batched FP32 random-collision workload (for profiling)")
     args = ap.parse_args()
+    if args.batched_only:
+        print(json.dumps(batched_variant(steps=args.steps, with_oracle=False)), flush=True)
+        return
     if args.copy_probe:
         import torch
         x = torch.rand(2048 * 2048, dtype=torch.float64, device="cuda")
@@ -190,6 +257,8 @@ def main():
         r2, ms2 = timed_rate(s2, 2048, max(args.steps // 4, 100), 50)
         var["fp64_2048"] = {"value": r2, "ms_per_step": ms2, "roofline": roofline(2048, "fp64", kernel_times(s2))}
         print("[bench] fp64 2048^2: %.3e pt*steps/s" % r2, file=sys.stderr, flush=True)
+        var["batched_fp32_config4"] = batched_variant(o_workers=min(os.cpu_count() or 1, 16),
+                                                      with_oracle=not args.no_baseline)
         result["variants"] = var
 
     print(json.dumps(result), flush=True)

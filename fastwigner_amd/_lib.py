@@ -76,6 +76,28 @@ class RhoExprs(ctypes.Structure):
         "consts", "V", "K", "diff_V", "diff_K", "abs_boundary_x", "abs_boundary_p")]
 
 
+FW_BATCH_NPRM = 8
+
+
+class BatchDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_instances", ctypes.c_int), ("precision", ctypes.c_int), ("device", ctypes.c_int),
+        ("mode", ctypes.c_int), ("dt", ctypes.c_double), ("phase_time_shift", ctypes.c_double),
+        ("e_for_n", ctypes.c_int * 5), ("blocks_per_cu", ctypes.c_int),
+    ]
+
+
+class BatchInstance(ctypes.Structure):
+    _fields_ = [
+        ("grid_dim", ctypes.c_int), ("x_amplitude", ctypes.c_double), ("p_amplitude", ctypes.c_double),
+        ("params", ctypes.c_double * FW_BATCH_NPRM),
+    ]
+
+
+class BatchExprs(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_char_p) for n in ("consts", "param_names", "V", "K", "diff_V", "diff_K")]
+
+
 def _b(s):
     return None if s is None else s.encode()
 
@@ -119,6 +141,36 @@ def rho_exprs(V, K, consts=None, diff_V=None, diff_K=None, abs_boundary_x=None, 
     e = RhoExprs()
     e.consts, e.V, e.K, e.diff_V, e.diff_K = _b(consts), _b(V), _b(K), _b(diff_V), _b(diff_K)
     e.abs_boundary_x, e.abs_boundary_p = _b(abs_boundary_x), _b(abs_boundary_p)
+    return e
+
+
+def batch_desc(n, dt, precision=1, device=0, mode=0, shift=0.0, e_for_n=None, blocks_per_cu=0):
+    d = BatchDesc()
+    d.n_instances, d.precision, d.device, d.mode = int(n), int(precision), int(device), int(mode)
+    d.dt, d.phase_time_shift = float(dt), float(shift)
+    for k, e in enumerate(e_for_n or [0] * 5):
+        d.e_for_n[k] = int(e)
+    d.blocks_per_cu = int(blocks_per_cu)
+    return d
+
+
+def batch_instances(specs):
+    """specs: list of (grid_dim, x_amplitude, p_amplitude, params)."""
+    arr = (BatchInstance * len(specs))()
+    for i, (n, xa, pa, prm) in enumerate(specs):
+        arr[i].grid_dim, arr[i].x_amplitude, arr[i].p_amplitude = int(n), float(xa), float(pa)
+        prm = list(prm)
+        if len(prm) > FW_BATCH_NPRM:
+            raise ValueError("at most %d parameters per instance" % FW_BATCH_NPRM)
+        for k, v in enumerate(prm):
+            arr[i].params[k] = float(v)
+    return arr
+
+
+def batch_exprs(V, K, param_names="", consts=None, diff_V=None, diff_K=None):
+    e = BatchExprs()
+    e.consts, e.param_names, e.V, e.K = _b(consts), _b(param_names), _b(V), _b(K)
+    e.diff_V, e.diff_K = _b(diff_V), _b(diff_K)
     return e
 
 
@@ -171,6 +223,17 @@ PROTOTYPES = {
     "fw_rho_wigner_time": [_P, _PD, _D, _PD],
     "fw_rho_set_timing": [_P, _I],
     "fw_rho_kernel_times": [_P, _PD],
+    "fw_batch_check": [ctypes.POINTER(BatchDesc), ctypes.POINTER(BatchInstance), ctypes.POINTER(BatchExprs)],
+    "fw_batch_create": [ctypes.POINTER(BatchDesc), ctypes.POINTER(BatchInstance), ctypes.POINTER(BatchExprs), _PP],
+    "fw_batch_destroy": [_P],
+    "fw_batch_set_state": [_P, _I, _PD],
+    "fw_batch_get_state": [_P, _I, _PD],
+    "fw_batch_set_dissipator": [_P, _I, _PD, _D],
+    "fw_batch_gibbs_dissipators": [_P, _PD],
+    "fw_batch_bloch_propagate": [_P, _I, _D, _PD, _PI],
+    "fw_batch_propagate": [_P, _PD, _I, _PD],
+    "fw_batch_info": [_P, ctypes.POINTER(ctypes.c_longlong), _PI, _PI],
+    "fw_batch_step_time": [_P, _I, _PD],
 }
 
 _lib = None

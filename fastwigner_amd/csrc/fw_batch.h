@@ -1,0 +1,521 @@
+// fw_batch.h -- batched split-operator propagation of many independent Wigner
+// functions on one gfx950 device: each instance has its own square grid
+// (N x N, N a power of two in [64, 1024]), amplitudes, potential parameters,
+// collision rate and temperature. Embedded in libfastwigner.so and compiled by
+// hipRTC with a prelude holding:
+//   FW_REAL, FW_FP32, FW_PHASE_SHIFT, FW_B_NMAX (largest N of the batch),
+//   FW_B_HAS_<N> (1 for every N present), FW_B_E<N> (elements per thread of
+//   the FFT of length N), FW_B_EMAX, the time step `dt`, the caller's constants,
+//   and the spliced expressions
+//     double V(double X, double t, const double* prm), K(P, t, prm),
+//     fw_diff_V(X, t, prm), fw_diff_K(P, t, prm)
+//   in which the caller's parameter names are bound to prm[0..7].
+//
+// Work decomposition. A step of one instance is the three passes of the single
+// grid propagator (V columns, K rows, V columns; see fw_device.h), cut into
+// tiles of P line pairs, one tile per 256-thread block. Tasks (pass, instance,
+// tile) sit in eight queues, one per XCD: each instance's tasks go to one
+// queue, so the column tiles that share 128-byte lines, and the passes that
+// re-read the instance, stay on one XCD's L2. Queues are ordered pass-major.
+//
+// One launch per step (mode 0): a persistent grid takes tasks from its XCD's
+// queue (then from the others), and a task of pass p of instance i waits until
+// every tile of pass p-1 of instance i has been released (per-instance
+// counters; agent-scope release/acquire). Tasks earlier in a queue never wait
+// on later ones and a task is only taken by a running block, so the waits
+// always end; each wait is also bounded (spin_limit), after which the block
+// raises the error flag and every block drains out.
+// One launch per pass (mode 1): grid = tasks of that pass, one task per block,
+// no waits.
+#pragma once
+
+#include "fw_common.h"
+
+namespace fwk {
+
+constexpr int BT = 256;   // threads per block of the batched kernels
+
+// One instance (host mirror in fw_batch.cpp; field order and types must match).
+struct BInst {
+  long long off;          // element offset of W and W0s
+  int n;                  // grid dimension N (N x N)
+  int ntile;              // tiles per pass (N / (2 P))
+  int part;               // first partial row of the instance
+  int dissip;             // 1: random-collision dissipator on
+  int nbloch;             // Bloch runs: steps of this instance
+  int pad_;
+  double dX, dP, dLambda, dTheta;
+  double w_coeff;         // exp(-gamma dt / 2)
+  double dbeta;           // Bloch step of this instance
+  double prm[8];          // parameters of the spliced expressions
+};
+
+struct BTask {
+  int inst;
+  int code;               // pass << 16 | tile; inst < 0: no work (padding)
+};
+
+struct BArgs {
+  real_t* W;
+  const real_t* W0s;
+  const BInst* inst;
+  const BTask* tasks;     // mode 0: queues concatenated; mode 1: passes concatenated
+  const int* qoff;        // [9] queue offsets (mode 0)
+  const tw_t* tw;         // exp(-2 pi i j / FW_B_NMAX)
+  const double* times;    // t after the increment, per chunk-local step
+  double* partials;       // NSER per tile of the last V pass
+  double* series;         // [slot][instance][NSER]
+  unsigned int* qhead;    // [chunk step][8]
+  unsigned int* done;     // [chunk step][instance][3]
+  int* fresh;             // [instance]: W normalised, partials stale
+  int* err;               // dependency wait exceeded spin_limit
+  const double* vkmin;    // Bloch: [instance][2] V_min, K_min
+  int step;               // chunk-local step (series slot, counters)
+  int gstep;              // global step (Bloch: instances stop after nbloch)
+  int ninst;
+  int mode;
+  int task_base;          // mode 1: first task of this launch
+  int spin_limit;
+  double cv, ck;          // Bloch exponent factors (1/4, 1/2 or 1/2, 1)
+};
+
+enum { BK_MOYAL = 0, BK_BLOCH = 1 };
+
+// ---------------------------------------------------------------------------
+// Phase factors evaluated on the fly (FP64 argument; the FP32 form reduces it
+// mod 2 pi in FP64 before its FP32 sincos)
+// ---------------------------------------------------------------------------
+FWD cpx b_vfactor(double X, double Th, double tau, const double* prm) {
+  return unit_phase(-0.5 * ::dt * (::V(X - 0.5 * Th, tau, prm) - ::V(X + 0.5 * Th, tau, prm)), 1.0);
+}
+FWD cpx b_kfactor(double Lm, double P, double tau, const double* prm) {
+  return unit_phase(-::dt * (::K(P + 0.5 * Lm, tau, prm) - ::K(P - 0.5 * Lm, tau, prm)), 1.0);
+}
+FWD cpx b_bloch_v(const BArgs& a, int i, double X, double Th, const double* prm, double db) {
+  const double t0 = a.times[0];
+  const double f = exp(-a.cv * db * (::V(X - 0.5 * Th, t0, prm) + ::V(X + 0.5 * Th, t0, prm) - a.vkmin[2 * i]));
+  return mk((real_t)f, (real_t)0);
+}
+FWD cpx b_bloch_k(const BArgs& a, int i, double Lm, double P, const double* prm, double db) {
+  const double t0 = a.times[0];
+  const double f = exp(-a.ck * db * (::K(P + 0.5 * Lm, t0, prm) + ::K(P - 0.5 * Lm, t0, prm) - a.vkmin[2 * i + 1]));
+  return mk((real_t)f, (real_t)0);
+}
+
+FWD void b_pair_apply(cpx& zk, cpx& zm, cpx fa, cpx fb, real_t s) {
+  const real_t h = (real_t)0.5 * s;
+  const cpx A = mk(h * (zk.x + zm.x), h * (zk.y - zm.y));
+  const cpx B = mk(h * (zk.y + zm.y), h * (zm.x - zk.x));
+  const cpx Ap = cmul(A, fa), Bp = cmul(B, fb);
+  zk = mk(Ap.x - Bp.y, Ap.y + Bp.x);
+  zm = mk(Ap.x + Bp.y, Bp.x - Ap.y);
+}
+FWD cpx b_self_apply(cpx z, cpx fa, cpx fb, real_t s) { return mk(z.x * fa.x * s, z.y * fb.x * s); }
+
+// ---------------------------------------------------------------------------
+// V pass of one column tile: P column pairs c = 2 (tile P + q), q < P.
+// FIRST: deferred normalisation (+ dissipator) on load. LAST: dissipator + the
+// nine sums into partials.
+// ---------------------------------------------------------------------------
+template <int N, int E, int KIND, bool FIRST, bool LAST>
+FWD void b_vpass(const BArgs& a, int i, int tile, int tid, cpx* lds, const tw_t* twl, double* red) {
+  constexpr int TPL = N / E, P = BT / TPL;
+  static_assert(P >= 1 && P * TPL == BT, "a tile must fill the block");
+  using L = Line<N, E, P, false, FW_B_NMAX>;
+  const BInst* in = a.inst + i;
+  const int q = tid % P, t = tid / P;
+  const int c = (tile * P + q) * 2;
+  real_t* __restrict__ W = a.W + in->off;
+  const double* prm = in->prm;
+  const double tnow = a.times[a.step];
+  const double tau = tnow + FW_PHASE_SHIFT * ::dt;
+  const double dX = in->dX, dP = in->dP, dTh = in->dTheta;
+  const bool dis = KIND == BK_MOYAL && in->dissip;
+  const real_t wc = (real_t)in->w_coeff;
+
+  cpx v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = *reinterpret_cast<const cpx*>(W + (size_t)(t + TPL * e) * N + c);
+
+  if constexpr (FIRST) {
+    // the previous step's norm (its sums are published by the K pass)
+    double nrm = 1.0;
+    if (a.fresh[i] == 0)
+      nrm = 1.0 / (sum_partials<BT>(a.partials + (size_t)in->part * NSER, in->ntile, 0, red) * (dX * dP));
+    const real_t rn = (real_t)nrm;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      cpx w = v[e];
+      w.x *= rn; w.y *= rn;
+      if (dis) {
+        const cpx g = *reinterpret_cast<const cpx*>(a.W0s + in->off + (size_t)(t + TPL * e) * N + c);
+        w.x = w.x * wc + g.x;
+        w.y = w.y * wc + g.y;
+      }
+      v[e] = w;
+    }
+  }
+  __syncthreads();
+  L::template fwd<true>(v, lds, q, t, twl);
+
+  const double Xa = dX * (c - 0.5 * N), Xb = dX * (c + 1 - 0.5 * N);
+  const double db = in->dbeta;
+  {
+    const real_t s = (real_t)1 / (real_t)N;
+    // not unrolled: the factors are evaluated on the fly, and unrolling lets
+    // the scheduler hoist all of them at once (register pressure)
+#pragma unroll 1
+    for (int e = 0; e < E / 2; ++e) {
+      const int k = t + TPL * e;
+      if (k == 0) {
+        const double ThN = dTh * (N / 2);
+        cpx fa0, fb0, faN, fbN;
+        if constexpr (KIND == BK_BLOCH) {
+          fa0 = b_bloch_v(a, i, Xa, 0.0, prm, db); fb0 = b_bloch_v(a, i, Xb, 0.0, prm, db);
+          faN = b_bloch_v(a, i, Xa, ThN, prm, db); fbN = b_bloch_v(a, i, Xb, ThN, prm, db);
+        } else {
+          fa0 = b_vfactor(Xa, 0.0, tau, prm); fb0 = b_vfactor(Xb, 0.0, tau, prm);
+          faN = b_vfactor(Xa, ThN, tau, prm); fbN = b_vfactor(Xb, ThN, tau, prm);
+        }
+        const int i0 = L::lidx(q, 0), iN = L::lidx(q, N / 2);
+        lds[i0] = b_self_apply(lds[i0], fa0, fb0, s);
+        lds[iN] = b_self_apply(lds[iN], faN, fbN, s);
+      } else {
+        const double Th = dTh * k;
+        cpx fa, fb;
+        if constexpr (KIND == BK_BLOCH) {
+          fa = b_bloch_v(a, i, Xa, Th, prm, db); fb = b_bloch_v(a, i, Xb, Th, prm, db);
+        } else {
+          fa = b_vfactor(Xa, Th, tau, prm); fb = b_vfactor(Xb, Th, tau, prm);
+        }
+        const int ik = L::lidx(q, k), im = L::lidx(q, N - k);
+        cpx zk = lds[ik], zm = lds[im];
+        b_pair_apply(zk, zm, fa, fb, s);
+        lds[ik] = zk;
+        lds[im] = zm;
+      }
+    }
+  }
+  L::sync();
+  L::load_natural(v, lds, q, t);
+  L::sync();
+  L::inv(v, lds, q, t, twl);
+
+  double sums[NSER];
+#pragma unroll
+  for (int k = 0; k < NSER; ++k) sums[k] = 0.0;
+  double ca = 0.0, cb = 0.0;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int n = t + TPL * e;
+    const size_t idx = (size_t)n * N + c;
+    cpx w = v[e];
+    if constexpr (LAST) {
+      const double Pn = dP * (n - 0.5 * N);
+      if (dis) {
+        const cpx g = *reinterpret_cast<const cpx*>(a.W0s + in->off + idx);
+        w.x = w.x * wc + g.x;
+        w.y = w.y * wc + g.y;
+      }
+      const double wa = (double)w.x, wb = (double)w.y, ws = wa + wb;
+      ca += wa; cb += wb;
+      sums[2] += Pn * ws;
+      sums[4] += Pn * Pn * ws;
+      sums[5] += ::fw_diff_K(Pn, tnow, prm) * ws;
+      sums[7] += ::K(Pn, tnow, prm) * ws;
+      sums[8] += wa * wa + wb * wb;
+    }
+    *reinterpret_cast<cpx*>(W + idx) = w;
+  }
+  if constexpr (LAST) {
+    sums[0] = ca + cb;
+    sums[1] = Xa * ca + Xb * cb;
+    sums[3] = Xa * Xa * ca + Xb * Xb * cb;
+    sums[6] = ::fw_diff_V(Xa, tnow, prm) * ca + ::fw_diff_V(Xb, tnow, prm) * cb;
+    sums[7] += ::V(Xa, tnow, prm) * ca + ::V(Xb, tnow, prm) * cb;
+    write_partials<BT>(sums, red, a.partials + (size_t)in->part * NSER, tile);
+  }
+}
+
+// K pass of one row tile: P row pairs r = 2 (tile P + q).
+template <int N, int E, int KIND>
+FWD void b_kpass(const BArgs& a, int i, int tile, int tid, cpx* lds, const tw_t* twl, double* red) {
+  constexpr int TPL = N / E, P = BT / TPL;
+  using L = Line<N, E, P, true, FW_B_NMAX>;
+  const BInst* in = a.inst + i;
+  const int t = tid % TPL, q = tid / TPL;
+  const int r = (tile * P + q) * 2;
+  const double* prm = in->prm;
+  const double tnow = a.times[a.step];
+  const double tau = tnow + FW_PHASE_SHIFT * ::dt;
+  real_t* __restrict__ Wa = a.W + in->off + (size_t)r * N;
+  real_t* __restrict__ Wb = Wa + N;
+  if (tile == 0) {
+    // Every V tile of this step has read `fresh` and the previous step's
+    // partials, and the last V pass has not yet overwritten them: publish the
+    // previous step's sums to series[step], or clear `fresh` at a run's start.
+    const int fr = a.fresh[i];
+    __syncthreads();
+    if (fr == 0) {
+      const double* pp = a.partials + (size_t)in->part * NSER;
+#pragma unroll 1
+      for (int k = 0; k < NSER; ++k) {
+        const double v = sum_partials<BT>(pp, in->ntile, k, red);
+        if (tid == 0) a.series[((size_t)a.step * a.ninst + i) * NSER + k] = v;
+      }
+    } else if (tid == 0) {
+      a.fresh[i] = 0;
+    }
+  }
+
+  cpx v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = mk(Wa[t + TPL * e], Wb[t + TPL * e]);
+  __syncthreads();
+  L::template fwd<true>(v, lds, q, t, twl);
+  {
+    const double dP = in->dP, dL = in->dLambda, db = in->dbeta;
+    const double Pa = dP * (r - 0.5 * N), Pb = dP * (r + 1 - 0.5 * N);
+    const real_t s = (real_t)1 / (real_t)N;
+    // not unrolled: the factors are evaluated on the fly, and unrolling lets
+    // the scheduler hoist all of them at once (register pressure)
+#pragma unroll 1
+    for (int e = 0; e < E / 2; ++e) {
+      const int k = t + TPL * e;
+      if (k == 0) {
+        const double LN = dL * (N / 2);
+        cpx fa0, fb0, faN, fbN;
+        if constexpr (KIND == BK_BLOCH) {
+          fa0 = b_bloch_k(a, i, 0.0, Pa, prm, db); fb0 = b_bloch_k(a, i, 0.0, Pb, prm, db);
+          faN = b_bloch_k(a, i, LN, Pa, prm, db); fbN = b_bloch_k(a, i, LN, Pb, prm, db);
+        } else {
+          fa0 = b_kfactor(0.0, Pa, tau, prm); fb0 = b_kfactor(0.0, Pb, tau, prm);
+          faN = b_kfactor(LN, Pa, tau, prm); fbN = b_kfactor(LN, Pb, tau, prm);
+        }
+        const int i0 = L::lidx(q, 0), iN = L::lidx(q, N / 2);
+        lds[i0] = b_self_apply(lds[i0], fa0, fb0, s);
+        lds[iN] = b_self_apply(lds[iN], faN, fbN, s);
+      } else {
+        const double Lm = dL * k;
+        cpx fa, fb;
+        if constexpr (KIND == BK_BLOCH) {
+          fa = b_bloch_k(a, i, Lm, Pa, prm, db); fb = b_bloch_k(a, i, Lm, Pb, prm, db);
+        } else {
+          fa = b_kfactor(Lm, Pa, tau, prm); fb = b_kfactor(Lm, Pb, tau, prm);
+        }
+        const int ik = L::lidx(q, k), im = L::lidx(q, N - k);
+        cpx zk = lds[ik], zm = lds[im];
+        b_pair_apply(zk, zm, fa, fb, s);
+        lds[ik] = zk;
+        lds[im] = zm;
+      }
+    }
+  }
+  L::sync();
+  L::load_natural(v, lds, q, t);
+  L::sync();
+  L::inv(v, lds, q, t, twl);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    Wa[t + TPL * e] = v[e].x;
+    Wb[t + TPL * e] = v[e].y;
+  }
+}
+
+template <int N, int E, int KIND>
+FWD void b_run_n(const BArgs& a, int i, int pass, int tile, int tid, cpx* lds, const tw_t* twl, double* red) {
+  if (pass == 0) b_vpass<N, E, KIND, true, false>(a, i, tile, tid, lds, twl, red);
+  else if (pass == 1) b_kpass<N, E, KIND>(a, i, tile, tid, lds, twl, red);
+  else b_vpass<N, E, KIND, false, true>(a, i, tile, tid, lds, twl, red);
+}
+
+template <int KIND>
+FWD void b_run(const BArgs& a, int i, int pass, int tile, int tid, cpx* lds, const tw_t* twl, double* red) {
+  switch (a.inst[i].n) {
+#if FW_B_HAS_64
+    case 64: b_run_n<64, FW_B_E64, KIND>(a, i, pass, tile, tid, lds, twl, red); break;
+#endif
+#if FW_B_HAS_128
+    case 128: b_run_n<128, FW_B_E128, KIND>(a, i, pass, tile, tid, lds, twl, red); break;
+#endif
+#if FW_B_HAS_256
+    case 256: b_run_n<256, FW_B_E256, KIND>(a, i, pass, tile, tid, lds, twl, red); break;
+#endif
+#if FW_B_HAS_512
+    case 512: b_run_n<512, FW_B_E512, KIND>(a, i, pass, tile, tid, lds, twl, red); break;
+#endif
+#if FW_B_HAS_1024
+    case 1024: b_run_n<1024, FW_B_E1024, KIND>(a, i, pass, tile, tid, lds, twl, red); break;
+#endif
+    default: break;
+  }
+}
+
+constexpr int B_LDS = BT * FW_B_EMAX + (BT * FW_B_EMAX) / 16 + 1;
+
+// Wait until every tile of pass `pass` of instance i is released (thread 0
+// spins, bounded), then acquire. Returns false if the wait gave up.
+FWD bool b_wait(const BArgs& a, int i, int pass, unsigned int target) {
+  __shared__ int ok;
+  if (threadIdx.x == 0) {
+    unsigned int* c = a.done + ((size_t)a.step * a.ninst + i) * 3 + pass;
+    int it = 0, good = 1;
+    while (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      __builtin_amdgcn_s_sleep(2);
+      if (++it > a.spin_limit) {
+        __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        good = 0;
+        break;
+      }
+    }
+    ok = good;
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return ok != 0;
+}
+
+FWD void b_release(const BArgs& a, int i, int pass) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (threadIdx.x == 0)
+    __hip_atomic_fetch_add(a.done + ((size_t)a.step * a.ninst + i) * 3 + pass, 1u, __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int KIND>
+FWD void b_step(const BArgs& a) {
+  __shared__ cpx lds[B_LDS];
+  __shared__ tw_t twl[TwLds<FW_B_NMAX>::SIZE];
+  __shared__ double red[NSER * 16];
+  __shared__ int s_task;
+  for (int j = threadIdx.x; j < FW_B_NMAX; j += BT) twl[twpad(j)] = a.tw[j];
+  __syncthreads();
+  if (a.mode == 1) {
+    const BTask tk = a.tasks[a.task_base + blockIdx.x];
+    const int i = tk.inst, pass = tk.code >> 16, tile = tk.code & 0xffff;   // uniform (scalar loads)
+    if (i < 0 || (KIND == BK_BLOCH && a.gstep >= a.inst[i].nbloch)) return;
+    b_run<KIND>(a, i, pass, tile, threadIdx.x, lds, twl, red);
+    return;
+  }
+  const int home = blockIdx.x & 7;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      int got = -1;
+      if (__hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+        for (int k = 0; k < 8 && got < 0; ++k) {
+          const int qn = (home + k) & 7;
+          const int len = a.qoff[qn + 1] - a.qoff[qn];
+          if (len == 0) continue;
+          unsigned int* h = a.qhead + (size_t)a.step * 8 + qn;
+          if (__hip_atomic_load(h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= (unsigned int)len) continue;
+          const unsigned int slot = __hip_atomic_fetch_add(h, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (slot < (unsigned int)len) got = a.qoff[qn] + (int)slot;
+        }
+      }
+      s_task = got;
+    }
+    __syncthreads();
+    // the task is the same for the whole block: make that visible to the
+    // compiler, so the instance fields are scalar loads and the size switch a
+    // uniform branch
+    const int task = __builtin_amdgcn_readfirstlane(s_task);
+    __syncthreads();
+    if (task < 0) break;
+    const BTask tk = a.tasks[task];
+    const int i = tk.inst, pass = tk.code >> 16, tile = tk.code & 0xffff;
+    bool ok = true;
+    if (pass > 0) ok = b_wait(a, i, pass - 1, (unsigned int)a.inst[i].ntile);
+    const bool active = !(KIND == BK_BLOCH && a.gstep >= a.inst[i].nbloch);
+    // the lane index is re-materialised per task: otherwise the compiler keeps
+    // every lane-dependent LDS/global offset of every size and pass variant
+    // live across the task loop (loop-invariant hoisting), > 200 VGPRs
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    if (ok && active) b_run<KIND>(a, i, pass, tile, tid, lds, twl, red);
+    if (pass < 2) b_release(a, i, pass);
+    __syncthreads();
+  }
+}
+
+// End of a run: W /= sum(W) dXdP from the last V pass's partials; block 0 of
+// each instance publishes the sums to series[slot]. Grid: ninst x 8 blocks.
+FWD void b_norm(const BArgs& a, int slot) {
+  __shared__ double red[16];
+  const int i = blockIdx.x >> 3, part = blockIdx.x & 7;
+  const BInst* in = a.inst + i;
+  if (a.fresh[i]) return;
+  const double* pp = a.partials + (size_t)in->part * NSER;
+  const double S0 = sum_partials<BT>(pp, in->ntile, 0, red);
+  if (part == 0) {
+    double tot[NSER];
+#pragma unroll
+    for (int k = 0; k < NSER; ++k) tot[k] = (k == 0) ? S0 : sum_partials<BT>(pp, in->ntile, k, red);
+    if (threadIdx.x == 0)
+      for (int k = 0; k < NSER; ++k) a.series[((size_t)slot * a.ninst + i) * NSER + k] = tot[k];
+  }
+  const real_t c = (real_t)(1.0 / (S0 * (in->dX * in->dP)));
+  const size_t n = (size_t)in->n * in->n;
+  real_t* W = a.W + in->off;
+  for (size_t j = (size_t)part * BT + threadIdx.x; j < n; j += 8 * BT) W[j] *= c;
+}
+
+// Minima of V(X-Th/2)+V(X+Th/2) and K(P+L/2)+K(P-L/2) at times[0] per instance
+// (as vk_min of the single grid). One block per instance.
+FWD void b_vkmin(const BArgs& a, double* out) {
+  __shared__ double rv[4], rk[4];
+  const int i = blockIdx.x, tid = threadIdx.x;
+  const BInst* in = a.inst + i;
+  const double* prm = in->prm;
+  const int N = in->n;
+  const double t0 = a.times[0];
+  double mv = __builtin_inf(), mkk = __builtin_inf();
+  const int nh = N / 2 + 1;
+  for (int idx = tid; idx < nh * N; idx += BT) {
+    const int k = idx / N, j = idx % N;
+    const double X = in->dX * (j - 0.5 * N), Th = in->dTheta * k;
+    mv = fmin(mv, ::V(X - 0.5 * Th, t0, prm) + ::V(X + 0.5 * Th, t0, prm));
+    const int r = idx / nh, l = idx % nh;
+    const double P = in->dP * (r - 0.5 * N), Lm = in->dLambda * l;
+    mkk = fmin(mkk, ::K(P + 0.5 * Lm, t0, prm) + ::K(P - 0.5 * Lm, t0, prm));
+  }
+  mv = wave_min(mv);
+  mkk = wave_min(mkk);
+  if ((tid & 63) == 0) { rv[tid >> 6] = mv; rk[tid >> 6] = mkk; }
+  __syncthreads();
+  if (tid == 0) {
+    for (int w = 1; w < BT / 64; ++w) { mv = fmin(mv, rv[w]); mkk = fmin(mkk, rk[w]); }
+    out[2 * i] = mv;
+    out[2 * i + 1] = mkk;
+  }
+}
+
+}  // namespace fwk
+
+extern "C" __global__ void __launch_bounds__(fwk::BT) fw_batch_step(fwk::BArgs a) {
+  fwk::b_step<fwk::BK_MOYAL>(a);
+}
+extern "C" __global__ void __launch_bounds__(fwk::BT) fw_batch_bloch_step(fwk::BArgs a) {
+  fwk::b_step<fwk::BK_BLOCH>(a);
+}
+extern "C" __global__ void __launch_bounds__(fwk::BT) fw_batch_norm(fwk::BArgs a, int slot) { fwk::b_norm(a, slot); }
+extern "C" __global__ void __launch_bounds__(fwk::BT) fw_batch_vkmin(fwk::BArgs a, double* out) {
+  fwk::b_vkmin(a, out);
+}
+// W0s <- (1 - w_i) W for every instance with w_i < 1 (the random-collision
+// source from the Gibbs states held in W). Grid: ninst x 64 blocks.
+extern "C" __global__ void __launch_bounds__(fwk::BT) fw_batch_gibbs(fwk::BArgs a, fwk::real_t* w0s) {
+  const int i = blockIdx.x >> 6, part = blockIdx.x & 63;
+  const fwk::BInst* in = a.inst + i;
+  const fwk::real_t c = (fwk::real_t)(1.0 - in->w_coeff);
+  const size_t n = (size_t)in->n * in->n;
+  for (size_t j = (size_t)part * fwk::BT + threadIdx.x; j < n; j += 64 * fwk::BT)
+    w0s[in->off + j] = c * a.W[in->off + j];
+}
+// zero n words (chunk counters); fill an int array
+extern "C" __global__ void __launch_bounds__(256) fw_batch_zero(unsigned int* p, int n) {
+  for (int j = blockIdx.x * 256 + threadIdx.x; j < n; j += gridDim.x * 256) p[j] = 0u;
+}
+extern "C" __global__ void __launch_bounds__(256) fw_batch_fill_int(int* p, int n, int v) {
+  for (int j = blockIdx.x * 256 + threadIdx.x; j < n; j += gridDim.x * 256) p[j] = v;
+}

@@ -215,7 +215,9 @@ template <int N> struct TwLds { static constexpr int SIZE = N + N / 16 + 1; };
 // interleaved mapping: q = tid % P, t = tid / P (columns: a wave reads 2P
 // adjacent columns of several rows per instruction).
 // ---------------------------------------------------------------------------
-template <int N, int E, int P, bool LM>
+// TWN: length of the twiddle table passed in (a table for a multiple of N
+// serves N with stride TWN/N; the batched kernels share one table).
+template <int N, int E, int P, bool LM, int TWN = N>
 struct Line {
   static_assert((N & (N - 1)) == 0 && N >= 2, "line length must be a power of two");
   static_assert(E <= N && (E & (E - 1)) == 0, "elements per thread must be a power of two <= N");
@@ -247,7 +249,7 @@ struct Line {
   template <int R, int NS, bool TOREG>
   FWD static void stage(cpx (&v)[E], cpx* lds, int q, int t, const tw_t* __restrict__ tw) {
     constexpr int B = E / R;
-    constexpr int TWS = N / (NS * R);
+    constexpr int TWS = (N / (NS * R)) * (TWN / N);
 #pragma unroll
     for (int b = 0; b < B; ++b) {
       const int j = t + TPL * b;
@@ -303,18 +305,37 @@ struct Line {
 
 enum { NSER = 9 };   // sums per step published by the propagators
 
+#if FW_FP32
+// sin and cos of |x| <= pi/4 + 1e-6 by their Taylor series through x^9 and
+// x^10 (truncation below 2.5e-9 relative, under half an FP32 ulp), Horner in
+// FP32. The argument comes already reduced, so no large-argument path is
+// needed.
+FWD void sincos_quarter(float x, float& s, float& c) {
+  const float x2 = x * x;
+  s = x * (1.0f + x2 * (-1.0f / 6.0f + x2 * (1.0f / 120.0f + x2 * (-1.0f / 5040.0f + x2 * (1.0f / 362880.0f)))));
+  c = 1.0f + x2 * (-0.5f + x2 * (1.0f / 24.0f + x2 * (-1.0f / 720.0f + x2 * (1.0f / 40320.0f +
+                                                                            x2 * (-1.0f / 3628800.0f)))));
+}
+#endif
+
 FWD cpx unit_phase(double ph, double m) {
 #if FW_FP32
-  // FP32 form: argument and its mod-2pi reduction in FP64, sincos in FP32.
-  const double two_pi = 6.283185307179586476925;
-  const double r = ph - two_pi * rint(ph * 0.15915494309189533577);
+  // FP32 form: the argument and its reduction to [-pi/4, pi/4] (quadrant q) in
+  // FP64, the series in FP32 with a first-order correction for the rounding of
+  // the reduced argument, then the quadrant rotation.
+  const double pio2 = 1.57079632679489661923;
+  const double k = rint(ph * 0.63661977236758134308);
+  const double r = fma(-k, pio2, ph);
   const float rf = (float)r;
-  const float rl = (float)(r - (double)rf);   // argument rounding, corrected to first order
+  const float rl = (float)(r - (double)rf);
   float s, c;
-  sincosf(rf, &s, &c);
+  sincos_quarter(rf, s, c);
   const float cc = c - s * rl, ss = s + c * rl;
+  const int q = (int)((long long)k & 3);
+  const float cq = (q == 0) ? cc : (q == 1) ? -ss : (q == 2) ? -cc : ss;
+  const float sq = (q == 0) ? ss : (q == 1) ? cc : (q == 2) ? -ss : -cc;
   const float mf = (float)m;
-  return mk(cc * mf, ss * mf);
+  return mk(cq * mf, sq * mf);
 #else
   double s, c;
   sincos(ph, &s, &c);

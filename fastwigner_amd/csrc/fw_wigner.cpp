@@ -347,7 +347,7 @@ int ensure_graph(fw_wigner* h, int kind, const KArgs& base) {
   if (rc != FW_OK) return rc;
   if (e != hipSuccess) return fw::fail(FW_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
   e = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
-  hipGraphDestroy(g);
+  (void)hipGraphDestroy(g);
   if (e != hipSuccess) return fw::fail(FW_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
   h->gkind = kind;
   return FW_OK;

@@ -225,6 +225,75 @@ int fw_rho_wigner_time(fw_rho* h, const double* host_w_init, double t, double* o
 int fw_rho_set_timing(fw_rho* h, int enable);
 int fw_rho_kernel_times(fw_rho* h, double us_out[3]);
 
+
+/* ------------------------------------------------------------------------ */
+/* Batched propagator: many independent Wigner functions per launch          */
+/* No counterpart in the reference, which runs one grid per Python object    */
+/* (WignerMoyalCUDA1D, wigner_moyal_cuda_1d.py:10-731; the random-collision  */
+/* step wigner_random_collision_model_fftw_1d.py:56-76; the Bloch thermal    */
+/* state wigner_bloch_fftw_1d.py:27-109): each instance here is that         */
+/* propagator on its own square grid with its own parameters.                */
+/* ------------------------------------------------------------------------ */
+
+typedef struct fw_batch fw_batch;
+
+#define FW_BATCH_NPRM 8
+
+typedef struct fw_batch_desc {
+  int n_instances;
+  int precision;           /* FW_FP32 (W and FFTs in FP32; phases, sums in FP64) or FW_FP64 */
+  int device;
+  int mode;                /* 0: one persistent launch per step; 1: one launch per pass */
+  double dt;               /* common time step */
+  double phase_time_shift; /* V, K at t_{n+1} + shift*dt */
+  int e_for_n[5];          /* FFT elements per thread for N = 64, 128, 256, 512, 1024 (0 = default) */
+  int blocks_per_cu;       /* mode 0: persistent blocks per CU (0 = occupancy limit) */
+} fw_batch_desc;
+
+typedef struct fw_batch_instance {
+  int grid_dim;                  /* N: the instance's grid is N x N, N in {64, 128, 256, 512, 1024} */
+  double x_amplitude, p_amplitude;
+  double params[FW_BATCH_NPRM];  /* values of the parameter names in the expressions */
+} fw_batch_instance;
+
+typedef struct fw_batch_exprs {
+  const char* consts;      /* C declarations, verbatim (may be NULL) */
+  const char* param_names; /* comma-separated names bound to params[0..7], e.g. "a2,a3,a4" */
+  const char* V;           /* V(X, t) in terms of X, t, dt, the constants and the parameters */
+  const char* K;           /* K(P, t) */
+  const char* diff_V;      /* NULL = "0." */
+  const char* diff_K;      /* NULL = "0." */
+} fw_batch_exprs;
+
+/* Compile the batch's kernels for the target without a device. */
+int fw_batch_check(const fw_batch_desc* desc, const fw_batch_instance* inst, const fw_batch_exprs* exprs);
+int fw_batch_create(const fw_batch_desc* desc, const fw_batch_instance* inst, const fw_batch_exprs* exprs,
+                    fw_batch** out);
+int fw_batch_destroy(fw_batch* h);
+/* W_i <- host array of N_i*N_i doubles (row-major, x contiguous), no normalisation. */
+int fw_batch_set_state(fw_batch* h, int i, const double* host_w);
+int fw_batch_get_state(fw_batch* h, int i, double* host_w);
+/* Random-collision source of instance i: W <- w W + host_w0s around every step
+ * (host_w0s = (1-w) W_Gibbs, wigner_random_collision_model_fftw_1d.py:63-74); NULL disables. */
+int fw_batch_set_dissipator(fw_batch* h, int i, const double* host_w0s, double w_coeff);
+/* Every instance with w_coeff[i] < 1 takes W0s = (1 - w_coeff[i]) W_i from its current state
+ * (the Gibbs states left by fw_batch_bloch_propagate); w_coeff[i] = 1 disables it. */
+int fw_batch_gibbs_dissipators(fw_batch* h, const double* w_coeff);
+/* Imaginary-time steps of every instance from its current state: instance i takes steps[i]
+ * steps of dbeta[i] (exponents of `convention`, FW_BLOCH_CUDA or FW_BLOCH_FFTW, potentials at
+ * t_initial, wigner_bloch_fftw_1d.py:27-109) with renormalisation; W is normalised at the end. */
+int fw_batch_bloch_propagate(fw_batch* h, int convention, double t_initial, const double* dbeta,
+                             const int* steps);
+/* `steps` real-time steps of every instance; times[s] = t after the s-th increment. After each
+ * step each W_i is renormalised; series (optional) gets steps * n_instances * FW_NSER sums over
+ * the unnormalised W_i of each step. One graph replay per 64 steps. */
+int fw_batch_propagate(fw_batch* h, const double* times, int steps, double* series);
+/* total grid points, tasks per step, persistent grid size */
+int fw_batch_info(fw_batch* h, long long* points, int* tasks_per_step, int* grid);
+/* Device time of one step (us), averaged over `steps` steps replayed from the graph between
+ * HIP events on the handle's stream. The state advances by `steps` steps. */
+int fw_batch_step_time(fw_batch* h, int steps, double* us_per_step);
+
 #ifdef __cplusplus
 }
 #endif

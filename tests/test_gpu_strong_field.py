@@ -1,0 +1,94 @@
+"""Strong-field Wigner propagation (the usage of strong_field_physics_cuda_1d.py:24-82
+and strong_field_physics_fftw_1d.py:35-90 with the Wigner propagator): soft-core
+V = -1/sqrt(x^2+1) + x E(t) with E(t) from `functions`,
+E(t) = E0 sin^2(pi t/T) sin(omega t), E0 = 0.05, omega = 0.06, T = 4 cycles,
+dt = 0.05, x in [-60, 60), p in [-6, 6), the x-p mask |cos(pi x / 2 X_amplitude)|^(1/8)
+after every step, and the dipole acceleration -<V'(x) > (P_average_RHS, with
+diff_V carrying E(t), strong_field_physics_cuda_1d.py:81) recorded every step.
+The initial state is the field-free ground state from the Bloch propagator.
+"""
+import numpy as np
+import pytest
+
+from oracle import cexpr
+from oracle import fw_oracle as fo
+
+pytestmark = pytest.mark.gpu
+
+OMEGA, E0 = 0.06, 0.05
+T_PULSE = 4 * 2 * np.pi / OMEGA
+FUNCTIONS = ("__device__ double E(double t)\n{{\n"
+             "  return (t < T_pulse) ? E0 * pow(sin(M_PI * t / T_pulse), 2) * sin(omega * t) : 0.;\n}}\n")
+V = "-1. / sqrt(X * X + 1.) + X * E(t)"
+DV = "X / pow(X * X + 1., 1.5) + E(t)"
+MASK = "pow(fabs(cos(0.5 * M_PI * X / X_amplitude)), 1. / 8.)"
+
+
+def E_np(t):
+    return np.where(t < T_PULSE, E0 * np.sin(np.pi * t / T_PULSE) ** 2 * np.sin(OMEGA * t), 0.)
+
+
+def params(n, dt=0.05, **kw):
+    d = dict(X_gridDIM=n, X_amplitude=60., P_gridDIM=n, P_amplitude=6., dt=dt, t=0., E0=E0, omega=OMEGA,
+             T_pulse=T_PULSE, functions=FUNCTIONS, V=V, K="0.5 * P * P", diff_V=DV, diff_K="P",
+             abs_boundary_x_p=MASK)
+    d.update(kw)
+    return d
+
+
+def ground_state(n):
+    from fastwigner_amd.wigner_bloch import WignerBlochHIP1D
+    b = WignerBlochHIP1D(dbeta=0.05, t_initial=0., **{k: v for k, v in params(n).items() if k != "abs_boundary_x_p"})
+    return b.get_ground_state(abs_tol_purity=1e-9).get()
+
+
+def test_strong_field_parity_256(gpu):
+    """256^2, 400 steps in the pulse's rising edge with the field on, against the oracle."""
+    from fastwigner_amd.wigner_moyal import WignerMoyalHIP1D
+    n, steps, t0 = 256, 400, 0.25 * T_PULSE
+    W0 = ground_state(n)
+    g = WignerMoyalHIP1D(**params(n, t=t0))
+    g.set_wignerfunction(W0)
+    cx = dict(X_amplitude=60., dt=0.05)
+    o = fo.MoyalOracle(n, 60., n, 6., 0.05, t=t0,
+                       V=lambda X, t: -1. / np.sqrt(X * X + 1.) + X * E_np(t), K=lambda P, t: 0.5 * P * P,
+                       diff_V=lambda X, t: X / (X * X + 1.) ** 1.5 + E_np(t), diff_K=lambda P, t: P,
+                       mask_x_p=cexpr.make(MASK, cx, ("X", "P", "t")))
+    o.set_wignerfunction(W0)
+    W = g.propagate(steps).get()
+    o.propagate(steps)
+    err = np.linalg.norm(W - o.W) / np.linalg.norm(o.W)
+    dip = np.array(g.P_average_RHS)
+    dref = np.array([d["P_average_RHS"] for d in o.series])
+    print("strong field 256^2 rel L2 %.2e, dipole max abs diff %.2e" % (err, np.max(np.abs(dip - dref))))
+    assert err <= 1e-12
+    assert np.max(np.abs(dip - dref)) <= 1e-11 * np.max(np.abs(dref))
+
+
+def test_config3_full_pulse_fp64_fp32(gpu):
+    """Config 3 at its full size: 1024^2, the whole 4-cycle pulse (8378 steps),
+    FP64 and FP32 from the same ground state. Checked by what holds at any size:
+    normalisation, the ground state barely moves before the pulse rises, a
+    finite dipole series every step; and the FP32 form against FP64."""
+    from fastwigner_amd.wigner_moyal import WignerMoyalHIP1D
+    n = 1024
+    steps = int(round(T_PULSE / 0.05))
+    W0 = ground_state(n)
+    out = {}
+    for prec in ("fp64", "fp32"):
+        g = WignerMoyalHIP1D(precision=prec, **params(n))
+        g.set_wignerfunction(W0)
+        g.propagate(steps)
+        W = g.wignerfunction.get()
+        out[prec] = (W, np.array(g.P_average_RHS), np.array(g.X_average))
+        assert abs(W.sum() * g.dX * g.dP - 1.0) <= (1e-12 if prec == "fp64" else 1e-6)
+    W64, d64, x64 = out["fp64"]
+    W32, d32, x32 = out["fp32"]
+    assert len(d64) == steps and np.all(np.isfinite(d64))
+    # the field is ~E0 (pi t/T)^2 omega t at the start: the ground state barely moves
+    assert np.max(np.abs(x64[:20])) <= 1e-6
+    werr = np.linalg.norm(W32 - W64) / np.linalg.norm(W64)
+    derr = np.max(np.abs(d32 - d64)) / np.max(np.abs(d64))
+    print("config 3: %d steps, FP32 vs FP64: W rel L2 %.2e, dipole max rel %.2e" % (steps, werr, derr))
+    assert werr <= 1e-3
+    assert derr <= 1e-3
