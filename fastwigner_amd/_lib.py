@@ -98,6 +98,12 @@ class BatchExprs(ctypes.Structure):
     _fields_ = [(n, ctypes.c_char_p) for n in ("consts", "param_names", "V", "K", "diff_V", "diff_K")]
 
 
+class FnoDesc(ctypes.Structure):
+    _fields_ = [("grid_dim", ctypes.c_int), ("width", ctypes.c_int), ("modes", ctypes.c_int),
+                ("layers", ctypes.c_int), ("max_batch", ctypes.c_int), ("device", ctypes.c_int),
+                ("x_amplitude", ctypes.c_double), ("p_amplitude", ctypes.c_double)]
+
+
 def _b(s):
     return None if s is None else s.encode()
 
@@ -180,6 +186,7 @@ _I = ctypes.c_int
 _PD = ctypes.POINTER(ctypes.c_double)
 _PI = ctypes.POINTER(ctypes.c_int)
 _PP = ctypes.POINTER(ctypes.c_void_p)
+_PF = ctypes.POINTER(ctypes.c_float)
 
 # name -> (argtypes); every entry point returns int
 PROTOTYPES = {
@@ -234,6 +241,20 @@ PROTOTYPES = {
     "fw_batch_propagate": [_P, _PD, _I, _PD],
     "fw_batch_info": [_P, ctypes.POINTER(ctypes.c_longlong), _PI, _PI],
     "fw_batch_step_time": [_P, _I, _PD],
+    "fw_batch_state_ptr": [_P, _I, _PP, _PI],
+    "fw_fno_create": [ctypes.POINTER(FnoDesc), _PP],
+    "fw_fno_destroy": [_P],
+    "fw_fno_param_count": [_P, ctypes.POINTER(ctypes.c_longlong)],
+    "fw_fno_set_params": [_P, _PF],
+    "fw_fno_get_params": [_P, _PF],
+    "fw_fno_forward": [_P, _PF, _PF, _I, _PF],
+    "fw_fno_loss_grad": [_P, _PF, _PF, _PF, _I, _PD, _PF],
+    "fw_fno_dataset_reserve": [_P, _I],
+    "fw_fno_dataset_put": [_P, _I, _P, _P, _PF],
+    "fw_fno_train_step": [_P, _PI, _I, ctypes.c_float, _PD],
+    "fw_fno_eval_pairs": [_P, _PI, _I, _PD],
+    "fw_fno_infer_time": [_P, _I, _I, _PD],
+    "fw_fno_gemm_test": [_I, _PF, _PF, _PF, _I, _I, _I, _I, _I],
 }
 
 _lib = None
@@ -277,3 +298,9 @@ def call(name, *args):
 def dptr(a):
     """ctypes double* to a contiguous float64 numpy array."""
     return a.ctypes.data_as(_PD)
+
+
+def fptr(a):
+    """ctypes float* to a contiguous float32 numpy array."""
+    assert a.dtype == "float32" and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_PF)

@@ -656,6 +656,15 @@ extern "C" int fw_batch_propagate(fw_batch* h, const double* times, int steps, d
   return FW_OK;
 }
 
+extern "C" int fw_batch_state_ptr(fw_batch* h, int i, void** dev_ptr, int* elem_size) {
+  fw::clear_error();
+  FWCHK(instance_check(h, i));
+  if (!dev_ptr) return fw::fail(FW_ERR_INVALID_ARG, "dev_ptr is NULL");
+  *dev_ptr = (char*)h->W + (size_t)h->inst[i].off * h->esz;
+  if (elem_size) *elem_size = (int)h->esz;
+  return FW_OK;
+}
+
 extern "C" int fw_batch_info(fw_batch* h, long long* points, int* tasks_per_step, int* grid) {
   if (!h) return fw::fail(FW_ERR_INVALID_ARG, "NULL handle");
   if (points) *points = h->points;

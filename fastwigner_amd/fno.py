@@ -1,0 +1,202 @@
+"""Fourier-neural-operator surrogate of k split-operator steps, trained on the
+GPU by the library's own forward/backward kernels (csrc/fw_fno.cpp) on
+trajectories of the batched propagator, with the plain propagator as the
+judge and fallback.
+
+The reference has no learned component: this is new work on top of the
+split-operator step it defines (wigner_moyal_cuda_1d.py:496-540 and the
+random-collision step, wigner_random_collision_model_fftw_1d.py:56-76).
+
+Network: lift 8 -> C, L Fourier layers (spectral convolution on M x M retained
+modes plus pointwise C x C mixing, GELU), projection C -> 1 added to the input
+W (the untrained network, with a zero projection, is the identity map).
+Channel mixing runs on BF16 MFMA with FP32 accumulation; the truncated
+transforms on FP32 MFMA.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .batch import QUARTIC_DV, QUARTIC_V, WignerBatchHIP1D, gaussian_state, random_quartic_instances
+
+
+class FNOSurrogate:
+    def __init__(self, grid_dim=256, width=32, modes=16, layers=4, max_batch=8, k_steps=20, dt=0.01, amplitude=8.,
+                 seed=0, device=0):
+        self.N, self.C, self.M, self.L = int(grid_dim), int(width), int(modes), int(layers)
+        self.max_batch, self.k, self.dt, self.amplitude = int(max_batch), int(k_steps), float(dt), float(amplitude)
+        d = _lib.FnoDesc()
+        d.grid_dim, d.width, d.modes, d.layers, d.max_batch, d.device = self.N, self.C, self.M, self.L, \
+            self.max_batch, int(device)
+        d.x_amplitude = d.p_amplitude = self.amplitude
+        h = ctypes.c_void_p()
+        _lib.call("fw_fno_create", ctypes.byref(d), ctypes.byref(h))
+        self._handle = h
+        n = ctypes.c_longlong()
+        _lib.call("fw_fno_param_count", self._handle, ctypes.byref(n))
+        self.nparam = n.value
+        self.set_params(self.initial_params(seed))
+        self.npairs = 0
+        self.history = []
+
+    def __del__(self):
+        h = getattr(self, "_handle", None)
+        if h:
+            try:
+                _lib.load().fw_fno_destroy(h)
+            except Exception:
+                pass
+            self._handle = None
+
+    # -- parameters ------------------------------------------------------------------
+    def layout(self):
+        C, nm = self.C, 2 * self.M * self.M
+        o, out = 0, {}
+        out["Pw"] = (o, 8 * C); o += 8 * C
+        out["pb"] = (o, C); o += C
+        for l in range(self.L):
+            out["R%d" % l] = (o, nm * C * C * 2); o += nm * C * C * 2
+            out["Wp%d" % l] = (o, C * C); o += C * C
+            out["b%d" % l] = (o, C); o += C
+        out["Q"] = (o, C); o += C
+        out["qb"] = (o, 1); o += 1
+        return out
+
+    def initial_params(self, seed):
+        """Lift and pointwise weights ~ N(0, 1/fan_in); spectral weights ~ U(-1/C^2, 1/C^2);
+        biases and projection zero."""
+        rng = np.random.default_rng(seed)
+        P = np.zeros(self.nparam, dtype=np.float32)
+        lay = self.layout()
+        o, n = lay["Pw"]
+        P[o:o + n] = rng.standard_normal(n) / np.sqrt(8.0)
+        s = 1.0 / (self.C * self.C)
+        for l in range(self.L):
+            o, n = lay["R%d" % l]
+            P[o:o + n] = rng.uniform(-s, s, n)
+            o, n = lay["Wp%d" % l]
+            P[o:o + n] = rng.standard_normal(n) / np.sqrt(self.C)
+        return P
+
+    def set_params(self, P):
+        P = np.ascontiguousarray(P, dtype=np.float32)
+        assert P.size == self.nparam
+        _lib.call("fw_fno_set_params", self._handle, _lib.fptr(P))
+
+    def get_params(self):
+        P = np.empty(self.nparam, dtype=np.float32)
+        _lib.call("fw_fno_get_params", self._handle, _lib.fptr(P))
+        return P
+
+    # -- inference and gradients (host arrays) ---------------------------------------------
+    def predict(self, W, feat):
+        """W(t + k dt) for a batch: W [B, N, N], feat [B, 5] = (a2, a3, a4, gamma, kT)."""
+        W = np.ascontiguousarray(W, dtype=np.float32)
+        feat = np.ascontiguousarray(feat, dtype=np.float32)
+        out = np.empty_like(W)
+        for s in range(0, W.shape[0], self.max_batch):
+            e = min(s + self.max_batch, W.shape[0])
+            o = np.empty((e - s,) + W.shape[1:], dtype=np.float32)
+            _lib.call("fw_fno_forward", self._handle, _lib.fptr(np.ascontiguousarray(W[s:e])),
+                      _lib.fptr(np.ascontiguousarray(feat[s:e])), e - s, _lib.fptr(o))
+            out[s:e] = o
+        return out
+
+    def loss_grad(self, W, feat, target, want_grad=True):
+        W = np.ascontiguousarray(W, dtype=np.float32)
+        feat = np.ascontiguousarray(feat, dtype=np.float32)
+        target = np.ascontiguousarray(target, dtype=np.float32)
+        loss = ctypes.c_double()
+        g = np.empty(self.nparam, dtype=np.float32) if want_grad else None
+        _lib.call("fw_fno_loss_grad", self._handle, _lib.fptr(W), _lib.fptr(feat), _lib.fptr(target), W.shape[0],
+                  ctypes.byref(loss), _lib.fptr(g) if want_grad else None)
+        return loss.value, g
+
+    # -- training data from the batched propagator --------------------------------------------
+    def build_dataset(self, n_instances, pairs_per_instance, seed=4, prepare_dbeta=0.01):
+        """Trajectories of the batched random-collision propagator (FP32) on N x N
+        grids: n_instances seeded quartic systems (the same generator as the
+        batched workload, grid fixed to N), each contributing the pairs
+        (W(j k dt), W((j+1) k dt)) for j < pairs_per_instance. Instances whose
+        Wigner-Bloch Gibbs state is unphysical (purity > 1; quartic unbounded
+        below on the theta-extended grid) are left out. Returns the kept instances."""
+        inst, _ = random_quartic_instances(n_instances, seed=seed, sizes=(self.N,), amplitude=self.amplitude)
+        b = WignerBatchHIP1D(inst, V=QUARTIC_V, K="0.5*P*P", dt=self.dt, param_names=("a2", "a3", "a4"),
+                             diff_V=QUARTIC_DV, diff_K="P", precision="fp32")
+        b.prepare_random_collision(dbeta=prepare_dbeta)
+        dXdP = (2 * self.amplitude / self.N) ** 2
+        keep = []
+        for i in range(len(inst)):
+            Wg = b.get_wignerfunction(i)
+            if 2 * np.pi * np.sum(Wg * Wg) * dXdP <= 1.0 + 1e-3:
+                keep.append(i)
+        for i, d in enumerate(inst):
+            b.set_wignerfunction(i, gaussian_state(self.N, self.amplitude, d["x0"], d["p0"]))
+        total = len(keep) * pairs_per_instance
+        _lib.call("fw_fno_dataset_reserve", self._handle, total)
+        ptrs = []
+        for i in keep:
+            p = ctypes.c_void_p()
+            _lib.call("fw_batch_state_ptr", b._handle, i, ctypes.byref(p), None)
+            ptrs.append(p)
+        feats = np.array([[inst[i]["params"][0], inst[i]["params"][1], inst[i]["params"][2], inst[i]["gamma"],
+                           inst[i]["kT"]] for i in keep], dtype=np.float32)
+        for j in range(pairs_per_instance):
+            for q, i in enumerate(keep):
+                _lib.call("fw_fno_dataset_put", self._handle, j * len(keep) + q, ptrs[q], None,
+                          _lib.fptr(np.ascontiguousarray(feats[q])))
+            b.propagate(self.k, record=False)
+            for q, i in enumerate(keep):
+                _lib.call("fw_fno_dataset_put", self._handle, j * len(keep) + q, None, ptrs[q],
+                          _lib.fptr(np.ascontiguousarray(feats[q])))
+        self.npairs = total
+        self.dataset_instances = [inst[i] for i in keep]
+        self.dataset_feats = feats
+        del b
+        return self.dataset_instances
+
+    # -- training ----------------------------------------------------------------------
+    def train(self, steps, lr=1e-3, batch=None, seed=0, pairs=None, lr_final=None, log_every=0):
+        """Adam steps on random minibatches of the training pairs (indices `pairs`,
+        default all); cosine decay from lr to lr_final."""
+        batch = batch or self.max_batch
+        rng = np.random.default_rng(seed)
+        pool = np.arange(self.npairs) if pairs is None else np.asarray(pairs)
+        lr_final = lr * 0.05 if lr_final is None else lr_final
+        loss = ctypes.c_double()
+        for s in range(steps):
+            idx = np.ascontiguousarray(rng.choice(pool, size=batch, replace=False).astype(np.int32))
+            lr_s = lr_final + 0.5 * (lr - lr_final) * (1 + np.cos(np.pi * s / max(steps - 1, 1)))
+            want = log_every and (s % log_every == 0 or s == steps - 1)
+            _lib.call("fw_fno_train_step", self._handle, idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), batch,
+                      ctypes.c_float(lr_s), ctypes.byref(loss) if want else None)
+            if want:
+                self.history.append((s, loss.value))
+        return self.history
+
+    def eval_pairs(self, pairs):
+        pairs = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32))
+        out = np.empty(len(pairs))
+        _lib.call("fw_fno_eval_pairs", self._handle, pairs.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), len(pairs),
+                  _lib.dptr(out))
+        return out
+
+    def infer_time_us(self, batch=None, reps=20):
+        us = ctypes.c_double()
+        _lib.call("fw_fno_infer_time", self._handle, batch or self.max_batch, reps, ctypes.byref(us))
+        return us.value
+
+
+def judged_jump(surrogate, W0, feat, judge_states, dXdP, l2_tol=1e-3, norm_tol=1e-4):
+    """One k-step jump of each sample with the surrogate, judged against the plain
+    propagator's result: accepted when the relative L2 error is <= l2_tol and the
+    norm drift |sum(W) dXdP - 1| <= norm_tol, otherwise replaced by the judge.
+    Returns (states, accepted mask, relative L2 errors, norm drifts)."""
+    pred = surrogate.predict(W0, feat).astype(np.float64)
+    err = np.linalg.norm((pred - judge_states).reshape(len(pred), -1), axis=1) / \
+        np.linalg.norm(judge_states.reshape(len(pred), -1), axis=1)
+    drift = np.abs(pred.reshape(len(pred), -1).sum(axis=1) * dXdP - 1.0)
+    ok = (err <= l2_tol) & (drift <= norm_tol)
+    out = np.where(ok[:, None, None], pred, judge_states)
+    return out, ok, err, drift

@@ -294,6 +294,56 @@ int fw_batch_info(fw_batch* h, long long* points, int* tasks_per_step, int* grid
  * HIP events on the handle's stream. The state advances by `steps` steps. */
 int fw_batch_step_time(fw_batch* h, int steps, double* us_per_step);
 
+/* Device address of instance i's W (N*N elements of elem_size bytes, row-major), valid until
+ * fw_batch_destroy; the caller may read it between calls (e.g. to copy training pairs). */
+int fw_batch_state_ptr(fw_batch* h, int i, void** dev_ptr, int* elem_size);
+
+/* ------------------------------------------------------------------------ */
+/* Fourier-neural-operator surrogate of k propagator steps                   */
+/* No counterpart in the reference. Its network is trained on trajectories  */
+/* of the batched propagator and judged by the plain propagator (the        */
+/* Moyal / random-collision step of wigner_moyal_cuda_1d.py:496-540 and     */
+/* wigner_random_collision_model_fftw_1d.py:56-76).                          */
+/* ------------------------------------------------------------------------ */
+
+typedef struct fw_fno fw_fno;
+
+typedef struct fw_fno_desc {
+  int grid_dim;           /* N: inputs are N x N Wigner functions, N a power of two in [64, 1024] */
+  int width;              /* channels C (divides 256) */
+  int modes;              /* retained modes M per axis (ky < M; kp < M or kp >= N - M) */
+  int layers;             /* Fourier layers */
+  int max_batch;          /* samples per forward / training step */
+  int device;
+  double x_amplitude, p_amplitude;
+} fw_fno_desc;
+
+/* Per-sample features after W: a2, a3, a4, gamma, kT (5 floats). Parameters are one FP32
+ * vector: Pw[8][C], pb[C], per layer {R[2M*M][C][C][re,im], Wp[C][C], b[C]}, Q[C], qb. */
+int fw_fno_create(const fw_fno_desc* desc, fw_fno** out);
+int fw_fno_destroy(fw_fno* h);
+int fw_fno_param_count(fw_fno* h, long long* n);
+int fw_fno_set_params(fw_fno* h, const float* host_params); /* also resets the Adam state */
+int fw_fno_get_params(fw_fno* h, float* host_params);
+/* prediction W(t + k dt) for `batch` samples (host arrays: batch*N*N inputs, batch*5 features) */
+int fw_fno_forward(fw_fno* h, const float* host_in, const float* host_feat, int batch, float* host_out);
+/* loss L = mean_b |pred_b - target_b|^2 / |target_b|^2 and, if host_grad != NULL, dL/dparams */
+int fw_fno_loss_grad(fw_fno* h, const float* host_in, const float* host_feat, const float* host_target, int batch,
+                     double* loss, float* host_grad);
+/* device-resident training set of (input, target, features) pairs; fw_fno_dataset_put copies N*N
+ * floats from device addresses (either may be NULL to leave that half of pair k unchanged) */
+int fw_fno_dataset_reserve(fw_fno* h, int pairs);
+int fw_fno_dataset_put(fw_fno* h, int k, const void* dev_in, const void* dev_target, const float feat[5]);
+/* one Adam step (lr; betas 0.9, 0.999) on the pairs host_idx[0..batch) */
+int fw_fno_train_step(fw_fno* h, const int* host_idx, int batch, float lr, double* loss);
+/* relative L2 error of the prediction for each listed pair */
+int fw_fno_eval_pairs(fw_fno* h, const int* host_idx, int n, double* rel_l2_out);
+/* device time of one forward pass over `batch` samples (us), averaged over reps */
+int fw_fno_infer_time(fw_fno* h, int batch, int reps, double* us_per_batch);
+/* C = A B through the MFMA GEMM kernel (bf16 != 0: BF16 operands, else FP32), host arrays; A is M x K
+ * (K x M if transA), B is K x N (N x K if transB), C is M x N row-major. For testing the kernel. */
+int fw_fno_gemm_test(int bf16, const float* A, const float* B, float* C, int M, int N, int K, int transA, int transB);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,186 @@
+"""numpy restatement of the Fourier-neural-operator surrogate (forward, loss,
+backward) -- TEST INFRASTRUCTURE ONLY: the checker of the HIP kernels in
+fastwigner_amd/csrc/fw_fno.cpp, never called by the product.
+
+The reference has no learned component; this network is defined by the build
+(the surrogate of k split-operator steps). The restatement uses numpy's FFTs
+for the truncated transforms (the HIP code multiplies by cosine/sine matrices)
+and float64 arithmetic. With ``bf16=True`` it rounds to bfloat16 (round to
+nearest even, through float32) exactly the operands the HIP channel-mixing
+GEMMs round: the mode-mixing inputs and weights, and the pointwise-mixing
+inputs and weights, forward and backward. Pinned in tests/test_fno_oracle.py
+by central finite differences (bf16=False) and by the identity-weight case,
+where the spectral convolution reduces to a low-pass filter.
+
+Parameter vector (float32 on the GPU, float64 here), in this order:
+  Pw[8][C], pb[C], for each layer {R[2M*M][C][C][2], Wp[C][C], b[C]}, Q[C], qb[1]
+Features: f = [W, x/X_amplitude, p/P_amplitude, a2, a3, a4, gamma, kT].
+"""
+import numpy as np
+from scipy.special import erf
+
+
+def bf16(x):
+    """Round to bfloat16 (nearest even) via float32; returned as float64."""
+    u = np.asarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+def gelu(x):
+    return 0.5 * x * (1.0 + erf(x / np.sqrt(2.0)))
+
+
+def gelu_d(x):
+    return 0.5 * (1.0 + erf(x / np.sqrt(2.0))) + x * np.exp(-0.5 * x * x) / np.sqrt(2.0 * np.pi)
+
+
+class FNOOracle:
+    def __init__(self, N=256, C=32, M=16, L=4, bf16=True):
+        self.N, self.C, self.M, self.L, self.use_bf16 = N, C, M, L, bf16
+        self.nm = 2 * M * M
+        o = 0
+        self.o_Pw = o; o += 8 * C
+        self.o_pb = o; o += C
+        self.o_R, self.o_Wp, self.o_b = [], [], []
+        for _ in range(L):
+            self.o_R.append(o); o += self.nm * C * C * 2
+            self.o_Wp.append(o); o += C * C
+            self.o_b.append(o); o += C
+        self.o_Q = o; o += C
+        self.o_qb = o; o += 1
+        self.nparam = o
+        self.kp = np.concatenate([np.arange(M), np.arange(N - M, N)])
+
+    # -- helpers -----------------------------------------------------------------------
+    def _r(self, x):
+        return bf16(x) if self.use_bf16 else x
+
+    def unpack(self, P):
+        C, nm = self.C, self.nm
+        d = dict(Pw=P[self.o_Pw:self.o_Pw + 8 * C].reshape(8, C), pb=P[self.o_pb:self.o_pb + C],
+                 Q=P[self.o_Q:self.o_Q + C], qb=P[self.o_qb])
+        d["R"] = [P[o:o + nm * C * C * 2].reshape(nm, C, C, 2) for o in self.o_R]
+        d["Wp"] = [P[o:o + C * C].reshape(C, C) for o in self.o_Wp]
+        d["b"] = [P[o:o + C] for o in self.o_b]
+        return d
+
+    def features(self, W, feat):
+        B, N = W.shape[0], self.N
+        g = (np.arange(N) - 0.5 * N) * (2.0 / N)
+        f = np.empty((B, N, N, 8))
+        f[..., 0] = W
+        f[..., 1] = g[np.newaxis, np.newaxis, :]
+        f[..., 2] = g[np.newaxis, :, np.newaxis]
+        for j in range(5):
+            f[..., 3 + j] = np.asarray(feat)[:, j][:, np.newaxis, np.newaxis]
+        return f
+
+    def modes(self, v):
+        """Z[b, m = kpi*M + ky, c] = 2-D DFT of each channel on the retained modes."""
+        F = np.fft.rfft2(v, axes=(1, 2))            # [B, N, N/2+1, C]
+        Z = F[:, self.kp, :self.M, :]               # [B, 2M, M, C]
+        return Z.reshape(v.shape[0], self.nm, self.C)
+
+    def inverse(self, Y):
+        """numpy irfft2 of the spectrum Y on the retained modes (zero elsewhere)."""
+        B, N = Y.shape[0], self.N
+        full = np.zeros((B, N, N // 2 + 1, Y.shape[-1]), dtype=complex)
+        full[:, self.kp, :self.M, :] = Y.reshape(B, 2 * self.M, self.M, -1)
+        return np.fft.irfft2(full, s=(N, N), axes=(1, 2))
+
+    def mix(self, Z, R):
+        """Y[b, m, o] = sum_c Z[b, m, c] R[m, c, o] with the real 2C x 2O block
+        [[Rr, Ri], [-Ri, Rr]] and bf16-rounded operands."""
+        Zr, Zi = self._r(Z.real), self._r(Z.imag)
+        Rr, Ri = self._r(R[..., 0]), self._r(R[..., 1])
+        Yr = np.einsum("bmc,mco->bmo", Zr, Rr, optimize=True) - np.einsum("bmc,mco->bmo", Zi, Ri, optimize=True)
+        Yi = np.einsum("bmc,mco->bmo", Zr, Ri, optimize=True) + np.einsum("bmc,mco->bmo", Zi, Rr, optimize=True)
+        return Yr + 1j * Yi
+
+    # -- forward -------------------------------------------------------------------------
+    def forward(self, P, W, feat, keep=False):
+        p = self.unpack(P)
+        f = self.features(W, feat)
+        v = [np.einsum("bpxj,jc->bpxc", f, p["Pw"], optimize=True) + p["pb"]]
+        pres, Zs = [], []
+        for l in range(self.L):
+            Z = self.modes(v[l])
+            Zs.append(Z)
+            S = self.inverse(self.mix(Z, p["R"][l]))
+            Pt = np.einsum("bpxc,co->bpxo", self._r(v[l]), self._r(p["Wp"][l]), optimize=True)
+            pre = S + Pt + p["b"][l]
+            pres.append(pre)
+            v.append(gelu(pre) if l < self.L - 1 else pre)
+        out = W + np.einsum("bpxc,c->bpx", v[-1], p["Q"], optimize=True) + p["qb"]
+        if keep:
+            return out, dict(f=f, v=v, pre=pres, Z=Zs, p=p)
+        return out
+
+    def loss(self, out, target):
+        se = np.sum((out - target) ** 2, axis=(1, 2))
+        st = np.sum(target ** 2, axis=(1, 2))
+        return float(np.mean(se / st)), st
+
+    # -- backward ------------------------------------------------------------------------
+    def loss_grad(self, P, W, feat, target):
+        out, k = self.forward(P, W, feat, keep=True)
+        L, st = self.loss(out, target)
+        B, N, C, M = W.shape[0], self.N, self.C, self.M
+        p = k["p"]
+        dout = 2.0 * (out - target) / st[:, None, None] / B
+        g = np.zeros(self.nparam)
+        g[self.o_Q:self.o_Q + C] = np.einsum("bpxc,bpx->c", k["v"][-1], dout, optimize=True)
+        g[self.o_qb] = dout.sum()
+        dv = dout[..., None] * p["Q"]
+        cky = np.where(np.arange(M) == 0, 1.0, 2.0)
+        for l in range(self.L - 1, -1, -1):
+            dpre = dv * gelu_d(k["pre"][l]) if l < self.L - 1 else dv
+            g[self.o_b[l]:self.o_b[l] + C] = dpre.sum(axis=(0, 1, 2))
+            Wp = p["Wp"][l]
+            dvn = np.einsum("bpxo,co->bpxc", self._r(dpre), self._r(Wp), optimize=True)
+            g[self.o_Wp[l]:self.o_Wp[l] + C * C] = np.einsum("bpxc,bpxo->co", self._r(k["v"][l]), self._r(dpre),
+                                                             optimize=True).ravel()
+            # adjoint of the inverse transform: dY = (c_ky / N^2) rfft2(dpre) on the retained modes
+            D = np.fft.rfft2(dpre, axes=(1, 2))[:, self.kp, :M, :] * (cky[None, None, :, None] / (N * N))
+            dY = D.reshape(B, self.nm, C)
+            dYr, dYi = self._r(dY.real), self._r(dY.imag)
+            R = p["R"][l]
+            Rr, Ri = self._r(R[..., 0]), self._r(R[..., 1])
+            # dZr = dYr Rr^T + dYi Ri^T ; dZi = -dYr Ri^T + dYi Rr^T  (transpose of [[Rr, Ri], [-Ri, Rr]])
+            dZr = (np.einsum("bmo,mco->bmc", dYr, Rr, optimize=True)
+                   + np.einsum("bmo,mco->bmc", dYi, Ri, optimize=True))
+            dZi = (-np.einsum("bmo,mco->bmc", dYr, Ri, optimize=True)
+                   + np.einsum("bmo,mco->bmc", dYi, Rr, optimize=True))
+            Zr, Zi = self._r(k["Z"][l].real), self._r(k["Z"][l].imag)
+            # dW = [Zr; Zi]^T [dYr, dYi]; dRr = dW11 + dW22, dRi = dW12 - dW21
+            dRr = (np.einsum("bmc,bmo->mco", Zr, dYr, optimize=True)
+                   + np.einsum("bmc,bmo->mco", Zi, dYi, optimize=True))
+            dRi = (np.einsum("bmc,bmo->mco", Zr, dYi, optimize=True)
+                   - np.einsum("bmc,bmo->mco", Zi, dYr, optimize=True))
+            gR = np.stack([dRr, dRi], axis=-1)
+            g[self.o_R[l]:self.o_R[l] + gR.size] = gR.ravel()
+            # adjoint of the forward transform: dv = Re sum_modes (dZr + i dZi) e^{+i theta}
+            full = np.zeros((B, N, N, C), dtype=complex)
+            full[:, self.kp, :M, :] = (dZr + 1j * dZi).reshape(B, 2 * M, M, C)
+            dvn = dvn + np.real(np.fft.ifft2(full, axes=(1, 2))) * (N * N)
+            dv = dvn
+        f = k["f"]
+        g[self.o_Pw:self.o_Pw + 8 * C] = np.einsum("bpxj,bpxc->jc", f, dv, optimize=True).ravel()
+        g[self.o_pb:self.o_pb + C] = dv.sum(axis=(0, 1, 2))
+        return L, g
+
+
+def init_params(o, seed=0, spectral_scale=None):
+    """Initial parameters: lift and pointwise ~ N(0, 1/fan_in), spectral ~ U(-s, s)
+    with s = 1/(C*C) (as the original FNO), projection zero (the untrained network is
+    the identity map W -> W)."""
+    rng = np.random.default_rng(seed)
+    C = o.C
+    P = np.zeros(o.nparam)
+    P[o.o_Pw:o.o_Pw + 8 * C] = rng.standard_normal(8 * C) / np.sqrt(8)
+    s = spectral_scale if spectral_scale is not None else 1.0 / (C * C)
+    for l in range(o.L):
+        P[o.o_R[l]:o.o_R[l] + o.nm * C * C * 2] = rng.uniform(-s, s, o.nm * C * C * 2)
+        P[o.o_Wp[l]:o.o_Wp[l] + C * C] = rng.standard_normal(C * C) / np.sqrt(C)
+    return P

@@ -1,0 +1,82 @@
+"""Pins the numpy FNO restatement (oracle/fno_oracle.py) before it checks the
+HIP forward/backward kernels: central finite differences of the loss
+(float64, no bf16 rounding), the spectral convolution with identity mode
+weights against a direct DFT sum (no FFT), and the bf16 rounding rule."""
+import numpy as np
+
+from oracle import fno_oracle as fo
+
+
+def test_bf16_rounding_rule():
+    x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, 1.0 + 2 ** -7, -2.5, 3.14159265])
+    # ties go to even: 1 + 2^-8 -> 1, 1 + 3 2^-8 -> 1 + 2^-6; pi -> 3.140625
+    assert np.array_equal(fo.bf16(x), [1.0, 1.0, 1.0 + 2 ** -6, 1.0 + 2 ** -7, -2.5, 3.140625])
+
+
+def test_spectral_identity_is_lowpass_by_direct_dft():
+    N, C, M = 8, 3, 2
+    o = fo.FNOOracle(N=N, C=C, M=M, L=1, bf16=False)
+    rng = np.random.default_rng(0)
+    v = rng.standard_normal((1, N, N, C))
+    R = np.zeros((o.nm, C, C, 2))
+    R[:, np.arange(C), np.arange(C), 0] = 1.0
+    S = o.inverse(o.mix(o.modes(v), R))
+    # direct: sum over the retained (kp, ky) of c_ky Re(Vhat e^{+i theta}) / N^2
+    p = np.arange(N)[:, None]
+    x = np.arange(N)[None, :]
+    ref = np.zeros((N, N, C))
+    for kp in list(range(M)) + list(range(N - M, N)):
+        for ky in range(M):
+            e = np.exp(-2j * np.pi * (kp * p + ky * x) / N)
+            vh = np.einsum("px,pxc->c", e, v[0])
+            ref += (1.0 if ky == 0 else 2.0) * np.real(vh[None, None, :] * np.conj(e)[..., None]) / N ** 2
+    assert np.max(np.abs(S[0] - ref)) < 1e-12
+
+
+def test_loss_gradient_by_finite_differences():
+    o = fo.FNOOracle(N=16, C=4, M=2, L=2, bf16=False)
+    rng = np.random.default_rng(1)
+    P = fo.init_params(o, seed=2, spectral_scale=0.3)
+    P[o.o_Q:o.o_Q + o.C] = rng.standard_normal(o.C) * 0.5
+    P[o.o_qb] = 0.01
+    for l in range(o.L):
+        P[o.o_b[l]:o.o_b[l] + o.C] = rng.standard_normal(o.C) * 0.1
+    W = rng.random((2, 16, 16))
+    feat = rng.random((2, 5))
+    tgt = W + 0.1 * rng.standard_normal(W.shape)
+    L0, g = o.loss_grad(P, W, feat, tgt)
+    picks = np.concatenate([rng.choice(o.nparam, 60, replace=False), np.arange(o.o_Q, o.nparam),
+                            np.arange(o.o_Pw, o.o_Pw + 8)])
+    h = 1e-6
+    for i in picks:
+        Pp, Pm = P.copy(), P.copy()
+        Pp[i] += h
+        Pm[i] -= h
+        num = (o.loss(o.forward(Pp, W, feat), tgt)[0] - o.loss(o.forward(Pm, W, feat), tgt)[0]) / (2 * h)
+        assert abs(num - g[i]) <= 1e-6 * max(1.0, abs(num)) + 1e-9, (i, num, g[i])
+
+
+def test_planted_adjoint_mistake_is_caught():
+    """The finite-difference check fails for a wrong adjoint (a dropped factor
+    2 on the ky >= 1 modes of the inverse transform's adjoint)."""
+    o = fo.FNOOracle(N=16, C=4, M=2, L=1, bf16=False)
+    rng = np.random.default_rng(3)
+    P = fo.init_params(o, seed=4, spectral_scale=0.3)
+    P[o.o_Q:o.o_Q + o.C] = rng.standard_normal(o.C)
+    W = rng.random((1, 16, 16))
+    feat = rng.random((1, 5))
+    tgt = W + 0.1 * rng.standard_normal(W.shape)
+    i = o.o_R[0] + 1 * (o.C * o.C) * 2 + 1        # mode m = 1 (kp = 0, ky = 1): imaginary weight
+    _, g = o.loss_grad(P, W, feat, tgt)
+    Pp, Pm = P.copy(), P.copy()
+    Pp[i] += 1e-6
+    Pm[i] -= 1e-6
+    num = (o.loss(o.forward(Pp, W, feat), tgt)[0] - o.loss(o.forward(Pm, W, feat), tgt)[0]) / 2e-6
+    assert abs(num - g[i]) <= 1e-6 * max(1.0, abs(num))
+    orig = np.where
+    try:
+        fo.np.where = lambda c, a, b: orig(c, a, a) if np.shape(c) == (o.M,) else orig(c, a, b)
+        _, g_bad = o.loss_grad(P, W, feat, tgt)
+    finally:
+        fo.np.where = orig
+    assert abs(num - g_bad[i]) > 1e-3 * abs(num)

@@ -1,0 +1,118 @@
+"""FNO surrogate kernels against the numpy oracle (oracle/fno_oracle.py, which
+rounds to bf16 where the HIP channel-mixing GEMMs do), training on batched
+propagator trajectories, and the judged jump with fallback."""
+import numpy as np
+import pytest
+
+from oracle import fno_oracle as fo
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b))
+
+
+@pytest.mark.parametrize("bf16", [0, 1])
+@pytest.mark.parametrize("M,N,K,ta,tb", [(70, 45, 100, 0, 0), (64, 64, 64, 1, 0), (33, 130, 17, 0, 1),
+                                         (128, 8, 300, 1, 1)])
+def test_mfma_gemm(gpu, bf16, M, N, K, ta, tb):
+    from fastwigner_amd import _lib
+    rng = np.random.default_rng(M * N + K)
+    A = rng.standard_normal((K, M) if ta else (M, K)).astype(np.float32)
+    B = rng.standard_normal((N, K) if tb else (K, N)).astype(np.float32)
+    C = np.empty((M, N), dtype=np.float32)
+    _lib.call("fw_fno_gemm_test", bf16, _lib.fptr(A), _lib.fptr(B), _lib.fptr(C), M, N, K, ta, tb)
+    Am = A.T if ta else A
+    Bm = B.T if tb else B
+    if bf16:
+        ref = fo.bf16(Am) @ fo.bf16(Bm)
+    else:
+        ref = Am.astype(np.float64) @ Bm.astype(np.float64)
+    assert rel(C, ref) <= 2e-6
+
+
+def random_problem(o, B, seed):
+    rng = np.random.default_rng(seed)
+    P = fo.init_params(o, seed=seed, spectral_scale=3.0 / (o.C * o.C))
+    P[o.o_Q:o.o_Q + o.C] = rng.standard_normal(o.C) * 0.05
+    P[o.o_qb] = 1e-3
+    for l in range(o.L):
+        P[o.o_b[l]:o.o_b[l] + o.C] = rng.standard_normal(o.C) * 0.1
+    P = P.astype(np.float32).astype(np.float64)
+    from fastwigner_amd.batch import gaussian_state
+    W = np.stack([gaussian_state(o.N, 8., rng.uniform(-2, 2), rng.uniform(-2, 2)) for _ in range(B)])
+    feat = rng.uniform(0, 1, (B, 5))
+    tgt = W + 0.01 * rng.standard_normal(W.shape) * W.max()
+    return P, W.astype(np.float32).astype(np.float64), feat.astype(np.float32).astype(np.float64), \
+        tgt.astype(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("N,C,M,L", [(64, 32, 8, 2), (256, 32, 16, 4)])
+def test_forward_and_gradients_vs_oracle(gpu, N, C, M, L):
+    """Network output and every parameter gradient from the HIP kernels against
+    the float64 oracle with the same bf16 rounding; the last case is the
+    surrogate's own size (256^2, 32 channels, 16 x 16 modes, 4 layers)."""
+    from fastwigner_amd.fno import FNOSurrogate
+    B = 2
+    o = fo.FNOOracle(N=N, C=C, M=M, L=L, bf16=True)
+    s = FNOSurrogate(grid_dim=N, width=C, modes=M, layers=L, max_batch=B)
+    P, W, feat, tgt = random_problem(o, B, seed=N + L)
+    s.set_params(P)
+    out = s.predict(W, feat)
+    ref = o.forward(P, W, feat)
+    e_out = rel(out - W, ref - W)
+    loss, g = s.loss_grad(W, feat, tgt)
+    L0, gref = o.loss_grad(P, W, feat, tgt)
+    e_g = rel(g, gref)
+    groups = {}
+    lay = s.layout()
+    for name, (off, n) in lay.items():
+        groups[name] = rel(g[off:off + n], gref[off:off + n]) if np.any(gref[off:off + n]) else 0.0
+    print("N=%d C=%d M=%d L=%d: network output rel %.2e, loss %.6e vs %.6e, gradient rel %.2e, worst group %s %.2e"
+          % (N, C, M, L, e_out, loss, L0, e_g, max(groups, key=groups.get), max(groups.values())))
+    # the bf16 roundings can fall on the other side of a tie from FP32 vs FP64
+    # accumulation; each such flip moves one mixed operand by 2^-8 relative
+    assert e_out <= 5e-4
+    assert abs(loss - L0) <= 1e-4 * L0
+    assert e_g <= 2e-3
+    assert max(groups.values()) <= 2e-2
+
+
+def test_training_on_propagator_trajectories_beats_identity(gpu):
+    """Train on pairs (W(t), W(t + 20 dt)) of the batched random-collision
+    propagator at 256^2 and show the gain on held-out instances over the
+    method without the network (W(t + 20 dt) ~ W(t), which is also the
+    untrained network since its projection starts at zero)."""
+    from fastwigner_amd.fno import FNOSurrogate
+    s = FNOSurrogate(grid_dim=256, max_batch=8, seed=0)
+    kept = s.build_dataset(n_instances=48, pairs_per_instance=4, seed=11)
+    n_inst = len(kept)
+    assert n_inst >= 40
+    # pair index j * n_inst + q: hold out the last 8 instances
+    train = [j * n_inst + q for j in range(4) for q in range(n_inst - 8)]
+    held = [j * n_inst + q for j in range(4) for q in range(n_inst - 8, n_inst)]
+    e0 = s.eval_pairs(held)
+    s.train(400, lr=2e-3, pairs=train, log_every=50)
+    e1 = s.eval_pairs(held)
+    print("held-out relative L2: identity %.3e -> trained %.3e (gain x%.1f); loss history %s"
+          % (np.mean(e0), np.mean(e1), np.mean(e0) / np.mean(e1), s.history[::2]))
+    assert np.mean(e1) < 0.7 * np.mean(e0)
+
+
+def test_judged_jump_falls_back_to_the_propagator(gpu):
+    """The judge replaces every prediction whose error or norm drift exceeds the
+    bounds by the propagator's state; accepted ones are within the bounds."""
+    from fastwigner_amd.fno import FNOSurrogate, judged_jump
+    from fastwigner_amd.batch import gaussian_state
+    s = FNOSurrogate(grid_dim=256, max_batch=8, seed=0)
+    rng = np.random.default_rng(5)
+    W0 = np.stack([gaussian_state(256, 8., rng.uniform(-2, 2), rng.uniform(-2, 2)) for _ in range(4)])
+    feat = rng.uniform(0, 1, (4, 5))
+    judge = W0.copy()
+    judge[2:] *= 1.01          # two samples the identity-like network cannot match to 1e-3
+    dXdP = (16. / 256) ** 2
+    out, ok, err, drift = judged_jump(s, W0, feat, judge, dXdP)
+    assert ok.tolist() == [True, True, False, False]
+    assert np.array_equal(out[2:], judge[2:])
+    assert np.all(err[:2] <= 1e-3) and np.all(drift[:2] <= 1e-4)
