@@ -257,6 +257,10 @@ def main():
         r2, ms2 = timed_rate(s2, 2048, max(args.steps // 4, 100), 50)
         var["fp64_2048"] = {"value": r2, "ms_per_step": ms2, "roofline": roofline(2048, "fp64", kernel_times(s2))}
         print("[bench] fp64 2048^2: %.3e pt*steps/s" % r2, file=sys.stderr, flush=True)
+        print("[bench] surrogate: training on propagator trajectories", file=sys.stderr, flush=True)
+        from fastwigner_amd.fno import surrogate_experiment
+        var["surrogate_fno_config5"] = surrogate_experiment(
+            20000, held_out=512, epochs=1.0, log=lambda m: print(m, file=sys.stderr, flush=True))
         var["batched_fp32_config4"] = batched_variant(o_workers=min(os.cpu_count() or 1, 16),
                                                       with_oracle=not args.no_baseline)
         result["variants"] = var

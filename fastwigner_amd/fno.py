@@ -192,11 +192,85 @@ def judged_jump(surrogate, W0, feat, judge_states, dXdP, l2_tol=1e-3, norm_tol=1
     """One k-step jump of each sample with the surrogate, judged against the plain
     propagator's result: accepted when the relative L2 error is <= l2_tol and the
     norm drift |sum(W) dXdP - 1| <= norm_tol, otherwise replaced by the judge.
-    Returns (states, accepted mask, relative L2 errors, norm drifts)."""
+    Returns (states, accepted mask, relative L2 errors, norm drifts). The norm
+    drift is that of the raw network output; the accepted states are then
+    renormalised like every propagator step (W /= sum(W) dXdP)."""
     pred = surrogate.predict(W0, feat).astype(np.float64)
     err = np.linalg.norm((pred - judge_states).reshape(len(pred), -1), axis=1) / \
         np.linalg.norm(judge_states.reshape(len(pred), -1), axis=1)
     drift = np.abs(pred.reshape(len(pred), -1).sum(axis=1) * dXdP - 1.0)
     ok = (err <= l2_tol) & (drift <= norm_tol)
+    pred = pred / (pred.reshape(len(pred), -1).sum(axis=1)[:, None, None] * dXdP)
     out = np.where(ok[:, None, None], pred, judge_states)
     return out, ok, err, drift
+
+
+def surrogate_experiment(n_pairs=20000, pairs_per_instance=20, held_out=512, epochs=2.0, batch=8, lr=2e-3,
+                         seed=4, held_seed=5, log=None):
+    """The learned-surrogate workload: a 4-layer, 32-channel, 16 x 16-mode FNO on
+    256^2 Wigner functions jumping k = 20 steps of dt = 0.01, trained on about
+    n_pairs trajectory pairs of the batched random-collision propagator (random
+    quartic potentials, Gaussian initial states, seeded), then evaluated on
+    `held_out` new seeds against the FP64 split-operator judge (the batched
+    propagator in FP64), with fallback where the relative L2 error exceeds 1e-3
+    or the norm drift 1e-4. Returns a dict of measurements."""
+    import time
+    say = log or (lambda *a: None)
+    s = FNOSurrogate(grid_dim=256, width=32, modes=16, layers=4, max_batch=batch, k_steps=20, dt=0.01, seed=0)
+    n_inst = int(np.ceil(n_pairs / pairs_per_instance / 0.97))
+    t0 = time.perf_counter()
+    kept = s.build_dataset(n_inst, pairs_per_instance, seed=seed)
+    t_data = time.perf_counter() - t0
+    say("[fno] dataset: %d pairs from %d instances in %.1f s" % (s.npairs, len(kept), t_data))
+    steps = int(epochs * s.npairs / batch)
+    t0 = time.perf_counter()
+    s.train(steps, lr=lr, batch=batch, log_every=max(steps // 10, 1))
+    t_train = time.perf_counter() - t0
+    say("[fno] %d Adam steps in %.1f s, loss %s" % (steps, t_train, [round(l, 5) for _, l in s.history]))
+    # held-out seeds and the FP64 judge
+    inst, _ = random_quartic_instances(held_out, seed=held_seed, sizes=(256,), amplitude=8.)
+    judge = WignerBatchHIP1D(inst, V=QUARTIC_V, K="0.5*P*P", dt=0.01, param_names=("a2", "a3", "a4"),
+                             diff_V=QUARTIC_DV, diff_K="P", precision="fp64")
+    judge.prepare_random_collision(dbeta=0.01)
+    dXdP = (16. / 256) ** 2
+    phys = [i for i in range(held_out)
+            if 2 * np.pi * np.sum(judge.get_wignerfunction(i) ** 2) * dXdP <= 1.0 + 1e-3]
+    W0 = np.stack([gaussian_state(256, 8., d["x0"], d["p0"]) for d in inst])
+    for i in range(held_out):
+        judge.set_wignerfunction(i, W0[i])
+    t0 = time.perf_counter()
+    judge.propagate(20, record=False)
+    t_judge = time.perf_counter() - t0
+    WJ = np.stack([judge.get_wignerfunction(i) for i in phys])
+    feat = np.array([[inst[i]["params"][0], inst[i]["params"][1], inst[i]["params"][2], inst[i]["gamma"],
+                      inst[i]["kT"]] for i in phys])
+    W0p = W0[phys] / (W0[phys].sum(axis=(1, 2), keepdims=True) * dXdP)
+    _, ok, err, drift = judged_jump(s, W0p, feat, WJ, dXdP)
+    ident = np.linalg.norm((W0p - WJ).reshape(len(phys), -1), axis=1) / np.linalg.norm(WJ.reshape(len(phys), -1),
+                                                                                        axis=1)
+    us = s.infer_time_us(batch, reps=20)
+    rate = batch * 256 * 256 * 20 / (us * 1e-6)
+    out = dict(pairs=s.npairs, train_instances=len(kept), adam_steps=steps, batch=batch, train_s=round(t_train, 1),
+               data_s=round(t_data, 1), held_out=len(phys), held_out_unphysical=held_out - len(phys),
+               rel_l2_median=float(np.median(err)), rel_l2_mean=float(np.mean(err)), rel_l2_max=float(np.max(err)),
+               identity_rel_l2_mean=float(np.mean(ident)), norm_drift_max=float(np.max(drift)),
+               accepted=int(ok.sum()), fallback=int((~ok).sum()),
+               infer_us_per_batch=round(us, 1), value=rate, unit="grid-point*steps/s",
+               judge_fp64_s_for_held_out=round(t_judge, 3))
+    say("[fno] held-out: rel L2 median %.3e (identity %.3e), accepted %d of %d; %.3e pt*steps/s"
+        % (out["rel_l2_median"], out["identity_rel_l2_mean"], out["accepted"], len(phys), rate))
+    return out
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+    import sys
+    ap = argparse.ArgumentParser(description="train and judge the FNO surrogate on propagator trajectories")
+    ap.add_argument("--pairs", type=int, default=20000)
+    ap.add_argument("--held-out", type=int, default=512)
+    ap.add_argument("--epochs", type=float, default=2.0)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    a = ap.parse_args()
+    print(json.dumps(surrogate_experiment(a.pairs, held_out=a.held_out, epochs=a.epochs, lr=a.lr,
+                                          log=lambda m: print(m, file=sys.stderr, flush=True))))
