@@ -47,8 +47,9 @@ def make(n, precision):
 
 
 def kernel_times(s, steps=64):
-    """Mean device time (us) of v_first, k, v_last with HIP events on the
-    propagator's stream."""
+    """Mean device time (us) of the step kernels with HIP events on the
+    propagator's stream: {name: us}. Two-pass steps run fw_vv and fw_k_norm,
+    three-pass steps fw_v_first, fw_k and fw_v_last."""
     import ctypes
     from fastwigner_amd import _lib
     _lib.call("fw_wigner_set_timing", s._handle, 1)
@@ -56,7 +57,11 @@ def kernel_times(s, steps=64):
     out = (ctypes.c_double * 3)()
     _lib.call("fw_wigner_kernel_times", s._handle, out)
     _lib.call("fw_wigner_set_timing", s._handle, 0)
-    return list(out)
+    passes = ctypes.c_int()
+    _lib.call("fw_wigner_step_passes", s._handle, ctypes.byref(passes))
+    if passes.value == 2:
+        return {"fw_vv": out[0], "fw_k_norm": out[1]}
+    return {"fw_v_first": out[0], "fw_k": out[1], "fw_v_last": out[2]}
 
 
 def measured_traffic(kernel, n, precision):
@@ -74,14 +79,14 @@ def measured_traffic(kernel, n, precision):
 
 
 def roofline(n, precision, ktimes):
-    names = ["v_first", "k", "v_last"]
-    i = int(np.argmax(ktimes))
+    name = max(ktimes, key=ktimes.get)
     bytes_per_launch = BYTES_PER_PASS[precision] * n * n
-    achieved = bytes_per_launch / (ktimes[i] * 1e-6) / 1e9
-    return {"bound": "hbm", "kernel": "fw_" + names[i], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+    achieved = bytes_per_launch / (ktimes[name] * 1e-6) / 1e9
+    return {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic("fw_" + names[i], n, precision),
-            "bytes_per_launch": bytes_per_launch, "mean_us": [round(x, 3) for x in ktimes]}
+            "traffic": measured_traffic(name, n, precision),
+            "bytes_per_launch": bytes_per_launch, "passes_per_step": len(ktimes),
+            "mean_us": {k: round(v, 3) for k, v in ktimes.items()}}
 
 
 def timed_rate(s, n, steps, warmup):

@@ -13,7 +13,7 @@ GRID_CONST_NAMES = frozenset(
 
 # Keyword arguments of this implementation that are not physics constants.
 CONTROL_KWARGS = frozenset(["precision", "phase_time_shift", "device", "tiles", "bloch_convention",
-                            "phase_tables"])
+                            "phase_tables", "step_passes"])
 
 
 def bind_kwargs(obj, kwargs):

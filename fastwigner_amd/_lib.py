@@ -52,7 +52,7 @@ class WignerDesc(ctypes.Structure):
         ("precision", ctypes.c_int), ("device", ctypes.c_int),
         ("col_e", ctypes.c_int), ("col_p", ctypes.c_int),
         ("row_e", ctypes.c_int), ("row_p", ctypes.c_int),
-        ("phase_tables", ctypes.c_int),
+        ("phase_tables", ctypes.c_int), ("step_passes", ctypes.c_int),
     ]
 
 
@@ -108,7 +108,8 @@ def _b(s):
     return None if s is None else s.encode()
 
 
-def wigner_desc(nx, xamp, npp, pamp, dt, shift=0.5, precision=0, device=0, tiles=None, phase_tables=0):
+def wigner_desc(nx, xamp, npp, pamp, dt, shift=0.5, precision=0, device=0, tiles=None, phase_tables=0,
+                step_passes=0):
     d = WignerDesc()
     d.x_grid_dim, d.p_grid_dim = int(nx), int(npp)
     d.x_amplitude, d.p_amplitude = float(xamp), float(pamp)
@@ -117,6 +118,7 @@ def wigner_desc(nx, xamp, npp, pamp, dt, shift=0.5, precision=0, device=0, tiles
     if tiles:
         d.col_e, d.col_p, d.row_e, d.row_p = [int(v) for v in tiles]
     d.phase_tables = int(phase_tables)
+    d.step_passes = int(step_passes)
     return d
 
 
@@ -211,6 +213,7 @@ PROTOTYPES = {
     "fw_wigner_snapshot": [_P, _I],
     "fw_wigner_set_timing": [_P, _I],
     "fw_wigner_kernel_times": [_P, _PD],
+    "fw_wigner_step_passes": [_P, _PI],
     "fw_rho_grid_constants": [ctypes.POINTER(RhoDesc), _PD],
     "fw_rho_check": [ctypes.POINTER(RhoDesc), ctypes.POINTER(RhoExprs)],
     "fw_rho_create": [ctypes.POINTER(RhoDesc), ctypes.POINTER(RhoExprs), _PP],

@@ -1,10 +1,12 @@
 """Autotuner of the FFT tile schedules (elements per thread E, lines per block P)
-of the Wigner V pass (columns) and K pass (rows).
+of the Wigner column passes (V) and row passes (K).
 
 Each candidate is compiled, run for a few steps on the device, and timed with
-HIP events on the propagator's stream (fw_wigner_kernel_times). The V-pass and
-K-pass schedules are independent kernels and are chosen independently. The
-winners are written in the format of ``fastwigner_amd/tuning.json``.
+HIP events on the propagator's stream (fw_wigner_kernel_times) in the default
+two-pass steps (the fused VV column pass and the normalising K row pass). The
+column and row schedules are independent kernels and are chosen
+independently. The winners are written in the format of
+``fastwigner_amd/tuning.json``.
 
 Usage (on the GPU):
     python -m fastwigner_amd.autotune --grids 1024 2048 --precision fp64 fp32 --out tuning.json
@@ -22,18 +24,25 @@ PHYS = dict(X_amplitude=8., P_amplitude=8., dt=0.01, t=0., V="0.5*X*X+0.1*X*X*X*
             diff_V="X+0.4*X*X*X", diff_K="P")
 
 
-def candidates(n_line, n_lines_pairs, esz_c, max_threads=1024, max_lds=160 * 1024):
+def twiddle_entries(n, fp32):
+    """LDS twiddle entries (16 bytes each): the full padded table in FP32, the
+    two-level table (n/S + S, S = min(64, n)) in FP64."""
+    s = min(64, n)
+    return n * 17 // 16 + 1 if fp32 else n // s + s
+
+
+def candidates(n_line, n_lines_pairs, esz_c, fp32, max_threads=1024, max_lds=160 * 1024):
     out = []
     for e in (4, 8, 16, 32):
         if e > n_line:
             continue
         tpl = n_line // e
-        for p in (1, 2, 4, 8):
+        for p in (1, 2, 4, 8, 16):
             nt = p * tpl
             if nt < 64 or nt > max_threads or n_lines_pairs % p:
                 continue
             raw = p * n_line
-            if (raw + raw // 16 + 1) * esz_c + (n_line * 17 // 16 + 1) * 16 + 1152 > max_lds:
+            if (raw + raw // 16 + 1) * esz_c + twiddle_entries(n_line, fp32) * 16 + 1152 > max_lds:
                 continue
             out.append((e, p))
     return out
@@ -53,8 +62,8 @@ def measure(n, prec, tiles, steps=48):
 
 def tune(n, prec, log=sys.stderr):
     esz_c = 8 if prec else 16
-    col = candidates(n, n // 2, esz_c)
-    row = candidates(n, n // 2, esz_c)
+    col = candidates(n, n // 2, esz_c, bool(prec))
+    row = candidates(n, n // 2, esz_c, bool(prec))
     m = max(len(col), len(row))
     best_col, best_row = (None, 1e30), (None, 1e30)
     rows = []
@@ -67,16 +76,16 @@ def tune(n, prec, log=sys.stderr):
         except _lib.FastWignerError as ex:
             print("  tiles %s failed: %s" % (tiles, str(ex).splitlines()[0]), file=log, flush=True)
             continue
-        vt = t[0] + t[2]
+        vt = t[0]   # the fused column pass (vv)
         rows.append({"tiles": tiles, "us": t})
-        print("  %dx%d %s tiles %s: v_first %.2f k %.2f v_last %.2f us" % (
-            n, n, "fp32" if prec else "fp64", tiles, t[0], t[1], t[2]), file=log, flush=True)
+        print("  %dx%d %s tiles %s: vv %.2f k %.2f us" % (
+            n, n, "fp32" if prec else "fp64", tiles, t[0], t[1]), file=log, flush=True)
         if i < len(col) and vt < best_col[1]:
             best_col = (c, vt)
         if i < len(row) and t[1] < best_row[1]:
             best_row = (r, t[1])
     tiles = (best_col[0][0], best_col[0][1], best_row[0][0], best_row[0][1])
-    return tiles, {"v_pass_us": best_col[1] / 2, "k_pass_us": best_row[1], "sweep": rows}
+    return tiles, {"vv_pass_us": best_col[1], "k_pass_us": best_row[1], "sweep": rows}
 
 
 def main():
@@ -91,11 +100,11 @@ def main():
             prec = 1 if p == "fp32" else 0
             tiles, info = tune(n, prec)
             print("[autotune] wigner %dx%d %s -> %s" % (n, n, p, tiles), file=sys.stderr, flush=True)
-            result[tuning.key("wigner", n, n, prec)] = {"tiles": list(tiles), "v_pass_us": info["v_pass_us"],
+            result[tuning.key("wigner", n, n, prec)] = {"tiles": list(tiles), "vv_pass_us": info["vv_pass_us"],
                                                          "k_pass_us": info["k_pass_us"]}
             if args.out:
                 tuning.store("wigner", n, n, prec, tiles,
-                             {"v_pass_us": round(info["v_pass_us"], 3), "k_pass_us": round(info["k_pass_us"], 3)},
+                             {"vv_pass_us": round(info["vv_pass_us"], 3), "k_pass_us": round(info["k_pass_us"], 3)},
                              path=args.out)
     print(json.dumps(result))
 

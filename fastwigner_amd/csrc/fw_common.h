@@ -209,6 +209,55 @@ FWD int xcd_remap(int b, int nb) {
 FWD int twpad(int i) { return i + (i >> 4); }
 template <int N> struct TwLds { static constexpr int SIZE = N + N / 16 + 1; };
 
+// Twiddle table of length TWN as staged in LDS. Full (TWO = false): all TWN
+// entries, padded (twpad). Two-level (TWO = true, FP64 only): w_j = A[j / S]
+// B[j % S] with A[h] = w_{h S}, B[l] = w_l, S = min(64, TWN): TWN/S + S
+// entries (1.5 KiB at TWN = 2048 instead of 32 KiB) for one more complex
+// product per twiddle. Entry e of the staged table is entry src(e) of the
+// global full table, stored at dst(e).
+template <int TWN, bool TWO>
+struct TwTab {
+  static constexpr int COUNT = TWN;
+  static constexpr int SIZE = TWN + TWN / 16 + 1;
+  FWD static int src(int e) { return e; }
+  FWD static int dst(int e) { return twpad(e); }
+  FWD static tw_t get(const tw_t* tw, int idx) { return tw[twpad(idx)]; }
+};
+#if !FW_FP32
+template <int TWN>
+struct TwTab<TWN, true> {
+  static constexpr int S = TWN < 64 ? TWN : 64;
+  static constexpr int NA = TWN / S;
+  static constexpr int COUNT = NA + S;
+  static constexpr int SIZE = NA + S;
+  FWD static int src(int e) { return e < NA ? e * S : e - NA; }
+  FWD static int dst(int e) { return e; }
+  FWD static tw_t get(const tw_t* tw, int idx) { return cmul(tw[idx / S], tw[NA + (idx % S)]); }
+};
+#endif
+
+// Loads the staged entries of a TwTab from global memory into registers (issue
+// early) and stores them into LDS (before the first barrier of the FFT).
+template <class T, int NT>
+struct TwStage {
+  static constexpr int R = (T::COUNT + NT - 1) / NT;
+  tw_t r[R];
+  FWD void load(const tw_t* __restrict__ g, int tid) {
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const int e = tid + i * NT;
+      if (e < T::COUNT) r[i] = g[T::src(e)];
+    }
+  }
+  FWD void store(tw_t* l, int tid) const {
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const int e = tid + i * NT;
+      if (e < T::COUNT) l[T::dst(e)] = r[i];
+    }
+  }
+};
+
 // ---------------------------------------------------------------------------
 // Line FFT engine: P lines of length N per block, E complex elements per thread.
 // LM (line-major) lane mapping: t = tid % TPL, q = tid / TPL (rows, contiguous);
@@ -217,8 +266,9 @@ template <int N> struct TwLds { static constexpr int SIZE = N + N / 16 + 1; };
 // ---------------------------------------------------------------------------
 // TWN: length of the twiddle table passed in (a table for a multiple of N
 // serves N with stride TWN/N; the batched kernels share one table).
-template <int N, int E, int P, bool LM, int TWN = N>
+template <int N, int E, int P, bool LM, int TWN = N, bool TWO = false>
 struct Line {
+  using TW = TwTab<TWN, TWO>;
   static_assert((N & (N - 1)) == 0 && N >= 2, "line length must be a power of two");
   static_assert(E <= N && (E & (E - 1)) == 0, "elements per thread must be a power of two <= N");
   static constexpr int TPL = N / E;
@@ -259,7 +309,7 @@ struct Line {
       if constexpr (NS > 1) {
         const int k = j & (NS - 1);
 #pragma unroll
-        for (int m = 1; m < R; ++m) a[m] = twmul(a[m], tw[twpad(m * k * TWS)]);
+        for (int m = 1; m < R; ++m) a[m] = twmul(a[m], TW::get(tw, m * k * TWS));
       }
       Dft<R>::run(a);
       if constexpr (TOREG) {

@@ -81,6 +81,7 @@ enum {
   F_VTAB = 64,       // V factors from the cached table
   F_KTAB = 128,      // K factors from the cached table
   F_FROM_PARTIALS = 256,  // apply_norm: norm from the last V pass's partials (end of a run)
+  F_KNORM = 512,     // two-pass steps: the K pass applies the previous step's deferred norm
 };
 
 // two adjacent factors (columns c, c+1) in one load
@@ -164,15 +165,59 @@ FWD bool previous_step(const Args& a, int blk, double* red, double& nrm) {
 // ---------------------------------------------------------------------------
 // V pass: columns (along p), lines = pairs of adjacent columns.
 // ---------------------------------------------------------------------------
-using VL = Line<FW_NP, FW_COL_E, FW_COL_P, false>;
-using KL = Line<FW_NX, FW_ROW_E, FW_ROW_P, true>;
+// Two-level twiddle tables (TwTab<., true>) shrink the LDS footprint but cost a
+// second LDS read and a product per twiddle; measured slower on gfx950 for
+// these passes, which are LDS-bandwidth bound, so the full table is used.
+#define FW_TW2 0
+using VL = Line<FW_NP, FW_COL_E, FW_COL_P, false, FW_NP, FW_TW2>;
+using KL = Line<FW_NX, FW_ROW_E, FW_ROW_P, true, FW_NX, FW_TW2>;
+
+// Multiply the packed spectrum of column pair (Xa, Xb) held in LDS by the V
+// factors at time tau (masks at tnow), from the table row cache ftab or
+// evaluated here.
+template <bool use_tab>
+FWD void v_multiply(cpx* lds, int q, int t, double Xa, double Xb, double tau, double tnow, const cpx2* ftab) {
+  using L = VL;
+  constexpr int N = FW_NP, E = FW_COL_E, TPL = L::TPL;
+  const real_t s = (real_t)1 / (real_t)N;
+#pragma unroll
+  for (int e = 0; e < E / 2; ++e) {
+    const int k = t + TPL * e;
+    if (k == 0) {
+      const double ThN = ::dTheta * (N / 2);
+      cpx fa0, fb0, faN, fbN;
+      if constexpr (use_tab) {
+        fa0 = ftab[0].a; fb0 = ftab[0].b; faN = ftab[E / 2].a; fbN = ftab[E / 2].b;
+      } else {
+        fa0 = vfactor(Xa, 0.0, tau, tnow); fb0 = vfactor(Xb, 0.0, tau, tnow);
+        faN = vfactor(Xa, ThN, tau, tnow); fbN = vfactor(Xb, ThN, tau, tnow);
+      }
+      const int i0 = L::lidx(q, 0), iN = L::lidx(q, N / 2);
+      lds[i0] = self_apply(lds[i0], fa0, fb0, s);
+      lds[iN] = self_apply(lds[iN], faN, fbN, s);
+    } else {
+      const double Th = ::dTheta * k;
+      cpx fa, fb;
+      if constexpr (use_tab) {
+        fa = ftab[e].a; fb = ftab[e].b;
+      } else {
+        fa = vfactor(Xa, Th, tau, tnow); fb = vfactor(Xb, Th, tau, tnow);
+      }
+      const int ik = L::lidx(q, k), im = L::lidx(q, N - k);
+      cpx zk = lds[ik], zm = lds[im];
+      pair_apply(zk, zm, fa, fb, s);
+      lds[ik] = zk;
+      lds[im] = zm;
+    }
+  }
+}
 
 template <int FL>
 FWD void vpass(const Args& a) {
   using L = VL;
   constexpr int N = FW_NP, E = FW_COL_E, P = FW_COL_P, TPL = L::TPL, NT = L::NT;
   __shared__ cpx lds[L::LDS];
-  __shared__ tw_t twl[TwLds<N>::SIZE];
+  __shared__ tw_t twl[L::TW::SIZE];
   __shared__ double red[NSER * 16];
   if (a.scal[SC_STOP] != 0.0) return;
   const int tid = threadIdx.x;
@@ -188,10 +233,8 @@ FWD void vpass(const Args& a) {
   real_t* __restrict__ W = a.W;
 
   // issue every global load of the pass up front: twiddles, W, factor table
-  constexpr int TWN = (N + NT - 1) / NT;
-  tw_t twr[TWN];
-#pragma unroll
-  for (int i = 0; i < TWN; ++i) twr[i] = a.tw_col[(tid + i * NT) & (N - 1)];
+  TwStage<typename L::TW, NT> tws;
+  tws.load(a.tw_col, tid);
   cpx v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = *reinterpret_cast<const cpx*>(W + (size_t)(t + TPL * e) * FW_NX + c);
@@ -202,9 +245,7 @@ FWD void vpass(const Args& a) {
       ftab[e] = *reinterpret_cast<const cpx2*>(a.vtab + (size_t)(t + TPL * e) * FW_NX + c);
     ftab[E / 2] = *reinterpret_cast<const cpx2*>(a.vtab + (size_t)(N / 2) * FW_NX + c);
   }
-#pragma unroll
-  for (int i = 0; i < TWN; ++i)
-    if ((N % NT) == 0 || tid + i * NT < N) twl[twpad(tid + i * NT)] = twr[i];
+  tws.store(twl, tid);
 
   if constexpr (FL & F_FIRST) {
     double nrm = 1.0;
@@ -233,39 +274,7 @@ FWD void vpass(const Args& a) {
   // spectral multiply on the theta x representation
   const double Xa = ::dX * (c - 0.5 * ::X_gridDIM);
   const double Xb = ::dX * (c + 1 - 0.5 * ::X_gridDIM);
-  {
-    const real_t s = (real_t)1 / (real_t)N;
-#pragma unroll
-    for (int e = 0; e < E / 2; ++e) {
-      const int k = t + TPL * e;
-      if (k == 0) {
-        const double ThN = ::dTheta * (N / 2);
-        cpx fa0, fb0, faN, fbN;
-        if constexpr (use_tab) {
-          fa0 = ftab[0].a; fb0 = ftab[0].b; faN = ftab[E / 2].a; fbN = ftab[E / 2].b;
-        } else {
-          fa0 = vfactor(Xa, 0.0, tau, tnow); fb0 = vfactor(Xb, 0.0, tau, tnow);
-          faN = vfactor(Xa, ThN, tau, tnow); fbN = vfactor(Xb, ThN, tau, tnow);
-        }
-        const int i0 = L::lidx(q, 0), iN = L::lidx(q, N / 2);
-        lds[i0] = self_apply(lds[i0], fa0, fb0, s);
-        lds[iN] = self_apply(lds[iN], faN, fbN, s);
-      } else {
-        const double Th = ::dTheta * k;
-        cpx fa, fb;
-        if constexpr (use_tab) {
-          fa = ftab[e].a; fb = ftab[e].b;
-        } else {
-          fa = vfactor(Xa, Th, tau, tnow); fb = vfactor(Xb, Th, tau, tnow);
-        }
-        const int ik = L::lidx(q, k), im = L::lidx(q, N - k);
-        cpx zk = lds[ik], zm = lds[im];
-        pair_apply(zk, zm, fa, fb, s);
-        lds[ik] = zk;
-        lds[im] = zm;
-      }
-    }
-  }
+  v_multiply<use_tab>(lds, q, t, Xa, Xb, tau, tnow, ftab);
   L::sync();
   L::load_natural(v, lds, q, t);
   L::sync();
@@ -311,6 +320,95 @@ FWD void vpass(const Args& a) {
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// Fused column pass of two-pass steps: the closing V/2 of step s and the
+// opening V/2 of step s+1 in one read and one write of W. Between them, in
+// registers: the x-p mask and the nine sums of step s (partials; the norm is
+// applied by the next K pass, which is linear like everything before it).
+// ---------------------------------------------------------------------------
+FWD void vvpass(const Args& a) {
+  using L = VL;
+  constexpr int N = FW_NP, E = FW_COL_E, P = FW_COL_P, TPL = L::TPL, NT = L::NT;
+  __shared__ cpx lds[L::LDS];
+  __shared__ tw_t twl[L::TW::SIZE];
+  __shared__ double red[NSER * 16];
+  if (a.scal[SC_STOP] != 0.0) return;
+  const int tid = threadIdx.x;
+  const int q = tid % P, t = tid / P;
+  const int nb = gridDim.x;
+  const int blk = xcd_remap(blockIdx.x, nb);
+  const int c = (blk * P + q) * 2;
+  const double t0 = a.times[a.step], t1 = a.times[a.step + 1];
+  constexpr bool use_tab = FW_VTAB;
+  real_t* __restrict__ W = a.W;
+
+  TwStage<typename L::TW, NT> tws;
+  tws.load(a.tw_col, tid);
+  cpx v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = *reinterpret_cast<const cpx*>(W + (size_t)(t + TPL * e) * FW_NX + c);
+  cpx2 ftab[E / 2 + 1];
+  if constexpr (use_tab) {
+#pragma unroll
+    for (int e = 0; e < E / 2; ++e)
+      ftab[e] = *reinterpret_cast<const cpx2*>(a.vtab + (size_t)(t + TPL * e) * FW_NX + c);
+    ftab[E / 2] = *reinterpret_cast<const cpx2*>(a.vtab + (size_t)(N / 2) * FW_NX + c);
+  }
+  tws.store(twl, tid);
+  // the K pass of this step has read `fresh`; the next K pass uses the partials
+  if (blk == 0 && tid == 0 && a.scal[SC_FRESH] != 0.0) a.scal[SC_FRESH] = 0.0;
+  __syncthreads();
+
+  const double Xa = ::dX * (c - 0.5 * ::X_gridDIM);
+  const double Xb = ::dX * (c + 1 - 0.5 * ::X_gridDIM);
+  // closing V/2 of step s
+  L::template fwd<true>(v, lds, q, t, twl);
+  v_multiply<use_tab>(lds, q, t, Xa, Xb, t0 + FW_PHASE_SHIFT * ::dt, t0, ftab);
+  L::sync();
+  L::load_natural(v, lds, q, t);
+  L::sync();
+  L::inv(v, lds, q, t, twl);
+  // end of step s: x-p mask, sums
+  double sums[NSER];
+#pragma unroll
+  for (int i = 0; i < NSER; ++i) sums[i] = 0.0;
+  double ca = 0.0, cb = 0.0;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int n = t + TPL * e;
+    cpx w = v[e];
+    const double Pn = ::dP * (n - 0.5 * ::P_gridDIM);
+    if (a.flags & F_MASK_XP) {
+      w.x *= (real_t)::fw_mask_x_p(Xa, Pn, t0);
+      w.y *= (real_t)::fw_mask_x_p(Xb, Pn, t0);
+    }
+    const double wa = (double)w.x, wb = (double)w.y, ws = wa + wb;
+    ca += wa; cb += wb;
+    sums[2] += Pn * ws;
+    sums[4] += Pn * Pn * ws;
+    sums[5] += ::fw_diff_K(Pn, t0) * ws;
+    sums[7] += ::K(Pn, t0) * ws;
+    sums[8] += wa * wa + wb * wb;
+    v[e] = w;
+  }
+  sums[0] = ca + cb;
+  sums[1] = Xa * ca + Xb * cb;
+  sums[3] = Xa * Xa * ca + Xb * Xb * cb;
+  sums[6] = ::fw_diff_V(Xa, t0) * ca + ::fw_diff_V(Xb, t0) * cb;
+  sums[7] += ::V(Xa, t0) * ca + ::V(Xb, t0) * cb;
+  write_partials<NT>(sums, red, a.partials, blk);
+  // opening V/2 of step s+1
+  L::template fwd<true>(v, lds, q, t, twl);
+  v_multiply<use_tab>(lds, q, t, Xa, Xb, t1 + FW_PHASE_SHIFT * ::dt, t1, ftab);
+  L::sync();
+  L::load_natural(v, lds, q, t);
+  L::sync();
+  L::inv(v, lds, q, t, twl);
+#pragma unroll
+  for (int e = 0; e < E; ++e) *reinterpret_cast<cpx*>(W + (size_t)(t + TPL * e) * FW_NX + c) = v[e];
+}
+
 // ---------------------------------------------------------------------------
 // K pass: rows (along x), lines = pairs of adjacent rows.
 // ---------------------------------------------------------------------------
@@ -320,7 +418,8 @@ FWD void kpass(const Args& a) {
   constexpr int N = FW_NX, E = FW_ROW_E, P = FW_ROW_P, TPL = L::TPL, NT = L::NT;
   constexpr int KW = FW_NX / 2 + 1;
   __shared__ cpx lds[L::LDS];
-  __shared__ tw_t twl[TwLds<N>::SIZE];
+  __shared__ tw_t twl[L::TW::SIZE];
+  __shared__ double red[(FL & F_KNORM) ? NSER * 16 : 1];
   if (a.scal[SC_STOP] != 0.0) return;
   const int tid = threadIdx.x;
   const int t = tid % TPL, q = tid / TPL;
@@ -332,12 +431,11 @@ FWD void kpass(const Args& a) {
   constexpr bool use_tab = (FL & F_BLOCH) || FW_KTAB;
   real_t* __restrict__ Wa = a.W + (size_t)r * FW_NX;
   real_t* __restrict__ Wb = Wa + FW_NX;
-  if (blk == 0 && tid == 0 && a.scal[SC_FRESH] != 0.0) a.scal[SC_FRESH] = 0.0;
+  if constexpr (!(FL & F_KNORM))
+    if (blk == 0 && tid == 0 && a.scal[SC_FRESH] != 0.0) a.scal[SC_FRESH] = 0.0;
 
-  constexpr int TWN = (N + NT - 1) / NT;
-  tw_t twr[TWN];
-#pragma unroll
-  for (int i = 0; i < TWN; ++i) twr[i] = a.tw_row[(tid + i * NT) & (N - 1)];
+  TwStage<typename L::TW, NT> tws;
+  tws.load(a.tw_row, tid);
   cpx v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = mk(Wa[t + TPL * e], Wb[t + TPL * e]);
@@ -351,10 +449,21 @@ FWD void kpass(const Args& a) {
     fta[E / 2] = a.ktab[(size_t)r * KW + N / 2];
     ftb[E / 2] = a.ktab[(size_t)(r + 1) * KW + N / 2];
   }
+  tws.store(twl, tid);
+  if constexpr (FL & F_KNORM) {
+    // two-pass steps: the previous step's norm (and its published sums)
+    double nrm = 1.0;
+    if (a.scal[SC_FRESH] == 0.0) {
+      if (!previous_step<NT>(a, blk, red, nrm)) return;
+    } else {
+      __syncthreads();
+    }
+    const real_t rn = (real_t)nrm;
 #pragma unroll
-  for (int i = 0; i < TWN; ++i)
-    if ((N % NT) == 0 || tid + i * NT < N) twl[twpad(tid + i * NT)] = twr[i];
-  __syncthreads();
+    for (int e = 0; e < E; ++e) { v[e].x *= rn; v[e].y *= rn; }
+  } else {
+    __syncthreads();
+  }
   L::template fwd<true>(v, lds, q, t, twl);
   {
     const double Pa = ::dP * (r - 0.5 * ::P_gridDIM);
@@ -539,6 +648,8 @@ FWD void vk_min(const Args& a) {
 FW_VK(fw_v_first, fwk::F_FIRST)
 FW_VK(fw_v_last, fwk::F_LAST)
 FW_KK(fw_k, 0)
+FW_KK(fw_k_norm, fwk::F_KNORM)
+extern "C" __global__ void __launch_bounds__(fwk::VL::NT) fw_vv(fwk::Args a) { fwk::vvpass(a); }
 extern "C" __global__ void __launch_bounds__(fwk::HT) fw_moments(fwk::Args a) { fwk::moments(a); }
 extern "C" __global__ void __launch_bounds__(fwk::HT) fw_apply_norm(fwk::Args a) { fwk::apply_norm(a); }
 extern "C" __global__ void fw_set_scalar(double* p, double v) { *p = v; }

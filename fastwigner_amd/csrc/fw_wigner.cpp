@@ -71,7 +71,8 @@ struct fw_wigner {
   hipModule_t mod = nullptr, mod_bloch = nullptr, mod_dissip = nullptr;
   hipFunction_t f_v_first = nullptr, f_v_last = nullptr, f_v_first_dissip = nullptr, f_v_last_dissip = nullptr,
                 f_k = nullptr, f_bv_first = nullptr, f_bv_last = nullptr, f_bk = nullptr, f_moments = nullptr,
-                f_apply_norm = nullptr, f_set_scalar = nullptr, f_vk_min = nullptr;
+                f_apply_norm = nullptr, f_set_scalar = nullptr, f_vk_min = nullptr, f_vv = nullptr,
+                f_k_norm = nullptr;
   hipStream_t stream = nullptr;
 
   void* W = nullptr;
@@ -101,6 +102,8 @@ struct fw_wigner {
 
   hipGraphExec_t gexec = nullptr;
   int gkind = -1;
+  hipGraphExec_t gexec2 = nullptr;   // two-pass Moyal steps
+  int last_passes = 3;
 
   std::map<std::string, ExprMod> init_mods, avg_mods;
 
@@ -122,6 +125,8 @@ int validate(const fw_wigner_desc* d, const fw_wigner_exprs* x) {
   if (!std::isfinite(d->dt)) return fw::fail(FW_ERR_INVALID_ARG, "dt must be finite");
   if (d->precision != FW_FP64 && d->precision != FW_FP32)
     return fw::fail(FW_ERR_INVALID_ARG, "precision must be FW_FP64 or FW_FP32");
+  if (d->step_passes != 0 && d->step_passes != 2 && d->step_passes != 3)
+    return fw::fail(FW_ERR_INVALID_ARG, "step_passes must be 0, 2 or 3");
   if (!x->V || !x->K) return fw::fail(FW_ERR_INVALID_ARG, "V and K expressions are required");
   return FW_OK;
 }
@@ -156,7 +161,12 @@ int resolve_tiles(const fw_wigner_desc* d, Tiles& t) {
   const size_t lds_c = (size_t)t.col_p * d->p_grid_dim, lds_r = (size_t)t.row_p * d->x_grid_dim;
   // gfx950: 160 KiB of LDS per workgroup (exchange buffer + twiddle table + reduction scratch)
   const size_t tws = 16;
-  const size_t twc = (size_t)d->p_grid_dim * 17 / 16 + 1, twr = (size_t)d->x_grid_dim * 17 / 16 + 1;
+  // FP64: two-level twiddle tables (N/S + S entries, S = min(64, N)); FP32: the full padded table
+  auto tw_entries = [&](int n) -> size_t {
+    const int sq = n < 64 ? n : 64;
+    return fp32 ? (size_t)n * 17 / 16 + 1 : (size_t)(n / sq + sq);
+  };
+  const size_t twc = tw_entries(d->p_grid_dim), twr = tw_entries(d->x_grid_dim);
   if ((lds_c + lds_c / 16 + 1) * esz + twc * tws + 1152 > 160 * 1024 ||
       (lds_r + lds_r / 16 + 1) * esz + twr * tws > 160 * 1024)
     return fw::fail(FW_ERR_INVALID_ARG, "tile needs more than 160 KiB of LDS");
@@ -427,6 +437,108 @@ int run_steps(fw_wigner* h, int kind, const KArgs& base0, const double* times, i
   return FW_OK;
 }
 
+// Two-pass Moyal steps: V/2 (opening, step 0), then per step s a normalising K
+// pass and a fused VV pass (closing V/2 of s + opening V/2 of s+1), the last
+// step closing with the plain V/2 pass. The K pass of step s applies the norm
+// of step s-1 and publishes its sums to chunk-local slot s, as the first pass
+// of three-pass steps does, so the series bookkeeping is the same.
+int enqueue_fused(fw_wigner* h, int s, bool last, const KArgs& base, hipEvent_t* ev) {
+  KArgs a = base;
+  a.step = s;
+  if (ev) HIPCHK(hipEventRecord(ev[0], h->stream));
+  FWCHK(launch(h, h->f_k_norm, h->row_nb, h->row_nt, a));
+  if (ev) HIPCHK(hipEventRecord(ev[1], h->stream));
+  FWCHK(launch(h, last ? h->f_v_last : h->f_vv, h->col_nb, h->col_nt, a));
+  if (ev) HIPCHK(hipEventRecord(ev[2], h->stream));
+  return FW_OK;
+}
+
+int ensure_graph_fused(fw_wigner* h, const KArgs& base) {
+  if (h->gexec2) return FW_OK;
+  hipGraph_t g;
+  HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  int rc = FW_OK;
+  for (int s = 0; s < kChunk && rc == FW_OK; ++s) rc = enqueue_fused(h, s, false, base, nullptr);
+  hipError_t e = hipStreamEndCapture(h->stream, &g);
+  if (rc != FW_OK) return rc;
+  if (e != hipSuccess) return fw::fail(FW_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  e = hipGraphInstantiate(&h->gexec2, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return fw::fail(FW_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  return FW_OK;
+}
+
+int run_steps_fused(fw_wigner* h, const KArgs& base0, const double* times, int steps, double* series) {
+  KArgs base = base0;
+  FWCHK(ensure_tables(h, 0, base));
+  FWCHK(set_scalar(h, fw::SC_STOP, 0.0));
+  FWCHK(set_scalar(h, fw::SC_FRESH, 1.0));
+  const bool timing = h->timing;
+  if (!timing && steps > kChunk) FWCHK(ensure_graph_fused(h, base));
+  std::vector<double> slots((size_t)FW_NSER * (kChunk + 1));
+  std::vector<hipEvent_t> evs;
+  double acc[2] = {0, 0};
+  int nacc = 0;
+  for (int off = 0; off < steps; off += kChunk) {
+    const int n = std::min(kChunk, steps - off);
+    const int nt = std::min(kChunk + 1, steps - off);   // t of steps off .. off+n (the next chunk's first)
+    HIPCHK(hipMemcpyAsync(h->times, times + off, sizeof(double) * nt, hipMemcpyHostToDevice, h->stream));
+    if (off == 0) {
+      KArgs a = base;
+      a.step = 0;
+      a.flags |= fw::F_FIRST;
+      FWCHK(launch(h, h->f_v_first, h->col_nb, h->col_nt, a));
+    }
+    const bool has_last = off + n == steps;
+    if (!timing && !has_last && n == kChunk) {
+      HIPCHK(hipGraphLaunch(h->gexec2, h->stream));
+    } else if (timing) {
+      evs.resize(3 * n);
+      for (auto& e : evs) HIPCHK(hipEventCreate(&e));
+      for (int s = 0; s < n; ++s) FWCHK(enqueue_fused(h, s, has_last && s == n - 1, base, &evs[3 * s]));
+      HIPCHK(hipStreamSynchronize(h->stream));
+      for (int s = 0; s < n; ++s) {
+        float ms0 = 0, ms1 = 0;
+        HIPCHK(hipEventElapsedTime(&ms0, evs[3 * s + 1], evs[3 * s + 2]));
+        HIPCHK(hipEventElapsedTime(&ms1, evs[3 * s], evs[3 * s + 1]));
+        if (!(has_last && s == n - 1)) {
+          acc[0] += ms0 * 1000.0;
+          acc[1] += ms1 * 1000.0;
+          ++nacc;
+        }
+      }
+      for (auto& e : evs) (void)hipEventDestroy(e);
+    } else {
+      for (int s = 0; s < n; ++s) FWCHK(enqueue_fused(h, s, has_last && s == n - 1, base, nullptr));
+    }
+    if (series) {
+      HIPCHK(hipMemcpyAsync(slots.data(), h->series, sizeof(double) * FW_NSER * n, hipMemcpyDeviceToHost,
+                            h->stream));
+      HIPCHK(hipStreamSynchronize(h->stream));
+      for (int s = 0; s < n; ++s) {
+        const int g = off + s - 1;
+        if (g >= 0) std::memcpy(series + (size_t)g * FW_NSER, &slots[(size_t)s * FW_NSER], sizeof(double) * FW_NSER);
+      }
+    }
+  }
+  if (nacc) {
+    h->ktimes[0] = acc[0] / nacc;
+    h->ktimes[1] = acc[1] / nacc;
+    h->ktimes[2] = 0.0;
+  }
+  KArgs a = base;
+  a.flags |= fw::F_FROM_PARTIALS;
+  a.step = kChunk;
+  FWCHK(launch(h, h->f_apply_norm, 1024, 256, a));
+  FWCHK(set_scalar(h, fw::SC_NORM, 1.0));
+  if (series)
+    HIPCHK(hipMemcpyAsync(series + (size_t)(steps - 1) * FW_NSER, h->series + (size_t)kChunk * FW_NSER,
+                          sizeof(double) * FW_NSER, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->last_passes = 2;
+  return FW_OK;
+}
+
 int ensure_kset(fw_wigner* h, int kset) {
   hipModule_t* m = kset == 2 ? &h->mod_bloch : &h->mod_dissip;
   if (*m) return FW_OK;
@@ -520,7 +632,8 @@ extern "C" int fw_wigner_create(const fw_wigner_desc* desc, const fw_wigner_expr
              {"fw_k", &h->f_k},               {"fw_moments", &h->f_moments},
              {"fw_apply_norm", &h->f_apply_norm}, {"fw_set_scalar", &h->f_set_scalar},
              {"fw_vk_min", &h->f_vk_min},   {"fw_build_vtab", &h->f_build_vtab},
-             {"fw_build_ktab", &h->f_build_ktab}};
+             {"fw_build_ktab", &h->f_build_ktab}, {"fw_vv", &h->f_vv},
+             {"fw_k_norm", &h->f_k_norm}};
   for (auto& f : fns)
     if ((rc = getfn(h->mod, f.n, f.f)) != FW_OK) return bail(rc);
 
@@ -532,7 +645,7 @@ extern "C" int fw_wigner_create(const fw_wigner_desc* desc, const fw_wigner_expr
   const size_t npts = (size_t)h->NX * h->NP;
   const int maxnb = std::max(std::max(h->col_nb, h->row_nb), 1024);
   if ((rc = alloc(&h->W, npts * h->esz)) != FW_OK) return bail(rc);
-  if ((rc = alloc((void**)&h->times, sizeof(double) * kChunk)) != FW_OK) return bail(rc);
+  if ((rc = alloc((void**)&h->times, sizeof(double) * (kChunk + 1))) != FW_OK) return bail(rc);
   if ((rc = alloc((void**)&h->scal, sizeof(double) * fw::NSCAL)) != FW_OK) return bail(rc);
   if ((rc = alloc((void**)&h->partials, sizeof(double) * FW_NSER * maxnb)) != FW_OK) return bail(rc);
   if ((rc = alloc((void**)&h->counter, sizeof(unsigned int) * 4)) != FW_OK) return bail(rc);
@@ -550,6 +663,7 @@ extern "C" int fw_wigner_destroy(fw_wigner* h) {
   if (!h) return FW_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+  if (h->gexec2) (void)hipGraphExecDestroy(h->gexec2);
   for (auto* m : {&h->init_mods, &h->avg_mods})
     for (auto& kv : *m)
       if (kv.second.mod) (void)hipModuleUnload(kv.second.mod);
@@ -662,6 +776,8 @@ extern "C" int fw_wigner_propagate(fw_wigner* h, const double* times, int steps,
   if (steps < 0) return fw::fail(FW_ERR_INVALID_ARG, "steps must be >= 0");
   if (steps == 0) return FW_OK;
   KArgs a = base_args(h);
+  if (!h->dissip && h->d.step_passes != 3) return run_steps_fused(h, a, times, steps, series);
+  h->last_passes = 3;
   return run_steps(h, 0, a, times, steps, series, nullptr);
 }
 
@@ -794,6 +910,12 @@ extern "C" int fw_wigner_snapshot(fw_wigner* h, int which) {
 extern "C" int fw_wigner_set_timing(fw_wigner* h, int enable) {
   if (!h) return fw::fail(FW_ERR_INVALID_ARG, "NULL handle");
   h->timing = enable != 0;
+  return FW_OK;
+}
+
+extern "C" int fw_wigner_step_passes(fw_wigner* h, int* passes) {
+  if (!h || !passes) return fw::fail(FW_ERR_INVALID_ARG, "NULL argument");
+  *passes = h->last_passes;
   return FW_OK;
 }
 

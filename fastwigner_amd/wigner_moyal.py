@@ -18,6 +18,7 @@ Additional keyword arguments:
                       0.0 the FFTW class, -0.5 the midpoint rule
     alpha             the FFTW class's absorbing boundary (wigner_moyal_fftw_1d.py:135-156),
                       translated into an abs_boundary_x_theta expression
+    step_passes       0 (default) or 2: Moyal steps in two fused passes; 3: three passes
     phase_tables      1 / 0 / -1: cache the V and K factors in tables always /
                       when the strings do not reference t (default) / never
     device, tiles     HIP device ordinal; (col_e, col_p, row_e, row_p) FFT tiles
@@ -120,7 +121,7 @@ class WignerMoyalHIP1D:
         tiles = getattr(self, "tiles", None) or tuning.lookup("wigner", self.X_gridDIM, self.P_gridDIM, self._prec)
         desc = _lib.wigner_desc(self.X_gridDIM, self.X_amplitude, self.P_gridDIM, self.P_amplitude, self.dt,
                                 self.phase_time_shift, self._prec, self.device, tiles,
-                                int(getattr(self, "phase_tables", 0)))
+                                int(getattr(self, "phase_tables", 0)), int(getattr(self, "step_passes", 0)))
         ex = _lib.wigner_exprs(
             V=self.V, K=self.K, consts=self.hip_consts,
             diff_V=getattr(self, "diff_V", None) if isinstance(getattr(self, "diff_V", None), str) else None,

@@ -77,6 +77,8 @@ typedef struct fw_wigner_desc {
   int phase_tables;        /* cache the V/K factors in tables (as the FFTW class caches _expV/_expK,
                               wigner_moyal_fftw_1d.py:164-213): 0 = when the V (K) string and its
                               mask do not contain the token t, 1 = always, -1 = never */
+  int step_passes;         /* Moyal steps without the dissipator: 0 or 2 = two passes per step (the
+                              closing V/2 of a step fused with the opening V/2 of the next), 3 = three */
 } fw_wigner_desc;
 
 typedef struct fw_wigner_exprs {
@@ -149,11 +151,14 @@ int fw_wigner_set_dissipator(fw_wigner* h, const double* host_w0s, double w_coef
  * (wigner_bloch_cuda_1d.py:186,209,218): which = 0 save, 1 restore. */
 int fw_wigner_snapshot(fw_wigner* h, int which);
 
-/* Mean device time of the three step kernels over the last propagate call, in
- * microseconds (v_first, k, v_last), measured with HIP events on the handle's
- * stream when timing is enabled. */
+/* Mean device time of the step kernels over the last propagate call, in
+ * microseconds, measured with HIP events on the handle's stream when timing is
+ * enabled: (v_first, k, v_last) for three-pass steps, (vv, k_norm, 0) for
+ * two-pass steps (fw_wigner_step_passes tells which ran). */
 int fw_wigner_set_timing(fw_wigner* h, int enable);
 int fw_wigner_kernel_times(fw_wigner* h, double us_out[3]);
+/* Passes per step of the last Moyal propagate call (2 or 3). */
+int fw_wigner_step_passes(fw_wigner* h, int* passes);
 
 /* ------------------------------------------------------------------------ */
 /* Density-matrix propagators (von Neumann, Bloch)                           */

@@ -50,12 +50,13 @@ def gaussian(o, x0=1.0, p0=0.5):
     return np.exp(-(o.X - x0) ** 2 - (o.P - p0) ** 2) / np.pi
 
 
-@pytest.mark.parametrize("tables", [0, -1])
-def test_config1_parity_fp64(gpu, tables):
+@pytest.mark.parametrize("tables,passes", [(0, 0), (-1, 0), (0, 3)])
+def test_config1_parity_fp64(gpu, tables, passes):
     """Config 1: 256^2, x,p in [-8,8), V=0.5x^2+0.1x^4, dt=0.01, 1000 steps,
     <x>,<p>,<H>, purity logged each step; with cached factor tables (default for
-    time-independent V, K) and with the factors evaluated on the fly."""
-    g, o = make_pair(256, 8., 256, 8., 0.01, V_Q, K_Q, DV_Q, DK_Q, phase_tables=tables)
+    time-independent V, K) and with the factors evaluated on the fly; in two
+    fused passes per step (default) and in three."""
+    g, o = make_pair(256, 8., 256, 8., 0.01, V_Q, K_Q, DV_Q, DK_Q, phase_tables=tables, step_passes=passes)
     W0 = gaussian(o)
     g.set_wignerfunction(W0)
     o.set_wignerfunction(W0)
@@ -73,6 +74,11 @@ def test_config1_parity_fp64(gpu, tables):
     assert np.max(np.abs(np.array(g.hamiltonian_average) - oh)) <= 1e-11 * np.max(np.abs(oh))
     assert np.max(np.abs(np.array(g.purity_series) - opu)) <= 1e-11
     assert abs(g.t - o.t) == 0.0
+    from fastwigner_amd import _lib
+    import ctypes
+    n = ctypes.c_int()
+    _lib.call("fw_wigner_step_passes", g._handle, ctypes.byref(n))
+    assert n.value == (3 if passes == 3 else 2)
 
 
 def test_harmonic_closed_form_gpu(gpu):
