@@ -25,10 +25,9 @@ PHYS = dict(X_amplitude=8., P_amplitude=8., dt=0.01, t=0., V="0.5*X*X+0.1*X*X*X*
 
 
 def twiddle_entries(n, fp32):
-    """LDS twiddle entries (16 bytes each): the full padded table in FP32, the
-    two-level table (n/S + S, S = min(64, n)) in FP64."""
-    s = min(64, n)
-    return n * 17 // 16 + 1 if fp32 else n // s + s
+    """LDS twiddle entries (16 bytes each): the full padded table in FP32; none
+    in FP64, whose twiddles are computed in the kernel."""
+    return n * 17 // 16 + 1 if fp32 else 1
 
 
 def candidates(n_line, n_lines_pairs, esz_c, fp32, max_threads=1024, max_lds=160 * 1024):

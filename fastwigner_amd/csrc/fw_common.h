@@ -215,7 +215,7 @@ template <int N> struct TwLds { static constexpr int SIZE = N + N / 16 + 1; };
 // entries (1.5 KiB at TWN = 2048 instead of 32 KiB) for one more complex
 // product per twiddle. Entry e of the staged table is entry src(e) of the
 // global full table, stored at dst(e).
-template <int TWN, bool TWO>
+template <int TWN, int MODE>
 struct TwTab {
   static constexpr int COUNT = TWN;
   static constexpr int SIZE = TWN + TWN / 16 + 1;
@@ -224,8 +224,21 @@ struct TwTab {
   FWD static tw_t get(const tw_t* tw, int idx) { return tw[twpad(idx)]; }
 };
 #if !FW_FP32
+// MODE 2 (FP64): no table; w_idx = exp(-2 pi i idx / TWN) by sincospi
 template <int TWN>
-struct TwTab<TWN, true> {
+struct TwTab<TWN, 2> {
+  static constexpr int COUNT = 0;
+  static constexpr int SIZE = 1;
+  FWD static int src(int e) { return e; }
+  FWD static int dst(int e) { return e; }
+  FWD static tw_t get(const tw_t*, int idx) {
+    double sn, cs;
+    sincospi(-2.0 * (double)idx / (double)TWN, &sn, &cs);
+    return mk(cs, sn);
+  }
+};
+template <int TWN>
+struct TwTab<TWN, 1> {
   static constexpr int S = TWN < 64 ? TWN : 64;
   static constexpr int NA = TWN / S;
   static constexpr int COUNT = NA + S;
@@ -240,7 +253,7 @@ struct TwTab<TWN, true> {
 // early) and stores them into LDS (before the first barrier of the FFT).
 template <class T, int NT>
 struct TwStage {
-  static constexpr int R = (T::COUNT + NT - 1) / NT;
+  static constexpr int R = (T::COUNT + NT - 1) / NT > 0 ? (T::COUNT + NT - 1) / NT : 1;
   tw_t r[R];
   FWD void load(const tw_t* __restrict__ g, int tid) {
 #pragma unroll
@@ -266,7 +279,7 @@ struct TwStage {
 // ---------------------------------------------------------------------------
 // TWN: length of the twiddle table passed in (a table for a multiple of N
 // serves N with stride TWN/N; the batched kernels share one table).
-template <int N, int E, int P, bool LM, int TWN = N, bool TWO = false>
+template <int N, int E, int P, bool LM, int TWN = N, int TWO = 0>
 struct Line {
   using TW = TwTab<TWN, TWO>;
   static_assert((N & (N - 1)) == 0 && N >= 2, "line length must be a power of two");
@@ -308,8 +321,17 @@ struct Line {
       for (int m = 0; m < R; ++m) a[m] = v[b + m * B];
       if constexpr (NS > 1) {
         const int k = j & (NS - 1);
+        if constexpr (TWO == 2) {
+          cpx w[R];
+          w[1] = TW::get(tw, k * TWS);
 #pragma unroll
-        for (int m = 1; m < R; ++m) a[m] = twmul(a[m], TW::get(tw, m * k * TWS));
+          for (int m = 2; m < R; ++m) w[m] = cmul(w[m / 2], w[m - m / 2]);
+#pragma unroll
+          for (int m = 1; m < R; ++m) a[m] = cmul(a[m], w[m]);
+        } else {
+#pragma unroll
+          for (int m = 1; m < R; ++m) a[m] = twmul(a[m], TW::get(tw, m * k * TWS));
+        }
       }
       Dft<R>::run(a);
       if constexpr (TOREG) {

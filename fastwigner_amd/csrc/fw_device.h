@@ -165,12 +165,24 @@ FWD bool previous_step(const Args& a, int blk, double* red, double& nrm) {
 // ---------------------------------------------------------------------------
 // V pass: columns (along p), lines = pairs of adjacent columns.
 // ---------------------------------------------------------------------------
-// Two-level twiddle tables (TwTab<., true>) shrink the LDS footprint but cost a
-// second LDS read and a product per twiddle; measured slower on gfx950 for
-// these passes, which are LDS-bandwidth bound, so the full table is used.
-#define FW_TW2 0
+// Twiddles of the FP64 steps are computed, not staged (TwTab mode 2: one FP64
+// sincospi per butterfly, the powers by products): no table in LDS (32 KiB per
+// block at N = 2048), no table loads, twice the blocks per CU; measured 20-27%
+// faster per pass than the staged full table (mode 0) or the two-level table
+// (mode 1) on gfx950. The FP32 form keeps the full compensated table.
+#ifndef FW_TW2
+#define FW_TW2 (FW_FP32 ? 0 : 2)
+#endif
 using VL = Line<FW_NP, FW_COL_E, FW_COL_P, false, FW_NP, FW_TW2>;
 using KL = Line<FW_NX, FW_ROW_E, FW_ROW_P, true, FW_NX, FW_TW2>;
+
+// The V factor table is stored by column pair: entry (k, x) at
+// ((x/2)(N_p/2+1) + k) 2 + x%2, so a column pass reads each pair's N_p/2+1
+// factors contiguously (row-major storage would make these reads as
+// fragmented as the column reads of W).
+FWD size_t vtab_index(int x, int k) {
+  return ((size_t)(x >> 1) * (FW_NP / 2 + 1) + k) * 2 + (x & 1);
+}
 
 // Multiply the packed spectrum of column pair (Xa, Xb) held in LDS by the V
 // factors at time tau (masks at tnow), from the table row cache ftab or
@@ -242,8 +254,8 @@ FWD void vpass(const Args& a) {
   if constexpr (use_tab) {
 #pragma unroll
     for (int e = 0; e < E / 2; ++e)
-      ftab[e] = *reinterpret_cast<const cpx2*>(a.vtab + (size_t)(t + TPL * e) * FW_NX + c);
-    ftab[E / 2] = *reinterpret_cast<const cpx2*>(a.vtab + (size_t)(N / 2) * FW_NX + c);
+      ftab[e] = *reinterpret_cast<const cpx2*>(a.vtab + vtab_index(c, t + TPL * e));
+    ftab[E / 2] = *reinterpret_cast<const cpx2*>(a.vtab + vtab_index(c, N / 2));
   }
   tws.store(twl, tid);
 
@@ -352,8 +364,8 @@ FWD void vvpass(const Args& a) {
   if constexpr (use_tab) {
 #pragma unroll
     for (int e = 0; e < E / 2; ++e)
-      ftab[e] = *reinterpret_cast<const cpx2*>(a.vtab + (size_t)(t + TPL * e) * FW_NX + c);
-    ftab[E / 2] = *reinterpret_cast<const cpx2*>(a.vtab + (size_t)(N / 2) * FW_NX + c);
+      ftab[e] = *reinterpret_cast<const cpx2*>(a.vtab + vtab_index(c, t + TPL * e));
+    ftab[E / 2] = *reinterpret_cast<const cpx2*>(a.vtab + vtab_index(c, N / 2));
   }
   tws.store(twl, tid);
   // the K pass of this step has read `fresh`; the next K pass uses the partials
@@ -584,7 +596,7 @@ FWD void build_vtab(const Args& a, cpx* tab) {
   for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
     const int k = (int)(idx / FW_NX), j = (int)(idx % FW_NX);
     const double X = ::dX * (j - 0.5 * ::X_gridDIM), Th = ::dTheta * k;
-    tab[idx] = (a.flags & F_BLOCH) ? bloch_vfactor(a, X, Th) : vfactor<true>(X, Th, tau, tnow);
+    tab[vtab_index(j, k)] = (a.flags & F_BLOCH) ? bloch_vfactor(a, X, Th) : vfactor<true>(X, Th, tau, tnow);
   }
 }
 FWD void build_ktab(const Args& a, cpx* tab) {

@@ -161,11 +161,8 @@ int resolve_tiles(const fw_wigner_desc* d, Tiles& t) {
   const size_t lds_c = (size_t)t.col_p * d->p_grid_dim, lds_r = (size_t)t.row_p * d->x_grid_dim;
   // gfx950: 160 KiB of LDS per workgroup (exchange buffer + twiddle table + reduction scratch)
   const size_t tws = 16;
-  // FP64: two-level twiddle tables (N/S + S entries, S = min(64, N)); FP32: the full padded table
-  auto tw_entries = [&](int n) -> size_t {
-    const int sq = n < 64 ? n : 64;
-    return fp32 ? (size_t)n * 17 / 16 + 1 : (size_t)(n / sq + sq);
-  };
+  // FP64: twiddles computed in the kernel (no LDS table); FP32: the full padded table
+  auto tw_entries = [&](int n) -> size_t { return fp32 ? (size_t)n * 17 / 16 + 1 : (size_t)1; };
   const size_t twc = tw_entries(d->p_grid_dim), twr = tw_entries(d->x_grid_dim);
   if ((lds_c + lds_c / 16 + 1) * esz + twc * tws + 1152 > 160 * 1024 ||
       (lds_r + lds_r / 16 + 1) * esz + twr * tws > 160 * 1024)
