@@ -147,10 +147,12 @@ def batched_variant(steps=500, o_workers=1, with_oracle=True):
     # one step moves W in three passes plus the dissipator's two reads of W0s: 32 B per point (FP32)
     us = b.step_time_us(640)
     bytes_step = 32 * b.total_points
-    out["roofline"] = {"bound": "hbm", "kernel": "fw_batch_step", "achieved": round(bytes_step / (us * 1e-6) / 1e9, 1),
+    out["roofline"] = {"bound": "hbm", "kernel": "fw_batch_step (one step: %s)" % (
+                           "one persistent launch" if b.mode == 0 else "three launches, one per pass"),
+                       "achieved": round(bytes_step / (us * 1e-6) / 1e9, 1),
                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": round(bytes_step / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
-                       "bytes_per_launch": bytes_step, "mean_us": round(us, 3)}
+                       "bytes_per_step": bytes_step, "mean_us_per_step": round(us, 3)}
     print("[bench] batched fp32: %.3e pt*steps/s, %.1f us/step (device %.1f us)" % (rate, el * 1e6 / steps, us),
           file=sys.stderr, flush=True)
     if with_oracle:

@@ -6,8 +6,8 @@ wigner_random_collision_model_fftw_1d.py:16-94 with its Gibbs state from
 WignerBlochFFTW1D, wigner_bloch_fftw_1d.py:27-109). ``WignerBatchHIP1D`` holds
 hundreds of such propagators with different square grids, potential
 parameters, collision rates and temperatures, and advances all of them with
-one persistent launch per step (or one per pass), replayed from a hipGraph per
-64 steps.
+one launch per pass (mode 1, the default) or one persistent launch per step
+(mode 0), replayed from a hipGraph per 64 steps.
 
 Each instance is exactly the single-grid step: three fused passes (V columns,
 K rows, V columns), the random-collision dissipator W <- w W + (1-w) W_Gibbs

@@ -295,8 +295,9 @@ int fw_batch_bloch_propagate(fw_batch* h, int convention, double t_initial, cons
 int fw_batch_propagate(fw_batch* h, const double* times, int steps, double* series);
 /* total grid points, tasks per step, persistent grid size */
 int fw_batch_info(fw_batch* h, long long* points, int* tasks_per_step, int* grid);
-/* Device time of one step (us), averaged over `steps` steps replayed from the graph between
- * HIP events on the handle's stream. The state advances by `steps` steps. */
+/* Device time of one step (us), averaged over the floor(steps/64)*64 steps replayed from the
+ * graph between HIP events on the handle's stream (steps >= 64). The state advances by that
+ * many steps, with V and K evaluated at times (s+1)*dt of each 64-step chunk. */
 int fw_batch_step_time(fw_batch* h, int steps, double* us_per_step);
 
 /* Device address of instance i's W (N*N elements of elem_size bytes, row-major), valid until
