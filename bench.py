@@ -120,7 +120,8 @@ def batched_variant(steps=500, o_workers=1, with_oracle=True):
     {128,256,512,1024}^2 (rng seed 4), quartic potentials, Gaussian initial
     states, gamma ~ U[0, 0.1], kT ~ U[0.1, 1] (Gibbs states from the batched
     Bloch run), FP32 with FP64 phases and sums, dt = 0.01, `steps` steps with
-    one persistent launch per step replayed from a hipGraph. Rate in total
+    one launch per pass over all instances, replayed from a hipGraph per 64
+    steps. Rate in total
     grid-point*steps/s; error of the three smallest instances against the FP64
     oracle."""
     from fastwigner_amd.batch import WignerBatchHIP1D, gaussian_state, random_quartic_instances, QUARTIC_V, \

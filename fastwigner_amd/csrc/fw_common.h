@@ -390,6 +390,47 @@ FWD void sincos_quarter(float x, float& s, float& c) {
 }
 #endif
 
+// FP64 sine and cosine of the phase factors, branch-free so that the factors
+// of a thread interleave. Cody-Waite reduction by pi/2 with a two-term split
+// (C1 = pi/2 rounded to double, C2 = pi/2 - C1 rounded): for |x| >= 1 every bit
+// of x - k C1 is at least 2^-52 and the difference is below 1, so the first
+// FMA is exact; the second rounds once; the neglected k (pi/2 - C1 - C2) is
+// below 1.5e-33 |k|, i.e. under 1e-17 for |x| < 2^42. The reduced
+// |r| <= pi/4 + 2^-50 goes through the Taylor series of sin to r^17 and of cos
+// to r^18 (truncation below 1e-19 at pi/4), then the quadrant rotation.
+// Accuracy: about 1 ulp for |x| < 2^42 (phases in radians); beyond that the
+// neglected term grows as |x| 1e-33.
+FWD void sincos_fp64(double x, double& s, double& c) {
+  const double k = rint(x * 0.63661977236758134308);
+  const double C1 = 1.5707963267948966192e+00;
+  const double C2 = 6.1232339957367658e-17;
+  double r = fma(-k, C1, x);
+  r = fma(-k, C2, r);
+  const double r2 = r * r;
+  double ps = -1.0 / 355687428096000.0;                 // -1/17!
+  ps = fma(ps, r2, 1.0 / 1307674368000.0);              // 1/15!
+  ps = fma(ps, r2, -1.0 / 6227020800.0);                // -1/13!
+  ps = fma(ps, r2, 1.0 / 39916800.0);                   // 1/11!
+  ps = fma(ps, r2, -1.0 / 362880.0);                    // -1/9!
+  ps = fma(ps, r2, 1.0 / 5040.0);                       // 1/7!
+  ps = fma(ps, r2, -1.0 / 120.0);                       // -1/5!
+  ps = fma(ps, r2, 1.0 / 6.0);                          // 1/3!
+  const double sn = fma(-r * r2, ps, r);                // r - r^3/3! + ...
+  double pc = 1.0 / 6402373705728000.0;                 // 1/18!
+  pc = fma(pc, r2, -1.0 / 20922789888000.0);            // -1/16!
+  pc = fma(pc, r2, 1.0 / 87178291200.0);                // 1/14!
+  pc = fma(pc, r2, -1.0 / 479001600.0);                 // -1/12!
+  pc = fma(pc, r2, 1.0 / 3628800.0);                    // 1/10!
+  pc = fma(pc, r2, -1.0 / 40320.0);                     // -1/8!
+  pc = fma(pc, r2, 1.0 / 720.0);                        // 1/6!
+  pc = fma(pc, r2, -1.0 / 24.0);                        // -1/4!
+  pc = fma(pc, r2, 0.5);                                // 1/2!
+  const double cs = fma(-r2, pc, 1.0);                  // 1 - r^2/2! + ...
+  const int q = (int)((long long)k & 3);
+  c = (q == 0) ? cs : (q == 1) ? -sn : (q == 2) ? -cs : sn;
+  s = (q == 0) ? sn : (q == 1) ? cs : (q == 2) ? -sn : -cs;
+}
+
 FWD cpx unit_phase(double ph, double m) {
 #if FW_FP32
   // FP32 form: the argument and its reduction to [-pi/4, pi/4] (quadrant q) in
@@ -410,16 +451,16 @@ FWD cpx unit_phase(double ph, double m) {
   return mk(cq * mf, sq * mf);
 #else
   double s, c;
-  sincos(ph, &s, &c);
+  sincos_fp64(ph, s, c);
   return mk(c * m, s * m);
 #endif
 }
 
 // Table entries are built once, so they take the FP64 sincos in both forms and
-// are rounded once to the storage precision.
+// are rounded once to the storage precision (in FP64 they equal unit_phase).
 FWD cpx unit_phase_exact(double ph, double m) {
   double s, c;
-  sincos(ph, &s, &c);
+  sincos_fp64(ph, s, c);
   return mk((real_t)(c * m), (real_t)(s * m));
 }
 
@@ -431,6 +472,31 @@ FWD double wave_sum(double v) {
   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
 }
+
+// Block reductions over the NT threads of a block. NT >= 64: each wave adds
+// its lanes with cross-lane reads and lane 0 stores one slot per wave. NT < 64:
+// the block is one partial wave whose missing lanes hold undefined data that a
+// cross-lane read would pick up, so every thread stores its own slot. After a
+// barrier, total() adds the RN slots of quantity i in order (the same bits in
+// every block). red: >= ns * RN doubles.
+template <int NT>
+struct BlockRed {
+  static constexpr int RN = NT >= 64 ? NT / 64 : NT;
+  FWD static void put(double v, double* red, int ns, int i) {
+    if constexpr (NT >= 64) {
+      v = wave_sum(v);
+      if ((threadIdx.x & 63) == 0) red[(threadIdx.x >> 6) * ns + i] = v;
+    } else {
+      red[threadIdx.x * ns + i] = v;
+    }
+  }
+  FWD static double total(const double* red, int ns, int i) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < RN; ++w) t += red[w * ns + i];
+    return t;
+  }
+};
 
 // Block-reduce NSER sums into partials[blk], then the last block to finish adds
 // the partials of all blocks (every thread takes a strided share, Neumaier-
@@ -501,9 +567,9 @@ FWD bool block_reduce_publish(double (&s)[NSER], double* red, double* partials, 
 // adds the partials itself: every block takes the same strided order and the
 // same tree, so every block obtains the same bits.
 // ---------------------------------------------------------------------------
+// red: >= BlockRed<NT>::RN doubles.
 template <int NT, int NS = NSER>
 FWD double sum_partials(const double* __restrict__ partials, int nb, int i, double* red) {
-  constexpr int NW = (NT + 63) / 64;
   double sum = 0.0, comp = 0.0;
   for (int b = threadIdx.x; b < nb; b += NT) {
     const double x = partials[(size_t)b * NS + i];
@@ -511,31 +577,20 @@ FWD double sum_partials(const double* __restrict__ partials, int nb, int i, doub
     comp += (fabs(sum) >= fabs(x)) ? (sum - tt) + x : (x - tt) + sum;
     sum = tt;
   }
-  double v = wave_sum(sum + comp);
   __syncthreads();
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  BlockRed<NT>::put(sum + comp, red, 1, 0);
   __syncthreads();
-  double tot = 0.0;
-#pragma unroll
-  for (int w = 0; w < NW; ++w) tot += red[w];
-  return tot;
+  return BlockRed<NT>::total(red, 1, 0);
 }
 
-// Block sums of the last V pass -> partials[blk] (plain stores).
+// Block sums of the last V pass -> partials[blk] (plain stores; blocks with
+// fewer threads than sums store several). red: >= NS * BlockRed<NT>::RN doubles.
 template <int NT, int NS = NSER>
 FWD void write_partials(double (&s)[NS], double* red, double* partials, int blk) {
-  constexpr int NW = (NT + 63) / 64;
 #pragma unroll
-  for (int i = 0; i < NS; ++i) {
-    const double v = wave_sum(s[i]);
-    if ((threadIdx.x & 63) == 0) red[(threadIdx.x >> 6) * NS + i] = v;
-  }
+  for (int i = 0; i < NS; ++i) BlockRed<NT>::put(s[i], red, NS, i);
   __syncthreads();
-  if (threadIdx.x < NS) {
-    double acc = 0.0;
-    for (int w = 0; w < NW; ++w) acc += red[w * NS + threadIdx.x];
-    partials[(size_t)blk * NS + threadIdx.x] = acc;
-  }
+  for (int i = threadIdx.x; i < NS; i += NT) partials[(size_t)blk * NS + i] = BlockRed<NT>::total(red, NS, i);
 }
 
 FWD double wave_min(double v) {

@@ -136,30 +136,79 @@ FWD cpx self_apply(cpx z, cpx fa, cpx fb, real_t s) {
   return mk(z.x * fa.x * s, z.y * fb.x * s);
 }
 
-// Norm factor of the previous step; block 0 also publishes its NSER sums to
-// series[slot] and raises the purity guard. Returns false when the step must
-// not run (guard tripped).
+// Partial sums of the column passes of Wigner steps are stored by sum: sum i
+// of block b at partials[i * nbp + b]. The S0 row, which every block of the
+// next pass adds for the deferred norm, is then nbp contiguous doubles (a
+// block-major layout makes each block read all NSER * nbp of them).
+// red: >= NSER * BlockRed<NT>::RN doubles.
 template <int NT>
-FWD bool previous_step(const Args& a, int blk, double* red, double& nrm) {
-  const double dxdp = ::dX * ::dP;
-  const double S0 = sum_partials<NT>(a.partials, a.nbp, 0, red);
-  nrm = 1.0 / (S0 * dxdp);
-  bool stop = false;
-  if (a.guard > 0.0) {
-    const double S2 = sum_partials<NT>(a.partials, a.nbp, 8, red);
-    stop = !(2.0 * M_PI * S2 / (S0 * S0 * dxdp) < a.guard);
-  }
-  if (blk == 0) {
-    double tot[NSER];
+FWD void write_partial_rows(double (&s)[NSER], double* red, double* partials, int blk, int nbp) {
 #pragma unroll
-    for (int i = 0; i < NSER; ++i) tot[i] = (i == 0) ? S0 : sum_partials<NT>(a.partials, a.nbp, i, red);
-    if (threadIdx.x == 0) {
+  for (int i = 0; i < NSER; ++i) BlockRed<NT>::put(s[i], red, NSER, i);
+  __syncthreads();
+  for (int i = threadIdx.x; i < NSER; i += NT) partials[(size_t)i * nbp + blk] = BlockRed<NT>::total(red, NSER, i);
+}
+
+// The S0 row of the previous column pass, loaded with the pass's own global
+// loads and added in sum_partials' order (b = tid, tid + NT, ..., compensated,
+// then waves in order), so every block obtains the same bits.
+template <int NT>
+struct NormRow {
+  static constexpr int NBP = FW_NX / 2 / FW_COL_P;  // blocks of the column pass
+  static constexpr int PL = (NBP + NT - 1) / NT;
+  double x[PL];
+  FWD void load(const double* __restrict__ partials) {
 #pragma unroll
-      for (int i = 0; i < NSER; ++i) a.series[(size_t)a.step * NSER + i] = tot[i];
-      if (stop) a.scal[SC_STOP] = (double)(a.step + 1);
+    for (int j = 0; j < PL; ++j) {
+      const int b = threadIdx.x + j * NT;
+      x[j] = (b < NBP) ? partials[b] : 0.0;
     }
   }
-  return !stop;
+  // block-reduction slots into red[0..BlockRed<NT>::RN); a barrier must
+  // separate this from total()
+  FWD void reduce_waves(double* red) const {
+    double sum = 0.0, comp = 0.0;
+#pragma unroll
+    for (int j = 0; j < PL; ++j) {
+      if (threadIdx.x + j * NT < NBP) {
+        const double tt = sum + x[j];
+        comp += (fabs(sum) >= fabs(x[j])) ? (sum - tt) + x[j] : (x[j] - tt) + sum;
+        sum = tt;
+      }
+    }
+    BlockRed<NT>::put(sum + comp, red, 1, 0);
+  }
+  FWD static double total(const double* red) { return BlockRed<NT>::total(red, 1, 0); }
+};
+
+// Block b of the nb blocks of a pass adds rows b, b + nb, ... (< NSER) of the
+// previous column pass's partials and publishes them to series[step] (called
+// by whole blocks, after their own work).
+template <int NT>
+FWD void publish_sums(const Args& a, int blk, int nb, double* red) {
+  for (int i = blk; i < NSER; i += nb) {
+    const double tot = sum_partials<NT, 1>(a.partials + (size_t)i * a.nbp, a.nbp, 0, red);
+    if (threadIdx.x == 0) a.series[(size_t)a.step * NSER + i] = tot;
+  }
+}
+
+// Deferred norm of the previous step from its S0 (preloaded in a NormRow whose
+// reduction slots are in red, and a barrier passed). With the purity guard
+// armed, a tripped guard raises SC_STOP, publishes the sums and returns false.
+template <int NT>
+FWD bool previous_norm(const Args& a, int blk, double* red, double& nrm) {
+  const double dxdp = ::dX * ::dP;
+  const double S0 = NormRow<NT>::total(red);
+  nrm = 1.0 / (S0 * dxdp);
+  if (a.guard > 0.0) {
+    const double S2 = sum_partials<NT, 1>(a.partials + (size_t)8 * a.nbp, a.nbp, 0, red);
+    if (!(2.0 * M_PI * S2 / (S0 * S0 * dxdp) < a.guard)) {
+      if (blk == 0 && threadIdx.x == 0) a.scal[SC_STOP] = (double)(a.step + 1);
+      publish_sums<NT>(a, blk, gridDim.x, red);
+      return false;
+    }
+  }
+  return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -182,6 +231,17 @@ using KL = Line<FW_NX, FW_ROW_E, FW_ROW_P, true, FW_NX, FW_TW2>;
 // fragmented as the column reads of W).
 FWD size_t vtab_index(int x, int k) {
   return ((size_t)(x >> 1) * (FW_NP / 2 + 1) + k) * 2 + (x & 1);
+}
+
+// V factors of column pair (c, c+1) that a thread of a column pass applies,
+// from the table: slot e < E/2 for theta index k = t + TPL e, slot E/2 for
+// k = N/2 (used by t = 0 only).
+FWD void v_factors(const Args& a, int c, int t, cpx2 (&ftab)[FW_COL_E / 2 + 1]) {
+  constexpr int N = FW_NP, E = FW_COL_E, TPL = VL::TPL;
+#pragma unroll
+  for (int e = 0; e < E / 2; ++e)
+    ftab[e] = *reinterpret_cast<const cpx2*>(a.vtab + vtab_index(c, t + TPL * e));
+  ftab[E / 2] = *reinterpret_cast<const cpx2*>(a.vtab + vtab_index(c, N / 2));
 }
 
 // Multiply the packed spectrum of column pair (Xa, Xb) held in LDS by the V
@@ -230,7 +290,7 @@ FWD void vpass(const Args& a) {
   constexpr int N = FW_NP, E = FW_COL_E, P = FW_COL_P, TPL = L::TPL, NT = L::NT;
   __shared__ cpx lds[L::LDS];
   __shared__ tw_t twl[L::TW::SIZE];
-  __shared__ double red[NSER * 16];
+  __shared__ double red[NSER * BlockRed<NT>::RN];
   if (a.scal[SC_STOP] != 0.0) return;
   const int tid = threadIdx.x;
   const int q = tid % P, t = tid / P;
@@ -250,22 +310,19 @@ FWD void vpass(const Args& a) {
   cpx v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = *reinterpret_cast<const cpx*>(W + (size_t)(t + TPL * e) * FW_NX + c);
+  // the previous step's S0 row (deferred norm), issued before the table loads
+  const bool deferred = (FL & F_FIRST) && a.scal[SC_FRESH] == 0.0;
+  NormRow<NT> norm_row;
+  if (deferred) norm_row.load(a.partials);
   cpx2 ftab[E / 2 + 1];
-  if constexpr (use_tab) {
-#pragma unroll
-    for (int e = 0; e < E / 2; ++e)
-      ftab[e] = *reinterpret_cast<const cpx2*>(a.vtab + vtab_index(c, t + TPL * e));
-    ftab[E / 2] = *reinterpret_cast<const cpx2*>(a.vtab + vtab_index(c, N / 2));
-  }
+  if constexpr (use_tab) v_factors(a, c, t, ftab);
   tws.store(twl, tid);
+  if (deferred) norm_row.reduce_waves(red);
+  __syncthreads();
 
   if constexpr (FL & F_FIRST) {
     double nrm = 1.0;
-    if (a.scal[SC_FRESH] == 0.0) {
-      if (!previous_step<NT>(a, blk, red, nrm)) return;
-    } else {
-      __syncthreads();
-    }
+    if (deferred && !previous_norm<NT>(a, blk, red, nrm)) return;
     const real_t rn = (real_t)nrm;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -278,8 +335,6 @@ FWD void vpass(const Args& a) {
       }
       v[e] = w;
     }
-  } else {
-    __syncthreads();
   }
   L::template fwd<true>(v, lds, q, t, twl);
 
@@ -328,8 +383,12 @@ FWD void vpass(const Args& a) {
     sums[3] = Xa * Xa * ca + Xb * Xb * cb;
     sums[6] = ::fw_diff_V(Xa, tnow) * ca + ::fw_diff_V(Xb, tnow) * cb;
     sums[7] += ::V(Xa, tnow) * ca + ::V(Xb, tnow) * cb;
-    write_partials<NT>(sums, red, a.partials, blk);
+    // the previous step's sums were published by the K pass (three-pass steps
+    // open with this pass only after a fresh start or a K pass)
+    write_partial_rows<NT>(sums, red, a.partials, blk, nb);
   }
+  if constexpr (FL & F_FIRST)
+    if (deferred) publish_sums<NT>(a, blk, gridDim.x, red);
 }
 
 
@@ -344,7 +403,7 @@ FWD void vvpass(const Args& a) {
   constexpr int N = FW_NP, E = FW_COL_E, P = FW_COL_P, TPL = L::TPL, NT = L::NT;
   __shared__ cpx lds[L::LDS];
   __shared__ tw_t twl[L::TW::SIZE];
-  __shared__ double red[NSER * 16];
+  __shared__ double red[NSER * BlockRed<NT>::RN];
   if (a.scal[SC_STOP] != 0.0) return;
   const int tid = threadIdx.x;
   const int q = tid % P, t = tid / P;
@@ -352,7 +411,7 @@ FWD void vvpass(const Args& a) {
   const int blk = xcd_remap(blockIdx.x, nb);
   const int c = (blk * P + q) * 2;
   const double t0 = a.times[a.step], t1 = a.times[a.step + 1];
-  constexpr bool use_tab = FW_VTAB;
+  constexpr bool use_tab = FW_VTAB;   // one table read serves both half-steps
   real_t* __restrict__ W = a.W;
 
   TwStage<typename L::TW, NT> tws;
@@ -361,12 +420,7 @@ FWD void vvpass(const Args& a) {
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = *reinterpret_cast<const cpx*>(W + (size_t)(t + TPL * e) * FW_NX + c);
   cpx2 ftab[E / 2 + 1];
-  if constexpr (use_tab) {
-#pragma unroll
-    for (int e = 0; e < E / 2; ++e)
-      ftab[e] = *reinterpret_cast<const cpx2*>(a.vtab + vtab_index(c, t + TPL * e));
-    ftab[E / 2] = *reinterpret_cast<const cpx2*>(a.vtab + vtab_index(c, N / 2));
-  }
+  if constexpr (use_tab) v_factors(a, c, t, ftab);
   tws.store(twl, tid);
   // the K pass of this step has read `fresh`; the next K pass uses the partials
   if (blk == 0 && tid == 0 && a.scal[SC_FRESH] != 0.0) a.scal[SC_FRESH] = 0.0;
@@ -409,7 +463,7 @@ FWD void vvpass(const Args& a) {
   sums[3] = Xa * Xa * ca + Xb * Xb * cb;
   sums[6] = ::fw_diff_V(Xa, t0) * ca + ::fw_diff_V(Xb, t0) * cb;
   sums[7] += ::V(Xa, t0) * ca + ::V(Xb, t0) * cb;
-  write_partials<NT>(sums, red, a.partials, blk);
+  write_partial_rows<NT>(sums, red, a.partials, blk, nb);
   // opening V/2 of step s+1
   L::template fwd<true>(v, lds, q, t, twl);
   v_multiply<use_tab>(lds, q, t, Xa, Xb, t1 + FW_PHASE_SHIFT * ::dt, t1, ftab);
@@ -431,7 +485,7 @@ FWD void kpass(const Args& a) {
   constexpr int KW = FW_NX / 2 + 1;
   __shared__ cpx lds[L::LDS];
   __shared__ tw_t twl[L::TW::SIZE];
-  __shared__ double red[(FL & F_KNORM) ? NSER * 16 : 1];
+  __shared__ double red[(FL & F_KNORM) ? BlockRed<NT>::RN : 1];
   if (a.scal[SC_STOP] != 0.0) return;
   const int tid = threadIdx.x;
   const int t = tid % TPL, q = tid / TPL;
@@ -451,6 +505,10 @@ FWD void kpass(const Args& a) {
   cpx v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = mk(Wa[t + TPL * e], Wb[t + TPL * e]);
+  // two-pass steps: the previous step's S0 row (deferred norm), issued before the table loads
+  const bool deferred = (FL & F_KNORM) && a.scal[SC_FRESH] == 0.0;
+  NormRow<NT> norm_row;
+  if (deferred) norm_row.load(a.partials);
   cpx fta[E / 2 + 1], ftb[E / 2 + 1];
   if constexpr (use_tab) {
 #pragma unroll
@@ -462,19 +520,14 @@ FWD void kpass(const Args& a) {
     ftb[E / 2] = a.ktab[(size_t)(r + 1) * KW + N / 2];
   }
   tws.store(twl, tid);
+  if (deferred) norm_row.reduce_waves(red);
+  __syncthreads();
   if constexpr (FL & F_KNORM) {
-    // two-pass steps: the previous step's norm (and its published sums)
     double nrm = 1.0;
-    if (a.scal[SC_FRESH] == 0.0) {
-      if (!previous_step<NT>(a, blk, red, nrm)) return;
-    } else {
-      __syncthreads();
-    }
+    if (deferred && !previous_norm<NT>(a, blk, red, nrm)) return;
     const real_t rn = (real_t)nrm;
 #pragma unroll
     for (int e = 0; e < E; ++e) { v[e].x *= rn; v[e].y *= rn; }
-  } else {
-    __syncthreads();
   }
   L::template fwd<true>(v, lds, q, t, twl);
   {
@@ -521,6 +574,9 @@ FWD void kpass(const Args& a) {
     Wa[t + TPL * e] = v[e].x;
     Wb[t + TPL * e] = v[e].y;
   }
+  // the previous step's sums to the series, after this block's own work
+  if constexpr (FL & F_KNORM)
+    if (deferred) publish_sums<NT>(a, blk, gridDim.x, red);
 }
 
 // ---------------------------------------------------------------------------
@@ -564,20 +620,18 @@ FWD void moments(const Args& a) {
 }
 
 // W *= norm: the norm is scal[SC_NORM], or, at the end of a run
-// (F_FROM_PARTIALS), that of the last step, whose sums block 0 publishes to
-// series[step].
+// (F_FROM_PARTIALS), that of the last step, whose sums blocks 0..NSER-1
+// publish to series[step] (the grid has at least NSER blocks).
 FWD void apply_norm(const Args& a) {
   __shared__ double red[16];
   double c;
   if (a.flags & F_FROM_PARTIALS) {
-    const double S0 = sum_partials<HT>(a.partials, a.nbp, 0, red);
+    const double S0 = sum_partials<HT, 1>(a.partials, a.nbp, 0, red);
     c = 1.0 / (S0 * (::dX * ::dP));
-    if (blockIdx.x == 0) {
-      double tot[NSER];
-#pragma unroll
-      for (int i = 0; i < NSER; ++i) tot[i] = (i == 0) ? S0 : sum_partials<HT>(a.partials, a.nbp, i, red);
-      if (threadIdx.x == 0)
-        for (int i = 0; i < NSER; ++i) a.series[(size_t)a.step * NSER + i] = tot[i];
+    if (blockIdx.x < NSER) {
+      const double tot =
+          blockIdx.x == 0 ? S0 : sum_partials<HT, 1>(a.partials + (size_t)blockIdx.x * a.nbp, a.nbp, 0, red);
+      if (threadIdx.x == 0) a.series[(size_t)a.step * NSER + blockIdx.x] = tot;
     }
   } else {
     c = a.scal[SC_NORM];

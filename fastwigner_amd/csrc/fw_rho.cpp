@@ -132,7 +132,7 @@ int tiles(const fw_rho_desc* d, int& re, int& rp, int& ce, int& cp) {
   if (rp * (n / re) > 1024 || cp * (n / ce) > 1024) return fw::fail(FW_ERR_INVALID_ARG, "tile needs more than 1024 threads");
   const size_t tws = ((size_t)n * 17 / 16 + 1) * 16;
   const size_t lr = (size_t)rp * n, lc = (size_t)cp * n;
-  if ((lr + lr / 16 + 1) * esz + tws + 20 * 16 * 8 > 160 * 1024 || (lc + lc / 16 + 1) * esz + tws + 20 * 16 * 8 > 160 * 1024)
+  if ((lr + lr / 16 + 1) * esz + tws + 20 * 32 * 8 > 160 * 1024 || (lc + lc / 16 + 1) * esz + tws + 20 * 32 * 8 > 160 * 1024)
     return fw::fail(FW_ERR_INVALID_ARG, "tile needs more than 160 KiB of LDS");
   return FW_OK;
 }

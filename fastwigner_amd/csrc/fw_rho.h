@@ -102,7 +102,7 @@ FWD void rho_row(const RArgs& a) {
   constexpr int TWN = (N + NT - 1) / NT;
   __shared__ cpx lds[L::LDS];
   __shared__ tw_t twl[TwLds<N>::SIZE];
-  __shared__ double red[RSER * 16];
+  __shared__ double red[RSER * 32];
   if (a.scal[RC_STOP] != 0.0) return;
   const int tid = threadIdx.x;
   const int t = tid % TPL, q = tid / TPL;
@@ -280,7 +280,7 @@ FWD void rho_diag_finish(const RArgs& a, int nblk) {
   constexpr int TPL = L::TPL;
   __shared__ cpx lds[L::LDS];
   __shared__ tw_t twl[TwLds<FW_N>::SIZE];
-  __shared__ double red[RSER * 16];
+  __shared__ double red[RSER * 32];
   const int t = threadIdx.x;
   for (int r = t; r < FW_N; r += TPL) twl[twpad(r)] = a.tw[r];
   cpx v[E];
@@ -338,7 +338,7 @@ FWD void rho_apply_norm(const RArgs& a, int from_partials) {
 // Sums over the current rho (trace family + purity) into scal[8..8+R_PUR];
 // scal[0] + i scal[3] = 1/(Tr rho dX). One block.
 FWD void rho_moments(const RArgs& a) {
-  __shared__ double red[RSER * 16];
+  __shared__ double red[RSER * 32];
   const int tid = threadIdx.x;
   const double tnow = a.times[a.step];
   double s[RSER];
@@ -383,7 +383,7 @@ FWD void rho_wigner_pass(const RArgs& a, const cpx* __restrict__ src, cpx* __res
   constexpr int TPL = L::TPL, NT = L::NT;
   __shared__ cpx lds[L::LDS];
   __shared__ tw_t twl[TwLds<N>::SIZE];
-  __shared__ double red[RSER * 16];
+  __shared__ double red[RSER * 32];
   const int tid = threadIdx.x;
   const int t = (MODE == 0) ? tid % TPL : tid / P;
   const int q = (MODE == 0) ? tid / TPL : tid % P;

@@ -127,6 +127,8 @@ int validate(const fw_wigner_desc* d, const fw_wigner_exprs* x) {
     return fw::fail(FW_ERR_INVALID_ARG, "precision must be FW_FP64 or FW_FP32");
   if (d->step_passes != 0 && d->step_passes != 2 && d->step_passes != 3)
     return fw::fail(FW_ERR_INVALID_ARG, "step_passes must be 0, 2 or 3");
+  if (d->phase_tables < -1 || d->phase_tables > 1)
+    return fw::fail(FW_ERR_INVALID_ARG, "phase_tables must be -1, 0 or 1");
   if (!x->V || !x->K) return fw::fail(FW_ERR_INVALID_ARG, "V and K expressions are required");
   return FW_OK;
 }

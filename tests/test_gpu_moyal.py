@@ -50,12 +50,12 @@ def gaussian(o, x0=1.0, p0=0.5):
     return np.exp(-(o.X - x0) ** 2 - (o.P - p0) ** 2) / np.pi
 
 
-@pytest.mark.parametrize("tables,passes", [(0, 0), (-1, 0), (0, 3)])
+@pytest.mark.parametrize("tables,passes", [(0, 0), (-1, 0), (0, 3), (-1, 3)])
 def test_config1_parity_fp64(gpu, tables, passes):
     """Config 1: 256^2, x,p in [-8,8), V=0.5x^2+0.1x^4, dt=0.01, 1000 steps,
     <x>,<p>,<H>, purity logged each step; with cached factor tables (default for
-    time-independent V, K) and with the factors evaluated on the fly; in two
-    fused passes per step (default) and in three."""
+    time-independent V, K) and with the factors evaluated at every half-step;
+    in two fused passes per step (default) and in three."""
     g, o = make_pair(256, 8., 256, 8., 0.01, V_Q, K_Q, DV_Q, DK_Q, phase_tables=tables, step_passes=passes)
     W0 = gaussian(o)
     g.set_wignerfunction(W0)
@@ -118,6 +118,41 @@ def test_rectangular_masks_time_dependent(gpu, nx, npp):
     assert rel_l2(Wg, o.W) <= 1e-12
     dip = np.array([s["P_average_RHS"] for s in o.series])
     assert np.max(np.abs(np.array(g.P_average_RHS) - dip)) <= 1e-11
+
+
+@pytest.mark.parametrize("nx,npp,passes", [(64, 32, 0), (64, 32, 3), (16, 16, 0), (32, 64, 0)])
+def test_small_grid_observables(gpu, nx, npp, passes):
+    """Grids whose column blocks hold fewer threads than there are sums (4 or 2
+    threads at N_p = 32 or 16) and fewer than a wave: every logged observable
+    against the oracle, in two- and three-pass steps."""
+    g, o = make_pair(nx, 8., npp, 8., 0.01, V_Q, K_Q, DV_Q, DK_Q, step_passes=passes)
+    W0 = gaussian(o)
+    g.set_wignerfunction(W0)
+    o.set_wignerfunction(W0)
+    Wg = g.propagate(130).get()
+    o.propagate(130)
+    assert rel_l2(Wg, o.W) <= 1e-12
+    for k, ours in (("X_average", g.X_average), ("P_average", g.P_average), ("X_average_RHS", g.X_average_RHS),
+                    ("P_average_RHS", g.P_average_RHS), ("hamiltonian_average", g.hamiltonian_average),
+                    ("purity", g.purity_series)):
+        ref = np.array([s[k] for s in o.series])
+        assert np.max(np.abs(np.array(ours) - ref)) <= 1e-11 * max(1.0, np.max(np.abs(ref))), k
+
+
+@pytest.mark.parametrize("tables", [0, -1])
+def test_time_independent_masks(gpu, tables):
+    """All three absorbing-boundary strings with time-independent V and K: the
+    masks enter the cached tables (0) and the per-half-step factors (-1) alike."""
+    masks = dict(abs_boundary_x_theta="pow(fabs(cos(0.5*M_PI*X_minus/X_amplitude)*cos(0.5*M_PI*X_plus/X_amplitude)), 0.01)",
+                 abs_boundary_lambda_p="exp(-1e-4*Lambda*Lambda)",
+                 abs_boundary_x_p="pow(fabs(cos(0.5*M_PI*X/X_amplitude)), 0.125)")
+    g, o = make_pair(256, 10., 128, 6., 0.02, V_Q, K_Q, DV_Q, DK_Q, masks=masks, phase_tables=tables)
+    W0 = np.exp(-(o.X + 0.5) ** 2 - (o.P - 0.3) ** 2)
+    g.set_wignerfunction(W0)
+    o.set_wignerfunction(W0)
+    Wg = g.propagate(300).get()
+    o.propagate(300)
+    assert rel_l2(Wg, o.W) <= 1e-12
 
 
 @pytest.mark.parametrize("tiles", [(16, 1, 16, 1), (8, 2, 8, 2), (16, 4, 4, 4), (32, 1, 32, 1), (4, 8, 2, 8)])
