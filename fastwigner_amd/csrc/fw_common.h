@@ -467,10 +467,31 @@ FWD cpx unit_phase_exact(double ph, double m) {
 // ---------------------------------------------------------------------------
 // Reductions
 // ---------------------------------------------------------------------------
+// Lane-to-lane move of a double through DPP (two 32-bit moves, same control).
+template <int CTRL>
+FWD double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+FWD double read_lane(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+// Sum over the 64 lanes of a full wave, the same bits in every lane. Within
+// each row of 16 lanes by DPP (quad_perm [1,0,3,2], quad_perm [2,3,0,1],
+// row_half_mirror, row_mirror: VALU moves, no LDS traffic), then the four row
+// sums read from lanes 0, 16, 32, 48. A cross-lane shuffle through
+// ds_bpermute costs two LDS instructions per step and double.
 FWD double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  v += dpp_move<0xB1>(v);
+  v += dpp_move<0x4E>(v);
+  v += dpp_move<0x141>(v);
+  v += dpp_move<0x140>(v);
+  return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
 }
 
 // Block reductions over the NT threads of a block. NT >= 64: each wave adds
