@@ -253,20 +253,24 @@ struct TwTab<TWN, 1> {
 // early) and stores them into LDS (before the first barrier of the FFT).
 template <class T, int NT>
 struct TwStage {
-  static constexpr int R = (T::COUNT + NT - 1) / NT > 0 ? (T::COUNT + NT - 1) / NT : 1;
-  tw_t r[R];
+  static constexpr int R = (T::COUNT + NT - 1) / NT;   // 0: nothing staged
+  tw_t r[R > 0 ? R : 1];
+  // COUNT a multiple of NT (the usual case): every e = tid + i NT (i < R) is
+  // below COUNT, so no bounds test, which would make the compiler keep r[] in
+  // scratch memory
+  static constexpr bool EVEN = T::COUNT % NT == 0;
   FWD void load(const tw_t* __restrict__ g, int tid) {
 #pragma unroll
     for (int i = 0; i < R; ++i) {
       const int e = tid + i * NT;
-      if (e < T::COUNT) r[i] = g[T::src(e)];
+      if (EVEN || e < T::COUNT) r[i] = g[T::src(e)];
     }
   }
   FWD void store(tw_t* l, int tid) const {
 #pragma unroll
     for (int i = 0; i < R; ++i) {
       const int e = tid + i * NT;
-      if (e < T::COUNT) l[T::dst(e)] = r[i];
+      if (EVEN || e < T::COUNT) l[T::dst(e)] = r[i];
     }
   }
 };
@@ -370,6 +374,120 @@ struct Line {
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e].y = -v[e].y;
     fwd<false>(v, lds, q, t, tw);
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e].y = -v[e].y;
+  }
+
+  // ---- Pair-fused spectral step: forward FFT, multiply, inverse FFT ----
+  // The separation of two packed real lines pairs spectral positions k and
+  // N - k. The forward FFT's last stage (radix RL, butterflies j < S = N/RL,
+  // butterfly j holding positions j + m S) runs with a map that gives each
+  // thread the butterflies j and S - j of a slot, i.e. conjugate partners, so
+  // the pair multiply happens in registers; and the inverse FFT's first stage,
+  // of radix RL too, takes exactly the positions j + m S of butterfly j as its
+  // inputs. Against transforms with the multiply between them through LDS
+  // (spectrum out, pair gather and write back, natural reload) this saves two
+  // LDS rounds and three barriers per spectral step.
+  static constexpr bool PAIRED = E >= 4;
+  static constexpr int RL = PAIRED ? (E / 2 < N / 2 ? E / 2 : N / 2) : 1;
+  static constexpr int S = N / RL;
+  static constexpr int BS = PAIRED ? E / (2 * RL) : 0;   // slots per thread
+  // A thread has E/2 items (pairs; item 0 of thread 0 is the DC/Nyquist pair
+  // of self-partners). item_k: the spectral index k <= N/2 whose factors item i
+  // of thread t applies; thread 0 also applies k = N/2.
+  FWD static int item_k(int t, int i) {
+    const int sl = t + TPL * (i / RL), m = i % RL;
+    if (sl == 0) return m < RL / 2 ? m * S : S / 2 + (m - RL / 2) * S;
+    const int p = sl + m * S;
+    return m < RL / 2 ? p : N - p;
+  }
+  // forward stages before the last one; result in LDS (natural order, synced)
+  template <int NS>
+  FWD static void fwd_pre(cpx (&v)[E], cpx* lds, int q, int t, const tw_t* __restrict__ tw) {
+    constexpr int REM = S / NS;
+    constexpr int R = REM < RMAX ? REM : RMAX;
+    stage<R, NS, false>(v, lds, q, t, tw);
+    sync();
+    if constexpr (NS * R < S) {
+      load_natural(v, lds, q, t);
+      sync();
+      fwd_pre<NS * R>(v, lds, q, t, tw);
+    }
+  }
+  // butterfly h (0, 1) of slot sl
+  FWD static int slot_bfly(int sl, int h) { return sl == 0 ? (h == 0 ? 0 : S / 2) : (h == 0 ? sl : S - sl); }
+  // MUL: pair(i, zk, zm) for item i (zk at k = item_k, zm at N - k) and
+  // self(z0, zn) for DC and Nyquist (thread 0, with item 0).
+  template <class MUL>
+  FWD static void spectral(cpx (&v)[E], cpx* lds, int q, int t, const tw_t* __restrict__ tw, MUL& mul) {
+    static_assert(PAIRED, "the pair-fused step needs E >= 4");
+    constexpr int TWS = TWN / N;
+    fwd_pre<1>(v, lds, q, t, tw);
+    // last forward stage, slot map; v[(2 b + h) RL + m] = output m of butterfly h of slot b
+#pragma unroll
+    for (int b = 0; b < BS; ++b) {
+      const int sl = t + TPL * b;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int j = slot_bfly(sl, h);
+        cpx a[RL];
+#pragma unroll
+        for (int m = 0; m < RL; ++m) a[m] = lds[lidx(q, j + m * S)];
+        if constexpr (TWO == 2) {
+          cpx w[RL];
+          w[1] = TW::get(tw, j * TWS);
+#pragma unroll
+          for (int m = 2; m < RL; ++m) w[m] = cmul(w[m / 2], w[m - m / 2]);
+#pragma unroll
+          for (int m = 1; m < RL; ++m) a[m] = cmul(a[m], w[m]);
+        } else {
+#pragma unroll
+          for (int m = 1; m < RL; ++m) a[m] = twmul(a[m], TW::get(tw, m * j * TWS));
+        }
+        Dft<RL>::run(a);
+#pragma unroll
+        for (int m = 0; m < RL; ++m) v[(2 * b + h) * RL + m] = a[m];
+      }
+    }
+    sync();   // every read of this exchange precedes the inverse's writes
+    // pair multiply in registers
+#pragma unroll
+    for (int b = 0; b < BS; ++b) {
+      const int sl = t + TPL * b;
+      cpx* y1 = v + 2 * b * RL;
+      cpx* y2 = y1 + RL;
+      if (sl == 0) {
+        mul.self(y1[0], y1[RL / 2]);
+#pragma unroll
+        for (int m = 1; m < RL / 2; ++m) mul.pair(b * RL + m, y1[m], y1[RL - m]);
+#pragma unroll
+        for (int m = 0; m < RL / 2; ++m) mul.pair(b * RL + RL / 2 + m, y2[m], y2[RL - 1 - m]);
+      } else {
+#pragma unroll
+        for (int m = 0; m < RL / 2; ++m) mul.pair(b * RL + m, y1[m], y2[RL - 1 - m]);
+#pragma unroll
+        for (int m = RL / 2; m < RL; ++m) mul.pair(b * RL + m, y2[RL - 1 - m], y1[m]);
+      }
+    }
+    // inverse (conjugation identity): first stage, radix RL, NS = 1, on the slot butterflies
+#pragma unroll
+    for (int b = 0; b < BS; ++b) {
+      const int sl = t + TPL * b;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int j = slot_bfly(sl, h);
+        cpx a[RL];
+#pragma unroll
+        for (int m = 0; m < RL; ++m) a[m] = mk(v[(2 * b + h) * RL + m].x, -v[(2 * b + h) * RL + m].y);
+        Dft<RL>::run(a);
+#pragma unroll
+        for (int m = 0; m < RL; ++m) lds[lidx(q, j * RL + m)] = a[m];
+      }
+    }
+    sync();
+    load_natural(v, lds, q, t);
+    sync();
+    fwd<false, RL>(v, lds, q, t, tw);
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e].y = -v[e].y;
   }

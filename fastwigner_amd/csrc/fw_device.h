@@ -233,56 +233,117 @@ FWD size_t vtab_index(int x, int k) {
   return ((size_t)(x >> 1) * (FW_NP / 2 + 1) + k) * 2 + (x & 1);
 }
 
-// V factors of column pair (c, c+1) that a thread of a column pass applies,
-// from the table: slot e < E/2 for theta index k = t + TPL e, slot E/2 for
-// k = N/2 (used by t = 0 only).
-FWD void v_factors(const Args& a, int c, int t, cpx2 (&ftab)[FW_COL_E / 2 + 1]) {
-  constexpr int N = FW_NP, E = FW_COL_E, TPL = VL::TPL;
-#pragma unroll
-  for (int e = 0; e < E / 2; ++e)
-    ftab[e] = *reinterpret_cast<const cpx2*>(a.vtab + vtab_index(c, t + TPL * e));
-  ftab[E / 2] = *reinterpret_cast<const cpx2*>(a.vtab + vtab_index(c, N / 2));
+// Spectral index k <= N/2 of pair item i of thread t in a pass's spectral
+// step: the pair-fused map of Line::spectral, or k = t + TPL i (E = 2).
+template <class L>
+FWD int spec_k(int t, int i) {
+  if constexpr (L::PAIRED) return L::item_k(t, i);
+  else return t + L::TPL * i;
 }
 
-// Multiply the packed spectrum of column pair (Xa, Xb) held in LDS by the V
-// factors at time tau (masks at tnow), from the table row cache ftab or
-// evaluated here.
-template <bool use_tab>
-FWD void v_multiply(cpx* lds, int q, int t, double Xa, double Xb, double tau, double tnow, const cpx2* ftab) {
-  using L = VL;
-  constexpr int N = FW_NP, E = FW_COL_E, TPL = L::TPL;
-  const real_t s = (real_t)1 / (real_t)N;
+// Spectral step of a pass: forward FFT, mul.pair on each conjugate pair
+// (k, N - k) and mul.self on DC and Nyquist, inverse FFT; registers to
+// registers at the natural positions. E >= 4: the pair-fused step; E = 2:
+// separate transforms with the multiply through LDS.
+template <class L, int E, class MUL>
+FWD void spectral_step(cpx (&v)[E], cpx* lds, int q, int t, const tw_t* __restrict__ twl, MUL& mul) {
+  if constexpr (L::PAIRED) {
+    L::spectral(v, lds, q, t, twl, mul);
+  } else {
+    constexpr int N = L::TPL * E, TPL = L::TPL;
+    L::template fwd<true>(v, lds, q, t, twl);
 #pragma unroll
-  for (int e = 0; e < E / 2; ++e) {
-    const int k = t + TPL * e;
-    if (k == 0) {
-      const double ThN = ::dTheta * (N / 2);
-      cpx fa0, fb0, faN, fbN;
-      if constexpr (use_tab) {
-        fa0 = ftab[0].a; fb0 = ftab[0].b; faN = ftab[E / 2].a; fbN = ftab[E / 2].b;
-      } else {
-        fa0 = vfactor(Xa, 0.0, tau, tnow); fb0 = vfactor(Xb, 0.0, tau, tnow);
-        faN = vfactor(Xa, ThN, tau, tnow); fbN = vfactor(Xb, ThN, tau, tnow);
-      }
-      const int i0 = L::lidx(q, 0), iN = L::lidx(q, N / 2);
-      lds[i0] = self_apply(lds[i0], fa0, fb0, s);
-      lds[iN] = self_apply(lds[iN], faN, fbN, s);
-    } else {
-      const double Th = ::dTheta * k;
-      cpx fa, fb;
-      if constexpr (use_tab) {
-        fa = ftab[e].a; fb = ftab[e].b;
-      } else {
-        fa = vfactor(Xa, Th, tau, tnow); fb = vfactor(Xb, Th, tau, tnow);
-      }
-      const int ik = L::lidx(q, k), im = L::lidx(q, N - k);
+    for (int e = 0; e < E / 2; ++e) {
+      const int k = t + TPL * e;
+      const int ik = L::lidx(q, k), im = L::lidx(q, k == 0 ? N / 2 : N - k);
       cpx zk = lds[ik], zm = lds[im];
-      pair_apply(zk, zm, fa, fb, s);
+      if (k == 0) mul.self(zk, zm);
+      else mul.pair(e, zk, zm);
       lds[ik] = zk;
       lds[im] = zm;
     }
+    L::sync();
+    L::load_natural(v, lds, q, t);
+    L::sync();
+    L::inv(v, lds, q, t, twl);
   }
 }
+
+// V factors of column pair (c, c+1) that a thread of a column pass applies,
+// from the table: slot i < E/2 for its pair item i, slot E/2 for k = N/2
+// (used by t = 0 only).
+FWD void v_factors(const Args& a, int c, int t, cpx2 (&ftab)[FW_COL_E / 2 + 1]) {
+  constexpr int N = FW_NP, E = FW_COL_E;
+#pragma unroll
+  for (int i = 0; i < E / 2; ++i)
+    ftab[i] = *reinterpret_cast<const cpx2*>(a.vtab + vtab_index(c, spec_k<VL>(t, i)));
+  ftab[E / 2] = *reinterpret_cast<const cpx2*>(a.vtab + vtab_index(c, N / 2));
+}
+
+// Pair multiply of the column passes: real-pair separation of columns a, b,
+// their V factors (from the table row cache ftab, or evaluated at time tau,
+// masks at tnow), repacking, scale 1/N.
+template <bool use_tab>
+struct VMul {
+  const cpx2 (&ftab)[FW_COL_E / 2 + 1];
+  double Xa, Xb, tau, tnow;
+  int t;
+  FWD void pair(int i, cpx& zk, cpx& zm) const {
+    cpx fa, fb;
+    if constexpr (use_tab) {
+      fa = ftab[i].a; fb = ftab[i].b;
+    } else {
+      const double Th = ::dTheta * spec_k<VL>(t, i);
+      fa = vfactor(Xa, Th, tau, tnow); fb = vfactor(Xb, Th, tau, tnow);
+    }
+    pair_apply(zk, zm, fa, fb, (real_t)1 / (real_t)FW_NP);
+  }
+  FWD void self(cpx& z0, cpx& zn) const {
+    cpx fa0, fb0, faN, fbN;
+    if constexpr (use_tab) {
+      fa0 = ftab[0].a; fb0 = ftab[0].b; faN = ftab[FW_COL_E / 2].a; fbN = ftab[FW_COL_E / 2].b;
+    } else {
+      const double ThN = ::dTheta * (FW_NP / 2);
+      fa0 = vfactor(Xa, 0.0, tau, tnow); fb0 = vfactor(Xb, 0.0, tau, tnow);
+      faN = vfactor(Xa, ThN, tau, tnow); fbN = vfactor(Xb, ThN, tau, tnow);
+    }
+    const real_t s = (real_t)1 / (real_t)FW_NP;
+    z0 = self_apply(z0, fa0, fb0, s);
+    zn = self_apply(zn, faN, fbN, s);
+  }
+};
+
+// Pair multiply of the row passes: rows a, b and their K factors.
+template <bool use_tab>
+struct KMul {
+  const cpx (&fta)[FW_ROW_E / 2 + 1];
+  const cpx (&ftb)[FW_ROW_E / 2 + 1];
+  double Pa, Pb, tau, tnow;
+  int t;
+  FWD void pair(int i, cpx& zk, cpx& zm) const {
+    cpx fa, fb;
+    if constexpr (use_tab) {
+      fa = fta[i]; fb = ftb[i];
+    } else {
+      const double Lm = ::dLambda * spec_k<KL>(t, i);
+      fa = kfactor(Lm, Pa, tau, tnow); fb = kfactor(Lm, Pb, tau, tnow);
+    }
+    pair_apply(zk, zm, fa, fb, (real_t)1 / (real_t)FW_NX);
+  }
+  FWD void self(cpx& z0, cpx& zn) const {
+    cpx fa0, fb0, faN, fbN;
+    if constexpr (use_tab) {
+      fa0 = fta[0]; fb0 = ftb[0]; faN = fta[FW_ROW_E / 2]; fbN = ftb[FW_ROW_E / 2];
+    } else {
+      const double LN = ::dLambda * (FW_NX / 2);
+      fa0 = kfactor(0.0, Pa, tau, tnow); fb0 = kfactor(0.0, Pb, tau, tnow);
+      faN = kfactor(LN, Pa, tau, tnow); fbN = kfactor(LN, Pb, tau, tnow);
+    }
+    const real_t s = (real_t)1 / (real_t)FW_NX;
+    z0 = self_apply(z0, fa0, fb0, s);
+    zn = self_apply(zn, faN, fbN, s);
+  }
+};
 
 template <int FL>
 FWD void vpass(const Args& a) {
@@ -336,16 +397,11 @@ FWD void vpass(const Args& a) {
       v[e] = w;
     }
   }
-  L::template fwd<true>(v, lds, q, t, twl);
-
   // spectral multiply on the theta x representation
   const double Xa = ::dX * (c - 0.5 * ::X_gridDIM);
   const double Xb = ::dX * (c + 1 - 0.5 * ::X_gridDIM);
-  v_multiply<use_tab>(lds, q, t, Xa, Xb, tau, tnow, ftab);
-  L::sync();
-  L::load_natural(v, lds, q, t);
-  L::sync();
-  L::inv(v, lds, q, t, twl);
+  VMul<use_tab> mul{ftab, Xa, Xb, tau, tnow, t};
+  spectral_step<L>(v, lds, q, t, twl, mul);
 
   double sums[NSER];
 #pragma unroll
@@ -429,12 +485,8 @@ FWD void vvpass(const Args& a) {
   const double Xa = ::dX * (c - 0.5 * ::X_gridDIM);
   const double Xb = ::dX * (c + 1 - 0.5 * ::X_gridDIM);
   // closing V/2 of step s
-  L::template fwd<true>(v, lds, q, t, twl);
-  v_multiply<use_tab>(lds, q, t, Xa, Xb, t0 + FW_PHASE_SHIFT * ::dt, t0, ftab);
-  L::sync();
-  L::load_natural(v, lds, q, t);
-  L::sync();
-  L::inv(v, lds, q, t, twl);
+  VMul<use_tab> mul0{ftab, Xa, Xb, t0 + FW_PHASE_SHIFT * ::dt, t0, t};
+  spectral_step<L>(v, lds, q, t, twl, mul0);
   // end of step s: x-p mask, sums
   double sums[NSER];
 #pragma unroll
@@ -464,13 +516,13 @@ FWD void vvpass(const Args& a) {
   sums[6] = ::fw_diff_V(Xa, t0) * ca + ::fw_diff_V(Xb, t0) * cb;
   sums[7] += ::V(Xa, t0) * ca + ::V(Xb, t0) * cb;
   write_partial_rows<NT>(sums, red, a.partials, blk, nb);
-  // opening V/2 of step s+1
-  L::template fwd<true>(v, lds, q, t, twl);
-  v_multiply<use_tab>(lds, q, t, Xa, Xb, t1 + FW_PHASE_SHIFT * ::dt, t1, ftab);
-  L::sync();
-  L::load_natural(v, lds, q, t);
-  L::sync();
-  L::inv(v, lds, q, t, twl);
+  // opening V/2 of step s+1. The thread index passes through an empty asm so
+  // that the compiler recomputes this step's twiddles and LDS indices instead
+  // of keeping those of the closing step live across it (register pressure).
+  int t2 = t;
+  asm volatile("" : "+v"(t2));
+  VMul<use_tab> mul1{ftab, Xa, Xb, t1 + FW_PHASE_SHIFT * ::dt, t1, t2};
+  spectral_step<L>(v, lds, q, t2, twl, mul1);
 #pragma unroll
   for (int e = 0; e < E; ++e) *reinterpret_cast<cpx*>(W + (size_t)(t + TPL * e) * FW_NX + c) = v[e];
 }
@@ -512,9 +564,10 @@ FWD void kpass(const Args& a) {
   cpx fta[E / 2 + 1], ftb[E / 2 + 1];
   if constexpr (use_tab) {
 #pragma unroll
-    for (int e = 0; e < E / 2; ++e) {
-      fta[e] = a.ktab[(size_t)r * KW + t + TPL * e];
-      ftb[e] = a.ktab[(size_t)(r + 1) * KW + t + TPL * e];
+    for (int i = 0; i < E / 2; ++i) {
+      const int k = spec_k<L>(t, i);
+      fta[i] = a.ktab[(size_t)r * KW + k];
+      ftb[i] = a.ktab[(size_t)(r + 1) * KW + k];
     }
     fta[E / 2] = a.ktab[(size_t)r * KW + N / 2];
     ftb[E / 2] = a.ktab[(size_t)(r + 1) * KW + N / 2];
@@ -529,46 +582,10 @@ FWD void kpass(const Args& a) {
 #pragma unroll
     for (int e = 0; e < E; ++e) { v[e].x *= rn; v[e].y *= rn; }
   }
-  L::template fwd<true>(v, lds, q, t, twl);
-  {
-    const double Pa = ::dP * (r - 0.5 * ::P_gridDIM);
-    const double Pb = ::dP * (r + 1 - 0.5 * ::P_gridDIM);
-    const real_t s = (real_t)1 / (real_t)N;
-#pragma unroll
-    for (int e = 0; e < E / 2; ++e) {
-      const int k = t + TPL * e;
-      if (k == 0) {
-        const double LN = ::dLambda * (N / 2);
-        cpx fa0, fb0, faN, fbN;
-        if constexpr (use_tab) {
-          fa0 = fta[0]; fb0 = ftb[0]; faN = fta[E / 2]; fbN = ftb[E / 2];
-        } else {
-          fa0 = kfactor(0.0, Pa, tau, tnow); fb0 = kfactor(0.0, Pb, tau, tnow);
-          faN = kfactor(LN, Pa, tau, tnow); fbN = kfactor(LN, Pb, tau, tnow);
-        }
-        const int i0 = L::lidx(q, 0), iN = L::lidx(q, N / 2);
-        lds[i0] = self_apply(lds[i0], fa0, fb0, s);
-        lds[iN] = self_apply(lds[iN], faN, fbN, s);
-      } else {
-        const double Lm = ::dLambda * k;
-        cpx fa, fb;
-        if constexpr (use_tab) {
-          fa = fta[e]; fb = ftb[e];
-        } else {
-          fa = kfactor(Lm, Pa, tau, tnow); fb = kfactor(Lm, Pb, tau, tnow);
-        }
-        const int ik = L::lidx(q, k), im = L::lidx(q, N - k);
-        cpx zk = lds[ik], zm = lds[im];
-        pair_apply(zk, zm, fa, fb, s);
-        lds[ik] = zk;
-        lds[im] = zm;
-      }
-    }
-  }
-  L::sync();
-  L::load_natural(v, lds, q, t);
-  L::sync();
-  L::inv(v, lds, q, t, twl);
+  const double Pa = ::dP * (r - 0.5 * ::P_gridDIM);
+  const double Pb = ::dP * (r + 1 - 0.5 * ::P_gridDIM);
+  KMul<use_tab> mul{fta, ftb, Pa, Pb, tau, tnow, t};
+  spectral_step<L>(v, lds, q, t, twl, mul);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     Wa[t + TPL * e] = v[e].x;
