@@ -63,7 +63,7 @@ struct BArgs {
   const int* qoff;        // [9] queue offsets (mode 0)
   const tw_t* tw;         // exp(-2 pi i j / FW_B_NMAX)
   const double* times;    // t after the increment, per chunk-local step
-  double* partials;       // NSER per tile of the last V pass
+  double* partials;       // per instance NSER rows of ntile sums (row k: sum k of every tile)
   double* series;         // [slot][instance][NSER]
   unsigned int* qhead;    // [chunk step][8]
   unsigned int* done;     // [chunk step][instance][3]
@@ -112,6 +112,63 @@ FWD void b_pair_apply(cpx& zk, cpx& zm, cpx fa, cpx fb, real_t s) {
 }
 FWD cpx b_self_apply(cpx z, cpx fa, cpx fb, real_t s) { return mk(z.x * fa.x * s, z.y * fb.x * s); }
 
+// Pair multiplies of the batched passes (the callbacks of spectral_step): the
+// factors of item i at spectral index spec_k<L>(t, i), evaluated here.
+template <class L, int N, int KIND>
+struct BVMul {
+  const BArgs& a;
+  const double* prm;
+  int i, t;
+  double Xa, Xb, tau, dTh, db;
+  FWD void factors(double Th, cpx& fa, cpx& fb) const {
+    if constexpr (KIND == BK_BLOCH) {
+      fa = b_bloch_v(a, i, Xa, Th, prm, db); fb = b_bloch_v(a, i, Xb, Th, prm, db);
+    } else {
+      fa = b_vfactor(Xa, Th, tau, prm); fb = b_vfactor(Xb, Th, tau, prm);
+    }
+  }
+  FWD void pair(int item, cpx& zk, cpx& zm) const {
+    cpx fa, fb;
+    factors(dTh * spec_k<L>(t, item), fa, fb);
+    b_pair_apply(zk, zm, fa, fb, (real_t)1 / (real_t)N);
+  }
+  FWD void self(cpx& z0, cpx& zn) const {
+    const real_t s = (real_t)1 / (real_t)N;
+    cpx fa, fb;
+    factors(0.0, fa, fb);
+    z0 = b_self_apply(z0, fa, fb, s);
+    factors(dTh * (N / 2), fa, fb);
+    zn = b_self_apply(zn, fa, fb, s);
+  }
+};
+template <class L, int N, int KIND>
+struct BKMul {
+  const BArgs& a;
+  const double* prm;
+  int i, t;
+  double Pa, Pb, tau, dL, db;
+  FWD void factors(double Lm, cpx& fa, cpx& fb) const {
+    if constexpr (KIND == BK_BLOCH) {
+      fa = b_bloch_k(a, i, Lm, Pa, prm, db); fb = b_bloch_k(a, i, Lm, Pb, prm, db);
+    } else {
+      fa = b_kfactor(Lm, Pa, tau, prm); fb = b_kfactor(Lm, Pb, tau, prm);
+    }
+  }
+  FWD void pair(int item, cpx& zk, cpx& zm) const {
+    cpx fa, fb;
+    factors(dL * spec_k<L>(t, item), fa, fb);
+    b_pair_apply(zk, zm, fa, fb, (real_t)1 / (real_t)N);
+  }
+  FWD void self(cpx& z0, cpx& zn) const {
+    const real_t s = (real_t)1 / (real_t)N;
+    cpx fa, fb;
+    factors(0.0, fa, fb);
+    z0 = b_self_apply(z0, fa, fb, s);
+    factors(dL * (N / 2), fa, fb);
+    zn = b_self_apply(zn, fa, fb, s);
+  }
+};
+
 // ---------------------------------------------------------------------------
 // V pass of one column tile: P column pairs c = 2 (tile P + q), q < P.
 // FIRST: deferred normalisation (+ dissipator) on load. LAST: dissipator + the
@@ -141,7 +198,7 @@ FWD void b_vpass(const BArgs& a, int i, int tile, int tid, cpx* lds, const tw_t*
     // the previous step's norm (its sums are published by the K pass)
     double nrm = 1.0;
     if (a.fresh[i] == 0)
-      nrm = 1.0 / (sum_partials<BT>(a.partials + (size_t)in->part * NSER, in->ntile, 0, red) * (dX * dP));
+      nrm = 1.0 / (sum_partials<BT, 1>(a.partials + (size_t)in->part * NSER, in->ntile, 0, red) * (dX * dP));
     const real_t rn = (real_t)nrm;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -156,50 +213,9 @@ FWD void b_vpass(const BArgs& a, int i, int tile, int tid, cpx* lds, const tw_t*
     }
   }
   __syncthreads();
-  L::template fwd<true>(v, lds, q, t, twl);
-
   const double Xa = dX * (c - 0.5 * N), Xb = dX * (c + 1 - 0.5 * N);
-  const double db = in->dbeta;
-  {
-    const real_t s = (real_t)1 / (real_t)N;
-    // not unrolled: the factors are evaluated on the fly, and unrolling lets
-    // the scheduler hoist all of them at once (register pressure)
-#pragma unroll 1
-    for (int e = 0; e < E / 2; ++e) {
-      const int k = t + TPL * e;
-      if (k == 0) {
-        const double ThN = dTh * (N / 2);
-        cpx fa0, fb0, faN, fbN;
-        if constexpr (KIND == BK_BLOCH) {
-          fa0 = b_bloch_v(a, i, Xa, 0.0, prm, db); fb0 = b_bloch_v(a, i, Xb, 0.0, prm, db);
-          faN = b_bloch_v(a, i, Xa, ThN, prm, db); fbN = b_bloch_v(a, i, Xb, ThN, prm, db);
-        } else {
-          fa0 = b_vfactor(Xa, 0.0, tau, prm); fb0 = b_vfactor(Xb, 0.0, tau, prm);
-          faN = b_vfactor(Xa, ThN, tau, prm); fbN = b_vfactor(Xb, ThN, tau, prm);
-        }
-        const int i0 = L::lidx(q, 0), iN = L::lidx(q, N / 2);
-        lds[i0] = b_self_apply(lds[i0], fa0, fb0, s);
-        lds[iN] = b_self_apply(lds[iN], faN, fbN, s);
-      } else {
-        const double Th = dTh * k;
-        cpx fa, fb;
-        if constexpr (KIND == BK_BLOCH) {
-          fa = b_bloch_v(a, i, Xa, Th, prm, db); fb = b_bloch_v(a, i, Xb, Th, prm, db);
-        } else {
-          fa = b_vfactor(Xa, Th, tau, prm); fb = b_vfactor(Xb, Th, tau, prm);
-        }
-        const int ik = L::lidx(q, k), im = L::lidx(q, N - k);
-        cpx zk = lds[ik], zm = lds[im];
-        b_pair_apply(zk, zm, fa, fb, s);
-        lds[ik] = zk;
-        lds[im] = zm;
-      }
-    }
-  }
-  L::sync();
-  L::load_natural(v, lds, q, t);
-  L::sync();
-  L::inv(v, lds, q, t, twl);
+  BVMul<L, N, KIND> mul{a, prm, i, t, Xa, Xb, tau, dTh, in->dbeta};
+  spectral_step<L>(v, lds, q, t, twl, mul);
 
   double sums[NSER];
 #pragma unroll
@@ -233,7 +249,9 @@ FWD void b_vpass(const BArgs& a, int i, int tile, int tid, cpx* lds, const tw_t*
     sums[3] = Xa * Xa * ca + Xb * Xb * cb;
     sums[6] = ::fw_diff_V(Xa, tnow, prm) * ca + ::fw_diff_V(Xb, tnow, prm) * cb;
     sums[7] += ::V(Xa, tnow, prm) * ca + ::V(Xb, tnow, prm) * cb;
-    write_partials<BT>(sums, red, a.partials + (size_t)in->part * NSER, tile);
+    write_partial_rows<BT>(sums, red, a.partials + (size_t)in->part * NSER, tile, in->ntile);
+    // every K tile of this step has read `fresh`: a run's first step ends here
+    if (tile == 0 && tid == 0 && a.fresh[i] != 0) a.fresh[i] = 0;
   }
 }
 
@@ -250,75 +268,29 @@ FWD void b_kpass(const BArgs& a, int i, int tile, int tid, cpx* lds, const tw_t*
   const double tau = tnow + FW_PHASE_SHIFT * ::dt;
   real_t* __restrict__ Wa = a.W + in->off + (size_t)r * N;
   real_t* __restrict__ Wb = Wa + N;
-  if (tile == 0) {
-    // Every V tile of this step has read `fresh` and the previous step's
-    // partials, and the last V pass has not yet overwritten them: publish the
-    // previous step's sums to series[step], or clear `fresh` at a run's start.
-    const int fr = a.fresh[i];
-    __syncthreads();
-    if (fr == 0) {
-      const double* pp = a.partials + (size_t)in->part * NSER;
-#pragma unroll 1
-      for (int k = 0; k < NSER; ++k) {
-        const double v = sum_partials<BT>(pp, in->ntile, k, red);
-        if (tid == 0) a.series[((size_t)a.step * a.ninst + i) * NSER + k] = v;
-      }
-    } else if (tid == 0) {
-      a.fresh[i] = 0;
-    }
-  }
-
+  // `fresh` is cleared by the last V pass of the step, after every K tile has read it
+  const int fr = a.fresh[i];
   cpx v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = mk(Wa[t + TPL * e], Wb[t + TPL * e]);
   __syncthreads();
-  L::template fwd<true>(v, lds, q, t, twl);
-  {
-    const double dP = in->dP, dL = in->dLambda, db = in->dbeta;
-    const double Pa = dP * (r - 0.5 * N), Pb = dP * (r + 1 - 0.5 * N);
-    const real_t s = (real_t)1 / (real_t)N;
-    // not unrolled: the factors are evaluated on the fly, and unrolling lets
-    // the scheduler hoist all of them at once (register pressure)
-#pragma unroll 1
-    for (int e = 0; e < E / 2; ++e) {
-      const int k = t + TPL * e;
-      if (k == 0) {
-        const double LN = dL * (N / 2);
-        cpx fa0, fb0, faN, fbN;
-        if constexpr (KIND == BK_BLOCH) {
-          fa0 = b_bloch_k(a, i, 0.0, Pa, prm, db); fb0 = b_bloch_k(a, i, 0.0, Pb, prm, db);
-          faN = b_bloch_k(a, i, LN, Pa, prm, db); fbN = b_bloch_k(a, i, LN, Pb, prm, db);
-        } else {
-          fa0 = b_kfactor(0.0, Pa, tau, prm); fb0 = b_kfactor(0.0, Pb, tau, prm);
-          faN = b_kfactor(LN, Pa, tau, prm); fbN = b_kfactor(LN, Pb, tau, prm);
-        }
-        const int i0 = L::lidx(q, 0), iN = L::lidx(q, N / 2);
-        lds[i0] = b_self_apply(lds[i0], fa0, fb0, s);
-        lds[iN] = b_self_apply(lds[iN], faN, fbN, s);
-      } else {
-        const double Lm = dL * k;
-        cpx fa, fb;
-        if constexpr (KIND == BK_BLOCH) {
-          fa = b_bloch_k(a, i, Lm, Pa, prm, db); fb = b_bloch_k(a, i, Lm, Pb, prm, db);
-        } else {
-          fa = b_kfactor(Lm, Pa, tau, prm); fb = b_kfactor(Lm, Pb, tau, prm);
-        }
-        const int ik = L::lidx(q, k), im = L::lidx(q, N - k);
-        cpx zk = lds[ik], zm = lds[im];
-        b_pair_apply(zk, zm, fa, fb, s);
-        lds[ik] = zk;
-        lds[im] = zm;
-      }
-    }
-  }
-  L::sync();
-  L::load_natural(v, lds, q, t);
-  L::sync();
-  L::inv(v, lds, q, t, twl);
+  const double dP = in->dP;
+  const double Pa = dP * (r - 0.5 * N), Pb = dP * (r + 1 - 0.5 * N);
+  BKMul<L, N, KIND> mul{a, prm, i, t, Pa, Pb, tau, in->dLambda, in->dbeta};
+  spectral_step<L>(v, lds, q, t, twl, mul);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     Wa[t + TPL * e] = v[e].x;
     Wb[t + TPL * e] = v[e].y;
+  }
+  // the previous step's sums to series[step]: tile k adds rows k, k + ntile, ...
+  // (the last V pass, which overwrites the partials, runs after this pass)
+  if (fr == 0) {
+    const double* pp = a.partials + (size_t)in->part * NSER;
+    for (int k = tile; k < NSER; k += in->ntile) {
+      const double v2 = sum_partials<BT, 1>(pp + (size_t)k * in->ntile, in->ntile, 0, red);
+      if (tid == 0) a.series[((size_t)a.step * a.ninst + i) * NSER + k] = v2;
+    }
   }
 }
 
@@ -389,15 +361,19 @@ FWD void b_step(const BArgs& a) {
   __shared__ tw_t twl[TwLds<FW_B_NMAX>::SIZE];
   __shared__ double red[NSER * 16];
   __shared__ int s_task;
-  for (int j = threadIdx.x; j < FW_B_NMAX; j += BT) twl[twpad(j)] = a.tw[j];
-  __syncthreads();
   if (a.mode == 1) {
     const BTask tk = a.tasks[a.task_base + blockIdx.x];
     const int i = tk.inst, pass = tk.code >> 16, tile = tk.code & 0xffff;   // uniform (scalar loads)
     if (i < 0 || (KIND == BK_BLOCH && a.gstep >= a.inst[i].nbloch)) return;
+    // the FFT of length n reads the table entries at stride FW_B_NMAX / n only
+    const int n = a.inst[i].n, stride = FW_B_NMAX / n;
+    for (int j = threadIdx.x; j < n; j += BT) twl[twpad(j * stride)] = a.tw[j * stride];
+    __syncthreads();
     b_run<KIND>(a, i, pass, tile, threadIdx.x, lds, twl, red);
     return;
   }
+  for (int j = threadIdx.x; j < FW_B_NMAX; j += BT) twl[twpad(j)] = a.tw[j];
+  __syncthreads();
   const int home = blockIdx.x & 7;
   for (;;) {
     if (threadIdx.x == 0) {
@@ -446,11 +422,11 @@ FWD void b_norm(const BArgs& a, int slot) {
   const BInst* in = a.inst + i;
   if (a.fresh[i]) return;
   const double* pp = a.partials + (size_t)in->part * NSER;
-  const double S0 = sum_partials<BT>(pp, in->ntile, 0, red);
+  const double S0 = sum_partials<BT, 1>(pp, in->ntile, 0, red);
   if (part == 0) {
     double tot[NSER];
 #pragma unroll
-    for (int k = 0; k < NSER; ++k) tot[k] = (k == 0) ? S0 : sum_partials<BT>(pp, in->ntile, k, red);
+    for (int k = 0; k < NSER; ++k) tot[k] = (k == 0) ? S0 : sum_partials<BT, 1>(pp + (size_t)k * in->ntile, in->ntile, 0, red);
     if (threadIdx.x == 0)
       for (int k = 0; k < NSER; ++k) a.series[((size_t)slot * a.ninst + i) * NSER + k] = tot[k];
   }

@@ -495,6 +495,43 @@ struct Line {
 
 enum { NSER = 9 };   // sums per step published by the propagators
 
+// Spectral index k <= N/2 of pair item i of thread t in a pass's spectral
+// step: the pair-fused map of Line::spectral, or k = t + TPL i (E = 2).
+template <class L>
+FWD int spec_k(int t, int i) {
+  if constexpr (L::PAIRED) return L::item_k(t, i);
+  else return t + L::TPL * i;
+}
+
+// Spectral step of a pass: forward FFT, mul.pair on each conjugate pair
+// (k, N - k) and mul.self on DC and Nyquist, inverse FFT; registers to
+// registers at the natural positions. E >= 4: the pair-fused step; E = 2:
+// separate transforms with the multiply through LDS.
+template <class L, int E, class MUL>
+FWD void spectral_step(cpx (&v)[E], cpx* lds, int q, int t, const tw_t* __restrict__ twl, MUL& mul) {
+  if constexpr (L::PAIRED) {
+    L::spectral(v, lds, q, t, twl, mul);
+  } else {
+    constexpr int N = L::TPL * E, TPL = L::TPL;
+    L::template fwd<true>(v, lds, q, t, twl);
+#pragma unroll
+    for (int e = 0; e < E / 2; ++e) {
+      const int k = t + TPL * e;
+      const int ik = L::lidx(q, k), im = L::lidx(q, k == 0 ? N / 2 : N - k);
+      cpx zk = lds[ik], zm = lds[im];
+      if (k == 0) mul.self(zk, zm);
+      else mul.pair(e, zk, zm);
+      lds[ik] = zk;
+      lds[im] = zm;
+    }
+    L::sync();
+    L::load_natural(v, lds, q, t);
+    L::sync();
+    L::inv(v, lds, q, t, twl);
+  }
+}
+
+
 #if FW_FP32
 // sin and cos of |x| <= pi/4 + 1e-6 by their Taylor series through x^9 and
 // x^10 (truncation below 2.5e-9 relative, under half an FP32 ulp), Horner in
@@ -730,6 +767,17 @@ FWD void write_partials(double (&s)[NS], double* red, double* partials, int blk)
   for (int i = 0; i < NS; ++i) BlockRed<NT>::put(s[i], red, NS, i);
   __syncthreads();
   for (int i = threadIdx.x; i < NS; i += NT) partials[(size_t)blk * NS + i] = BlockRed<NT>::total(red, NS, i);
+}
+
+// Block sums stored by sum: sum i of block blk at partials[i * nbp + blk], so
+// that a row (one sum over all nbp blocks) is contiguous for the readers that
+// add it. red: >= NS * BlockRed<NT>::RN doubles.
+template <int NT, int NS = NSER>
+FWD void write_partial_rows(double (&s)[NS], double* red, double* partials, int blk, int nbp) {
+#pragma unroll
+  for (int i = 0; i < NS; ++i) BlockRed<NT>::put(s[i], red, NS, i);
+  __syncthreads();
+  for (int i = threadIdx.x; i < NS; i += NT) partials[(size_t)i * nbp + blk] = BlockRed<NT>::total(red, NS, i);
 }
 
 FWD double wave_min(double v) {

@@ -136,19 +136,9 @@ FWD cpx self_apply(cpx z, cpx fa, cpx fb, real_t s) {
   return mk(z.x * fa.x * s, z.y * fb.x * s);
 }
 
-// Partial sums of the column passes of Wigner steps are stored by sum: sum i
-// of block b at partials[i * nbp + b]. The S0 row, which every block of the
-// next pass adds for the deferred norm, is then nbp contiguous doubles (a
-// block-major layout makes each block read all NSER * nbp of them).
-// red: >= NSER * BlockRed<NT>::RN doubles.
-template <int NT>
-FWD void write_partial_rows(double (&s)[NSER], double* red, double* partials, int blk, int nbp) {
-#pragma unroll
-  for (int i = 0; i < NSER; ++i) BlockRed<NT>::put(s[i], red, NSER, i);
-  __syncthreads();
-  for (int i = threadIdx.x; i < NSER; i += NT) partials[(size_t)i * nbp + blk] = BlockRed<NT>::total(red, NSER, i);
-}
-
+// Partial sums of the column passes of Wigner steps are stored by sum (see
+// write_partial_rows): the S0 row, which every block of the next pass adds for
+// the deferred norm, is nbp contiguous doubles.
 // The S0 row of the previous column pass, loaded with the pass's own global
 // loads and added in sum_partials' order (b = tid, tid + NT, ..., compensated,
 // then waves in order), so every block obtains the same bits.
@@ -231,42 +221,6 @@ using KL = Line<FW_NX, FW_ROW_E, FW_ROW_P, true, FW_NX, FW_TW2>;
 // fragmented as the column reads of W).
 FWD size_t vtab_index(int x, int k) {
   return ((size_t)(x >> 1) * (FW_NP / 2 + 1) + k) * 2 + (x & 1);
-}
-
-// Spectral index k <= N/2 of pair item i of thread t in a pass's spectral
-// step: the pair-fused map of Line::spectral, or k = t + TPL i (E = 2).
-template <class L>
-FWD int spec_k(int t, int i) {
-  if constexpr (L::PAIRED) return L::item_k(t, i);
-  else return t + L::TPL * i;
-}
-
-// Spectral step of a pass: forward FFT, mul.pair on each conjugate pair
-// (k, N - k) and mul.self on DC and Nyquist, inverse FFT; registers to
-// registers at the natural positions. E >= 4: the pair-fused step; E = 2:
-// separate transforms with the multiply through LDS.
-template <class L, int E, class MUL>
-FWD void spectral_step(cpx (&v)[E], cpx* lds, int q, int t, const tw_t* __restrict__ twl, MUL& mul) {
-  if constexpr (L::PAIRED) {
-    L::spectral(v, lds, q, t, twl, mul);
-  } else {
-    constexpr int N = L::TPL * E, TPL = L::TPL;
-    L::template fwd<true>(v, lds, q, t, twl);
-#pragma unroll
-    for (int e = 0; e < E / 2; ++e) {
-      const int k = t + TPL * e;
-      const int ik = L::lidx(q, k), im = L::lidx(q, k == 0 ? N / 2 : N - k);
-      cpx zk = lds[ik], zm = lds[im];
-      if (k == 0) mul.self(zk, zm);
-      else mul.pair(e, zk, zm);
-      lds[ik] = zk;
-      lds[im] = zm;
-    }
-    L::sync();
-    L::load_natural(v, lds, q, t);
-    L::sync();
-    L::inv(v, lds, q, t, twl);
-  }
 }
 
 // V factors of column pair (c, c+1) that a thread of a column pass applies,
