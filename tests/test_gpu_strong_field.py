@@ -69,12 +69,17 @@ def test_strong_field_parity_256(gpu):
     assert np.max(np.abs(dip - dref)) <= 1e-11 * np.max(np.abs(dref))
 
 
-def test_config3_full_pulse_fp64_fp32(gpu):
+def test_config3_full_pulse_fp64_fp32(gpu, tmp_path):
     """Config 3 at its full size: 1024^2, the whole 4-cycle pulse (8378 steps),
     FP64 and FP32 from the same ground state. Checked by what holds at any size:
     normalisation, the ground state barely moves before the pulse rises, a
-    finite dipole series every step; and the FP32 form against FP64."""
+    finite dipole series every step; and the FP32 form against FP64. The FP64
+    run is saved the way the reference's driver saves it (settings, a frame per
+    quarter of the pulse, the Ehrenfest series; strong_field_physics_cuda_1d.py:
+    110-122,227-237,348-354), and the HHG spectrum of its dipole series
+    (file_viwer.py:237-260) peaks at the laser frequency."""
     import time
+    from fastwigner_amd.results import ResultsWriter, hhg_spectrum, load_results
     from fastwigner_amd.wigner_moyal import WignerMoyalHIP1D
     n = 1024
     steps = int(round(T_PULSE / 0.05))
@@ -82,11 +87,25 @@ def test_config3_full_pulse_fp64_fp32(gpu):
     W0 = ground_state(n)
     print("config 3: ground state at %d^2 in %.1f s" % (n, time.perf_counter() - t0), flush=True)
     out = {}
+    path = str(tmp_path / "config3.npz")
     for prec in ("fp64", "fp32"):
         t0 = time.perf_counter()
         g = WignerMoyalHIP1D(precision=prec, **params(n))
         g.set_wignerfunction(W0)
-        g.propagate(steps)
+        if prec == "fp64":
+            tw = 0.0
+            with ResultsWriter(path, params(n), propagator=g) as rw:
+                done = 0
+                for q in range(4):
+                    chunk = (steps * (q + 1)) // 4 - done
+                    g.propagate(chunk)
+                    done += chunk
+                    t1 = time.perf_counter()
+                    rw.add_propagator_frame(g)
+                    tw += time.perf_counter() - t1
+                rw.set_ehrenfest(g)
+        else:
+            g.propagate(steps)
         W = g.wignerfunction.get()
         print("config 3: %s pulse (%d steps) in %.1f s" % (prec, steps, time.perf_counter() - t0), flush=True)
         out[prec] = (W, np.array(g.P_average_RHS), np.array(g.X_average))
@@ -101,3 +120,69 @@ def test_config3_full_pulse_fp64_fp32(gpu):
     print("config 3: %d steps, FP32 vs FP64: W rel L2 %.2e, dipole max rel %.2e" % (steps, werr, derr))
     assert werr <= 1e-3
     assert derr <= 1e-3
+    # the saved run reads back as written
+    r = load_results(path)
+    print("config 3: results file %.1f MiB, 4 frames of 8 MiB written in %.2f s"
+          % (__import__("os").path.getsize(path) / 2 ** 20, tw))
+    assert len(r["frames"]) == 4 and np.array_equal(r["frames"][-1]["wigner"], W64)
+    assert abs(r["frames"][-1]["t"] - steps * 0.05) < 1e-9 and r["settings"]["X_gridDIM"] == n
+    assert np.array_equal(r["ehrenfest"]["P_average_RHS"], d64)
+    # HHG spectrum of the dipole acceleration: the driving frequency dominates
+    om, spec = hhg_spectrum(d64, 0.05, OMEGA)
+    pos = om > 0.5
+    k = om[pos][np.argmax(spec[pos])]
+    print("config 3: HHG spectrum peak at %.3f omega" % k)
+    assert abs(k - 1.0) < 0.1
+
+
+# velocity gauge (strong_field_physics_fftw_1d.py:35-80): Mathieu lattice, K(p, t) = (p + A(t))^2 / 2
+VG_FUNCTIONS = ("__device__ double A(double t)\n{{\n"
+                "  return -F / omega * sin(omega * t) * pow(sin(omega * t / 16.), 2);\n}}\n")
+
+
+def vg_params(n, **kw):
+    d = dict(X_gridDIM=n, X_amplitude=8., P_gridDIM=n, P_amplitude=10., dt=0.01, t=0., V0=0.37, F=0.04,
+             omega=0.05698, kT=0.03, functions=VG_FUNCTIONS, V="-V0 * (1. + cos(2. * M_PI * X / X_amplitude))",
+             diff_V="V0 * 2. * M_PI / X_amplitude * sin(2. * M_PI * X / X_amplitude)",
+             K="0.5 * (P + A(t)) * (P + A(t))", diff_K="P + A(t)")
+    d.update(kw)
+    return d
+
+
+def A_np(t, F=0.04, omega=0.05698):
+    return -F / omega * np.sin(omega * t) * np.sin(omega * t / 16.) ** 2
+
+
+def test_velocity_gauge_mathieu_parity(gpu):
+    """The velocity-gauge usage of strong_field_physics_fftw_1d.py:35-80 at 256^2 (the
+    script runs 512^2): Mathieu lattice V = -V0 (1 + cos(2 pi x / X_amplitude)), the
+    time-dependent kinetic energy K(p, t) = (p + A(t))^2 / 2, so the K factors are
+    evaluated at every half-step (no table), the FFTW time argument t_{n+1}
+    (phase_time_shift = 0), and the initial state the FFTW-convention Gibbs state at
+    kT = 0.03 (:155-159). 600 steps from t = 300, where the field is on, against the
+    oracle; X_RHS = <p + A(t)> is the velocity-gauge current."""
+    from fastwigner_amd.wigner_bloch import WignerBlochHIP1D
+    from fastwigner_amd.wigner_moyal import WignerMoyalHIP1D
+    n, t0, steps = 256, 300., 600
+    b = WignerBlochHIP1D(bloch_convention="fftw", dbeta=0.01, t_initial=0.,
+                         **{k: v for k, v in vg_params(n).items() if k != "t"})
+    W0 = b.get_gibbs_state().get()
+    assert b.gibbs_steps == 3333
+    g = WignerMoyalHIP1D(phase_time_shift=0.0, **vg_params(n, t=t0))
+    g.set_wignerfunction(W0)
+    V0, L = 0.37, 8.
+    o = fo.MoyalOracle(n, 8., n, 10., 0.01, t=t0, phase_time_shift=0.0,
+                       V=lambda X, t: -V0 * (1. + np.cos(2. * np.pi * X / L)),
+                       K=lambda P, t: 0.5 * (P + A_np(t)) ** 2,
+                       diff_V=lambda X, t: V0 * 2. * np.pi / L * np.sin(2. * np.pi * X / L),
+                       diff_K=lambda P, t: P + A_np(t))
+    o.set_wignerfunction(W0)
+    W = g.propagate(steps).get()
+    o.propagate(steps)
+    err = np.linalg.norm(W - o.W) / np.linalg.norm(o.W)
+    cur = np.array(g.X_average_RHS)
+    cref = np.array([d["X_average_RHS"] for d in o.series])
+    print("velocity gauge 256^2: rel L2 %.2e, current max abs diff %.2e, max |A| %.3f"
+          % (err, np.max(np.abs(cur - cref)), np.max(np.abs(A_np(t0 + 0.01 * np.arange(steps))))))
+    assert err <= 1e-12
+    assert np.max(np.abs(cur - cref)) <= 1e-11 * np.max(np.abs(cref))
