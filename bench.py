@@ -15,7 +15,10 @@ Prints ONE JSON line. Extra fields:
   parity        relative L2 of the GPU state against the oracle after the
                 sample's steps (FP64 bar 1e-12)
   variants      FP32 form at 1024^2 and FP64 at 2048^2 with their own rates,
-                roofline fractions and errors against the FP64 oracle
+                roofline fractions and errors against the FP64 oracle; the
+                strong-field workload (config 3, time-dependent V); the
+                surrogate (config 5); the batched random-collision workload
+                (config 4)
 """
 import argparse
 import json
@@ -87,6 +90,41 @@ def roofline(n, precision, ktimes):
             "traffic": measured_traffic(name, n, precision),
             "bytes_per_launch": bytes_per_launch, "passes_per_step": len(ktimes),
             "mean_us": {k: round(v, 3) for k, v in ktimes.items()}}
+
+
+# config 3: strong-field Wigner propagation (strong_field_physics_cuda_1d.py:24-82 usage)
+OMEGA3, E03 = 0.06, 0.05
+T_PULSE3 = 4 * 2 * np.pi / OMEGA3
+CFG3 = dict(X_amplitude=60., P_amplitude=6., dt=0.05, t=0., E0=E03, omega=OMEGA3, T_pulse=T_PULSE3,
+            functions="__device__ double E(double t)\n{{\n  return (t < T_pulse) ? E0 * pow(sin(M_PI * t / T_pulse), 2)"
+                      " * sin(omega * t) : 0.;\n}}\n",
+            V="-1. / sqrt(X * X + 1.) + X * E(t)", K="0.5 * P * P", diff_V="X / pow(X * X + 1., 1.5) + E(t)",
+            diff_K="P", abs_boundary_x_p="pow(fabs(cos(0.5 * M_PI * X / X_amplitude)), 1. / 8.)")
+
+
+def strong_field_variant(n=1024, steps=2000, warmup=64):
+    """Config 3 at 1024^2, FP64: the field-free ground state (Bloch, get_ground_state),
+    then `steps` steps inside the pulse with the time-dependent V (factors evaluated
+    at every half-step, no table), the x-p mask after every step and the dipole
+    acceleration P_average_RHS recorded every step."""
+    from fastwigner_amd.wigner_bloch import WignerBlochHIP1D
+    from fastwigner_amd.wigner_moyal import WignerMoyalHIP1D
+    t0 = time.perf_counter()
+    b = WignerBlochHIP1D(X_gridDIM=n, P_gridDIM=n, dbeta=0.05, t_initial=0.,
+                         **{k: v for k, v in CFG3.items() if k != "abs_boundary_x_p"})
+    W0 = b.get_ground_state(abs_tol_purity=1e-8).get()
+    gs = time.perf_counter() - t0
+    del b
+    g = WignerMoyalHIP1D(X_gridDIM=n, P_gridDIM=n, **CFG3)
+    g.set_wignerfunction(W0)
+    rate, ms = timed_rate(g, n, steps, warmup)
+    out = {"value": rate, "unit": "grid-point*steps/s", "ms_per_step": ms, "steps": steps,
+           "t_window": [round(warmup * 0.05, 2), round((warmup + steps) * 0.05, 2)],
+           "ground_state_s": round(gs, 2), "roofline": roofline(n, "fp64", kernel_times(g)),
+           "dipole_finite": bool(np.all(np.isfinite(g.P_average_RHS)))}
+    print("[bench] strong field fp64 %dx%d: %.3e pt*steps/s, %.2f us/step" % (n, n, rate, ms * 1e3),
+          file=sys.stderr, flush=True)
+    return out
 
 
 def timed_rate(s, n, steps, warmup):
@@ -239,6 +277,10 @@ def main():
                                   "kind": "port",
                                   "sample": "%d steps of the same %dx%d FP64 workload with the scipy.fft oracle "
                                             "(oracle/fw_oracle.py), %.1f s" % (osteps, n, n, oel)}
+        _, osteps1, oel1 = oracle_sample(n, 6.0, 200, 1)
+        result["cpu_baseline"]["single_thread"] = {
+            "value": n * n * osteps1 / oel1, "unit": "grid-point*steps/s", "cores": 1,
+            "sample": "%d steps, %.1f s" % (osteps1, oel1)}
         g = make(n, "fp64")
         g.propagate(osteps)
         err = float(np.linalg.norm(g.wignerfunction.get() - o.W) / np.linalg.norm(o.W))
@@ -265,6 +307,8 @@ def main():
         r2, ms2 = timed_rate(s2, 2048, max(args.steps // 4, 100), 50)
         var["fp64_2048"] = {"value": r2, "ms_per_step": ms2, "roofline": roofline(2048, "fp64", kernel_times(s2))}
         print("[bench] fp64 2048^2: %.3e pt*steps/s" % r2, file=sys.stderr, flush=True)
+        del s2
+        var["strong_field_fp64_config3"] = strong_field_variant()
         print("[bench] surrogate: training on propagator trajectories", file=sys.stderr, flush=True)
         from fastwigner_amd.fno import surrogate_experiment
         var["surrogate_fno_config5"] = surrogate_experiment(

@@ -22,7 +22,7 @@ KCACHE = os.path.join(HERE, "_kcache")
 
 SOURCES = ["fw_runtime.cpp", "fw_wigner.cpp", "fw_rho.cpp", "fw_batch.cpp", "fw_fno.cpp"]
 EMBED = [("fw_common.h", "fw_common_h_src"), ("fw_device.h", "fw_device_h_src"), ("fw_rho.h", "fw_rho_h_src"),
-         ("fw_batch.h", "fw_batch_h_src"), ("fw_fno.h", "fw_fno_h_src")]
+         ("fw_batch.h", "fw_batch_h_src"), ("fw_fno.h", "fw_fno_h_src"), ("fw_math.h", "fw_math_h_src")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -187,6 +187,8 @@ std::string make_prelude(const fw_batch_desc* d, const fw_batch_instance* in, co
   s << "#define FW_B_EMAX " << emax << "\n";
   s << "#ifndef M_PI\n#define M_PI 3.14159265358979323846\n#endif\n";
   s << "const double dt = " << fw::fmt_double(d->dt) << ";\n";
+  // the caller's sin and cos: the branch-free FP64 ones of fw_math.h (see there)
+  s << "#include \"fw_math.h\"\n#define sin fw_user_sin\n#define cos fw_user_cos\n";
   s << "// ---- caller constants and functions ----\n";
   if (x->consts) s << x->consts << "\n";
   std::string bind;
@@ -200,6 +202,7 @@ std::string make_prelude(const fw_batch_desc* d, const fw_batch_instance* in, co
     << bind << "  return (" << or_default(x->diff_V, "0.") << ");\n}\n";
   s << "__device__ double fw_diff_K(double P, double t, const double* prm) {\n"
     << bind << "  return (" << or_default(x->diff_K, "0.") << ");\n}\n";
+  s << "#undef sin\n#undef cos\n";
   s << "#include \"fw_batch.h\"\n";
   return s.str();
 }

@@ -5,6 +5,8 @@
 // (density matrix). Needs FW_REAL (double|float) and FW_FP32 (0|1).
 #pragma once
 
+#include "fw_math.h"
+
 #ifndef FW_REAL
 #error "FW_REAL must be defined"
 #endif
@@ -292,12 +294,27 @@ struct Line {
   static constexpr int NT = P * TPL;
   static constexpr int RMAX = E < 16 ? E : 16;
   static constexpr int RAW = P * N;
+#ifndef FW_LDS_SWZ
+#define FW_LDS_SWZ 0
+#endif
+#if FW_LDS_SWZ == 1
+  static constexpr int LDS = RAW + (RAW >> 3) + 1;
+#elif FW_LDS_SWZ == 2
+  static constexpr int LDS = RAW;
+#else
   static constexpr int LDS = RAW + (RAW >> 4) + 1;
+#endif
   static constexpr bool WAVE_LOCAL = (NT <= 64) || (LM && TPL <= 64 && (64 % TPL) == 0);
 
   FWD static int lidx(int q, int i) {
     const int r = LM ? q * N + i : i * P + q;
+#if FW_LDS_SWZ == 1
+    return r + (r >> 3);
+#elif FW_LDS_SWZ == 2
+    return r ^ ((r >> 3) & 7);
+#else
     return r + (r >> 4);
+#endif
   }
   FWD static void sync() {
     if constexpr (WAVE_LOCAL) wave_sync();
@@ -544,47 +561,6 @@ FWD void sincos_quarter(float x, float& s, float& c) {
                                                                             x2 * (-1.0f / 3628800.0f)))));
 }
 #endif
-
-// FP64 sine and cosine of the phase factors, branch-free so that the factors
-// of a thread interleave. Cody-Waite reduction by pi/2 with a two-term split
-// (C1 = pi/2 rounded to double, C2 = pi/2 - C1 rounded): for |x| >= 1 every bit
-// of x - k C1 is at least 2^-52 and the difference is below 1, so the first
-// FMA is exact; the second rounds once; the neglected k (pi/2 - C1 - C2) is
-// below 1.5e-33 |k|, i.e. under 1e-17 for |x| < 2^42. The reduced
-// |r| <= pi/4 + 2^-50 goes through the Taylor series of sin to r^17 and of cos
-// to r^18 (truncation below 1e-19 at pi/4), then the quadrant rotation.
-// Accuracy: about 1 ulp for |x| < 2^42 (phases in radians); beyond that the
-// neglected term grows as |x| 1e-33.
-FWD void sincos_fp64(double x, double& s, double& c) {
-  const double k = rint(x * 0.63661977236758134308);
-  const double C1 = 1.5707963267948966192e+00;
-  const double C2 = 6.1232339957367658e-17;
-  double r = fma(-k, C1, x);
-  r = fma(-k, C2, r);
-  const double r2 = r * r;
-  double ps = -1.0 / 355687428096000.0;                 // -1/17!
-  ps = fma(ps, r2, 1.0 / 1307674368000.0);              // 1/15!
-  ps = fma(ps, r2, -1.0 / 6227020800.0);                // -1/13!
-  ps = fma(ps, r2, 1.0 / 39916800.0);                   // 1/11!
-  ps = fma(ps, r2, -1.0 / 362880.0);                    // -1/9!
-  ps = fma(ps, r2, 1.0 / 5040.0);                       // 1/7!
-  ps = fma(ps, r2, -1.0 / 120.0);                       // -1/5!
-  ps = fma(ps, r2, 1.0 / 6.0);                          // 1/3!
-  const double sn = fma(-r * r2, ps, r);                // r - r^3/3! + ...
-  double pc = 1.0 / 6402373705728000.0;                 // 1/18!
-  pc = fma(pc, r2, -1.0 / 20922789888000.0);            // -1/16!
-  pc = fma(pc, r2, 1.0 / 87178291200.0);                // 1/14!
-  pc = fma(pc, r2, -1.0 / 479001600.0);                 // -1/12!
-  pc = fma(pc, r2, 1.0 / 3628800.0);                    // 1/10!
-  pc = fma(pc, r2, -1.0 / 40320.0);                     // -1/8!
-  pc = fma(pc, r2, 1.0 / 720.0);                        // 1/6!
-  pc = fma(pc, r2, -1.0 / 24.0);                        // -1/4!
-  pc = fma(pc, r2, 0.5);                                // 1/2!
-  const double cs = fma(-r2, pc, 1.0);                  // 1 - r^2/2! + ...
-  const int q = (int)((long long)k & 3);
-  c = (q == 0) ? cs : (q == 1) ? -sn : (q == 2) ? -cs : sn;
-  s = (q == 0) ? sn : (q == 1) ? cs : (q == 2) ? -sn : -cs;
-}
 
 FWD cpx unit_phase(double ph, double m) {
 #if FW_FP32

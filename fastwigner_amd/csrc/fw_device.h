@@ -136,6 +136,34 @@ FWD cpx self_apply(cpx z, cpx fa, cpx fb, real_t s) {
   return mk(z.x * fa.x * s, z.y * fb.x * s);
 }
 
+// The x-p mask of column pair (Xa, Xb) at row momentum Pn: when the mask
+// string does not reference P (FW_MASK_XP_X_ONLY, set by the host), one value
+// per column, evaluated once per pass (ma, mb) instead of once per element.
+#ifndef FW_MASK_XP_X_ONLY
+#define FW_MASK_XP_X_ONLY 0
+#endif
+struct MaskXP {
+  bool on;
+  real_t ma, mb;
+  double Xa, Xb, tnow;
+  FWD MaskXP(bool on_, double Xa_, double Xb_, double tnow_) : on(on_), ma(1), mb(1), Xa(Xa_), Xb(Xb_), tnow(tnow_) {
+    if (FW_MASK_XP_X_ONLY && on) {
+      ma = (real_t)::fw_mask_x_p(Xa, 0.0, tnow);
+      mb = (real_t)::fw_mask_x_p(Xb, 0.0, tnow);
+    }
+  }
+  FWD void apply(cpx& w, double Pn) const {
+    if (!on) return;
+    if (FW_MASK_XP_X_ONLY) {
+      w.x *= ma;
+      w.y *= mb;
+    } else {
+      w.x *= (real_t)::fw_mask_x_p(Xa, Pn, tnow);
+      w.y *= (real_t)::fw_mask_x_p(Xb, Pn, tnow);
+    }
+  }
+};
+
 // Partial sums of the column passes of Wigner steps are stored by sum (see
 // write_partial_rows): the S0 row, which every block of the next pass adds for
 // the deferred norm, is nbp contiguous doubles.
@@ -234,68 +262,48 @@ FWD void v_factors(const Args& a, int c, int t, cpx2 (&ftab)[FW_COL_E / 2 + 1]) 
   ftab[E / 2] = *reinterpret_cast<const cpx2*>(a.vtab + vtab_index(c, N / 2));
 }
 
+// The same factors evaluated at time tau (masks at tnow), for time-dependent V.
+// One straight-line block before the spectral step: the evaluations share
+// their time-only subexpressions (a field E(t) in V, say), which the compiler
+// then computes once per pass instead of once per factor.
+FWD void v_factors_at(int c, int t, double tau, double tnow, cpx2 (&ftab)[FW_COL_E / 2 + 1]) {
+  constexpr int N = FW_NP, E = FW_COL_E;
+  const double Xa = ::dX * (c - 0.5 * ::X_gridDIM);
+  const double Xb = ::dX * (c + 1 - 0.5 * ::X_gridDIM);
+#pragma unroll
+  for (int i = 0; i <= E / 2; ++i) {
+    const double Th = ::dTheta * (i < E / 2 ? spec_k<VL>(t, i) : N / 2);
+    ftab[i].a = vfactor(Xa, Th, tau, tnow);
+    ftab[i].b = vfactor(Xb, Th, tau, tnow);
+  }
+}
+
 // Pair multiply of the column passes: real-pair separation of columns a, b,
-// their V factors (from the table row cache ftab, or evaluated at time tau,
-// masks at tnow), repacking, scale 1/N.
-template <bool use_tab>
+// their V factors from the register array ftab (read from the table or
+// evaluated before the step), repacking, scale 1/N.
 struct VMul {
   const cpx2 (&ftab)[FW_COL_E / 2 + 1];
-  double Xa, Xb, tau, tnow;
-  int t;
   FWD void pair(int i, cpx& zk, cpx& zm) const {
-    cpx fa, fb;
-    if constexpr (use_tab) {
-      fa = ftab[i].a; fb = ftab[i].b;
-    } else {
-      const double Th = ::dTheta * spec_k<VL>(t, i);
-      fa = vfactor(Xa, Th, tau, tnow); fb = vfactor(Xb, Th, tau, tnow);
-    }
-    pair_apply(zk, zm, fa, fb, (real_t)1 / (real_t)FW_NP);
+    pair_apply(zk, zm, ftab[i].a, ftab[i].b, (real_t)1 / (real_t)FW_NP);
   }
   FWD void self(cpx& z0, cpx& zn) const {
-    cpx fa0, fb0, faN, fbN;
-    if constexpr (use_tab) {
-      fa0 = ftab[0].a; fb0 = ftab[0].b; faN = ftab[FW_COL_E / 2].a; fbN = ftab[FW_COL_E / 2].b;
-    } else {
-      const double ThN = ::dTheta * (FW_NP / 2);
-      fa0 = vfactor(Xa, 0.0, tau, tnow); fb0 = vfactor(Xb, 0.0, tau, tnow);
-      faN = vfactor(Xa, ThN, tau, tnow); fbN = vfactor(Xb, ThN, tau, tnow);
-    }
     const real_t s = (real_t)1 / (real_t)FW_NP;
-    z0 = self_apply(z0, fa0, fb0, s);
-    zn = self_apply(zn, faN, fbN, s);
+    z0 = self_apply(z0, ftab[0].a, ftab[0].b, s);
+    zn = self_apply(zn, ftab[FW_COL_E / 2].a, ftab[FW_COL_E / 2].b, s);
   }
 };
 
 // Pair multiply of the row passes: rows a, b and their K factors.
-template <bool use_tab>
 struct KMul {
   const cpx (&fta)[FW_ROW_E / 2 + 1];
   const cpx (&ftb)[FW_ROW_E / 2 + 1];
-  double Pa, Pb, tau, tnow;
-  int t;
   FWD void pair(int i, cpx& zk, cpx& zm) const {
-    cpx fa, fb;
-    if constexpr (use_tab) {
-      fa = fta[i]; fb = ftb[i];
-    } else {
-      const double Lm = ::dLambda * spec_k<KL>(t, i);
-      fa = kfactor(Lm, Pa, tau, tnow); fb = kfactor(Lm, Pb, tau, tnow);
-    }
-    pair_apply(zk, zm, fa, fb, (real_t)1 / (real_t)FW_NX);
+    pair_apply(zk, zm, fta[i], ftb[i], (real_t)1 / (real_t)FW_NX);
   }
   FWD void self(cpx& z0, cpx& zn) const {
-    cpx fa0, fb0, faN, fbN;
-    if constexpr (use_tab) {
-      fa0 = fta[0]; fb0 = ftb[0]; faN = fta[FW_ROW_E / 2]; fbN = ftb[FW_ROW_E / 2];
-    } else {
-      const double LN = ::dLambda * (FW_NX / 2);
-      fa0 = kfactor(0.0, Pa, tau, tnow); fb0 = kfactor(0.0, Pb, tau, tnow);
-      faN = kfactor(LN, Pa, tau, tnow); fbN = kfactor(LN, Pb, tau, tnow);
-    }
     const real_t s = (real_t)1 / (real_t)FW_NX;
-    z0 = self_apply(z0, fa0, fb0, s);
-    zn = self_apply(zn, faN, fbN, s);
+    z0 = self_apply(z0, fta[0], ftb[0], s);
+    zn = self_apply(zn, fta[FW_ROW_E / 2], ftb[FW_ROW_E / 2], s);
   }
 };
 
@@ -331,6 +339,7 @@ FWD void vpass(const Args& a) {
   if (deferred) norm_row.load(a.partials);
   cpx2 ftab[E / 2 + 1];
   if constexpr (use_tab) v_factors(a, c, t, ftab);
+  else v_factors_at(c, t, tau, tnow, ftab);
   tws.store(twl, tid);
   if (deferred) norm_row.reduce_waves(red);
   __syncthreads();
@@ -354,13 +363,14 @@ FWD void vpass(const Args& a) {
   // spectral multiply on the theta x representation
   const double Xa = ::dX * (c - 0.5 * ::X_gridDIM);
   const double Xb = ::dX * (c + 1 - 0.5 * ::X_gridDIM);
-  VMul<use_tab> mul{ftab, Xa, Xb, tau, tnow, t};
+  VMul mul{ftab};
   spectral_step<L>(v, lds, q, t, twl, mul);
 
   double sums[NSER];
 #pragma unroll
   for (int i = 0; i < NSER; ++i) sums[i] = 0.0;
   double ca = 0.0, cb = 0.0;
+  const MaskXP mask((FL & F_LAST) && (a.flags & F_MASK_XP), Xa, Xb, tnow);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int n = t + TPL * e;
@@ -368,10 +378,7 @@ FWD void vpass(const Args& a) {
     cpx w = v[e];
     if constexpr (FL & F_LAST) {
       const double Pn = ::dP * (n - 0.5 * ::P_gridDIM);
-      if (a.flags & F_MASK_XP) {
-        w.x *= (real_t)::fw_mask_x_p(Xa, Pn, tnow);
-        w.y *= (real_t)::fw_mask_x_p(Xb, Pn, tnow);
-      }
+      mask.apply(w, Pn);
       if constexpr (FL & F_DISSIP) {
         const cpx g = *reinterpret_cast<const cpx*>(a.W0s + idx);
         w.x = w.x * (real_t)a.w_coeff + g.x;
@@ -431,6 +438,7 @@ FWD void vvpass(const Args& a) {
   for (int e = 0; e < E; ++e) v[e] = *reinterpret_cast<const cpx*>(W + (size_t)(t + TPL * e) * FW_NX + c);
   cpx2 ftab[E / 2 + 1];
   if constexpr (use_tab) v_factors(a, c, t, ftab);
+  else v_factors_at(c, t, t0 + FW_PHASE_SHIFT * ::dt, t0, ftab);
   tws.store(twl, tid);
   // the K pass of this step has read `fresh`; the next K pass uses the partials
   if (blk == 0 && tid == 0 && a.scal[SC_FRESH] != 0.0) a.scal[SC_FRESH] = 0.0;
@@ -439,22 +447,20 @@ FWD void vvpass(const Args& a) {
   const double Xa = ::dX * (c - 0.5 * ::X_gridDIM);
   const double Xb = ::dX * (c + 1 - 0.5 * ::X_gridDIM);
   // closing V/2 of step s
-  VMul<use_tab> mul0{ftab, Xa, Xb, t0 + FW_PHASE_SHIFT * ::dt, t0, t};
+  VMul mul0{ftab};
   spectral_step<L>(v, lds, q, t, twl, mul0);
   // end of step s: x-p mask, sums
   double sums[NSER];
 #pragma unroll
   for (int i = 0; i < NSER; ++i) sums[i] = 0.0;
   double ca = 0.0, cb = 0.0;
+  const MaskXP mask((a.flags & F_MASK_XP) != 0, Xa, Xb, t0);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int n = t + TPL * e;
     cpx w = v[e];
     const double Pn = ::dP * (n - 0.5 * ::P_gridDIM);
-    if (a.flags & F_MASK_XP) {
-      w.x *= (real_t)::fw_mask_x_p(Xa, Pn, t0);
-      w.y *= (real_t)::fw_mask_x_p(Xb, Pn, t0);
-    }
+    mask.apply(w, Pn);
     const double wa = (double)w.x, wb = (double)w.y, ws = wa + wb;
     ca += wa; cb += wb;
     sums[2] += Pn * ws;
@@ -475,7 +481,8 @@ FWD void vvpass(const Args& a) {
   // of keeping those of the closing step live across it (register pressure).
   int t2 = t;
   asm volatile("" : "+v"(t2));
-  VMul<use_tab> mul1{ftab, Xa, Xb, t1 + FW_PHASE_SHIFT * ::dt, t1, t2};
+  if constexpr (!use_tab) v_factors_at(c, t2, t1 + FW_PHASE_SHIFT * ::dt, t1, ftab);
+  VMul mul1{ftab};
   spectral_step<L>(v, lds, q, t2, twl, mul1);
 #pragma unroll
   for (int e = 0; e < E; ++e) *reinterpret_cast<cpx*>(W + (size_t)(t + TPL * e) * FW_NX + c) = v[e];
@@ -525,6 +532,16 @@ FWD void kpass(const Args& a) {
     }
     fta[E / 2] = a.ktab[(size_t)r * KW + N / 2];
     ftb[E / 2] = a.ktab[(size_t)(r + 1) * KW + N / 2];
+  } else {
+    // time-dependent K: evaluated here in one block (see v_factors_at)
+    const double Pa = ::dP * (r - 0.5 * ::P_gridDIM);
+    const double Pb = ::dP * (r + 1 - 0.5 * ::P_gridDIM);
+#pragma unroll
+    for (int i = 0; i <= E / 2; ++i) {
+      const double Lm = ::dLambda * (i < E / 2 ? spec_k<L>(t, i) : N / 2);
+      fta[i] = kfactor(Lm, Pa, tau, tnow);
+      ftb[i] = kfactor(Lm, Pb, tau, tnow);
+    }
   }
   tws.store(twl, tid);
   if (deferred) norm_row.reduce_waves(red);
@@ -536,9 +553,7 @@ FWD void kpass(const Args& a) {
 #pragma unroll
     for (int e = 0; e < E; ++e) { v[e].x *= rn; v[e].y *= rn; }
   }
-  const double Pa = ::dP * (r - 0.5 * ::P_gridDIM);
-  const double Pb = ::dP * (r + 1 - 0.5 * ::P_gridDIM);
-  KMul<use_tab> mul{fta, ftb, Pa, Pb, tau, tnow, t};
+  KMul mul{fta, ftb};
   spectral_step<L>(v, lds, q, t, twl, mul);
 #pragma unroll
   for (int e = 0; e < E; ++e) {

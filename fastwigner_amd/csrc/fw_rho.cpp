@@ -156,6 +156,8 @@ std::string make_prelude(const fw_rho_desc* d, const fw_rho_exprs* x, int re, in
   s << "const double dt = " << fw::fmt_double(d->dt) << ";\n";
   s << "const double dX = " << fw::fmt_double(dX) << ";\n";
   s << "const double dP = " << fw::fmt_double(dP) << ";\n";
+  // the caller's sin and cos: the branch-free FP64 ones of fw_math.h (see there)
+  s << "#include \"fw_math.h\"\n#define sin fw_user_sin\n#define cos fw_user_cos\n";
   s << "// ---- caller constants and functions ----\n";
   if (x->consts) s << x->consts << "\n";
   s << "// ---- spliced expressions (rho_vneumann_cuda_1d.py:631-684) ----\n";
@@ -167,6 +169,7 @@ std::string make_prelude(const fw_rho_desc* d, const fw_rho_exprs* x, int re, in
     << or_default(x->abs_boundary_x, "1.") << "); }\n";
   s << "__device__ double fw_mask_p(double P, double P_prime, double t) { return ("
     << or_default(x->abs_boundary_p, "1.") << "); }\n";
+  s << "#undef sin\n#undef cos\n";
   return s.str();
 }
 

@@ -30,6 +30,7 @@ extern const char fw_device_h_src[];
 extern const char fw_rho_h_src[] __attribute__((weak));
 extern const char fw_batch_h_src[] __attribute__((weak));
 extern const char fw_fno_h_src[] __attribute__((weak));
+extern const char fw_math_h_src[];
 
 namespace fw {
 
@@ -95,11 +96,12 @@ int compile(const std::string& src, const std::string& arch, std::vector<char>& 
                                    "-Wno-unused-but-set-variable"};
   // the embedded headers hipRTC may include
   std::vector<const char*> hdr_names, hdrs;
-  const char* names_all[] = {"fw_common.h", "fw_device.h", "fw_rho.h", "fw_batch.h", "fw_fno.h"};
+  const char* names_all[] = {"fw_common.h", "fw_device.h", "fw_rho.h", "fw_batch.h", "fw_fno.h", "fw_math.h"};
   const char* srcs_all[] = {fw_common_h_src, fw_device_h_src, &fw_rho_h_src ? fw_rho_h_src : nullptr,
-                            &fw_batch_h_src ? fw_batch_h_src : nullptr, &fw_fno_h_src ? fw_fno_h_src : nullptr};
+                            &fw_batch_h_src ? fw_batch_h_src : nullptr, &fw_fno_h_src ? fw_fno_h_src : nullptr,
+                            fw_math_h_src};
   std::string key_src = src + "\n//" + arch;
-  for (int i = 0; i < 5; ++i)
+  for (int i = 0; i < 6; ++i)
     if (srcs_all[i]) {
       hdr_names.push_back(names_all[i]);
       hdrs.push_back(srcs_all[i]);

@@ -172,15 +172,18 @@ int resolve_tiles(const fw_wigner_desc* d, Tiles& t) {
   return FW_OK;
 }
 
-// Does the expression reference the time argument (identifier token "t")?
-bool uses_t(const char* s) {
+// Does the expression reference the identifier token `tok` (e.g. "t", "P")?
+bool uses_token(const char* s, const char* tok) {
   if (!s) return false;
-  const size_t n = strlen(s);
+  const size_t n = strlen(s), m = strlen(tok);
   auto ident = [](char c) { return isalnum((unsigned char)c) || c == '_'; };
-  for (size_t i = 0; i < n; ++i)
-    if (s[i] == 't' && (i == 0 || !ident(s[i - 1])) && (i + 1 == n || !ident(s[i + 1]))) return true;
+  for (size_t i = 0; i + m <= n; ++i)
+    if (strncmp(s + i, tok, m) == 0 && (i == 0 || !ident(s[i - 1])) && (i + m == n || !ident(s[i + m])))
+      return true;
   return false;
 }
+// Does the expression reference the time argument (identifier token "t")?
+bool uses_t(const char* s) { return uses_token(s, "t"); }
 
 std::string or_default(const char* s, const char* d) { return (s && *s) ? std::string(s) : std::string(d); }
 
@@ -214,6 +217,9 @@ std::string make_prelude(const fw_wigner_desc* d, const fw_wigner_exprs* x, cons
   s << "const double dP = " << fw::fmt_double(dP) << ";\n";
   s << "const double dLambda = " << fw::fmt_double(dL) << ";\n";
   s << "const double dTheta = " << fw::fmt_double(dT) << ";\n";
+  s << "#define FW_MASK_XP_X_ONLY " << (uses_token(x->abs_boundary_x_p, "P") ? 0 : 1) << "\n";
+  // the caller's sin and cos: the branch-free FP64 ones of fw_math.h (see there)
+  s << "#include \"fw_math.h\"\n#define sin fw_user_sin\n#define cos fw_user_cos\n";
   s << "// ---- caller constants and functions ----\n";
   if (x->consts) s << x->consts << "\n";
   s << "// ---- spliced expressions ----\n";
@@ -229,6 +235,7 @@ std::string make_prelude(const fw_wigner_desc* d, const fw_wigner_exprs* x, cons
     << or_default(x->abs_boundary_lambda_p, "1.") << "); }\n";
   s << "__device__ double fw_mask_x_p(double X, double P, double t) { return ("
     << or_default(x->abs_boundary_x_p, "1.") << "); }\n";
+  s << "#undef sin\n#undef cos\n";
   return s.str();
 }
 
