@@ -112,60 +112,22 @@ FWD void b_pair_apply(cpx& zk, cpx& zm, cpx fa, cpx fb, real_t s) {
 }
 FWD cpx b_self_apply(cpx z, cpx fa, cpx fb, real_t s) { return mk(z.x * fa.x * s, z.y * fb.x * s); }
 
-// Pair multiplies of the batched passes (the callbacks of spectral_step): the
-// factors of item i at spectral index spec_k<L>(t, i), evaluated here.
-template <class L, int N, int KIND>
-struct BVMul {
-  const BArgs& a;
-  const double* prm;
-  int i, t;
-  double Xa, Xb, tau, dTh, db;
-  FWD void factors(double Th, cpx& fa, cpx& fb) const {
-    if constexpr (KIND == BK_BLOCH) {
-      fa = b_bloch_v(a, i, Xa, Th, prm, db); fb = b_bloch_v(a, i, Xb, Th, prm, db);
-    } else {
-      fa = b_vfactor(Xa, Th, tau, prm); fb = b_vfactor(Xb, Th, tau, prm);
-    }
-  }
+// Pair multiply of the batched passes (the callback of spectral_step) with the
+// factors of item i in registers: fa[i], fb[i] for lines a, b; slot E/2 holds
+// the factors at the Nyquist index. The factors are evaluated before the step
+// in one straight-line block (b_factors), where the evaluations share their
+// instance- and time-only subexpressions.
+template <int N, int E>
+struct BMul {
+  const cpx (&fa)[E / 2 + 1];
+  const cpx (&fb)[E / 2 + 1];
   FWD void pair(int item, cpx& zk, cpx& zm) const {
-    cpx fa, fb;
-    factors(dTh * spec_k<L>(t, item), fa, fb);
-    b_pair_apply(zk, zm, fa, fb, (real_t)1 / (real_t)N);
+    b_pair_apply(zk, zm, fa[item], fb[item], (real_t)1 / (real_t)N);
   }
   FWD void self(cpx& z0, cpx& zn) const {
     const real_t s = (real_t)1 / (real_t)N;
-    cpx fa, fb;
-    factors(0.0, fa, fb);
-    z0 = b_self_apply(z0, fa, fb, s);
-    factors(dTh * (N / 2), fa, fb);
-    zn = b_self_apply(zn, fa, fb, s);
-  }
-};
-template <class L, int N, int KIND>
-struct BKMul {
-  const BArgs& a;
-  const double* prm;
-  int i, t;
-  double Pa, Pb, tau, dL, db;
-  FWD void factors(double Lm, cpx& fa, cpx& fb) const {
-    if constexpr (KIND == BK_BLOCH) {
-      fa = b_bloch_k(a, i, Lm, Pa, prm, db); fb = b_bloch_k(a, i, Lm, Pb, prm, db);
-    } else {
-      fa = b_kfactor(Lm, Pa, tau, prm); fb = b_kfactor(Lm, Pb, tau, prm);
-    }
-  }
-  FWD void pair(int item, cpx& zk, cpx& zm) const {
-    cpx fa, fb;
-    factors(dL * spec_k<L>(t, item), fa, fb);
-    b_pair_apply(zk, zm, fa, fb, (real_t)1 / (real_t)N);
-  }
-  FWD void self(cpx& z0, cpx& zn) const {
-    const real_t s = (real_t)1 / (real_t)N;
-    cpx fa, fb;
-    factors(0.0, fa, fb);
-    z0 = b_self_apply(z0, fa, fb, s);
-    factors(dL * (N / 2), fa, fb);
-    zn = b_self_apply(zn, fa, fb, s);
+    z0 = b_self_apply(z0, fa[0], fb[0], s);
+    zn = b_self_apply(zn, fa[E / 2], fb[E / 2], s);
   }
 };
 
@@ -214,7 +176,17 @@ FWD void b_vpass(const BArgs& a, int i, int tile, int tid, cpx* lds, const tw_t*
   }
   __syncthreads();
   const double Xa = dX * (c - 0.5 * N), Xb = dX * (c + 1 - 0.5 * N);
-  BVMul<L, N, KIND> mul{a, prm, i, t, Xa, Xb, tau, dTh, in->dbeta};
+  cpx fa[E / 2 + 1], fb[E / 2 + 1];
+#pragma unroll
+  for (int k = 0; k <= E / 2; ++k) {
+    const double Th = dTh * (k < E / 2 ? spec_k<L>(t, k) : N / 2);
+    if constexpr (KIND == BK_BLOCH) {
+      fa[k] = b_bloch_v(a, i, Xa, Th, prm, in->dbeta); fb[k] = b_bloch_v(a, i, Xb, Th, prm, in->dbeta);
+    } else {
+      fa[k] = b_vfactor(Xa, Th, tau, prm); fb[k] = b_vfactor(Xb, Th, tau, prm);
+    }
+  }
+  BMul<N, E> mul{fa, fb};
   spectral_step<L>(v, lds, q, t, twl, mul);
 
   double sums[NSER];
@@ -274,9 +246,19 @@ FWD void b_kpass(const BArgs& a, int i, int tile, int tid, cpx* lds, const tw_t*
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = mk(Wa[t + TPL * e], Wb[t + TPL * e]);
   __syncthreads();
-  const double dP = in->dP;
+  const double dP = in->dP, dL = in->dLambda;
   const double Pa = dP * (r - 0.5 * N), Pb = dP * (r + 1 - 0.5 * N);
-  BKMul<L, N, KIND> mul{a, prm, i, t, Pa, Pb, tau, in->dLambda, in->dbeta};
+  cpx fa[E / 2 + 1], fb[E / 2 + 1];
+#pragma unroll
+  for (int k = 0; k <= E / 2; ++k) {
+    const double Lm = dL * (k < E / 2 ? spec_k<L>(t, k) : N / 2);
+    if constexpr (KIND == BK_BLOCH) {
+      fa[k] = b_bloch_k(a, i, Lm, Pa, prm, in->dbeta); fb[k] = b_bloch_k(a, i, Lm, Pb, prm, in->dbeta);
+    } else {
+      fa[k] = b_kfactor(Lm, Pa, tau, prm); fb[k] = b_kfactor(Lm, Pb, tau, prm);
+    }
+  }
+  BMul<N, E> mul{fa, fb};
   spectral_step<L>(v, lds, q, t, twl, mul);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
