@@ -17,8 +17,8 @@ Prints ONE JSON line. Extra fields:
   variants      FP32 form at 1024^2 and FP64 at 2048^2 with their own rates,
                 roofline fractions and errors against the FP64 oracle; the
                 strong-field workload (config 3, time-dependent V); the
-                surrogate (config 5); the batched random-collision workload
-                (config 4)
+                von Neumann density-matrix step (config 2); the surrogate
+                (config 5); the batched random-collision workload (config 4)
 """
 import argparse
 import json
@@ -124,6 +124,35 @@ def strong_field_variant(n=1024, steps=2000, warmup=64):
            "dipole_finite": bool(np.all(np.isfinite(g.P_average_RHS)))}
     print("[bench] strong field fp64 %dx%d: %.3e pt*steps/s, %.2f us/step" % (n, n, rate, ms * 1e3),
           file=sys.stderr, flush=True)
+    return out
+
+
+def rho_variant(n=1024, steps=1000, warmup=64):
+    """Config 2, density-matrix part: the double-well thermal state at 1024^2
+    (x in [-10, 10), kT = 0.2, RhoBloch from rho = I), then von Neumann steps
+    (dt = 0.005) with the x-family observables every step; complex128, three
+    passes per step of 32 B per matrix element each (96 B per element per step)."""
+    from fastwigner_amd.rho_bloch import RhoBlochHIP1D
+    from fastwigner_amd.rho_vneumann import RhoVNeumannHIP1D
+    V, K = "-0.5*X*X + X*X*X*X/20.", "0.5*P*P"
+    b = RhoBlochHIP1D(X_gridDIM=n, X_amplitude=10., kT=0.2, dbeta=0.005, t=0., V=V, K=K)
+    rho_th = b.get_gibbs_state().get()
+    del b
+    vn = RhoVNeumannHIP1D(X_gridDIM=n, X_amplitude=10., dt=0.005, t=0., V=V, K=K, diff_V="-X + X*X*X/5.",
+                          diff_K="P")
+    vn.set_rho(rho_th)
+    vn.propagate(warmup)
+    t0 = time.perf_counter()
+    vn.propagate(steps)
+    el = time.perf_counter() - t0
+    us = el * 1e6 / steps
+    achieved = 96.0 * n * n / (us * 1e-6) / 1e9
+    out = {"value": n * n * steps / el, "unit": "matrix-element*steps/s", "ms_per_step": el * 1e3 / steps,
+           "steps": steps, "dtype": "c128",
+           "roofline": {"bound": "hbm", "kernel": "von Neumann step (three passes)", "achieved": round(achieved, 1),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                        "traffic": None, "bytes_per_step": 96 * n * n}}
+    print("[bench] von Neumann c128 %dx%d: %.1f us/step" % (n, n, us), file=sys.stderr, flush=True)
     return out
 
 
@@ -309,6 +338,7 @@ def main():
         print("[bench] fp64 2048^2: %.3e pt*steps/s" % r2, file=sys.stderr, flush=True)
         del s2
         var["strong_field_fp64_config3"] = strong_field_variant()
+        var["rho_vneumann_c128_config2"] = rho_variant()
         print("[bench] surrogate: training on propagator trajectories", file=sys.stderr, flush=True)
         from fastwigner_amd.fno import surrogate_experiment
         var["surrogate_fno_config5"] = surrogate_experiment(

@@ -56,34 +56,37 @@ struct RArgs {
 FWD double xcoord(int j) { return ::dX * (j - 0.5 * ::X_gridDIM); }
 FWD double pcoord(int k) { return ::dP * ((k + FW_N / 2) % FW_N - 0.5 * FW_N); }
 
-// Complex norm of the previous step from the partials of pass C (same bits in
-// every block); block 0 publishes the previous step's sums to series[step] and
-// raises the purity guard. Returns false when the step must not run.
+// The partial sums of pass C are stored by sum (write_partial_rows): sum r of
+// block b at partials[r * nbp + b], so each row the next pass adds is nbp
+// contiguous doubles.
+// Complex norm of the previous step from the trace rows (the same bits in
+// every block). With the purity guard armed, `stop` tells whether the step
+// must not run; the caller then publishes the sums (rho_publish) and returns.
 template <int NT>
-FWD bool rho_previous(const RArgs& a, int blk, double* red, cpx& nrm) {
-  const double tr_re = sum_partials<NT, RSER>(a.partials, a.nbp, R_TR, red);
-  const double tr_im = sum_partials<NT, RSER>(a.partials, a.nbp, R_TR + 1, red);
+FWD void rho_previous(const RArgs& a, double* red, cpx& nrm, bool& stop) {
+  const double tr_re = sum_partials<NT, 1>(a.partials + (size_t)R_TR * a.nbp, a.nbp, 0, red);
+  const double tr_im = sum_partials<NT, 1>(a.partials + (size_t)(R_TR + 1) * a.nbp, a.nbp, 0, red);
   // c = 1 / (Tr rho * dX), complex, as rho /= trace(rho) * dX (rho_vneumann_cuda_1d.py:369)
   const double d = (tr_re * tr_re + tr_im * tr_im) * ::dX;
   nrm = mk((real_t)(tr_re / d), (real_t)(-tr_im / d));
-  bool stop = false;
+  stop = false;
   if (a.guard > 0.0) {
-    const double pur = sum_partials<NT, RSER>(a.partials, a.nbp, R_PUR, red);
+    const double pur = sum_partials<NT, 1>(a.partials + (size_t)R_PUR * a.nbp, a.nbp, 0, red);
     const double purity = pur * ::dX * ::dX / ((tr_re * tr_re + tr_im * tr_im) * ::dX * ::dX);
     stop = !(purity < a.guard);
   }
-  if (blk == 0) {
-    double tot[R_PUR + 1];
-#pragma unroll
-    for (int i = 0; i <= R_PUR; ++i)
-      tot[i] = (i == R_TR) ? tr_re : (i == R_TR + 1) ? tr_im : sum_partials<NT, RSER>(a.partials, a.nbp, i, red);
-    if (threadIdx.x == 0) {
-#pragma unroll
-      for (int i = 0; i <= R_PUR; ++i) a.series[(size_t)a.step * RSER + i] = tot[i];
-      if (stop) a.scal[RC_STOP] = (double)(a.step + 1);
-    }
+}
+
+// The previous step's sums 0..R_PUR to series[step]: block b of nb adds rows
+// b, b + nb, ...; block 0 raises the purity guard. Called by whole blocks,
+// after their own work.
+template <int NT>
+FWD void rho_publish(const RArgs& a, int blk, int nb, double* red, bool stop) {
+  for (int r = blk; r <= R_PUR; r += nb) {
+    const double tot = sum_partials<NT, 1>(a.partials + (size_t)r * a.nbp, a.nbp, 0, red);
+    if (threadIdx.x == 0) a.series[(size_t)a.step * RSER + r] = tot;
   }
-  return !stop;
+  if (stop && blk == 0 && threadIdx.x == 0) a.scal[RC_STOP] = (double)(a.step + 1);
 }
 
 FWD cpx rowfac(const cpx* __restrict__ u, int s, int j) { return u[(size_t)s * FW_N + j]; }
@@ -92,16 +95,23 @@ FWD cpx rowfac(const cpx* __restrict__ u, int s, int j) { return u[(size_t)s * F
 // Pass A / C: rows (along j). PASS_A: [norm], expV, forward FFT.
 // PASS_C: inverse FFT, expV, sums.
 // ---------------------------------------------------------------------------
-using RL = Line<FW_N, FW_ROW_E, FW_ROW_P, true>;
-using CL = Line<FW_N, FW_COL_E, FW_COL_P, false>;
+// FP64 steps compute their twiddles in the kernel (one sincospi per
+// butterfly, powers by products: no table staged in LDS, as the Wigner passes
+// do); the FP32 form keeps the staged compensated table.
+#ifndef FW_RHO_TW
+#define FW_RHO_TW (FW_FP32 ? 0 : 2)
+#endif
+using RL = Line<FW_N, FW_ROW_E, FW_ROW_P, true, FW_N, FW_RHO_TW>;
+using CL = Line<FW_N, FW_COL_E, FW_COL_P, false, FW_N, FW_RHO_TW>;
 
 template <bool PASS_A, int FL>
 FWD void rho_row(const RArgs& a) {
   using L = RL;
   constexpr int N = FW_N, E = FW_ROW_E, P = FW_ROW_P, TPL = L::TPL, NT = L::NT;
   constexpr int TWN = (N + NT - 1) / NT;
+  constexpr bool STAGE = L::TW::COUNT > 0;   // a twiddle table in LDS (FP32)
   __shared__ cpx lds[L::LDS];
-  __shared__ tw_t twl[TwLds<N>::SIZE];
+  __shared__ tw_t twl[STAGE ? TwLds<N>::SIZE : 1];
   __shared__ double red[RSER * 32];
   if (a.scal[RC_STOP] != 0.0) return;
   const int tid = threadIdx.x;
@@ -113,8 +123,10 @@ FWD void rho_row(const RArgs& a) {
   const double tnow = a.times[a.step];
 
   tw_t twr[TWN];
+  if constexpr (STAGE) {
 #pragma unroll
-  for (int r = 0; r < TWN; ++r) twr[r] = a.tw[(tid + r * NT) & (N - 1)];
+    for (int r = 0; r < TWN; ++r) twr[r] = a.tw[(tid + r * NT) & (N - 1)];
+  }
   cpx v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = row[t + TPL * e];
@@ -122,16 +134,25 @@ FWD void rho_row(const RArgs& a) {
 #pragma unroll
   for (int e = 0; e < E; ++e) fj[e] = rowfac(a.uv, a.step, t + TPL * e);
   const cpx fi = cconj(rowfac(a.uv, a.step, i));
+  if constexpr (STAGE) {
 #pragma unroll
-  for (int r = 0; r < TWN; ++r)
-    if ((N % NT) == 0 || tid + r * NT < N) twl[twpad(tid + r * NT)] = twr[r];
+    for (int r = 0; r < TWN; ++r)
+      if ((N % NT) == 0 || tid + r * NT < N) twl[twpad(tid + r * NT)] = twr[r];
+  }
   const double Xi = xcoord(i);
 
+  bool deferred = false;   // pass A of a step after the first: the previous step's norm and sums
   if constexpr (PASS_A) {
     cpx nrm = mk((real_t)1, (real_t)0);
     if constexpr (FL & RF_FIRST) {
       if (a.scal[RC_FRESH] == 0.0) {
-        if (!rho_previous<NT>(a, blk, red, nrm)) return;
+        deferred = true;
+        bool stop;
+        rho_previous<NT>(a, red, nrm, stop);
+        if (stop) {
+          rho_publish<NT>(a, blk, nb, red, true);
+          return;
+        }
       } else {
         __syncthreads();
       }
@@ -150,6 +171,8 @@ FWD void rho_row(const RArgs& a) {
     L::template fwd<false>(v, lds, q, t, twl);
 #pragma unroll
     for (int e = 0; e < E; ++e) row[t + TPL * e] = v[e];
+    // the previous step's sums, after this block's own work
+    if (deferred) rho_publish<NT>(a, blk, nb, red, false);
   } else {
     __syncthreads();
     L::inv(v, lds, q, t, twl);
@@ -177,7 +200,7 @@ FWD void rho_row(const RArgs& a) {
         s[R_V] += wr * vv; s[R_V + 1] += wi * vv;
       }
     }
-    write_partials<NT, RSER>(s, red, a.partials, blk);
+    write_partial_rows<NT, RSER>(s, red, a.partials, blk, nb);
   }
 }
 
@@ -189,8 +212,9 @@ FWD void rho_col(const RArgs& a) {
   using L = CL;
   constexpr int N = FW_N, E = FW_COL_E, P = FW_COL_P, TPL = L::TPL, NT = L::NT;
   constexpr int TWN = (N + NT - 1) / NT;
+  constexpr bool STAGE = L::TW::COUNT > 0;   // a twiddle table in LDS (FP32)
   __shared__ cpx lds[L::LDS];
-  __shared__ tw_t twl[TwLds<N>::SIZE];
+  __shared__ tw_t twl[STAGE ? TwLds<N>::SIZE : 1];
   if (a.scal[RC_STOP] != 0.0) return;
   const int tid = threadIdx.x;
   const int q = tid % P, t = tid / P;
@@ -201,8 +225,10 @@ FWD void rho_col(const RArgs& a) {
   const double tnow = a.times[a.step];
 
   tw_t twr[TWN];
+  if constexpr (STAGE) {
 #pragma unroll
-  for (int r = 0; r < TWN; ++r) twr[r] = a.tw[(tid + r * NT) & (N - 1)];
+    for (int r = 0; r < TWN; ++r) twr[r] = a.tw[(tid + r * NT) & (N - 1)];
+  }
   cpx v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = a.rho[(size_t)(t + TPL * e) * N + j];
@@ -210,9 +236,11 @@ FWD void rho_col(const RArgs& a) {
 #pragma unroll
   for (int e = 0; e < E; ++e) wk[e] = cconj(rowfac(a.uk, a.step, t + TPL * e));
   const cpx wj = rowfac(a.uk, a.step, j);
+  if constexpr (STAGE) {
 #pragma unroll
-  for (int r = 0; r < TWN; ++r)
-    if ((N % NT) == 0 || tid + r * NT < N) twl[twpad(tid + r * NT)] = twr[r];
+    for (int r = 0; r < TWN; ++r)
+      if ((N % NT) == 0 || tid + r * NT < N) twl[twpad(tid + r * NT)] = twr[r];
+  }
   __syncthreads();
   L::template fwd<false>(v, lds, q, t, twl);
   const real_t sN = (real_t)1 / (real_t)N;
@@ -225,7 +253,12 @@ FWD void rho_col(const RArgs& a) {
     v[e] = cmul(v[e], f);
   }
   L::sync();
-  L::inv(v, lds, q, t, twl);
+  // the lane index passes through an empty asm, so that the compiler
+  // recomputes the inverse's twiddles and LDS indices instead of keeping the
+  // forward transform's live across it (register pressure)
+  int t2 = t;
+  asm volatile("" : "+v"(t2));
+  L::inv(v, lds, q, t2, twl);
 #pragma unroll
   for (int e = 0; e < E; ++e) a.rho[(size_t)(t + TPL * e) * N + j] = v[e];
 }
@@ -326,7 +359,9 @@ FWD void rho_apply_norm(const RArgs& a, int from_partials) {
   __shared__ double red[RSER * 4];
   cpx c;
   if (from_partials) {
-    (void)rho_previous<256>(a, blockIdx.x, red, c);   // called with guard = 0
+    bool stop;   // the guard is off (guard = 0) at the end of a run
+    rho_previous<256>(a, red, c, stop);
+    rho_publish<256>(a, blockIdx.x, gridDim.x, red, false);
   } else {
     c = mk((real_t)a.scal[0], (real_t)a.scal[3]);
   }
@@ -381,14 +416,16 @@ FWD void rho_wigner_pass(const RArgs& a, const cpx* __restrict__ src, cpx* __res
   using L = typename pick<MODE == 0, RL, CL>::type;
   constexpr int N = FW_N, E = (MODE == 0) ? FW_ROW_E : FW_COL_E, P = (MODE == 0) ? FW_ROW_P : FW_COL_P;
   constexpr int TPL = L::TPL, NT = L::NT;
+  constexpr bool STAGE = L::TW::COUNT > 0;
   __shared__ cpx lds[L::LDS];
-  __shared__ tw_t twl[TwLds<N>::SIZE];
+  __shared__ tw_t twl[STAGE ? TwLds<N>::SIZE : 1];
   __shared__ double red[RSER * 32];
   const int tid = threadIdx.x;
   const int t = (MODE == 0) ? tid % TPL : tid / P;
   const int q = (MODE == 0) ? tid / TPL : tid % P;
   const int line = blockIdx.x * P + q;
-  for (int r = tid; r < N; r += NT) twl[twpad(r)] = a.tw[r];
+  if constexpr (STAGE)
+    for (int r = tid; r < N; r += NT) twl[twpad(r)] = a.tw[r];
   auto at = [&](int e) -> size_t {
     const int n = t + TPL * e;
     return (MODE == 0) ? (size_t)line * N + n : (size_t)n * N + line;
