@@ -50,6 +50,19 @@ namespace fwk {
 // host before a run, cleared by the first K pass). SC_OUT.. = 9 moments.
 enum { SC_NORM = 0, SC_STOP = 1, SC_FRESH = 2, SC_OUT = 8 };
 
+// Weights of the theta-spectrum sums of the fused column pass (vvpass_theta),
+// entry k <= N_p/2, from F_k = sum_n f(P_n) exp(-2 pi i n k / N_p):
+//   0 < k < N_p/2: a = Im 2 F_k(P) (= dP N cot(pi k / N)), b = Re 2 F_k(P^2),
+//                  kk = 2 F_k(K), dk = 2 F_k(K') (re, im);
+//   k = 0, N_p/2:  a = F_k(P), b = F_k(P^2), kk = (F_k(K), 0), dk = (F_k(K'), 0).
+// For P_n = dP (n - N/2) and 0 < k < N/2, Re 2 F_k(P) = -dP N and Im F_k(P^2) = 0
+// exactly (F_k(P) = dP N (-1/2 + i cot(pi k/N)/2), F_k(P^2) = dP^2 N / (2 sin^2(pi k/N))),
+// so these two are not stored.
+struct __attribute__((aligned(16))) SumW {
+  double a, b;
+  double kk[2], dk[2];
+};
+
 struct Args {
   real_t* W;                   // state
   const real_t* W0s;           // (1-w) * Gibbs state for the dissipator, or null
@@ -62,6 +75,7 @@ struct Args {
   double* series;              // NSER sums per step
   const cpx* vtab;             // cached V factors, (N_p/2+1) x N_x, or null
   const cpx* ktab;             // cached K factors, N_p x (N_x/2+1), or null
+  const SumW* sw;              // theta-spectrum sum weights, N_p/2+1, or null
   int step;
   int flags;
   int nbp;                     // number of per-block partials of the last V pass
@@ -307,6 +321,68 @@ struct KMul {
   }
 };
 
+// ---------------------------------------------------------------------------
+// Sums on the theta spectrum. After the closing V/2 a column is
+//   a'_n = sum_k alpha_k exp(+2 pi i n k / N),   alpha_k = A_k f_k / N
+// (A_k its FFT along p, f_k the factor), so for a real function f of P
+//   sum_n f(P_n) a'_n = sum_k alpha_k conj(F_k),   sum_n a'_n^2 = N sum_k |alpha_k|^2
+// with F_k = sum_n f(P_n) exp(-2 pi i n k / N) (Parseval). a' is real, so k and
+// N - k contribute conjugate terms and the sums run over k <= N/2 with weights
+// c_k (SumW). When nothing acts on W in x-p between the closing V/2 of step s
+// and the opening V/2 of step s+1 (no x-p mask) and K, K' do not depend on t,
+// the column pass takes the nine sums of step s on alpha_k and multiplies by
+// the opening factors in the same registers: one FFT pair per pass instead of
+// two, the same W and the same sums in exact arithmetic.
+// ---------------------------------------------------------------------------
+#ifndef FW_THETA_SUMS
+#define FW_THETA_SUMS 0
+#endif
+struct VVMul {
+  const cpx2 (&f0)[FW_COL_E / 2 + 1];   // closing V/2 (step s)
+  const cpx2 (&f1)[FW_COL_E / 2 + 1];   // opening V/2 (step s+1)
+  const SumW* __restrict__ sw;          // weights, read at the multiply (registers)
+  int t;                                // lane of the line
+  double acc[4];                        // sum c_k Re(sigma_k conj F_k), sigma = alpha_a + alpha_b,
+                                        // for P, P^2, K', K
+  double pur;                           // sum c_k (|alpha_a|^2 + |alpha_b|^2)
+  double a0, b0;                        // alpha_0 of columns a, b (thread 0 of a line)
+  FWD void pair(int i, cpx& zk, cpx& zm) {
+    const real_t h = (real_t)0.5 / (real_t)FW_NP;
+    const cpx A = mk(h * (zk.x + zm.x), h * (zk.y - zm.y));
+    const cpx B = mk(h * (zk.y + zm.y), h * (zm.x - zk.x));
+    const cpx A1 = cmul(A, f0[i].a), B1 = cmul(B, f0[i].b);
+    const double sx = (double)A1.x + (double)B1.x, sy = (double)A1.y + (double)B1.y;
+    const SumW w = sw[spec_k<VL>(t, i)];
+    acc[0] += sy * w.a - sx * (::dP * FW_NP);
+    acc[1] += sx * w.b;
+    acc[2] += sx * w.dk[0] + sy * w.dk[1];
+    acc[3] += sx * w.kk[0] + sy * w.kk[1];
+    const double ax = A1.x, ay = A1.y, bx = B1.x, by = B1.y;
+    pur += 2.0 * (ax * ax + ay * ay + bx * bx + by * by);
+    const cpx A2 = cmul(A1, f1[i].a), B2 = cmul(B1, f1[i].b);
+    zk = mk(A2.x - B2.y, A2.y + B2.x);
+    zm = mk(A2.x + B2.y, B2.x - A2.y);
+  }
+  // DC and Nyquist: real part times the real part of the factor (c2r rule)
+  FWD void self(cpx& z0, cpx& zn) {
+    constexpr int E2 = FW_COL_E / 2;
+    const real_t s = (real_t)1 / (real_t)FW_NP;
+    const real_t ra = z0.x * s * f0[0].a.x, rb = z0.y * s * f0[0].b.x;
+    const real_t na = zn.x * s * f0[E2].a.x, nb = zn.y * s * f0[E2].b.x;
+    a0 = ra;
+    b0 = rb;
+    const double s0 = (double)ra + (double)rb, sn = (double)na + (double)nb;
+    const SumW w0 = sw[0], wn = sw[FW_NP / 2];
+    acc[0] += s0 * w0.a + sn * wn.a;
+    acc[1] += s0 * w0.b + sn * wn.b;
+    acc[2] += s0 * w0.dk[0] + sn * wn.dk[0];
+    acc[3] += s0 * w0.kk[0] + sn * wn.kk[0];
+    pur += (double)ra * ra + (double)rb * rb + (double)na * na + (double)nb * nb;
+    z0 = mk(ra * f1[0].a.x, rb * f1[0].b.x);
+    zn = mk(na * f1[E2].a.x, nb * f1[E2].b.x);
+  }
+};
+
 template <int FL>
 FWD void vpass(const Args& a) {
   using L = VL;
@@ -486,6 +562,110 @@ FWD void vvpass(const Args& a) {
   spectral_step<L>(v, lds, q, t2, twl, mul1);
 #pragma unroll
   for (int e = 0; e < E; ++e) *reinterpret_cast<cpx*>(W + (size_t)(t + TPL * e) * FW_NX + c) = v[e];
+}
+
+// The fused column pass with the sums on the theta spectrum (VVMul): no x-p
+// mask, K and K' independent of t (FW_THETA_SUMS, set by the host).
+FWD void vvpass_theta(const Args& a) {
+  using L = VL;
+  constexpr int N = FW_NP, E = FW_COL_E, P = FW_COL_P, TPL = L::TPL, NT = L::NT;
+  __shared__ cpx lds[L::LDS];
+  __shared__ tw_t twl[L::TW::SIZE];
+  __shared__ double red[NSER * BlockRed<NT>::RN];
+  if (a.scal[SC_STOP] != 0.0) return;
+  const int tid = threadIdx.x;
+  const int q = tid % P, t = tid / P;
+  const int nb = gridDim.x;
+  const int blk = xcd_remap(blockIdx.x, nb);
+  const int c = (blk * P + q) * 2;
+  const double t0 = a.times[a.step], t1 = a.times[a.step + 1];
+  constexpr bool use_tab = FW_VTAB;   // one table read serves both half-steps
+  real_t* __restrict__ W = a.W;
+
+  TwStage<typename L::TW, NT> tws;
+  tws.load(a.tw_col, tid);
+  cpx v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = *reinterpret_cast<const cpx*>(W + (size_t)(t + TPL * e) * FW_NX + c);
+  cpx2 f0[E / 2 + 1], f1[E / 2 + 1];
+  if constexpr (use_tab) {
+    v_factors(a, c, t, f0);
+#pragma unroll
+    for (int i = 0; i <= E / 2; ++i) f1[i] = f0[i];
+  } else {
+    v_factors_at(c, t, t0 + FW_PHASE_SHIFT * ::dt, t0, f0);
+    v_factors_at(c, t, t1 + FW_PHASE_SHIFT * ::dt, t1, f1);
+  }
+  tws.store(twl, tid);
+  // the K pass of this step has read `fresh`; the next K pass uses the partials
+  if (blk == 0 && tid == 0 && a.scal[SC_FRESH] != 0.0) a.scal[SC_FRESH] = 0.0;
+  __syncthreads();
+
+  VVMul mul{f0, f1, a.sw, t, {0.0, 0.0, 0.0, 0.0}, 0.0, 0.0, 0.0};
+  spectral_step<L>(v, lds, q, t, twl, mul);
+#pragma unroll
+  for (int e = 0; e < E; ++e) *reinterpret_cast<cpx*>(W + (size_t)(t + TPL * e) * FW_NX + c) = v[e];
+
+  // the nine sums of step s (column sums N alpha_0 on thread 0 of each line)
+  const double Xa = ::dX * (c - 0.5 * ::X_gridDIM);
+  const double Xb = ::dX * (c + 1 - 0.5 * ::X_gridDIM);
+  const double ca = (double)N * mul.a0, cb = (double)N * mul.b0;
+  double sums[NSER];
+  sums[0] = ca + cb;
+  sums[1] = Xa * ca + Xb * cb;
+  sums[2] = mul.acc[0];
+  sums[3] = Xa * Xa * ca + Xb * Xb * cb;
+  sums[4] = mul.acc[1];
+  sums[5] = mul.acc[2];
+  sums[6] = 0.0;
+  sums[7] = mul.acc[3];
+  if (t == 0) {
+    sums[6] = ::fw_diff_V(Xa, t0) * ca + ::fw_diff_V(Xb, t0) * cb;
+    sums[7] += ::V(Xa, t0) * ca + ::V(Xb, t0) * cb;
+  }
+  sums[8] = (double)N * mul.pur;
+  write_partial_rows<NT>(sums, red, a.partials, blk, nb);
+}
+
+// Theta-spectrum sum weights (SumW), one entry per k <= N_p/2, by direct
+// summation in FP64 (the angle index (n k) mod N_p is exact). K and K' do not
+// depend on t when these weights are used. acc[o] = (re, im) of F_k for
+// f = P, P^2, K', K.
+FWD void build_sumw(const Args& a, SumW* sw) {
+  constexpr int N = FW_NP;
+  const double tk = a.t_initial;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k <= N / 2; k += gridDim.x * blockDim.x) {
+    double acc[4][2] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+    for (int n = 0; n < N; ++n) {
+      const int m = (int)(((long long)n * k) & (N - 1));
+      double sn, cs;
+      sincospi(-2.0 * (double)m / (double)N, &sn, &cs);
+      const double P = ::dP * (n - 0.5 * ::P_gridDIM);
+      const double f[4] = {P, P * P, ::fw_diff_K(P, tk), ::K(P, tk)};
+#pragma unroll
+      for (int o = 0; o < 4; ++o) {
+        acc[o][0] += f[o] * cs;
+        acc[o][1] += f[o] * sn;
+      }
+    }
+    SumW w;
+    if (k == 0 || k == N / 2) {
+      w.a = acc[0][0];
+      w.b = acc[1][0];
+      w.dk[0] = acc[2][0];
+      w.dk[1] = 0.0;
+      w.kk[0] = acc[3][0];
+      w.kk[1] = 0.0;
+    } else {
+      w.a = 2.0 * acc[0][1];
+      w.b = 2.0 * acc[1][0];
+      w.dk[0] = 2.0 * acc[2][0];
+      w.dk[1] = 2.0 * acc[2][1];
+      w.kk[0] = 2.0 * acc[3][0];
+      w.kk[1] = 2.0 * acc[3][1];
+    }
+    sw[k] = w;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -701,7 +881,11 @@ FW_VK(fw_v_first, fwk::F_FIRST)
 FW_VK(fw_v_last, fwk::F_LAST)
 FW_KK(fw_k, 0)
 FW_KK(fw_k_norm, fwk::F_KNORM)
-extern "C" __global__ void __launch_bounds__(fwk::VL::NT) fw_vv(fwk::Args a) { fwk::vvpass(a); }
+extern "C" __global__ void __launch_bounds__(fwk::VL::NT) fw_vv(fwk::Args a) {
+  if constexpr (FW_THETA_SUMS) fwk::vvpass_theta(a);
+  else fwk::vvpass(a);
+}
+extern "C" __global__ void __launch_bounds__(256) fw_build_sumw(fwk::Args a, fwk::SumW* sw) { fwk::build_sumw(a, sw); }
 extern "C" __global__ void __launch_bounds__(fwk::HT) fw_moments(fwk::Args a) { fwk::moments(a); }
 extern "C" __global__ void __launch_bounds__(fwk::HT) fw_apply_norm(fwk::Args a) { fwk::apply_norm(a); }
 extern "C" __global__ void fw_set_scalar(double* p, double v) { *p = v; }

@@ -33,6 +33,7 @@ struct KArgs {
   double* series;
   const void* vtab;
   const void* ktab;
+  const void* sw;
   int step;
   int flags;
   int nbp;

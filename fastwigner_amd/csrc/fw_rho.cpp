@@ -43,6 +43,23 @@ struct Mod {
   hipFunction_t f1 = nullptr, f2 = nullptr;
 };
 
+// Diagonal partial kernel tiling: DC diagonals per block (lanes), 256 / DC
+// row subgroups, G row groups (the rows the one-block finish adds).
+void diag_geom(int n, int* dc, int* g) {
+  *dc = n < 64 ? n : 64;
+  const int w = 256 / *dc;
+  int gg = n / w;
+  if (gg > 16) gg = 16;
+  if (gg < 1) gg = 1;
+  *g = gg;
+}
+
+int diag_blocks(int n) {
+  int dc, g;
+  diag_geom(n, &dc, &g);
+  return (n / dc) * g;
+}
+
 int diag_e(int n) {
   int e = n / 256;
   if (e < 2) e = 2;
@@ -60,7 +77,7 @@ struct fw_rho {
   size_t csz = 16;   // bytes per complex element
   double dX = 0, dP = 0;
   int row_e = 0, row_p = 0, col_e = 0, col_p = 0;
-  int row_nb = 0, row_nt = 0, col_nb = 0, col_nt = 0, diag_rows = 16;
+  int row_nb = 0, row_nt = 0, col_nb = 0, col_nt = 0;
   std::string prelude, arch;
 
   hipModule_t mod = nullptr;
@@ -118,21 +135,28 @@ void grid(const fw_rho_desc* d, double* dX, double* dP) {
 int tiles(const fw_rho_desc* d, int& re, int& rp, int& ce, int& cp) {
   const int n = d->x_grid_dim;
   const size_t esz = d->precision == FW_FP32 ? 8 : 16;
-  re = d->row_e > 0 ? d->row_e : (n >= 256 ? 16 : (n >= 8 ? 8 : n));
+  // defaults: 256-thread row tiles of 8 elements per thread, column tiles of
+  // 16 elements per thread and 4 columns (fastest at 1024^2 FP64 on gfx950)
+  re = d->row_e > 0 ? d->row_e : (n >= 8 ? 8 : n);
   ce = d->col_e > 0 ? d->col_e : (n >= 256 ? 16 : (n >= 8 ? 8 : n));
   const int tpl = n / re;
-  rp = d->row_p > 0 ? d->row_p : (tpl >= 64 ? 1 : 64 / tpl);
+  rp = d->row_p > 0 ? d->row_p : (tpl >= 256 ? 1 : 256 / tpl);
   while (rp > 1 && n % rp) rp /= 2;
+  const size_t tws = ((size_t)n * 17 / 16 + 1) * 16;
+  auto lds = [&](int lines) {
+    const size_t l = (size_t)lines * n;
+    return (l + l / 16 + 1) * esz + tws + 20 * 32 * 8;
+  };
   cp = d->col_p > 0 ? d->col_p : 4;
   while (cp > 1 && n % cp) cp /= 2;
+  // the default column tile shrinks to what fits the 160 KiB of LDS of a CU
+  while (d->col_p <= 0 && cp > 1 && lds(cp) > 160 * 1024) cp /= 2;
   auto ok_e = [&](int e) { return fw::is_pow2(e) && e >= 2 && e <= n && e <= 32; };
   if (!ok_e(re) || !ok_e(ce)) return fw::fail(FW_ERR_INVALID_ARG, "elements per thread must be a power of two in [2, 32]");
   if (!fw::is_pow2(rp) || !fw::is_pow2(cp) || n % rp || n % cp)
     return fw::fail(FW_ERR_INVALID_ARG, "lines per block must be a power of two dividing N");
   if (rp * (n / re) > 1024 || cp * (n / ce) > 1024) return fw::fail(FW_ERR_INVALID_ARG, "tile needs more than 1024 threads");
-  const size_t tws = ((size_t)n * 17 / 16 + 1) * 16;
-  const size_t lr = (size_t)rp * n, lc = (size_t)cp * n;
-  if ((lr + lr / 16 + 1) * esz + tws + 20 * 32 * 8 > 160 * 1024 || (lc + lc / 16 + 1) * esz + tws + 20 * 32 * 8 > 160 * 1024)
+  if (lds(rp) > 160 * 1024 || lds(cp) > 160 * 1024)
     return fw::fail(FW_ERR_INVALID_ARG, "tile needs more than 160 KiB of LDS");
   return FW_OK;
 }
@@ -147,6 +171,11 @@ std::string make_prelude(const fw_rho_desc* d, const fw_rho_exprs* x, int re, in
   s << "#define FW_ROW_E " << re << "\n#define FW_ROW_P " << rp << "\n#define FW_COL_E " << ce << "\n#define FW_COL_P "
     << cp << "\n";
   s << "#define FW_DIAG_E " << diag_e(d->x_grid_dim) << "\n";
+  {
+    int dc, g;
+    diag_geom(d->x_grid_dim, &dc, &g);
+    s << "#define FW_DIAG_DC " << dc << "\n#define FW_DIAG_G " << g << "\n";
+  }
   s << "#define FW_PHASE_SHIFT (" << fw::fmt_double(d->phase_time_shift) << ")\n";
   s << "#define FW_HAS_MASK_X " << ((x->abs_boundary_x && *x->abs_boundary_x) ? 1 : 0) << "\n";
   s << "#define FW_HAS_MASK_P " << ((x->abs_boundary_p && *x->abs_boundary_p) ? 1 : 0) << "\n";
@@ -233,12 +262,8 @@ int enqueue_chunk(fw_rho* h, bool bloch, int n, bool momentum, const RArgs& b0, 
     FWCHK(args1(h, bloch ? h->f_bc : h->f_c, h->row_nb, h->row_nt, a));
     if (ev) HIPCHK(hipEventRecord((*ev)[4 * s + 3], h->stream));
     if (momentum) {
-      int rows = h->diag_rows;
-      void* p1[] = {&a, &rows};
-      FWCHK(launch(h, h->f_diag_p, h->N / rows, 256, p1));
-      int nblk = h->N / rows;
-      void* p2[] = {&a, &nblk};
-      FWCHK(launch(h, h->f_diag_f, 1, h->N / diag_e(h->N), p2));
+      FWCHK(args1(h, h->f_diag_p, diag_blocks(h->N), 256, a));
+      FWCHK(args1(h, h->f_diag_f, 1, h->N / diag_e(h->N), a));
     }
   }
   return FW_OK;
@@ -382,7 +407,6 @@ extern "C" int fw_rho_create(const fw_rho_desc* desc, const fw_rho_exprs* exprs,
   h->col_nt = cp * (h->N / ce);
   h->row_nb = h->N / rp;
   h->col_nb = h->N / cp;
-  h->diag_rows = h->N >= 16 ? 16 : h->N;
   h->prelude = make_prelude(desc, exprs, re, rp, ce, cp);
   h->arch = fw::target_arch(desc->device);
   auto bail = [&](int rc) {
@@ -549,12 +573,8 @@ extern "C" int fw_rho_moments(fw_rho* h, double t, double sums[FW_RSER]) {
   // momentum family of the current rho through the diagonal-sum kernels, into series slot 0
   RArgs a = base(h);
   a.step = 0;
-  int rows = h->diag_rows;
-  void* p1[] = {&a, &rows};
-  FWCHK(launch(h, h->f_diag_p, h->N / rows, 256, p1));
-  int nblk = h->N / rows;
-  void* p2[] = {&a, &nblk};
-  FWCHK(launch(h, h->f_diag_f, 1, h->N / diag_e(h->N), p2));
+  FWCHK(args1(h, h->f_diag_p, diag_blocks(h->N), 256, a));
+  FWCHK(args1(h, h->f_diag_f, 1, h->N / diag_e(h->N), a));
   HIPCHK(hipMemcpyAsync(sums, h->scal + RC_OUT, sizeof(double) * 11, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipMemcpyAsync(sums + 11, h->series + 11, sizeof(double) * 9, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));

@@ -290,24 +290,48 @@ FWD void rho_rows(const RArgs& a, cpx* uv, cpx* uk) {
 // n_k = (1/N) sum_d c_d exp(2 pi i k d / N) and Tr(rho g(P)) = sum_k g(P_k) n_k,
 // which is the reference's fft(axis 1)/N -> g(P) -> ifft -> trace
 // (rho_vneumann_cuda_1d.py:452-472).
+// Partial kernel: block (g, c) sums diagonals c*DC .. c*DC + DC - 1 over row
+// group g (N/G rows); lane l of a DC-wide thread row takes diagonal c*DC + l, so
+// a thread row reads DC consecutive (wrapped) elements of one row of rho.
+// The row subgroups of a block are combined in LDS, in a fixed order, and the
+// G group sums land in aux[g * N + d] for the one-block finish.
 // ---------------------------------------------------------------------------
-constexpr int DT = 256;   // threads of the diagonal kernels
-FWD void rho_diag_partial(const RArgs& a, int rows_per_block) {
-  const int i0 = blockIdx.x * rows_per_block;
-  for (int d = threadIdx.x; d < FW_N; d += DT) {
-    double cr = 0.0, ci = 0.0;
-    for (int r = 0; r < rows_per_block; ++r) {
-      const int i = i0 + r;
-      const cpx z = a.rho[(size_t)i * FW_N + ((i - d) & (FW_N - 1))];
-      cr += (double)z.x;
-      ci += (double)z.y;
+constexpr int DT = 256;   // threads of the diagonal partial kernel
+constexpr int DDC = FW_DIAG_DC, DW = DT / DDC, DG = FW_DIAG_G, DRG = FW_N / DG;
+static_assert(DT % DDC == 0 && FW_N % DDC == 0 && FW_N % DG == 0, "diagonal tiling");
+
+FWD void rho_diag_partial(const RArgs& a) {
+  __shared__ double red[2 * DT];
+  constexpr int NC = FW_N / DDC;
+  const int l = threadIdx.x % DDC, w = threadIdx.x / DDC;
+  const int g = blockIdx.x / NC, d = (blockIdx.x % NC) * DDC + l;
+  double cr = 0.0, ci = 0.0;
+  if (w < DRG) {
+    constexpr int RT = (DRG + DW - 1) / DW;   // rows per thread (the last ones may be absent)
+#pragma unroll 8
+    for (int r = 0; r < RT; ++r) {
+      const int i = g * DRG + w + r * DW;
+      if ((DRG % DW) == 0 || w + r * DW < DRG) {
+        const cpx z = a.rho[(size_t)i * FW_N + ((i - d) & (FW_N - 1))];
+        cr += (double)z.x;
+        ci += (double)z.y;
+      }
     }
-    a.aux[(size_t)blockIdx.x * FW_N + d] = mk((real_t)cr, (real_t)ci);
+  }
+  red[threadIdx.x] = cr;
+  red[DT + threadIdx.x] = ci;
+  __syncthreads();
+  if (w == 0) {
+    for (int k = 1; k < DW; ++k) {
+      cr += red[k * DDC + l];
+      ci += red[DT + k * DDC + l];
+    }
+    a.aux[(size_t)g * FW_N + d] = mk((real_t)cr, (real_t)ci);
   }
 }
 
 template <int E>
-FWD void rho_diag_finish(const RArgs& a, int nblk) {
+FWD void rho_diag_finish(const RArgs& a) {
   // one block of N/E threads (a single line of the FFT engine)
   using L = Line<FW_N, E, 1, true>;
   constexpr int TPL = L::TPL;
@@ -321,7 +345,8 @@ FWD void rho_diag_finish(const RArgs& a, int nblk) {
   for (int e = 0; e < E; ++e) {
     const int d = t + TPL * e;
     double cr = 0.0, ci = 0.0;
-    for (int b = 0; b < nblk; ++b) {
+#pragma unroll
+    for (int b = 0; b < DG; ++b) {
       const cpx z = a.aux[(size_t)b * FW_N + d];
       cr += (double)z.x;
       ci += (double)z.y;
@@ -499,11 +524,11 @@ extern "C" __global__ void __launch_bounds__(fwk::RL::NT) fw_rho_bc(fwk::RArgs a
 extern "C" __global__ void __launch_bounds__(256) fw_rho_rows(fwk::RArgs a, fwk::cpx* uv, fwk::cpx* uk) {
   fwk::rho_rows(a, uv, uk);
 }
-extern "C" __global__ void __launch_bounds__(256) fw_rho_diag_partial(fwk::RArgs a, int rows) {
-  fwk::rho_diag_partial(a, rows);
+extern "C" __global__ void __launch_bounds__(fwk::DT) fw_rho_diag_partial(fwk::RArgs a) {
+  fwk::rho_diag_partial(a);
 }
-extern "C" __global__ void __launch_bounds__(FW_N / FW_DIAG_E) fw_rho_diag_finish(fwk::RArgs a, int nblk) {
-  fwk::rho_diag_finish<FW_DIAG_E>(a, nblk);
+extern "C" __global__ void __launch_bounds__(FW_N / FW_DIAG_E) fw_rho_diag_finish(fwk::RArgs a) {
+  fwk::rho_diag_finish<FW_DIAG_E>(a);
 }
 extern "C" __global__ void __launch_bounds__(256) fw_rho_apply_norm(fwk::RArgs a, int from_partials) {
   fwk::rho_apply_norm(a, from_partials);

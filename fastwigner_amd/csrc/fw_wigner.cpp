@@ -95,10 +95,11 @@ struct fw_wigner {
 
   bool vtab_ok = false, ktab_ok = false;   // V (K) factors are time independent
   void* vtab = nullptr;
+  void* sw = nullptr;   // theta-spectrum sum weights (fused column pass), or null
   void* ktab = nullptr;
   int tab_kind = -1;          // what the tables hold: 0 Moyal, 1 Bloch
   double tab_dbeta = 0.0;
-  hipFunction_t f_build_vtab = nullptr, f_build_ktab = nullptr;
+  hipFunction_t f_build_vtab = nullptr, f_build_ktab = nullptr, f_build_sumw = nullptr;
 
   hipGraphExec_t gexec = nullptr;
   int gkind = -1;
@@ -195,6 +196,14 @@ bool tables_k(const fw_wigner_desc* d, const fw_wigner_exprs* x) {
   return d->phase_tables > 0 || (d->phase_tables == 0 && !uses_t(x->K) && !uses_t(x->abs_boundary_lambda_p));
 }
 
+// The fused column pass takes the step's sums on the theta spectrum when
+// nothing acts on W in x-p between its two half-steps (no x-p mask) and the
+// weights of the p sums are fixed (K and K' independent of t).
+bool theta_sums(const fw_wigner_exprs* x) {
+  const bool mask_xp = x->abs_boundary_x_p && *x->abs_boundary_x_p;
+  return !mask_xp && !uses_t(x->K) && !uses_t(x->diff_K);
+}
+
 std::string make_prelude(const fw_wigner_desc* d, const fw_wigner_exprs* x, const Tiles& t) {
   double dX, dP, dL, dT;
   grid_constants(d, &dX, &dP, &dL, &dT);
@@ -218,6 +227,7 @@ std::string make_prelude(const fw_wigner_desc* d, const fw_wigner_exprs* x, cons
   s << "const double dLambda = " << fw::fmt_double(dL) << ";\n";
   s << "const double dTheta = " << fw::fmt_double(dT) << ";\n";
   s << "#define FW_MASK_XP_X_ONLY " << (uses_token(x->abs_boundary_x_p, "P") ? 0 : 1) << "\n";
+  s << "#define FW_THETA_SUMS " << (theta_sums(x) ? 1 : 0) << "\n";
   // the caller's sin and cos: the branch-free FP64 ones of fw_math.h (see there)
   s << "#include \"fw_math.h\"\n#define sin fw_user_sin\n#define cos fw_user_cos\n";
   s << "// ---- caller constants and functions ----\n";
@@ -273,6 +283,7 @@ KArgs base_args(fw_wigner* h) {
   a.series = h->series;
   a.vtab = h->vtab;
   a.ktab = h->ktab;
+  a.sw = h->sw;
   a.w_coeff = h->w_coeff;
   a.flags = h->has_mask_xp ? fw::F_MASK_XP : 0;
   a.nbp = h->col_nb;
@@ -639,7 +650,7 @@ extern "C" int fw_wigner_create(const fw_wigner_desc* desc, const fw_wigner_expr
              {"fw_apply_norm", &h->f_apply_norm}, {"fw_set_scalar", &h->f_set_scalar},
              {"fw_vk_min", &h->f_vk_min},   {"fw_build_vtab", &h->f_build_vtab},
              {"fw_build_ktab", &h->f_build_ktab}, {"fw_vv", &h->f_vv},
-             {"fw_k_norm", &h->f_k_norm}};
+             {"fw_k_norm", &h->f_k_norm},     {"fw_build_sumw", &h->f_build_sumw}};
   for (auto& f : fns)
     if ((rc = getfn(h->mod, f.n, f.f)) != FW_OK) return bail(rc);
 
@@ -660,6 +671,15 @@ extern "C" int fw_wigner_create(const fw_wigner_desc* desc, const fw_wigner_expr
   if ((rc = fw::upload_twiddles(h->NX, h->fp32, &h->tw_row)) != FW_OK) return bail(rc);
   if (hipStreamCreate(&h->stream) != hipSuccess) return bail(fw::fail(FW_ERR_HIP, "hipStreamCreate failed"));
   if ((rc = set_scalar(h, fw::SC_NORM, 1.0)) != FW_OK) return bail(rc);
+  if (theta_sums(exprs)) {
+    if ((rc = alloc(&h->sw, (size_t)(h->NP / 2 + 1) * 48)) != FW_OK) return bail(rc);
+    KArgs a = base_args(h);
+    void* sw = h->sw;
+    void* params[] = {&a, &sw};
+    if (hipModuleLaunchKernel(h->f_build_sumw, (h->NP / 2 + 256) / 256, 1, 1, 256, 1, 1, 0, h->stream, params, nullptr) !=
+        hipSuccess)
+      return bail(fw::fail(FW_ERR_HIP, "fw_build_sumw launch failed"));
+  }
   if (hipStreamSynchronize(h->stream) != hipSuccess) return bail(fw::fail(FW_ERR_HIP, "initial sync failed"));
   *out = h;
   return FW_OK;
@@ -675,7 +695,7 @@ extern "C" int fw_wigner_destroy(fw_wigner* h) {
       if (kv.second.mod) (void)hipModuleUnload(kv.second.mod);
   for (hipModule_t m : {h->mod, h->mod_bloch, h->mod_dissip})
     if (m) (void)hipModuleUnload(m);
-  for (void* p : {h->W, h->W0s, h->snap, h->tw_col, h->tw_row, h->vtab, h->ktab, (void*)h->times, (void*)h->scal,
+  for (void* p : {h->W, h->W0s, h->snap, h->tw_col, h->tw_row, h->vtab, h->ktab, h->sw, (void*)h->times, (void*)h->scal,
                   (void*)h->partials, (void*)h->counter, (void*)h->series})
     if (p) (void)hipFree(p);
   if (h->stream) (void)hipStreamDestroy(h->stream);

@@ -155,9 +155,24 @@ def test_time_independent_masks(gpu, tables):
     assert rel_l2(Wg, o.W) <= 1e-12
 
 
-@pytest.mark.parametrize("tiles", [(16, 1, 16, 1), (8, 2, 8, 2), (16, 4, 4, 4), (32, 1, 32, 1), (4, 8, 2, 8)])
+SERIES = ("X_average", "P_average", "X_average_RHS", "P_average_RHS", "hamiltonian_average", "purity")
+
+
+def check_series(g, o, tol=1e-11):
+    ours = dict(X_average=g.X_average, P_average=g.P_average, X_average_RHS=g.X_average_RHS,
+                P_average_RHS=g.P_average_RHS, hamiltonian_average=g.hamiltonian_average, purity=g.purity_series)
+    for k in SERIES:
+        ref = np.array([s[k] for s in o.series])
+        assert np.max(np.abs(np.array(ours[k]) - ref)) <= tol * max(1.0, np.max(np.abs(ref))), k
+
+
+@pytest.mark.parametrize("tiles", [(16, 1, 16, 1), (8, 2, 8, 2), (16, 4, 4, 4), (32, 1, 32, 1), (4, 8, 2, 8),
+                                   (2, 4, 4, 4)])
 def test_tile_schedules_agree(gpu, tiles):
-    """Every FFT tile/radix schedule gives the oracle's answer."""
+    """Every FFT tile/radix schedule gives the oracle's answer, W and the
+    logged sums (the fused column pass takes the sums on the theta spectrum at
+    each schedule's own item-to-thread map; column E = 2 uses the separate
+    transforms)."""
     g, o = make_pair(256, 8., 256, 8., 0.01, V_Q, K_Q, DV_Q, DK_Q, tiles=tiles)
     W0 = gaussian(o)
     g.set_wignerfunction(W0)
@@ -165,6 +180,31 @@ def test_tile_schedules_agree(gpu, tiles):
     Wg = g.propagate(50).get()
     o.propagate(50)
     assert rel_l2(Wg, o.W) <= 1e-12
+    check_series(g, o)
+
+
+@pytest.mark.parametrize("nx,npp,tables", [(128, 64, 0), (64, 256, 0), (256, 128, 1)])
+def test_theta_sums_time_dependent_v(gpu, nx, npp, tables):
+    """The sums on the theta spectrum with a time-dependent V (the closing and
+    opening factors of the fused column pass evaluated at different times) or
+    a tabulated time-independent one, a non-polynomial K whose weights are
+    not those of P and P^2, and the x-theta and lambda-p masks (which sit in the
+    factors; no x-p mask, which would send the pass through x-p space)."""
+    consts = {"E0": 0.05, "om": 0.3}
+    masks = dict(abs_boundary_x_theta="pow(fabs(cos(0.5*M_PI*X_minus/X_amplitude)*cos(0.5*M_PI*X_plus/X_amplitude)), 0.01)",
+                 abs_boundary_lambda_p="exp(-1e-4*Lambda*Lambda)")
+    K = "0.5*P*P + 0.05*cos(P)"
+    dK = "P - 0.05*sin(P)"
+    V = "0.5*X*X + X*E0*sin(om*t)" if tables == 0 else V_Q
+    dV = "X + E0*sin(om*t)" if tables == 0 else DV_Q
+    g, o = make_pair(nx, 10., npp, 6., 0.02, V, K, dV, dK, consts=consts, masks=masks, phase_tables=tables)
+    W0 = np.exp(-(o.X + 0.5) ** 2 - (o.P - 0.3) ** 2)
+    g.set_wignerfunction(W0)
+    o.set_wignerfunction(W0)
+    Wg = g.propagate(200).get()
+    o.propagate(200)
+    assert rel_l2(Wg, o.W) <= 1e-12
+    check_series(g, o)
 
 
 @pytest.mark.parametrize("tables", [0, -1])
