@@ -245,11 +245,13 @@ PROTOTYPES = {
     "fw_batch_info": [_P, ctypes.POINTER(ctypes.c_longlong), _PI, _PI],
     "fw_batch_step_time": [_P, _I, _PD],
     "fw_batch_state_ptr": [_P, _I, _PP, _PI],
+    "fw_batch_copy_states": [_P, _P],
     "fw_fno_create": [ctypes.POINTER(FnoDesc), _PP],
     "fw_fno_destroy": [_P],
     "fw_fno_param_count": [_P, ctypes.POINTER(ctypes.c_longlong)],
     "fw_fno_set_params": [_P, _PF],
     "fw_fno_get_params": [_P, _PF],
+    "fw_fno_set_output": [_P, ctypes.c_float, _I],
     "fw_fno_forward": [_P, _PF, _PF, _I, _PF],
     "fw_fno_loss_grad": [_P, _PF, _PF, _PF, _I, _PD, _PF],
     "fw_fno_dataset_reserve": [_P, _I],

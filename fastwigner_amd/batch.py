@@ -108,6 +108,10 @@ class WignerBatchHIP1D:
         _lib.call("fw_batch_get_state", self._handle, int(i), _lib.dptr(out))
         return out
 
+    def copy_states_from(self, other):
+        """Every instance's W <- that of `other` (same instances and precision), on the device."""
+        _lib.call("fw_batch_copy_states", self._handle, other._handle)
+
     # -- thermal states and the dissipator ----------------------------------------------
     def prepare_random_collision(self, dbeta=0.01, bloch_convention="fftw", t_initial=0.):
         """Gibbs state of every instance with gamma > 0 at its kT by the batched

@@ -668,6 +668,21 @@ extern "C" int fw_batch_state_ptr(fw_batch* h, int i, void** dev_ptr, int* elem_
   return FW_OK;
 }
 
+extern "C" int fw_batch_copy_states(fw_batch* dst, fw_batch* src) {
+  fw::clear_error();
+  if (!dst || !src) return fw::fail(FW_ERR_INVALID_ARG, "NULL handle");
+  if (dst->inst.size() != src->inst.size() || dst->points != src->points || dst->esz != src->esz)
+    return fw::fail(FW_ERR_INVALID_ARG, "the batches differ in instances or precision");
+  for (size_t i = 0; i < dst->inst.size(); ++i)
+    if (dst->inst[i].n != src->inst[i].n || dst->inst[i].off != src->inst[i].off)
+      return fw::fail(FW_ERR_INVALID_ARG, "the batches differ in instance grids");
+  if (dst == src) return FW_OK;
+  HIPCHK(hipStreamSynchronize(src->stream));
+  HIPCHK(hipMemcpyAsync(dst->W, src->W, (size_t)dst->points * dst->esz, hipMemcpyDeviceToDevice, dst->stream));
+  HIPCHK(hipStreamSynchronize(dst->stream));
+  return FW_OK;
+}
+
 extern "C" int fw_batch_info(fw_batch* h, long long* points, int* tasks_per_step, int* grid) {
   if (!h) return fw::fail(FW_ERR_INVALID_ARG, "NULL handle");
   if (points) *points = h->points;

@@ -6,7 +6,9 @@
 //   features f(p, x) = [W, x/X_amplitude, p/P_amplitude, a2, a3, a4, gamma, kT]
 //   v0 = f Pw + pb                                   (lift, 8 -> C)
 //   v_{l+1} = act_l(S_l(v_l) + v_l Wp_l + b_l)       (act = GELU for l < L-1, identity for the last)
-//   out = W + v_L Q + qb                             (projection, C -> 1; residual on W)
+//   c = v_L Q + qb                                   (projection, C -> 1)
+//   out = W + s c, or W + s (c - mean c) per sample  (residual on W; output scale s, 1 by
+//                                                     default; zero-mass correction optional)
 // S_l is the truncated spectral convolution: the 2-D DFT of each channel kept
 // on the modes ky in [0, M) (the real-input half axis, along x) and
 // kp in [0, M) U [N-M, N) (along p), mixed per mode by complex C x C weights
@@ -236,24 +238,24 @@ __global__ void __launch_bounds__(256) reduce_rows(const float* part, float* out
   out[r] = accumulate ? out[r] + scale * s : scale * s;
 }
 
-// out[n] = W[n] + sum_c Q[c] v[n][c] + qb
+// out[n] = W[n] + scale (sum_c Q[c] v[n][c] + qb)
 __global__ void __launch_bounds__(256) proj_fwd(const float* W, const float* v, const float* Q, const float* qb,
-                                                float* out, long long total, int C) {
+                                                float* out, long long total, int C, float scale) {
   for (long long n = (long long)blockIdx.x * 256 + threadIdx.x; n < total; n += (long long)gridDim.x * 256) {
     float s = qb[0];
     for (int c = 0; c < C; ++c) s += Q[c] * v[n * C + c];
-    out[n] = W[n] + s;
+    out[n] = W[n] + scale * s;
   }
 }
 
-// dv[n][c] = Q[c] dout[n]; partials of sum_n v[n][c] dout[n] (C) and sum dout (1) per block
+// with d = scale dout[n]: dv[n][c] = Q[c] d; partials of sum_n v[n][c] d (C) and sum d (1) per block
 __global__ void __launch_bounds__(256) proj_bwd(const float* v, const float* dout, const float* Q, float* dv,
-                                                float* part, long long total, int C) {
+                                                float* part, long long total, int C, float scale) {
   __shared__ float red[256];
   const int c = threadIdx.x % C, lanes = 256 / C, li = threadIdx.x / C;
   float aq = 0.f, ab = 0.f;
   for (long long n = (long long)blockIdx.x * lanes + li; n < total; n += (long long)gridDim.x * lanes) {
-    const float d = dout[n];
+    const float d = scale * dout[n];
     dv[n * C + c] = Q[c] * d;
     aq += v[n * C + c] * d;
     if (c == 0) ab += d;
@@ -382,6 +384,25 @@ __global__ void __launch_bounds__(256) to_bf16_kernel(const float* s, bf16_t* d,
   for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) d[i] = to_bf16(s[i]);
 }
 
+// Per sample b (one block each): x[b][n] -= mean_n (x[b][n] - y[b][n]) (y = NULL: 0).
+// Forward of the zero-mass correction (x = out, y = W) and its backward (x = dout).
+__global__ void __launch_bounds__(256) center_kernel(float* x, const float* y, long long nn) {
+  __shared__ double r[256];
+  const int b = blockIdx.x;
+  float* xb = x + (long long)b * nn;
+  const float* yb = y ? y + (long long)b * nn : nullptr;
+  double s = 0;
+  for (long long i = threadIdx.x; i < nn; i += 256) s += (double)xb[i] - (yb ? (double)yb[i] : 0.0);
+  r[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) r[threadIdx.x] += r[threadIdx.x + k];
+    __syncthreads();
+  }
+  const float m = (float)(r[0] / (double)nn);
+  for (long long i = threadIdx.x; i < nn; i += 256) xb[i] -= m;
+}
+
 // loss L = (1/B) sum_b sum_n (out - t)^2 / sum_n t^2 ; dout = dL/dout. One block per sample.
 __global__ void __launch_bounds__(256) loss_kernel(const float* out, const float* tgt, float* dout, double* per_sample,
                                                    long long nn, float inv_b) {
@@ -448,6 +469,8 @@ struct fw_fno {
   // parameters: flat FP32 vector and offsets
   long long nparam = 0;
   long long o_Pw = 0, o_pb = 0, o_Q = 0, o_qb = 0;
+  float out_scale = 1.f;   // scale of the projection's output (small corrections with O(1) weights)
+  int zero_mass = 0;       // subtract each sample's mean correction (out keeps the mass of W)
   std::vector<long long> o_R, o_Wp, o_b;
   float* P = nullptr;      // master weights
   float* G = nullptr;      // gradients
@@ -559,7 +582,9 @@ int forward(fw_fno* h, int B) {
                        tot, C, l < h->L - 1 ? 1 : 0);
   }
   hipLaunchKernelGGL(fno::proj_fwd, dim3(grid_for(npts)), dim3(256), 0, h->stream, h->bin, h->v[h->L], h->P + h->o_Q,
-                     h->P + h->o_qb, h->out, npts, C);
+                     h->P + h->o_qb, h->out, npts, C, h->out_scale);
+  if (h->zero_mass)
+    hipLaunchKernelGGL(fno::center_kernel, dim3(B), dim3(256), 0, h->stream, h->out, h->bin, (long long)N * N);
   HIPCHK(hipGetLastError());
   return FW_OK;
 }
@@ -573,7 +598,7 @@ int backward(fw_fno* h, int B) {
   HIPCHK(hipMemsetAsync(h->G, 0, sizeof(float) * h->nparam, h->stream));
   // projection
   hipLaunchKernelGGL(fno::proj_bwd, dim3(nr), dim3(256), 0, h->stream, h->v[h->L], h->dout, h->P + h->o_Q, h->dv, h->part,
-                     npts, C);
+                     npts, C, h->out_scale);
   hipLaunchKernelGGL(fno::reduce_rows, dim3((C + 1 + 255) / 256), dim3(256), 0, h->stream, h->part, h->G + h->o_Q, nr,
                      C + 1, 1.f, 0);   // Q then qb (adjacent)
   float* dcur = h->dv;    // gradient w.r.t. v[l+1]
@@ -641,6 +666,9 @@ int loss(fw_fno* h, int B, bool want_grad, double* out) {
   const long long nn = (long long)h->N * h->N;
   hipLaunchKernelGGL(fno::loss_kernel, dim3(B), dim3(256), 0, h->stream, h->out, h->btg, want_grad ? h->dout : nullptr,
                      h->persample, nn, 1.f / B);
+  // zero-mass correction: dL/dc = s (dout - mean dout) per sample; proj_bwd applies s
+  if (want_grad && h->zero_mass)
+    hipLaunchKernelGGL(fno::center_kernel, dim3(B), dim3(256), 0, h->stream, h->dout, nullptr, nn);
   HIPCHK(hipGetLastError());
   if (out) {
     std::vector<double> ps(B);
@@ -830,6 +858,15 @@ extern "C" int fw_fno_set_params(fw_fno* h, const float* host) {
   h->t_adam = 0;
   FWCHK(refresh_weights(h));
   HIPCHK(hipStreamSynchronize(h->stream));
+  return FW_OK;
+}
+
+extern "C" int fw_fno_set_output(fw_fno* h, float scale, int zero_mass) {
+  fw::clear_error();
+  if (!h) return fw::fail(FW_ERR_INVALID_ARG, "NULL handle");
+  if (!(scale > 0.f) || !(scale < 1e30f)) return fw::fail(FW_ERR_INVALID_ARG, "the output scale must be positive");
+  h->out_scale = scale;
+  h->zero_mass = zero_mass ? 1 : 0;
   return FW_OK;
 }
 

@@ -304,6 +304,12 @@ int fw_batch_step_time(fw_batch* h, int steps, double* us_per_step);
  * fw_batch_destroy; the caller may read it between calls (e.g. to copy training pairs). */
 int fw_batch_state_ptr(fw_batch* h, int i, void** dev_ptr, int* elem_size);
 
+/* Every instance's W of src copied into dst on the device (both batches must have the same
+ * instance grids and precision; dst's dissipators and series are left as they are). No
+ * counterpart in the reference: it lets a second batch take a step of different length from
+ * the first one's states (the coarse base of the surrogate's training pairs). */
+int fw_batch_copy_states(fw_batch* dst, fw_batch* src);
+
 /* ------------------------------------------------------------------------ */
 /* Fourier-neural-operator surrogate of k propagator steps                   */
 /* No counterpart in the reference. Its network is trained on trajectories  */
@@ -331,6 +337,11 @@ int fw_fno_destroy(fw_fno* h);
 int fw_fno_param_count(fw_fno* h, long long* n);
 int fw_fno_set_params(fw_fno* h, const float* host_params); /* also resets the Adam state */
 int fw_fno_get_params(fw_fno* h, float* host_params);
+/* Output of the network: out = W + scale c with c = v_L Q + qb, or, with zero_mass != 0,
+ * out = W + scale (c - mean c) per sample, so that out has the mass (norm) of W. Defaults:
+ * scale 1, zero_mass 0. The scale lets a network learning a small correction keep O(1)
+ * projection weights. No counterpart in the reference (the surrogate is new). */
+int fw_fno_set_output(fw_fno* h, float scale, int zero_mass);
 /* prediction W(t + k dt) for `batch` samples (host arrays: batch*N*N inputs, batch*5 features) */
 int fw_fno_forward(fw_fno* h, const float* host_in, const float* host_feat, int batch, float* host_out);
 /* loss L = mean_b |pred_b - target_b|^2 / |target_b|^2 and, if host_grad != NULL, dL/dparams */

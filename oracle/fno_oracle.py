@@ -36,8 +36,10 @@ def gelu_d(x):
 
 
 class FNOOracle:
-    def __init__(self, N=256, C=32, M=16, L=4, bf16=True):
+    def __init__(self, N=256, C=32, M=16, L=4, bf16=True, out_scale=1.0, zero_mass=False):
         self.N, self.C, self.M, self.L, self.use_bf16 = N, C, M, L, bf16
+        # out = W + out_scale c, c = v_L Q + qb; with zero_mass, c - mean(c) per sample
+        self.out_scale, self.zero_mass = float(out_scale), bool(zero_mass)
         self.nm = 2 * M * M
         o = 0
         self.o_Pw = o; o += 8 * C
@@ -112,7 +114,10 @@ class FNOOracle:
             pre = S + Pt + p["b"][l]
             pres.append(pre)
             v.append(gelu(pre) if l < self.L - 1 else pre)
-        out = W + np.einsum("bpxc,c->bpx", v[-1], p["Q"], optimize=True) + p["qb"]
+        c = np.einsum("bpxc,c->bpx", v[-1], p["Q"], optimize=True) + p["qb"]
+        if self.zero_mass:
+            c = c - c.mean(axis=(1, 2), keepdims=True)
+        out = W + self.out_scale * c
         if keep:
             return out, dict(f=f, v=v, pre=pres, Z=Zs, p=p)
         return out
@@ -130,6 +135,9 @@ class FNOOracle:
         p = k["p"]
         dout = 2.0 * (out - target) / st[:, None, None] / B
         g = np.zeros(self.nparam)
+        dout = self.out_scale * dout     # from here on, the gradient w.r.t. the projection's output c
+        if self.zero_mass:
+            dout = dout - dout.mean(axis=(1, 2), keepdims=True)
         g[self.o_Q:self.o_Q + C] = np.einsum("bpxc,bpx->c", k["v"][-1], dout, optimize=True)
         g[self.o_qb] = dout.sum()
         dv = dout[..., None] * p["Q"]

@@ -3,6 +3,7 @@ HIP forward/backward kernels: central finite differences of the loss
 (float64, no bf16 rounding), the spectral convolution with identity mode
 weights against a direct DFT sum (no FFT), and the bf16 rounding rule."""
 import numpy as np
+import pytest
 
 from oracle import fno_oracle as fo
 
@@ -33,8 +34,9 @@ def test_spectral_identity_is_lowpass_by_direct_dft():
     assert np.max(np.abs(S[0] - ref)) < 1e-12
 
 
-def test_loss_gradient_by_finite_differences():
-    o = fo.FNOOracle(N=16, C=4, M=2, L=2, bf16=False)
+@pytest.mark.parametrize("out_scale,zero_mass", [(1.0, False), (0.05, False), (0.05, True)])
+def test_loss_gradient_by_finite_differences(out_scale, zero_mass):
+    o = fo.FNOOracle(N=16, C=4, M=2, L=2, bf16=False, out_scale=out_scale, zero_mass=zero_mass)
     rng = np.random.default_rng(1)
     P = fo.init_params(o, seed=2, spectral_scale=0.3)
     P[o.o_Q:o.o_Q + o.C] = rng.standard_normal(o.C) * 0.5
@@ -80,3 +82,15 @@ def test_planted_adjoint_mistake_is_caught():
     finally:
         fo.np.where = orig
     assert abs(num - g_bad[i]) > 1e-3 * abs(num)
+
+
+def test_zero_mass_output_keeps_the_mass_of_the_input():
+    o = fo.FNOOracle(N=16, C=4, M=2, L=1, bf16=False, out_scale=0.3, zero_mass=True)
+    rng = np.random.default_rng(3)
+    P = fo.init_params(o, seed=1, spectral_scale=0.3)
+    P[o.o_Q:o.o_Q + o.C] = rng.standard_normal(o.C)
+    P[o.o_qb] = 0.7
+    W = rng.random((3, 16, 16))
+    out = o.forward(P, W, rng.random((3, 5)))
+    assert np.allclose(out.sum(axis=(1, 2)), W.sum(axis=(1, 2)), rtol=0, atol=1e-12)
+    assert np.max(np.abs(out - W)) > 1e-3          # and still corrects W
