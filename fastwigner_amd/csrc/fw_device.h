@@ -329,10 +329,11 @@ struct KMul {
 // with F_k = sum_n f(P_n) exp(-2 pi i n k / N) (Parseval). a' is real, so k and
 // N - k contribute conjugate terms and the sums run over k <= N/2 with weights
 // c_k (SumW). When nothing acts on W in x-p between the closing V/2 of step s
-// and the opening V/2 of step s+1 (no x-p mask) and K, K' do not depend on t,
-// the column pass takes the nine sums of step s on alpha_k and multiplies by
-// the opening factors in the same registers: one FFT pair per pass instead of
-// two, the same W and the same sums in exact arithmetic.
+// and the opening V/2 of step s+1 but a mask of x alone (which scales a whole
+// column, so its spectrum too), and K, K' do not depend on t, the column pass
+// takes the nine sums of step s on alpha_k and multiplies by the opening
+// factors in the same registers: one FFT pair per pass instead of two, the
+// same W and the same sums in exact arithmetic.
 // ---------------------------------------------------------------------------
 #ifndef FW_THETA_SUMS
 #define FW_THETA_SUMS 0
@@ -340,6 +341,7 @@ struct KMul {
 struct VVMul {
   const cpx2 (&f0)[FW_COL_E / 2 + 1];   // closing V/2 (step s)
   const cpx2 (&f1)[FW_COL_E / 2 + 1];   // opening V/2 (step s+1)
+  real_t ma, mb;                        // x-only x-p mask of columns a, b (1 without a mask)
   const SumW* __restrict__ sw;          // weights, read at the multiply (registers)
   int t;                                // lane of the line
   double acc[4];                        // sum c_k Re(sigma_k conj F_k), sigma = alpha_a + alpha_b,
@@ -350,7 +352,7 @@ struct VVMul {
     const real_t h = (real_t)0.5 / (real_t)FW_NP;
     const cpx A = mk(h * (zk.x + zm.x), h * (zk.y - zm.y));
     const cpx B = mk(h * (zk.y + zm.y), h * (zm.x - zk.x));
-    const cpx A1 = cmul(A, f0[i].a), B1 = cmul(B, f0[i].b);
+    const cpx A1 = cscale(cmul(A, f0[i].a), ma), B1 = cscale(cmul(B, f0[i].b), mb);
     const double sx = (double)A1.x + (double)B1.x, sy = (double)A1.y + (double)B1.y;
     const SumW w = sw[spec_k<VL>(t, i)];
     acc[0] += sy * w.a - sx * (::dP * FW_NP);
@@ -367,8 +369,8 @@ struct VVMul {
   FWD void self(cpx& z0, cpx& zn) {
     constexpr int E2 = FW_COL_E / 2;
     const real_t s = (real_t)1 / (real_t)FW_NP;
-    const real_t ra = z0.x * s * f0[0].a.x, rb = z0.y * s * f0[0].b.x;
-    const real_t na = zn.x * s * f0[E2].a.x, nb = zn.y * s * f0[E2].b.x;
+    const real_t ra = z0.x * s * f0[0].a.x * ma, rb = z0.y * s * f0[0].b.x * mb;
+    const real_t na = zn.x * s * f0[E2].a.x * ma, nb = zn.y * s * f0[E2].b.x * mb;
     a0 = ra;
     b0 = rb;
     const double s0 = (double)ra + (double)rb, sn = (double)na + (double)nb;
@@ -565,7 +567,8 @@ FWD void vvpass(const Args& a) {
 }
 
 // The fused column pass with the sums on the theta spectrum (VVMul): no x-p
-// mask, K and K' independent of t (FW_THETA_SUMS, set by the host).
+// mask or one of x alone, K and K' independent of t (FW_THETA_SUMS, set by
+// the host).
 FWD void vvpass_theta(const Args& a) {
   using L = VL;
   constexpr int N = FW_NP, E = FW_COL_E, P = FW_COL_P, TPL = L::TPL, NT = L::NT;
@@ -601,14 +604,17 @@ FWD void vvpass_theta(const Args& a) {
   if (blk == 0 && tid == 0 && a.scal[SC_FRESH] != 0.0) a.scal[SC_FRESH] = 0.0;
   __syncthreads();
 
-  VVMul mul{f0, f1, a.sw, t, {0.0, 0.0, 0.0, 0.0}, 0.0, 0.0, 0.0};
+  const double Xa = ::dX * (c - 0.5 * ::X_gridDIM);
+  const double Xb = ::dX * (c + 1 - 0.5 * ::X_gridDIM);
+  // the x-only mask at t0, once per column (MaskXP); 1 without a mask
+  const MaskXP mask((a.flags & F_MASK_XP) != 0, Xa, Xb, t0);
+  static_assert(!FW_THETA_SUMS || FW_MASK_XP_X_ONLY, "the theta-spectrum pass takes x-only masks");
+  VVMul mul{f0, f1, mask.ma, mask.mb, a.sw, t, {0.0, 0.0, 0.0, 0.0}, 0.0, 0.0, 0.0};
   spectral_step<L>(v, lds, q, t, twl, mul);
 #pragma unroll
   for (int e = 0; e < E; ++e) *reinterpret_cast<cpx*>(W + (size_t)(t + TPL * e) * FW_NX + c) = v[e];
 
   // the nine sums of step s (column sums N alpha_0 on thread 0 of each line)
-  const double Xa = ::dX * (c - 0.5 * ::X_gridDIM);
-  const double Xb = ::dX * (c + 1 - 0.5 * ::X_gridDIM);
   const double ca = (double)N * mul.a0, cb = (double)N * mul.b0;
   double sums[NSER];
   sums[0] = ca + cb;

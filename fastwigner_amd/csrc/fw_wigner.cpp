@@ -197,11 +197,11 @@ bool tables_k(const fw_wigner_desc* d, const fw_wigner_exprs* x) {
 }
 
 // The fused column pass takes the step's sums on the theta spectrum when
-// nothing acts on W in x-p between its two half-steps (no x-p mask) and the
-// weights of the p sums are fixed (K and K' independent of t).
+// nothing acts on W in x-p between its two half-steps but a mask of x alone
+// (no P token: it scales whole columns) and the weights of the p sums are
+// fixed (K and K' independent of t).
 bool theta_sums(const fw_wigner_exprs* x) {
-  const bool mask_xp = x->abs_boundary_x_p && *x->abs_boundary_x_p;
-  return !mask_xp && !uses_t(x->K) && !uses_t(x->diff_K);
+  return !uses_token(x->abs_boundary_x_p, "P") && !uses_t(x->K) && !uses_t(x->diff_K);
 }
 
 std::string make_prelude(const fw_wigner_desc* d, const fw_wigner_exprs* x, const Tiles& t) {

@@ -100,14 +100,20 @@ def test_harmonic_closed_form_gpu(gpu):
     assert rel_l2(W, Wa) <= 1e-12
 
 
-@pytest.mark.parametrize("nx,npp", [(128, 64), (64, 256), (512, 512)])
-def test_rectangular_masks_time_dependent(gpu, nx, npp):
+@pytest.mark.parametrize("nx,npp,xp", [(128, 64, "x"), (64, 256, "x"), (512, 512, "x"), (128, 64, "xp"),
+                                       (256, 256, "xp")])
+def test_rectangular_masks_time_dependent(gpu, nx, npp, xp):
     """Non-square grids, all three absorbing-boundary strings and a
-    time-dependent potential V(x, t) = x^2/2 + x E(t) (strong-field pattern)."""
+    time-dependent potential V(x, t) = x^2/2 + x E(t) (strong-field pattern).
+    An x-p mask of x alone keeps the fused column pass on the theta spectrum;
+    one that also depends on p sends it through x-p space between its halves."""
     consts = {"E0": 0.05, "om": 0.3}
+    mxp = "pow(fabs(cos(0.5*M_PI*X/X_amplitude)), 0.125)"
+    if xp == "xp":
+        mxp += " * exp(-1e-3*P*P*P*P/(P_amplitude*P_amplitude))"
     masks = dict(abs_boundary_x_theta="pow(fabs(cos(0.5*M_PI*X_minus/X_amplitude)*cos(0.5*M_PI*X_plus/X_amplitude)), 0.01)",
                  abs_boundary_lambda_p="exp(-1e-4*Lambda*Lambda)",
-                 abs_boundary_x_p="pow(fabs(cos(0.5*M_PI*X/X_amplitude)), 0.125)")
+                 abs_boundary_x_p=mxp)
     g, o = make_pair(nx, 10., npp, 6., 0.02, "0.5*X*X + X*E0*sin(om*t)", "0.5*P*P",
                      "X + E0*sin(om*t)", "P", consts=consts, masks=masks)
     W0 = np.exp(-(o.X + 0.5) ** 2 - (o.P - 0.3) ** 2)
