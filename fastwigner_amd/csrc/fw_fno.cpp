@@ -173,11 +173,14 @@ __device__ __forceinline__ float gelu_d(float x) {
   return 0.5f * (1.f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * __expf(-0.5f * x * x);
 }
 
-// v0[n][c] = sum_j Pw[j][c] f_j(n) + pb[c]; f = [W, x/Xa, p/Pa, feat[b][0..4]]
+// v0[n][c] = sum_j Pw[j][c] f_j(n) + pb[c]; f = [W, x/Xa, p/Pa, feat[b][0..4]].
+// One thread per output (n, c): consecutive threads write consecutive channels.
 __global__ void __launch_bounds__(256) lift_fwd(const float* W, const float* feat, const float* Pw, const float* pb,
                                                 float* v0, int B, int N, int C, float xs, float ps) {
-  const long long total = (long long)B * N * N;
-  for (long long n = (long long)blockIdx.x * 256 + threadIdx.x; n < total; n += (long long)gridDim.x * 256) {
+  const long long total = (long long)B * N * N * C;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < total; i += (long long)gridDim.x * 256) {
+    const long long n = i / C;
+    const int c = (int)(i % C);
     const int b = (int)(n / ((long long)N * N));
     const int rem = (int)(n % ((long long)N * N));
     const int p = rem / N, x = rem % N;
@@ -187,12 +190,10 @@ __global__ void __launch_bounds__(256) lift_fwd(const float* W, const float* fea
     f[2] = ps * (p - 0.5f * N);
 #pragma unroll
     for (int j = 0; j < 5; ++j) f[3 + j] = feat[b * 5 + j];
-    for (int c = 0; c < C; ++c) {
-      float s = pb[c];
+    float s = pb[c];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) s += Pw[j * C + c] * f[j];
-      v0[n * C + c] = s;
-    }
+    for (int j = 0; j < 8; ++j) s += Pw[j * C + c] * f[j];
+    v0[i] = s;
   }
 }
 
@@ -384,23 +385,26 @@ __global__ void __launch_bounds__(256) to_bf16_kernel(const float* s, bf16_t* d,
   for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) d[i] = to_bf16(s[i]);
 }
 
-// Per sample b (one block each): x[b][n] -= mean_n (x[b][n] - y[b][n]) (y = NULL: 0).
-// Forward of the zero-mass correction (x = out, y = W) and its backward (x = dout).
-__global__ void __launch_bounds__(256) center_kernel(float* x, const float* y, long long nn) {
-  __shared__ double r[256];
+// Per sample b (one block of 1024 threads each): x[b][n] -= mean_n (x[b][n] - y[b][n])
+// (y = NULL: 0). Forward of the zero-mass correction (x = out, y = W) and its
+// backward (x = dout).
+__global__ void __launch_bounds__(1024) center_kernel(float* x, const float* y, long long nn) {
+  __shared__ double r[1024];
   const int b = blockIdx.x;
   float* xb = x + (long long)b * nn;
   const float* yb = y ? y + (long long)b * nn : nullptr;
   double s = 0;
-  for (long long i = threadIdx.x; i < nn; i += 256) s += (double)xb[i] - (yb ? (double)yb[i] : 0.0);
+#pragma unroll 8
+  for (long long i = threadIdx.x; i < nn; i += 1024) s += (double)xb[i] - (yb ? (double)yb[i] : 0.0);
   r[threadIdx.x] = s;
   __syncthreads();
-  for (int k = 128; k > 0; k >>= 1) {
+  for (int k = 512; k > 0; k >>= 1) {
     if (threadIdx.x < k) r[threadIdx.x] += r[threadIdx.x + k];
     __syncthreads();
   }
   const float m = (float)(r[0] / (double)nn);
-  for (long long i = threadIdx.x; i < nn; i += 256) xb[i] -= m;
+#pragma unroll 8
+  for (long long i = threadIdx.x; i < nn; i += 1024) xb[i] -= m;
 }
 
 // loss L = (1/B) sum_b sum_n (out - t)^2 / sum_n t^2 ; dout = dL/dout. One block per sample.
@@ -570,7 +574,7 @@ int forward(fw_fno* h, int B) {
   const long long npts = (long long)B * N * N, tot = npts * C;
   const float xs = (float)(2.0 * h->d.x_amplitude / N / h->d.x_amplitude);
   const float ps = (float)(2.0 * h->d.p_amplitude / N / h->d.p_amplitude);
-  hipLaunchKernelGGL(fno::lift_fwd, dim3(grid_for(npts)), dim3(256), 0, h->stream, h->bin, h->bft, h->P + h->o_Pw,
+  hipLaunchKernelGGL(fno::lift_fwd, dim3(grid_for(tot)), dim3(256), 0, h->stream, h->bin, h->bft, h->P + h->o_Pw,
                      h->P + h->o_pb, h->v[0], B, N, C, xs, ps);
   for (int l = 0; l < h->L; ++l) {
     float* acc = h->pre[l];
@@ -584,7 +588,7 @@ int forward(fw_fno* h, int B) {
   hipLaunchKernelGGL(fno::proj_fwd, dim3(grid_for(npts)), dim3(256), 0, h->stream, h->bin, h->v[h->L], h->P + h->o_Q,
                      h->P + h->o_qb, h->out, npts, C, h->out_scale);
   if (h->zero_mass)
-    hipLaunchKernelGGL(fno::center_kernel, dim3(B), dim3(256), 0, h->stream, h->out, h->bin, (long long)N * N);
+    hipLaunchKernelGGL(fno::center_kernel, dim3(B), dim3(1024), 0, h->stream, h->out, h->bin, (long long)N * N);
   HIPCHK(hipGetLastError());
   return FW_OK;
 }
@@ -668,7 +672,7 @@ int loss(fw_fno* h, int B, bool want_grad, double* out) {
                      h->persample, nn, 1.f / B);
   // zero-mass correction: dL/dc = s (dout - mean dout) per sample; proj_bwd applies s
   if (want_grad && h->zero_mass)
-    hipLaunchKernelGGL(fno::center_kernel, dim3(B), dim3(256), 0, h->stream, h->dout, nullptr, nn);
+    hipLaunchKernelGGL(fno::center_kernel, dim3(B), dim3(1024), 0, h->stream, h->dout, nullptr, nn);
   HIPCHK(hipGetLastError());
   if (out) {
     std::vector<double> ps(B);
