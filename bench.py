@@ -212,11 +212,17 @@ def batched_variant(steps=500, o_workers=1, with_oracle=True):
     out = {"value": rate, "unit": "grid-point*steps/s", "ms_per_step": el * 1e3 / steps, "steps": steps,
            "instances": 256, "total_points": b.total_points, "gibbs_prep_s": round(prep, 2),
            "tasks_per_step": b.tasks_per_step, "persistent_grid": b.grid}
-    # one step moves W in three passes plus the dissipator's two reads of W0s: 32 B per point (FP32)
+    # FP32 bytes a step must move: three passes read and write W (24 B per point) and the
+    # dissipator reads W0s twice (8 B); the two-pass step reads and writes W twice (16 B), and a
+    # dissipative instance also reads W0s and G = Vopen(W0s) once each (8 B)
     us = b.step_time_us(640)
-    bytes_step = 32 * b.total_points
+    passes = b.passes_per_step
+    pts = np.array([d["X_gridDIM"] ** 2 for d in inst], dtype=np.float64)
+    dis = np.array([float(w) < 1.0 for w in b.w_coeff])
+    bytes_step = int(np.sum(pts * ((16 if passes == 2 else 24) + 8 * dis)))
+    out["passes_per_step"] = passes
     out["roofline"] = {"bound": "hbm", "kernel": "fw_batch_step (one step: %s)" % (
-                           "one persistent launch" if b.mode == 0 else "three launches, one per pass"),
+                           "one persistent launch" if b.mode == 0 else "%d launches, one per pass" % passes),
                        "achieved": round(bytes_step / (us * 1e-6) / 1e9, 1),
                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": round(bytes_step / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,

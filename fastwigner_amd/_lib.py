@@ -246,6 +246,7 @@ PROTOTYPES = {
     "fw_batch_step_time": [_P, _I, _PD],
     "fw_batch_state_ptr": [_P, _I, _PP, _PI],
     "fw_batch_copy_states": [_P, _P],
+    "fw_batch_passes": [_P, _PI],
     "fw_fno_create": [ctypes.POINTER(FnoDesc), _PP],
     "fw_fno_destroy": [_P],
     "fw_fno_param_count": [_P, ctypes.POINTER(ctypes.c_longlong)],

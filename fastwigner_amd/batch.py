@@ -155,6 +155,13 @@ class WignerBatchHIP1D:
         if record:
             self.series = np.concatenate([self.series, ser], axis=0)
 
+    @property
+    def passes_per_step(self):
+        """Launches per real-time step (1 persistent, 3, or 2 for the fused two-pass steps)."""
+        n = ctypes.c_int()
+        _lib.call("fw_batch_passes", self._handle, ctypes.byref(n))
+        return n.value
+
     def step_time_us(self, steps=640):
         """Device time per step from graph replays between HIP events (advances the state)."""
         us = ctypes.c_double()

@@ -40,6 +40,7 @@ struct BTaskH {
 struct BArgsH {
   void* W;
   const void* W0s;
+  void* G;
   const void* inst;
   const void* tasks;
   const int* qoff;
@@ -53,8 +54,19 @@ struct BArgsH {
   int* err;
   const double* vkmin;
   int step, gstep, ninst, mode, task_base, spin_limit;
+  int vv, pad_;
   double cv, ck;
 };
+
+// Does the expression reference the identifier t (the time argument)?
+bool uses_t_token(const char* s) {
+  if (!s) return false;
+  auto ident = [](char c) { return isalnum((unsigned char)c) || c == '_'; };
+  const size_t n = strlen(s);
+  for (size_t i = 0; i < n; ++i)
+    if (s[i] == 't' && (i == 0 || !ident(s[i - 1])) && (i + 1 == n || !ident(s[i + 1]))) return true;
+  return false;
+}
 
 int size_index(int n) {
   for (int s = 0; s < kNSizes; ++s)
@@ -122,6 +134,8 @@ struct fw_batch {
 
   void* W = nullptr;
   void* W0s = nullptr;
+  void* G = nullptr;          // Vopen(W0s) of the two-pass steps
+  bool two_pass = false;      // mode 1 real-time runs in two passes per step (V independent of t)
   void* inst_d = nullptr;
   void* qtasks_d = nullptr;   // mode 0 queues
   void* ptasks_d = nullptr;   // mode 1, pass-major, XCD-interleaved
@@ -233,6 +247,7 @@ BArgsH base_args(fw_batch* h) {
   std::memset(&a, 0, sizeof a);
   a.W = h->W;
   a.W0s = h->W0s;
+  a.G = h->G;
   a.inst = h->inst_d;
   a.tasks = h->d.mode == 0 ? h->qtasks_d : h->ptasks_d;
   a.qoff = h->qoff_d;
@@ -281,12 +296,30 @@ int enqueue_step(fw_batch* h, int kind, int s, int gstep) {
   return FW_OK;
 }
 
+// one pass of the mode-1 task lists: pass 0 (column tiles), 1 (row tiles) or
+// 2 (column tiles), with the two-pass selector vv
+int enqueue_pass(fw_batch* h, int pass, int vv, int s, int gstep) {
+  BArgsH a = base_args(h);
+  a.step = s;
+  a.gstep = gstep;
+  a.vv = vv;
+  a.task_base = h->ptask_off[pass];
+  return launch(h, h->f_step, h->ptask_off[pass + 1] - h->ptask_off[pass], a);
+}
+
+// two-pass steps: the K pass of step s, then the fused column pass (closing
+// V/2 of s, opening V/2 of s+1)
+int enqueue_step2(fw_batch* h, int s, int gstep) {
+  FWCHK(enqueue_pass(h, 1, 1, s, gstep));
+  return enqueue_pass(h, 0, 1, s, gstep);
+}
+
 int ensure_graph(fw_batch* h) {
   if (h->gexec) return FW_OK;
   hipGraph_t g;
   HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
   int rc = zero_counters(h);
-  for (int s = 0; s < kChunk && rc == FW_OK; ++s) rc = enqueue_step(h, 0, s, s);
+  for (int s = 0; s < kChunk && rc == FW_OK; ++s) rc = h->two_pass ? enqueue_step2(h, s, s) : enqueue_step(h, 0, s, s);
   hipError_t e = hipStreamEndCapture(h->stream, &g);
   if (rc != FW_OK) return rc;
   if (e != hipSuccess) return fw::fail(FW_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
@@ -347,6 +380,7 @@ extern "C" int fw_batch_create(const fw_batch_desc* d, const fw_batch_instance* 
   FWCHK(plan(d, in, x, h->e_for, &h->nmax, h->prelude));
   h->d = *d;
   h->src.assign(in, in + d->n_instances);
+  h->two_pass = d->mode == 1 && !uses_t_token(x->V);
   h->fp32 = d->precision == FW_FP32;
   h->esz = h->fp32 ? 4 : 8;
   int ndev = 0;
@@ -470,7 +504,7 @@ extern "C" int fw_batch_destroy(fw_batch* h) {
   if (!h) return FW_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
-  void* bufs[] = {h->W, h->W0s, h->inst_d, h->qtasks_d, h->ptasks_d, h->qoff_d, h->tw, h->times, h->partials,
+  void* bufs[] = {h->W, h->W0s, h->G, h->inst_d, h->qtasks_d, h->ptasks_d, h->qoff_d, h->tw, h->times, h->partials,
                   h->series, h->counters, h->fresh, h->err, h->vkmin};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
@@ -521,10 +555,25 @@ static int ensure_w0s(fw_batch* h) {
   if (h->W0s) return FW_OK;
   HIPCHK(hipMalloc(&h->W0s, (size_t)h->points * h->esz));
   HIPCHK(hipMemsetAsync(h->W0s, 0, (size_t)h->points * h->esz, h->stream));
+  if (h->two_pass) {
+    HIPCHK(hipMalloc(&h->G, (size_t)h->points * h->esz));
+    HIPCHK(hipMemsetAsync(h->G, 0, (size_t)h->points * h->esz, h->stream));
+  }
   if (h->gexec) {
     (void)hipGraphExecDestroy(h->gexec);
     h->gexec = nullptr;
   }
+  return FW_OK;
+}
+
+// G = Vopen(W0s) of every dissipative instance, for the two-pass steps (V does
+// not depend on t there, so G is computed once per source).
+static int update_g(fw_batch* h) {
+  if (!h->two_pass || !h->G) return FW_OK;
+  const double t0 = 0.0;
+  HIPCHK(hipMemcpyAsync(h->times, &t0, sizeof(double), hipMemcpyHostToDevice, h->stream));
+  FWCHK(enqueue_pass(h, 0, 2, 0, 0));
+  HIPCHK(hipStreamSynchronize(h->stream));
   return FW_OK;
 }
 
@@ -550,7 +599,8 @@ extern "C" int fw_batch_set_dissipator(fw_batch* h, int i, const double* host_w0
   }
   b.dissip = 1;
   b.w_coeff = w_coeff;
-  return upload_instances(h);
+  FWCHK(upload_instances(h));
+  return update_g(h);
 }
 
 extern "C" int fw_batch_gibbs_dissipators(fw_batch* h, const double* w_coeff) {
@@ -568,6 +618,7 @@ extern "C" int fw_batch_gibbs_dissipators(fw_batch* h, const double* w_coeff) {
   void* w0s = h->W0s;
   void* params[] = {&a, &w0s};
   HIPCHK(hipModuleLaunchKernel(h->f_gibbs, h->d.n_instances * 64, 1, 1, kBT, 1, 1, 0, h->stream, params, nullptr));
+  FWCHK(update_g(h));
   HIPCHK(hipStreamSynchronize(h->stream));
   return FW_OK;
 }
@@ -632,7 +683,20 @@ extern "C" int fw_batch_propagate(fw_batch* h, const double* times, int steps, d
   for (int off = 0; off < steps; off += kChunk) {
     const int n = std::min(kChunk, steps - off);
     HIPCHK(hipMemcpyAsync(h->times, times + off, sizeof(double) * n, hipMemcpyHostToDevice, h->stream));
-    if (n == kChunk) {
+    if (h->two_pass) {
+      // VF, then (K, VV) per step; the run's last step is K, VL
+      if (off == 0) FWCHK(enqueue_pass(h, 0, 0, 0, 0));
+      const bool last = off + n == steps;
+      if (n == kChunk && !last) {
+        HIPCHK(hipGraphLaunch(h->gexec, h->stream));
+      } else {
+        for (int s = 0; s < n - (last ? 1 : 0); ++s) FWCHK(enqueue_step2(h, s, off + s));
+        if (last) {
+          FWCHK(enqueue_pass(h, 1, 1, n - 1, off + n - 1));
+          FWCHK(enqueue_pass(h, 2, 0, n - 1, off + n - 1));
+        }
+      }
+    } else if (n == kChunk) {
       HIPCHK(hipGraphLaunch(h->gexec, h->stream));
     } else {
       FWCHK(zero_counters(h));
@@ -683,6 +747,13 @@ extern "C" int fw_batch_copy_states(fw_batch* dst, fw_batch* src) {
   return FW_OK;
 }
 
+extern "C" int fw_batch_passes(fw_batch* h, int* passes_per_step) {
+  fw::clear_error();
+  if (!h || !passes_per_step) return fw::fail(FW_ERR_INVALID_ARG, "NULL argument");
+  *passes_per_step = h->d.mode == 0 ? 1 : (h->two_pass ? 2 : 3);
+  return FW_OK;
+}
+
 extern "C" int fw_batch_info(fw_batch* h, long long* points, int* tasks_per_step, int* grid) {
   if (!h) return fw::fail(FW_ERR_INVALID_ARG, "NULL handle");
   if (points) *points = h->points;
@@ -702,6 +773,7 @@ extern "C" int fw_batch_step_time(fw_batch* h, int steps, double* us_per_step) {
   HIPCHK(hipMemcpyAsync(h->times, t.data(), sizeof(double) * kChunk, hipMemcpyHostToDevice, h->stream));
   FWCHK(fill_int(h, h->fresh, h->d.n_instances, 1));
   FWCHK(fill_int(h, h->err, 1, 0));
+  if (h->two_pass) FWCHK(enqueue_pass(h, 0, 0, 0, 0));   // VF (not timed)
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
@@ -709,6 +781,10 @@ extern "C" int fw_batch_step_time(fw_batch* h, int steps, double* us_per_step) {
   for (int c = 0; c < chunks; ++c) HIPCHK(hipGraphLaunch(h->gexec, h->stream));
   HIPCHK(hipEventRecord(e1, h->stream));
   HIPCHK(hipEventSynchronize(e1));
+  if (h->two_pass) {   // close the run: K, VL (not timed)
+    FWCHK(enqueue_pass(h, 1, 1, kChunk - 1, kChunk - 1));
+    FWCHK(enqueue_pass(h, 2, 0, kChunk - 1, kChunk - 1));
+  }
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, e0, e1));
   (void)hipEventDestroy(e0);

@@ -26,7 +26,15 @@
 // always end; each wait is also bounded (spin_limit), after which the block
 // raises the error flag and every block drains out.
 // One launch per pass (mode 1): grid = tasks of that pass, one task per block,
-// no waits.
+// no waits. In mode 1, when V does not depend on t, a real-time run takes two
+// passes per step (BArgs.vv = 1): the K pass, which first applies the previous
+// step's deferred norm and dissipator as W = alpha Y + G, and one fused column
+// pass (b_vvpass) holding the closing V/2 of a step, its dissipator and sums,
+// and the opening V/2 of the next step. With U the state after the closing V/2
+// and dissipator and S its sum, the next step starts from w U / (S dXdP) + W0s,
+// and the opening V/2 is linear, so the pass stores Y = Vopen(U) and the K pass
+// forms (w / (S dXdP)) Y + G with G = Vopen(W0s) (b_gpass, once per source).
+// A run opens with the first V pass and closes with the last V pass.
 #pragma once
 
 #include "fw_common.h"
@@ -58,6 +66,7 @@ struct BTask {
 struct BArgs {
   real_t* W;
   const real_t* W0s;
+  real_t* G;              // Vopen(W0s) of the two-pass steps (dissipative instances), or null
   const BInst* inst;
   const BTask* tasks;     // mode 0: queues concatenated; mode 1: passes concatenated
   const int* qoff;        // [9] queue offsets (mode 0)
@@ -76,6 +85,9 @@ struct BArgs {
   int mode;
   int task_base;          // mode 1: first task of this launch
   int spin_limit;
+  int vv;                 // mode 1 Moyal: 0 three-pass step; 1 two-pass step (pass 0 tasks run
+                          // the fused column pass); 2: pass 0 tasks compute G from W0s
+  int pad_;
   double cv, ck;          // Bloch exponent factors (1/4, 1/2 or 1/2, 1)
 };
 
@@ -245,6 +257,27 @@ FWD void b_kpass(const BArgs& a, int i, int tile, int tid, cpx* lds, const tw_t*
   cpx v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = mk(Wa[t + TPL * e], Wb[t + TPL * e]);
+  if (KIND == BK_MOYAL && a.vv == 1 && fr == 0) {
+    // two-pass steps: W = alpha Y + G, the previous step's norm and dissipator
+    const bool dis = in->dissip;
+    cpx g[E];
+    if (dis) {
+      const real_t* __restrict__ Ga = a.G + in->off + (size_t)r * N;
+#pragma unroll
+      for (int e = 0; e < E; ++e) g[e] = mk(Ga[t + TPL * e], Ga[N + t + TPL * e]);
+    }
+    const double S0 = sum_partials<BT, 1>(a.partials + (size_t)in->part * NSER, in->ntile, 0, red);
+    const real_t al = (real_t)((dis ? in->w_coeff : 1.0) / (S0 * (in->dX * in->dP)));
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      v[e].x *= al;
+      v[e].y *= al;
+      if (dis) {
+        v[e].x += g[e].x;
+        v[e].y += g[e].y;
+      }
+    }
+  }
   __syncthreads();
   const double dP = in->dP, dL = in->dLambda;
   const double Pa = dP * (r - 0.5 * N), Pb = dP * (r + 1 - 0.5 * N);
@@ -276,11 +309,130 @@ FWD void b_kpass(const BArgs& a, int i, int tile, int tid, cpx* lds, const tw_t*
   }
 }
 
+// V factors of a column tile's items (V independent of t in the two-pass steps:
+// one set serves the closing and the opening half-steps)
+template <int N, int E, class L>
+FWD void b_vfactors(const BInst* in, int c, int t, double tau, cpx (&fa)[E / 2 + 1], cpx (&fb)[E / 2 + 1]) {
+  const double Xa = in->dX * (c - 0.5 * N), Xb = in->dX * (c + 1 - 0.5 * N);
+#pragma unroll
+  for (int k = 0; k <= E / 2; ++k) {
+    const double Th = in->dTheta * (k < E / 2 ? spec_k<L>(t, k) : N / 2);
+    fa[k] = b_vfactor(Xa, Th, tau, in->prm);
+    fb[k] = b_vfactor(Xb, Th, tau, in->prm);
+  }
+}
+
+// Fused column pass of the two-pass steps: closing V/2 of step s, dissipator,
+// the nine sums of step s, opening V/2 of step s+1; stores Y (unnormalised).
+template <int N, int E>
+FWD void b_vvpass(const BArgs& a, int i, int tile, int tid, cpx* lds, const tw_t* twl, double* red) {
+  constexpr int TPL = N / E, P = BT / TPL;
+  using L = Line<N, E, P, false, FW_B_NMAX>;
+  const BInst* in = a.inst + i;
+  const int q = tid % P, t = tid / P;
+  const int c = (tile * P + q) * 2;
+  real_t* __restrict__ W = a.W + in->off;
+  const double* prm = in->prm;
+  const double tnow = a.times[a.step];
+  const double tau = tnow + FW_PHASE_SHIFT * ::dt;
+  const double dX = in->dX, dP = in->dP;
+  const bool dis = in->dissip;
+  const real_t wc = (real_t)in->w_coeff;
+  cpx v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = *reinterpret_cast<const cpx*>(W + (size_t)(t + TPL * e) * N + c);
+  cpx fa[E / 2 + 1], fb[E / 2 + 1];
+  b_vfactors<N, E, L>(in, c, t, tau, fa, fb);
+  __syncthreads();
+  BMul<N, E> mul{fa, fb};
+  spectral_step<L>(v, lds, q, t, twl, mul);
+  // end of step s: dissipator, sums
+  const double Xa = dX * (c - 0.5 * N), Xb = dX * (c + 1 - 0.5 * N);
+  double sums[NSER];
+#pragma unroll
+  for (int k = 0; k < NSER; ++k) sums[k] = 0.0;
+  double ca = 0.0, cb = 0.0;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int n = t + TPL * e;
+    const double Pn = dP * (n - 0.5 * N);
+    cpx w = v[e];
+    if (dis) {
+      const cpx g = *reinterpret_cast<const cpx*>(a.W0s + in->off + (size_t)n * N + c);
+      w.x = w.x * wc + g.x;
+      w.y = w.y * wc + g.y;
+    }
+    const double wa = (double)w.x, wb = (double)w.y, ws = wa + wb;
+    ca += wa; cb += wb;
+    sums[2] += Pn * ws;
+    sums[4] += Pn * Pn * ws;
+    sums[5] += ::fw_diff_K(Pn, tnow, prm) * ws;
+    sums[7] += ::K(Pn, tnow, prm) * ws;
+    sums[8] += wa * wa + wb * wb;
+    v[e] = w;
+  }
+  sums[0] = ca + cb;
+  sums[1] = Xa * ca + Xb * cb;
+  sums[3] = Xa * Xa * ca + Xb * Xb * cb;
+  sums[6] = ::fw_diff_V(Xa, tnow, prm) * ca + ::fw_diff_V(Xb, tnow, prm) * cb;
+  sums[7] += ::V(Xa, tnow, prm) * ca + ::V(Xb, tnow, prm) * cb;
+  write_partial_rows<BT>(sums, red, a.partials + (size_t)in->part * NSER, tile, in->ntile);
+  // every K tile of this step has read `fresh`
+  if (tile == 0 && tid == 0 && a.fresh[i] != 0) a.fresh[i] = 0;
+  // opening V/2 of step s+1 (the lane index through an empty asm: the
+  // compiler recomputes this step's twiddles instead of keeping the first's)
+  int t2 = t;
+  asm volatile("" : "+v"(t2));
+  __syncthreads();
+  BMul<N, E> mul1{fa, fb};
+  spectral_step<L>(v, lds, q, t2, twl, mul1);
+#pragma unroll
+  for (int e = 0; e < E; ++e) *reinterpret_cast<cpx*>(W + (size_t)(t + TPL * e) * N + c) = v[e];
+}
+
+// G = Vopen(W0s) for a dissipative instance (the opening V/2 alone).
+template <int N, int E>
+FWD void b_gpass(const BArgs& a, int i, int tile, int tid, cpx* lds, const tw_t* twl) {
+  constexpr int TPL = N / E, P = BT / TPL;
+  using L = Line<N, E, P, false, FW_B_NMAX>;
+  const BInst* in = a.inst + i;
+  if (!in->dissip) return;   // uniform: one instance per block
+  const int q = tid % P, t = tid / P;
+  const int c = (tile * P + q) * 2;
+  const real_t* __restrict__ S = a.W0s + in->off;
+  real_t* __restrict__ G = a.G + in->off;
+  const double tau = a.times[a.step] + FW_PHASE_SHIFT * ::dt;
+  cpx v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = *reinterpret_cast<const cpx*>(S + (size_t)(t + TPL * e) * N + c);
+  cpx fa[E / 2 + 1], fb[E / 2 + 1];
+  b_vfactors<N, E, L>(in, c, t, tau, fa, fb);
+  __syncthreads();
+  BMul<N, E> mul{fa, fb};
+  spectral_step<L>(v, lds, q, t, twl, mul);
+#pragma unroll
+  for (int e = 0; e < E; ++e) *reinterpret_cast<cpx*>(G + (size_t)(t + TPL * e) * N + c) = v[e];
+}
+
 template <int N, int E, int KIND>
 FWD void b_run_n(const BArgs& a, int i, int pass, int tile, int tid, cpx* lds, const tw_t* twl, double* red) {
-  if (pass == 0) b_vpass<N, E, KIND, true, false>(a, i, tile, tid, lds, twl, red);
-  else if (pass == 1) b_kpass<N, E, KIND>(a, i, tile, tid, lds, twl, red);
-  else b_vpass<N, E, KIND, false, true>(a, i, tile, tid, lds, twl, red);
+  if (pass == 0) {
+    if constexpr (KIND == BK_MOYAL) {
+      if (a.vv == 1) {
+        b_vvpass<N, E>(a, i, tile, tid, lds, twl, red);
+        return;
+      }
+      if (a.vv == 2) {
+        b_gpass<N, E>(a, i, tile, tid, lds, twl);
+        return;
+      }
+    }
+    b_vpass<N, E, KIND, true, false>(a, i, tile, tid, lds, twl, red);
+  } else if (pass == 1) {
+    b_kpass<N, E, KIND>(a, i, tile, tid, lds, twl, red);
+  } else {
+    b_vpass<N, E, KIND, false, true>(a, i, tile, tid, lds, twl, red);
+  }
 }
 
 template <int KIND>

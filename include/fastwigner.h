@@ -293,6 +293,11 @@ int fw_batch_bloch_propagate(fw_batch* h, int convention, double t_initial, cons
  * step each W_i is renormalised; series (optional) gets steps * n_instances * FW_NSER sums over
  * the unnormalised W_i of each step. One graph replay per 64 steps. */
 int fw_batch_propagate(fw_batch* h, const double* times, int steps, double* series);
+/* Launches per real-time step: 1 (mode 0, persistent), 3 (mode 1), or 2 (mode 1 when V does not
+ * depend on t: the K pass with the previous step's deferred norm and dissipator, then one column
+ * pass with the closing V/2, dissipator and sums of a step and the opening V/2 of the next). No
+ * counterpart in the reference (the batch is new). */
+int fw_batch_passes(fw_batch* h, int* passes_per_step);
 /* total grid points, tasks per step, persistent grid size */
 int fw_batch_info(fw_batch* h, long long* points, int* tasks_per_step, int* grid);
 /* Device time of one step (us), averaged over the floor(steps/64)*64 steps replayed from the

@@ -39,9 +39,9 @@ def small_batch():
     return inst
 
 
-def make_batch(inst, precision, mode=0):
+def make_batch(inst, precision, mode=0, Vs=V):
     from fastwigner_amd.batch import WignerBatchHIP1D, gaussian_state
-    b = WignerBatchHIP1D(inst, V=V, K="0.5*P*P", dt=0.01, param_names=("a2", "a3", "a4"), diff_V=DV, diff_K="P",
+    b = WignerBatchHIP1D(inst, V=Vs, K="0.5*P*P", dt=0.01, param_names=("a2", "a3", "a4"), diff_V=DV, diff_K="P",
                          precision=precision, mode=mode)
     if any(d.get("gamma", 0) > 0 for d in inst):
         b.prepare_random_collision(dbeta=0.01)
@@ -50,15 +50,17 @@ def make_batch(inst, precision, mode=0):
     return b
 
 
-@pytest.mark.parametrize("mode", [0, 1])
-def test_batch_fp64_parity(gpu, mode):
+@pytest.mark.parametrize("mode,steps", [(0, 150), (1, 150), (1, 128), (1, 1), (1, 65)])
+def test_batch_fp64_parity(gpu, mode, steps):
     """Seven instances (64^2 .. 512^2, Moyal and random collision with their own
-    Gibbs states from the batched Bloch run), 150 steps (two graph chunks plus
-    a partial one), against the oracle per instance."""
+    Gibbs states from the batched Bloch run), against the oracle per instance:
+    150 steps (two graph chunks plus a partial one), and for the two-pass steps
+    of mode 1 also runs ending exactly on a chunk, of one step and of one chunk
+    plus one step."""
     from fastwigner_amd.batch import gaussian_state
     inst = small_batch()
     b = make_batch(inst, "fp64", mode)
-    steps = 150
+    assert b.passes_per_step == (1 if mode == 0 else 2)
     b.propagate(steps)
     for i, d in enumerate(inst):
         o = oracle_for(d, 0.01)
@@ -78,24 +80,32 @@ def test_batch_fp64_parity(gpu, mode):
 
 
 def test_batch_fp32_form_and_modes_agree(gpu):
-    """FP32 batch (FP64 phases and sums) against the FP64 oracle over 500 steps,
-    and the persistent launch (mode 0) bit-identical to one launch per pass."""
+    """FP32 batch (FP64 phases and sums) against the FP64 oracle over 500 steps.
+    The persistent launch (mode 0) is bit-identical to one launch per pass in
+    three passes per step (mode 1 with a V that names t, which keeps the three
+    passes); the two-pass steps (mode 1, V independent of t) meet the same
+    oracle bar and agree with them to FP32 rounding."""
     from fastwigner_amd.batch import gaussian_state
     inst = small_batch()[:4]
     b0 = make_batch(inst, "fp32", 0)
-    b1 = make_batch(inst, "fp32", 1)
-    b0.propagate(500)
-    b1.propagate(500)
+    b1 = make_batch(inst, "fp32", 1, Vs=V + " + 0.*t")
+    b2 = make_batch(inst, "fp32", 1)
+    assert (b0.passes_per_step, b1.passes_per_step, b2.passes_per_step) == (1, 3, 2)
+    for b in (b0, b1, b2):
+        b.propagate(500)
     for i, d in enumerate(inst):
-        W0, W1 = b0.get_wignerfunction(i), b1.get_wignerfunction(i)
+        W0, W1, W2 = b0.get_wignerfunction(i), b1.get_wignerfunction(i), b2.get_wignerfunction(i)
         assert np.array_equal(W0, W1), i
         assert np.array_equal(b0.series[:, i], b1.series[:, i]), i
         o = oracle_for(d, 0.01)
         o.set_wignerfunction(gaussian_state(d["X_gridDIM"], d["X_amplitude"], d["x0"], d["p0"]))
         o.propagate(500)
-        err = rel(W0, o.W)
-        print("fp32 instance %d N=%d: rel L2 vs fp64 oracle %.2e" % (i, d["X_gridDIM"], err))
-        assert err <= 1e-4, i
+        err, err2, d02 = rel(W0, o.W), rel(W2, o.W), rel(W2, W0)
+        print("fp32 instance %d N=%d: rel L2 vs fp64 oracle %.2e (two-pass %.2e; between them %.2e)"
+              % (i, d["X_gridDIM"], err, err2, d02))
+        assert err <= 1e-4 and err2 <= 1e-4, i
+        assert d02 <= 2e-5, i
+        assert np.max(np.abs(b2.series[:, i, 0] / b0.series[:, i, 0] - 1)) <= 1e-5, i
 
 
 def test_config4_full_batch(gpu):
