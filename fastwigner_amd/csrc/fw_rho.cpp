@@ -31,6 +31,8 @@ struct RArgs {
   double* series;
   const void* uv;
   const void* uk;
+  const double* dw;
+  unsigned int* dcount;
   int step;
   int flags;
   int nbp;
@@ -84,7 +86,9 @@ struct fw_rho {
   hipFunction_t f_a = nullptr, f_b = nullptr, f_c = nullptr, f_ba = nullptr, f_bb = nullptr, f_bc = nullptr,
                 f_rows = nullptr, f_diag_p = nullptr, f_diag_f = nullptr, f_norm = nullptr, f_moments = nullptr,
                 f_shx = nullptr, f_shy = nullptr, f_flip = nullptr, f_wscale = nullptr, f_set = nullptr,
-                f_vmin = nullptr;
+                f_vmin = nullptr, f_dweights = nullptr;
+  double* dw = nullptr;             // momentum weights (FW_DIAG_W), or null
+  unsigned int* dcount = nullptr;   // blocks of the diagonal pass done
   hipStream_t stream = nullptr;
 
   void* rho = nullptr;
@@ -161,6 +165,20 @@ int tiles(const fw_rho_desc* d, int& re, int& rp, int& ce, int& cp) {
   return FW_OK;
 }
 
+// Does the expression reference the identifier t (the time argument)?
+bool uses_t_tok(const char* s) {
+  if (!s) return false;
+  auto ident = [](char c) { return isalnum((unsigned char)c) || c == '_'; };
+  const size_t n = strlen(s);
+  for (size_t i = 0; i < n; ++i)
+    if (s[i] == 't' && (i == 0 || !ident(s[i - 1])) && (i + 1 == n || !ident(s[i + 1]))) return true;
+  return false;
+}
+
+// Momentum sums by precomputed weights (one pass, no FFT) when K and K' do not
+// depend on t; otherwise the group sums go through the one-block inverse FFT.
+bool diag_weights(const fw_rho_exprs* x) { return !uses_t_tok(x->K) && !uses_t_tok(x->diff_K); }
+
 std::string make_prelude(const fw_rho_desc* d, const fw_rho_exprs* x, int re, int rp, int ce, int cp) {
   double dX, dP;
   grid(d, &dX, &dP);
@@ -171,6 +189,7 @@ std::string make_prelude(const fw_rho_desc* d, const fw_rho_exprs* x, int re, in
   s << "#define FW_ROW_E " << re << "\n#define FW_ROW_P " << rp << "\n#define FW_COL_E " << ce << "\n#define FW_COL_P "
     << cp << "\n";
   s << "#define FW_DIAG_E " << diag_e(d->x_grid_dim) << "\n";
+  s << "#define FW_DIAG_W " << (diag_weights(x) ? 1 : 0) << "\n";
   {
     int dc, g;
     diag_geom(d->x_grid_dim, &dc, &g);
@@ -221,6 +240,8 @@ RArgs base(fw_rho* h) {
   a.scal = h->scal;
   a.partials = h->partials;
   a.series = h->series;
+  a.dw = h->dw;
+  a.dcount = h->dcount;
   a.uv = h->uv;
   a.uk = h->uk;
   a.nbp = h->row_nb;
@@ -263,7 +284,7 @@ int enqueue_chunk(fw_rho* h, bool bloch, int n, bool momentum, const RArgs& b0, 
     if (ev) HIPCHK(hipEventRecord((*ev)[4 * s + 3], h->stream));
     if (momentum) {
       FWCHK(args1(h, h->f_diag_p, diag_blocks(h->N), 256, a));
-      FWCHK(args1(h, h->f_diag_f, 1, h->N / diag_e(h->N), a));
+      if (!h->dw) FWCHK(args1(h, h->f_diag_f, 1, h->N / diag_e(h->N), a));
     }
   }
   return FW_OK;
@@ -427,7 +448,7 @@ extern "C" int fw_rho_create(const fw_rho_desc* desc, const fw_rho_exprs* exprs,
              {"fw_rho_moments", &h->f_moments}, {"fw_rho_shear_x", &h->f_shx},
              {"fw_rho_shear_y", &h->f_shy}, {"fw_rho_flip_ifft", &h->f_flip},
              {"fw_rho_wigner_scale", &h->f_wscale}, {"fw_rho_set_scalar", &h->f_set},
-             {"fw_rho_vmin", &h->f_vmin}};
+             {"fw_rho_vmin", &h->f_vmin}, {"fw_rho_diag_weights", &h->f_dweights}};
   for (auto& f : fns)
     if (hipModuleGetFunction(f.f, h->mod, f.n) != hipSuccess)
       return bail(fw::fail(FW_ERR_HIP, std::string("missing kernel ") + f.n));
@@ -450,6 +471,15 @@ extern "C" int fw_rho_create(const fw_rho_desc* desc, const fw_rho_exprs* exprs,
   if ((rc = alloc((void**)&h->series, sizeof(double) * RSER * (kChunk + 1))) != FW_OK) return bail(rc);
   if ((rc = fw::upload_twiddles(h->N, h->fp32, &h->tw)) != FW_OK) return bail(rc);
   if (hipStreamCreate(&h->stream) != hipSuccess) return bail(fw::fail(FW_ERR_HIP, "hipStreamCreate failed"));
+  if (diag_weights(exprs)) {
+    if ((rc = alloc((void**)&h->dw, sizeof(double) * 8 * h->N)) != FW_OK) return bail(rc);
+    if ((rc = alloc((void**)&h->dcount, sizeof(unsigned int))) != FW_OK) return bail(rc);
+    RArgs a = base(h);
+    double* dw = h->dw;
+    void* params[] = {&a, &dw};
+    if ((rc = launch(h, h->f_dweights, (h->N + 255) / 256, 256, params)) != FW_OK) return bail(rc);
+    if (hipStreamSynchronize(h->stream) != hipSuccess) return bail(fw::fail(FW_ERR_HIP, "weights sync failed"));
+  }
   *out = h;
   return FW_OK;
 }
@@ -463,7 +493,7 @@ extern "C" int fw_rho_destroy(fw_rho* h) {
       if (kv.second.mod) (void)hipModuleUnload(kv.second.mod);
   if (h->mod) (void)hipModuleUnload(h->mod);
   for (void* p : {h->rho, h->aux, h->wig, h->winit, h->snap, h->tw, h->uv, h->uk, (void*)h->times, (void*)h->scal,
-                  (void*)h->partials, (void*)h->series})
+                  (void*)h->partials, (void*)h->series, (void*)h->dw, (void*)h->dcount})
     if (p) (void)hipFree(p);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -574,7 +604,7 @@ extern "C" int fw_rho_moments(fw_rho* h, double t, double sums[FW_RSER]) {
   RArgs a = base(h);
   a.step = 0;
   FWCHK(args1(h, h->f_diag_p, diag_blocks(h->N), 256, a));
-  FWCHK(args1(h, h->f_diag_f, 1, h->N / diag_e(h->N), a));
+  if (!h->dw) FWCHK(args1(h, h->f_diag_f, 1, h->N / diag_e(h->N), a));
   HIPCHK(hipMemcpyAsync(sums, h->scal + RC_OUT, sizeof(double) * 11, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipMemcpyAsync(sums + 11, h->series + 11, sizeof(double) * 9, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));

@@ -46,6 +46,8 @@ struct RArgs {
   double* series;      // RSER sums per step
   const cpx* uv;       // per-step rows u_j (unitary) or real Bloch rows
   const cpx* uk;       // per-step rows w_k
+  const double* dw;    // momentum weights G_o[d] (re, im), o = P, P^2, K', K, or null
+  unsigned int* dcount;  // blocks of the diagonal pass done (the last one publishes; reset by it)
   int step;
   int flags;
   int nbp;             // number of partial rows written by pass C
@@ -296,6 +298,9 @@ FWD void rho_rows(const RArgs& a, cpx* uv, cpx* uk) {
 // The row subgroups of a block are combined in LDS, in a fixed order, and the
 // G group sums land in aux[g * N + d] for the one-block finish.
 // ---------------------------------------------------------------------------
+#ifndef FW_DIAG_W
+#define FW_DIAG_W 0   // 1: momentum sums by the weights G (K and K' independent of t)
+#endif
 constexpr int DT = 256;   // threads of the diagonal partial kernel
 constexpr int DDC = FW_DIAG_DC, DW = DT / DDC, DG = FW_DIAG_G, DRG = FW_N / DG;
 static_assert(DT % DDC == 0 && FW_N % DDC == 0 && FW_N % DG == 0, "diagonal tiling");
@@ -326,7 +331,76 @@ FWD void rho_diag_partial(const RArgs& a) {
       cr += red[k * DDC + l];
       ci += red[DT + k * DDC + l];
     }
+#if !FW_DIAG_W
     a.aux[(size_t)g * FW_N + d] = mk((real_t)cr, (real_t)ci);
+#endif
+  }
+#if FW_DIAG_W
+  // Tr(rho g(P)) = sum_d c_d G_d with G_d = (1/N) sum_k g(P_k) exp(2 pi i k d / N)
+  // (K and K' independent of t): this block's share over its diagonals, then
+  // the last block to finish adds the shares in block order and publishes them.
+  __syncthreads();
+  double* dr = red;   // 8 rows of DDC lanes (8 DDC <= 2 DT)
+  if (w == 0) {
+#pragma unroll
+    for (int o = 0; o < 4; ++o) {
+      const double gr = a.dw[((size_t)o * FW_N + d) * 2], gi = a.dw[((size_t)o * FW_N + d) * 2 + 1];
+      dr[(2 * o) * DDC + l] = cr * gr - ci * gi;
+      dr[(2 * o + 1) * DDC + l] = cr * gi + ci * gr;
+    }
+  }
+  __syncthreads();
+  double* part = reinterpret_cast<double*>(a.aux);   // 8 rows of gridDim.x shares
+  if (threadIdx.x < 8) {
+    double t = 0.0;
+    for (int k = 0; k < DDC; ++k) t += dr[threadIdx.x * DDC + k];
+    part[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = t;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  }
+  __shared__ int last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned int prev = __hip_atomic_fetch_add(a.dcount, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    last = prev == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (last) {
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (threadIdx.x < 8) {
+      double t = 0.0;
+      for (unsigned int b = 0; b < gridDim.x; ++b)
+        t += __hip_atomic_load(part + (size_t)threadIdx.x * gridDim.x + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      a.series[(size_t)a.step * RSER + R_P + threadIdx.x] = t;
+    }
+    if (threadIdx.x == 0) __hip_atomic_store(a.dcount, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+#endif
+}
+
+// Momentum weights G_o[d] = (1/N) sum_k g_o(P_k) exp(2 pi i k d / N) for
+// g = P, P^2, K'(P), K(P) (K independent of t), by direct summation in FP64
+// with the exact angle index (k d) mod N.
+FWD void rho_diag_weights(const RArgs& a, double* dw) {
+  const double tk = a.t_initial;
+  for (int d = blockIdx.x * blockDim.x + threadIdx.x; d < FW_N; d += gridDim.x * blockDim.x) {
+    double acc[4][2] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+    for (int k = 0; k < FW_N; ++k) {
+      const int m = (int)(((long long)k * d) & (FW_N - 1));
+      double sn, cs;
+      sincospi(2.0 * (double)m / (double)FW_N, &sn, &cs);
+      const double P = pcoord(k);
+      const double g[4] = {P, P * P, ::fw_diff_K(P, tk), ::K(P, tk)};
+#pragma unroll
+      for (int o = 0; o < 4; ++o) {
+        acc[o][0] += g[o] * cs;
+        acc[o][1] += g[o] * sn;
+      }
+    }
+#pragma unroll
+    for (int o = 0; o < 4; ++o) {
+      dw[((size_t)o * FW_N + d) * 2] = acc[o][0] / FW_N;
+      dw[((size_t)o * FW_N + d) * 2 + 1] = acc[o][1] / FW_N;
+    }
   }
 }
 
@@ -526,6 +600,9 @@ extern "C" __global__ void __launch_bounds__(256) fw_rho_rows(fwk::RArgs a, fwk:
 }
 extern "C" __global__ void __launch_bounds__(fwk::DT) fw_rho_diag_partial(fwk::RArgs a) {
   fwk::rho_diag_partial(a);
+}
+extern "C" __global__ void __launch_bounds__(256) fw_rho_diag_weights(fwk::RArgs a, double* dw) {
+  fwk::rho_diag_weights(a, dw);
 }
 extern "C" __global__ void __launch_bounds__(FW_N / FW_DIAG_E) fw_rho_diag_finish(fwk::RArgs a) {
   fwk::rho_diag_finish<FW_DIAG_E>(a);

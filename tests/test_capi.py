@@ -7,6 +7,7 @@ import glob
 import os
 import re
 
+import numpy as np
 import pytest
 
 from fastwigner_amd import _lib
@@ -109,3 +110,24 @@ def test_product_fails_loudly_without_gpu():
     with pytest.raises(_lib.NoDeviceError):
         WignerMoyalHIP1D(X_gridDIM=16, X_amplitude=8., P_gridDIM=16, P_amplitude=8., dt=0.01, t=0.,
                          V="0.5*X*X", K="0.5*P*P")
+
+
+def test_rho_record_matches_per_step_formula():
+    """The vectorised Ehrenfest recording of the density-matrix class equals its
+    per-step formula (normalised traces of the raw sums) on random sums."""
+    from fastwigner_amd.rho_vneumann import RhoVNeumannHIP1D
+
+    class Holder:
+        dX = 0.037
+
+    h = Holder()
+    names = ("X_average", "X_average_RHS", "P_average", "P_average_RHS", "hamiltonian_average", "purity_series")
+    for k in names:
+        setattr(h, k, [])
+    ser = np.random.default_rng(0).standard_normal((7, 20))
+    RhoVNeumannHIP1D._record(h, ser)
+    for i, row in enumerate(ser):
+        o = RhoVNeumannHIP1D._observables(h, row)
+        for k in names:
+            ref = o["purity" if k == "purity_series" else k]
+            assert abs(getattr(h, k)[i] - ref) <= 1e-14 * max(1.0, abs(ref)), k

@@ -35,8 +35,14 @@ def make(n=256, amp=10., dt=0.01, V=VQ, K=KQ, dV="X + 0.2*X*X*X", dK="P", consts
     return g, o
 
 
-def test_vneumann_parity_and_ehrenfest(gpu):
-    g, o = make()
+@pytest.mark.parametrize("K,dK", [(KQ, "P"), ("0.5*P*P + 0.05*cos(P)", "P - 0.05*sin(P)"),
+                                  ("0.5*P*P + 0.05*P*sin(0.3*t)", "P + 0.05*sin(0.3*t)")])
+def test_vneumann_parity_and_ehrenfest(gpu, K, dK):
+    """State and every Ehrenfest series against the oracle. The momentum family
+    goes through precomputed weights when K and K' do not depend on t (the
+    first two cases; the second is not a polynomial) and through the diagonal
+    group sums and one inverse FFT otherwise (the third)."""
+    g, o = make(K=K, dK=dK)
     r0 = gaussian_rho(o.Xv)
     g.set_rho(r0)
     o.set_rho(r0)
