@@ -49,22 +49,17 @@ def make(n, precision):
     return s
 
 
-def kernel_times(s, steps=64):
-    """Mean device time (us) of the step kernels with HIP events on the
-    propagator's stream: {name: us}. Two-pass steps run fw_vv and fw_k_norm,
-    three-pass steps fw_v_first, fw_k and fw_v_last."""
+def kernel_times(s, reps=640):
+    """Mean device time (us) per launch of the two-pass step kernels, fw_vv and
+    fw_k_norm (fw_wigner_kernel_bench): the step time from HIP events around
+    replays of the propagator's 64-step graph on its stream, split between the
+    two kernels by their shares measured with an event around every kernel of
+    one more chunk; the state is restored afterwards. {name: us}."""
     import ctypes
     from fastwigner_amd import _lib
-    _lib.call("fw_wigner_set_timing", s._handle, 1)
-    s.propagate(steps)
-    out = (ctypes.c_double * 3)()
-    _lib.call("fw_wigner_kernel_times", s._handle, out)
-    _lib.call("fw_wigner_set_timing", s._handle, 0)
-    passes = ctypes.c_int()
-    _lib.call("fw_wigner_step_passes", s._handle, ctypes.byref(passes))
-    if passes.value == 2:
-        return {"fw_vv": out[0], "fw_k_norm": out[1]}
-    return {"fw_v_first": out[0], "fw_k": out[1], "fw_v_last": out[2]}
+    out = (ctypes.c_double * 2)()
+    _lib.call("fw_wigner_kernel_bench", s._handle, float(s.t), int(reps), out)
+    return {"fw_vv": out[0], "fw_k_norm": out[1]}
 
 
 def measured_traffic(kernel, n, precision):

@@ -213,6 +213,7 @@ PROTOTYPES = {
     "fw_wigner_snapshot": [_P, _I],
     "fw_wigner_set_timing": [_P, _I],
     "fw_wigner_kernel_times": [_P, _PD],
+    "fw_wigner_kernel_bench": [_P, ctypes.c_double, _I, _PD],
     "fw_wigner_step_passes": [_P, _PI],
     "fw_rho_grid_constants": [ctypes.POINTER(RhoDesc), _PD],
     "fw_rho_check": [ctypes.POINTER(RhoDesc), ctypes.POINTER(RhoExprs)],

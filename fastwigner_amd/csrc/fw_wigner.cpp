@@ -945,6 +945,59 @@ extern "C" int fw_wigner_step_passes(fw_wigner* h, int* passes) {
   return FW_OK;
 }
 
+extern "C" int fw_wigner_kernel_bench(fw_wigner* h, double t, int reps, double us_out[2]) {
+  if (!h || !us_out || reps < 1) return fw::fail(FW_ERR_INVALID_ARG, "NULL argument or reps < 1");
+  if (h->dissip || h->d.step_passes == 3)
+    return fw::fail(FW_ERR_INVALID_ARG, "the kernel bench times the two-pass step kernels");
+  KArgs base = base_args(h);
+  FWCHK(ensure_tables(h, 0, base));
+  FWCHK(ensure_graph_fused(h, base));
+  const size_t bytes = (size_t)h->NX * h->NP * h->esz;
+  void* save = nullptr;
+  HIPCHK(hipMalloc(&save, bytes));
+  HIPCHK(hipMemcpyAsync(save, h->W, bytes, hipMemcpyDeviceToDevice, h->stream));
+  std::vector<double> tt(kChunk + 1);
+  for (int s = 0; s <= kChunk; ++s) tt[s] = t + (s + 1) * h->d.dt;
+  HIPCHK(hipMemcpyAsync(h->times, tt.data(), sizeof(double) * (kChunk + 1), hipMemcpyHostToDevice, h->stream));
+  FWCHK(set_scalar(h, fw::SC_STOP, 0.0));
+  FWCHK(set_scalar(h, fw::SC_FRESH, 1.0));
+  KArgs a = base;
+  a.step = 0;
+  a.flags |= fw::F_FIRST;
+  const int chunks = std::max(1, reps / kChunk);
+  std::vector<hipEvent_t> ev(2 + 3 * kChunk);
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  int rc = launch(h, h->f_v_first, h->col_nb, h->col_nt, a);
+  // the step time: graph replays (the propagator's own launch path) between two events
+  if (rc == FW_OK) rc = hipEventRecord(ev[0], h->stream) == hipSuccess ? FW_OK : FW_ERR_HIP;
+  for (int c = 0; c < chunks && rc == FW_OK; ++c) rc = hipGraphLaunch(h->gexec2, h->stream) == hipSuccess ? FW_OK : FW_ERR_HIP;
+  if (rc == FW_OK) rc = hipEventRecord(ev[1], h->stream) == hipSuccess ? FW_OK : FW_ERR_HIP;
+  // each kernel's share: one chunk with an event around every kernel
+  for (int s = 0; s < kChunk && rc == FW_OK; ++s) rc = enqueue_fused(h, s, false, base, &ev[2 + 3 * s]);
+  double tk = 0.0, tv = 0.0;
+  float step_ms = 0.f;
+  if (rc == FW_OK && hipStreamSynchronize(h->stream) == hipSuccess) {
+    (void)hipEventElapsedTime(&step_ms, ev[0], ev[1]);
+    for (int s = 0; s < kChunk; ++s) {
+      float m0 = 0, m1 = 0;
+      (void)hipEventElapsedTime(&m0, ev[2 + 3 * s], ev[3 + 3 * s]);
+      (void)hipEventElapsedTime(&m1, ev[3 + 3 * s], ev[4 + 3 * s]);
+      tk += m0;
+      tv += m1;
+    }
+    const double step_us = step_ms * 1000.0 / (chunks * kChunk);
+    us_out[0] = step_us * tv / (tv + tk);
+    us_out[1] = step_us * tk / (tv + tk);
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  // the state as it was before the call
+  HIPCHK(hipMemcpyAsync(h->W, save, bytes, hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  (void)hipFree(save);
+  if (rc != FW_OK) return fw::fail(FW_ERR_HIP, "kernel bench launch failed");
+  return FW_OK;
+}
+
 extern "C" int fw_wigner_kernel_times(fw_wigner* h, double us_out[3]) {
   if (!h || !us_out) return fw::fail(FW_ERR_INVALID_ARG, "NULL argument");
   for (int k = 0; k < 3; ++k) us_out[k] = h->ktimes[k];

@@ -157,6 +157,12 @@ int fw_wigner_snapshot(fw_wigner* h, int which);
  * two-pass steps (fw_wigner_step_passes tells which ran). */
 int fw_wigner_set_timing(fw_wigner* h, int enable);
 int fw_wigner_kernel_times(fw_wigner* h, double us_out[3]);
+/* Mean device time (us) per launch of the two step kernels of two-pass steps, fw_vv and
+ * fw_k_norm, in the propagator's own sequence from time t: the step time from HIP events
+ * around max(1, reps/64) replays of the 64-step graph on the handle's stream, split between
+ * the two kernels by their shares in one further chunk with an event around every kernel.
+ * W is restored afterwards. No counterpart in the reference (measurement). */
+int fw_wigner_kernel_bench(fw_wigner* h, double t, int reps, double us_out[2]);
 /* Passes per step of the last Moyal propagate call (2 or 3). */
 int fw_wigner_step_passes(fw_wigner* h, int* passes);
 
