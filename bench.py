@@ -2,7 +2,7 @@
 
 Workload (one "step" = one Strang step V/2-K-V/2 over the whole grid, plus the
 per-step renormalisation and the logged observables <x>, <p>, <H>, purity):
-config-1 physics (x, p in [-8, 8), V = 0.5x^2 + 0.1x^4, K = p^2/2, dt = 0.01,
+quartic physics (x, p in [-8, 8), V = 0.5x^2 + 0.1x^4, K = p^2/2, dt = 0.01,
 W0 = exp(-(x-1)^2 - (p-0.5)^2)/pi) on a 1024 x 1024 grid, FP64. The state is
 resident in HBM before the timed region; nothing crosses PCIe inside it except
 the per-chunk time/observable series (64 steps per chunk).
@@ -16,9 +16,10 @@ Prints ONE JSON line. Extra fields:
                 sample's steps (FP64 bar 1e-12)
   variants      FP32 form at 1024^2 and FP64 at 2048^2 with their own rates,
                 roofline fractions and errors against the FP64 oracle; the
-                strong-field workload (config 3, time-dependent V); the
-                von Neumann density-matrix step (config 2); the surrogate
-                (config 5); the batched random-collision workload (config 4)
+                strong-field pulse at 1024^2 (time-dependent V); the
+                double-well von Neumann density-matrix step at 1024^2; the
+                surrogate at 256^2; the batched random-collision workload
+                (256 instances)
 """
 import argparse
 import json
@@ -87,10 +88,10 @@ def roofline(n, precision, ktimes):
             "mean_us": {k: round(v, 3) for k, v in ktimes.items()}}
 
 
-# config 3: strong-field Wigner propagation (strong_field_physics_cuda_1d.py:24-82 usage)
-OMEGA3, E03 = 0.06, 0.05
-T_PULSE3 = 4 * 2 * np.pi / OMEGA3
-CFG3 = dict(X_amplitude=60., P_amplitude=6., dt=0.05, t=0., E0=E03, omega=OMEGA3, T_pulse=T_PULSE3,
+# strong-field Wigner propagation (strong_field_physics_cuda_1d.py:24-82 usage)
+SF_OMEGA, SF_E0 = 0.06, 0.05
+SF_T_PULSE = 4 * 2 * np.pi / SF_OMEGA
+SF = dict(X_amplitude=60., P_amplitude=6., dt=0.05, t=0., E0=SF_E0, omega=SF_OMEGA, T_pulse=SF_T_PULSE,
             functions="__device__ double E(double t)\n{{\n  return (t < T_pulse) ? E0 * pow(sin(M_PI * t / T_pulse), 2)"
                       " * sin(omega * t) : 0.;\n}}\n",
             V="-1. / sqrt(X * X + 1.) + X * E(t)", K="0.5 * P * P", diff_V="X / pow(X * X + 1., 1.5) + E(t)",
@@ -98,7 +99,7 @@ CFG3 = dict(X_amplitude=60., P_amplitude=6., dt=0.05, t=0., E0=E03, omega=OMEGA3
 
 
 def strong_field_variant(n=1024, steps=2000, warmup=64):
-    """Config 3 at 1024^2, FP64: the field-free ground state (Bloch, get_ground_state),
+    """Strong-field pulse at 1024^2, FP64: the field-free ground state (Bloch, get_ground_state),
     then `steps` steps inside the pulse with the time-dependent V (factors evaluated
     at every half-step, no table), the x-p mask after every step and the dipole
     acceleration P_average_RHS recorded every step."""
@@ -106,11 +107,11 @@ def strong_field_variant(n=1024, steps=2000, warmup=64):
     from fastwigner_amd.wigner_moyal import WignerMoyalHIP1D
     t0 = time.perf_counter()
     b = WignerBlochHIP1D(X_gridDIM=n, P_gridDIM=n, dbeta=0.05, t_initial=0.,
-                         **{k: v for k, v in CFG3.items() if k != "abs_boundary_x_p"})
+                         **{k: v for k, v in SF.items() if k != "abs_boundary_x_p"})
     W0 = b.get_ground_state(abs_tol_purity=1e-8).get()
     gs = time.perf_counter() - t0
     del b
-    g = WignerMoyalHIP1D(X_gridDIM=n, P_gridDIM=n, **CFG3)
+    g = WignerMoyalHIP1D(X_gridDIM=n, P_gridDIM=n, **SF)
     g.set_wignerfunction(W0)
     rate, ms = timed_rate(g, n, steps, warmup)
     out = {"value": rate, "unit": "grid-point*steps/s", "ms_per_step": ms, "steps": steps,
@@ -123,7 +124,7 @@ def strong_field_variant(n=1024, steps=2000, warmup=64):
 
 
 def rho_variant(n=1024, steps=1000, warmup=64):
-    """Config 2, density-matrix part: the double-well thermal state at 1024^2
+    """Double-well density matrix: the thermal state at 1024^2
     (x in [-10, 10), kT = 0.2, RhoBloch from rho = I), then von Neumann steps
     (dt = 0.005) with the x-family observables every step; complex128, three
     passes per step of 32 B per matrix element each (96 B per element per step)."""
@@ -292,7 +293,7 @@ def main():
         "metric": "Wigner grid-point*steps/s at 1024^2 (plain FP64 Moyal step)",
         "value": rate, "unit": "grid-point*steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic (config-1 Gaussian initial state)",
+        "data": "synthetic (Gaussian initial state W0 = exp(-(x-1)^2-(p-0.5)^2)/pi)",
         "config": {"workload": "plain FP64 Moyal split-operator step, %dx%d, x,p in [-8,8), "
                                "V=0.5x^2+0.1x^4, K=p^2/2, dt=0.01, renormalisation and <x>,<p>,<H>,purity "
                                "every step" % (n, n), "grid": [n, n], "parallelism": "single GPU"},
@@ -338,14 +339,14 @@ def main():
         var["fp64_2048"] = {"value": r2, "ms_per_step": ms2, "roofline": roofline(2048, "fp64", kernel_times(s2))}
         print("[bench] fp64 2048^2: %.3e pt*steps/s" % r2, file=sys.stderr, flush=True)
         del s2
-        var["strong_field_fp64_config3"] = strong_field_variant()
-        var["rho_vneumann_c128_config2"] = rho_variant()
+        var["strong_field_fp64_1024"] = strong_field_variant()
+        var["rho_vneumann_c128_1024"] = rho_variant()
         print("[bench] surrogate: training on propagator trajectories", file=sys.stderr, flush=True)
         from fastwigner_amd.fno import surrogate_experiment
-        var["surrogate_fno_config5"] = surrogate_experiment(
+        var["surrogate_fno_256"] = surrogate_experiment(
             20000, held_out=512, epochs=2.0, base_steps=2, out_scale=1e-2, zero_mass=True,
             log=lambda m: print(m, file=sys.stderr, flush=True))
-        var["batched_fp32_config4"] = batched_variant(o_workers=min(os.cpu_count() or 1, 16),
+        var["batched_fp32_256_instances"] = batched_variant(o_workers=min(os.cpu_count() or 1, 16),
                                                       with_oracle=not args.no_baseline)
         result["variants"] = var
 

@@ -25,8 +25,8 @@ V1, K1 = "0.5*X*X+0.1*X*X*X*X", "0.5*P*P"
 DV1, DK1 = "X+0.4*X*X*X", "P"
 
 
-def moyal_config1(n=256, steps=1000):
-    """Config 1 at its full size: 256^2, x,p in [-8,8), dt=0.01, 1000 steps;
+def moyal_quartic(n=256, steps=1000):
+    """The quartic Moyal workload at its full size: 256^2, x,p in [-8,8), dt=0.01, 1000 steps;
     series of <x>, <p>, <H>, purity per step and the final W on a 16x16
     subsample (the full W would be 512 KiB)."""
     o = fo.MoyalOracle(n, 8., n, 8., 0.01, V=cexpr.make(V1), K=cexpr.make(K1, args=("P", "t")),
@@ -60,7 +60,7 @@ def moyal_small(n=64, steps=200):
 
 
 def bloch_double_well(n=64):
-    """CUDA-convention Bloch Gibbs state of the config-2 double well at 64^2."""
+    """CUDA-convention Bloch Gibbs state of the double well (V = -x^2/2 + x^4/20) at 64^2."""
     V = "-0.5*X*X + X*X*X*X/20."
     o = fo.BlochOracle(n, 10., n, 10., 0.005, V=cexpr.make(V), K=cexpr.make(K1, args=("P", "t")),
                        kT=0.2, dbeta=0.005, convention="cuda")
@@ -90,7 +90,7 @@ def rho_vneumann(n=64, steps=100):
 
 def main():
     out = {
-        "moyal_config1_256.npz": moyal_config1,
+        "moyal_quartic_256.npz": moyal_quartic,
         "moyal_masks_64x32.npz": moyal_small,
         "bloch_double_well_64.npz": bloch_double_well,
         "random_collision_64.npz": random_collision,

@@ -28,7 +28,7 @@ def rel(a, b):
     ("bloch_double_well_64.npz", mg.bloch_double_well),
     ("random_collision_64.npz", mg.random_collision),
     ("rho_vneumann_64.npz", mg.rho_vneumann),
-    ("moyal_config1_256.npz", mg.moyal_config1),
+    ("moyal_quartic_256.npz", mg.moyal_quartic),
 ])
 def test_oracle_reproduces_golden(name, fn):
     g = load(name)
@@ -40,10 +40,10 @@ def test_oracle_reproduces_golden(name, fn):
             assert np.array_equal(np.asarray(d[k]), g[k]), (name, k)
 
 
-def test_config1_golden_physics():
-    """Config-1 fixture: normalised, purity conserved by the unitary Moyal
+def test_quartic_256_golden_physics():
+    """Quartic Moyal fixture (256^2): normalised, purity conserved by the unitary Moyal
     step (up to the grid's band limit), energy drift O(dt^2)."""
-    g = load("moyal_config1_256.npz")
+    g = load("moyal_quartic_256.npz")
     pur, H = g["purity"], g["hamiltonian_average"]
     assert abs(g["full_W_sum"] * (16. / 256) ** 2 - 1.0) <= 1e-13
     assert np.max(np.abs(pur - pur[0])) <= 1e-8

@@ -108,7 +108,7 @@ def test_batch_fp32_form_and_modes_agree(gpu):
         assert np.max(np.abs(b2.series[:, i, 0] / b0.series[:, i, 0] - 1)) <= 1e-5, i
 
 
-def test_config4_full_batch(gpu):
+def test_batched_256_instances_full(gpu):
     """The batched random-collision workload at full size: 256 instances with
     grids from {128, 256, 512, 1024}^2 (rng seed 4), FP32, 500 steps, one launch
     per step. Size-independent checks on every instance (finite sums, purity
@@ -126,7 +126,7 @@ def test_config4_full_batch(gpu):
     gibbs_purity = np.array([2 * np.pi * np.sum(b.get_wignerfunction(i) ** 2) * b.grids[i]["dX"] * b.grids[i]["dP"]
                              for i in range(256)])
     physical = gibbs_purity <= 1.0 + 1e-3
-    print("config 4: %d of 256 instances have an unphysical Gibbs state: %s"
+    print("256 instances: %d of 256 instances have an unphysical Gibbs state: %s"
           % ((~physical).sum(), np.flatnonzero(~physical).tolist()))
     assert physical.sum() >= 240
     for i, d in enumerate(inst):
@@ -140,7 +140,7 @@ def test_config4_full_batch(gpu):
         if physical[i]:
             over = max(over, float(np.max(ob["purity"])) - 1.0)
     # purity <= 1 up to the FP32 form's rounding (its W error is ~5e-5 relative L2)
-    print("config 4: largest purity - 1 over the physical instances and all steps: %.2e" % over)
+    print("256 instances: largest purity - 1 over the physical instances and all steps: %.2e" % over)
     assert over <= 1e-3
     small = sorted(np.flatnonzero(physical), key=lambda i: inst[i]["X_gridDIM"])[:3]
     for i in small:
@@ -152,6 +152,6 @@ def test_config4_full_batch(gpu):
         o.set_wignerfunction(gaussian_state(d["X_gridDIM"], 8., d["x0"], d["p0"]))
         o.propagate(500)
         err = rel(W, o.W)
-        print("config 4 instance %d (N=%d, gamma=%.3f, kT=%.2f): fp32 rel L2 vs fp64 oracle %.2e"
+        print("256 instances, instance %d (N=%d, gamma=%.3f, kT=%.2f): fp32 rel L2 vs fp64 oracle %.2e"
               % (i, d["X_gridDIM"], d["gamma"], d["kT"], err))
         assert err <= 1e-4

@@ -84,8 +84,8 @@ def test_ground_state_with_rollback(gpu):
     assert abs(b.get_purity() - 1.0) <= 1e-9
 
 
-def test_config2_double_well_gibbs(gpu):
-    """Config 2, Bloch part: 512^2, x,p in [-10,10), V = -x^2/2 + x^4/20,
+def test_double_well_gibbs_512(gpu):
+    """Double-well Bloch state: 512^2, x,p in [-10,10), V = -x^2/2 + x^4/20,
     kT = 0.2, dbeta = 0.005 -> 1000 steps (CUDA convention, state at kT)."""
     from fastwigner_amd.wigner_bloch import WignerBlochHIP1D
     V, K = "-0.5*X*X + X*X*X*X/20.", "0.5*P*P"
@@ -96,7 +96,7 @@ def test_config2_double_well_gibbs(gpu):
                        K=cexpr.make(K, args=("P", "t")))
     Wo = o.get_gibbs_state()
     err = rel_l2(W, Wo)
-    print("config2 Bloch 512^2 rel L2 vs oracle:", err)
+    print("double well Bloch 512^2 rel L2 vs oracle:", err)
     assert err <= 1e-12
 
 

@@ -20,9 +20,9 @@ def rel(a, b):
     return np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b))
 
 
-def test_config1_series(gpu):
+def test_quartic_256_series(gpu):
     from fastwigner_amd.wigner_moyal import WignerMoyalHIP1D
-    g = load("moyal_config1_256.npz")
+    g = load("moyal_quartic_256.npz")
     w = WignerMoyalHIP1D(X_gridDIM=256, X_amplitude=8., P_gridDIM=256, P_amplitude=8., dt=0.01, t=0.,
                          V="0.5*X*X+0.1*X*X*X*X", K="0.5*P*P", diff_V="X+0.4*X*X*X", diff_K="P")
     w.set_wignerfunction(np.exp(-(w.X - 1.) ** 2 - (w.P - 0.5) ** 2) / np.pi)

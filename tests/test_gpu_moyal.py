@@ -1,7 +1,7 @@
 """GPU parity of the plain Moyal propagator against the CPU oracle.
 
-Tolerances: FP64 relative L2 <= 1e-12 against the oracle (the
-FFT-to-FFT noise floor measured between pocketfft and torch on config 1 is
+Tolerances: FP64 relative L2 <= 1e-12 against the oracle (the parity bar; the
+FFT-to-FFT noise floor measured between pocketfft and torch on the quartic 256^2 workload is
 1.6e-13). FP32 form: relative L2 <= 1e-4 against the FP64 oracle.
 """
 import numpy as np
@@ -51,8 +51,8 @@ def gaussian(o, x0=1.0, p0=0.5):
 
 
 @pytest.mark.parametrize("tables,passes", [(0, 0), (-1, 0), (0, 3), (-1, 3)])
-def test_config1_parity_fp64(gpu, tables, passes):
-    """Config 1: 256^2, x,p in [-8,8), V=0.5x^2+0.1x^4, dt=0.01, 1000 steps,
+def test_quartic_256_parity_fp64(gpu, tables, passes):
+    """Quartic Moyal workload: 256^2, x,p in [-8,8), V=0.5x^2+0.1x^4, dt=0.01, 1000 steps,
     <x>,<p>,<H>, purity logged each step; with cached factor tables (default for
     time-independent V, K) and with the factors evaluated at every half-step;
     in two fused passes per step (default) and in three."""
@@ -63,7 +63,7 @@ def test_config1_parity_fp64(gpu, tables, passes):
     Wg = g.propagate(1000).get()
     o.propagate(1000)
     err = rel_l2(Wg, o.W)
-    print("config1 fp64 rel L2 vs oracle:", err)
+    print("quartic 256^2 fp64 rel L2 vs oracle:", err)
     assert err <= 1e-12
     ox = np.array([s["X_average"] for s in o.series])
     op = np.array([s["P_average"] for s in o.series])
@@ -216,8 +216,8 @@ def test_theta_sums_time_dependent_v(gpu, nx, npp, tables):
 @pytest.mark.parametrize("tables", [0, -1])
 def test_fp32_form_error(gpu, tables):
     """FP32 W and FFTs (compensated FP32 twiddles), FP64 phase argument with its
-    mod-2pi reduction, FP64 sums, against the FP64 oracle on config 1. Bar: the
-    north star's 1e-4 at 500 steps (the batched FP32 form's step count); 1000
+    mod-2pi reduction, FP64 sums, against the FP64 oracle on the quartic 256^2 workload. Bar: 1e-4
+    at 500 steps (the batched FP32 form's step count); 1000
     steps stay below 1.5e-4. For scale: plain float32 pocketfft drifts to 1.4e-4
     on the same 1000 steps."""
     g, o = make_pair(256, 8., 256, 8., 0.01, V_Q, K_Q, DV_Q, DK_Q, precision="fp32", phase_tables=tables)
@@ -229,7 +229,7 @@ def test_fp32_form_error(gpu, tables):
         Wg = g.propagate(chunk).get()
         o.propagate(chunk)
         errs.append(rel_l2(Wg, o.W))
-    print("config1 fp32 (tables=%d) rel L2 vs fp64 oracle at 500/1000 steps:" % tables, errs)
+    print("quartic 256^2 fp32 (tables=%d) rel L2 vs fp64 oracle at 500/1000 steps:" % tables, errs)
     assert errs[0] <= 1e-4 and errs[1] <= 1.5e-4
     ox = np.array([s["X_average"] for s in o.series])
     assert np.max(np.abs(np.array(g.X_average) - ox)) <= 1e-4

@@ -150,8 +150,8 @@ def test_rho_ground_state(gpu):
     assert abs(g.get_average((None, "0.5*P*P")) + g.get_average(("0.5*X*X",)) - 0.5) <= 1e-3
 
 
-def test_config2_density_matrix_1024(gpu):
-    """Config 2, density-matrix part at its full size: the thermal state of the
+def test_double_well_density_matrix_1024(gpu):
+    """Double-well density matrix at its full size: the thermal state of the
     double well at 1024^2 (x in [-10, 10)) from rho = I, then 2000 von Neumann
     steps with dt = 0.005. Checked by size-independent properties: trace,
     hermiticity, purity constant, energy conserved (the thermal state is
@@ -172,7 +172,7 @@ def test_config2_density_matrix_1024(gpu):
     pur = np.array(vn.purity_series)
     assert np.max(np.abs(pur - pur[0])) <= 1e-10
     H = np.array(vn.hamiltonian_average)
-    print("config2 vN 1024^2: energy drift over 2000 steps", np.max(np.abs(H - H[0])) / abs(H[0]))
+    print("double well vN 1024^2: energy drift over 2000 steps", np.max(np.abs(H - H[0])) / abs(H[0]))
     assert np.max(np.abs(H - H[0])) <= 1e-6 * abs(H[0])    # a thermal state is stationary
     assert abs(np.sum(Wb) * b.wigner_dxdp - 1.0) <= 1e-12
 

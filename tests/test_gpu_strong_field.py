@@ -69,8 +69,8 @@ def test_strong_field_parity_256(gpu):
     assert np.max(np.abs(dip - dref)) <= 1e-11 * np.max(np.abs(dref))
 
 
-def test_config3_full_pulse_fp64_fp32(gpu, tmp_path):
-    """Config 3 at its full size: 1024^2, the whole 4-cycle pulse (8378 steps),
+def test_strong_field_full_pulse_1024_fp64_fp32(gpu, tmp_path):
+    """The strong-field pulse at its full size: 1024^2, the whole 4-cycle pulse (8378 steps),
     FP64 and FP32 from the same ground state. Checked by what holds at any size:
     normalisation, the ground state barely moves before the pulse rises, a
     finite dipole series every step; and the FP32 form against FP64. The FP64
@@ -85,9 +85,9 @@ def test_config3_full_pulse_fp64_fp32(gpu, tmp_path):
     steps = int(round(T_PULSE / 0.05))
     t0 = time.perf_counter()
     W0 = ground_state(n)
-    print("config 3: ground state at %d^2 in %.1f s" % (n, time.perf_counter() - t0), flush=True)
+    print("strong field: ground state at %d^2 in %.1f s" % (n, time.perf_counter() - t0), flush=True)
     out = {}
-    path = str(tmp_path / "config3.npz")
+    path = str(tmp_path / "strong_field.npz")
     for prec in ("fp64", "fp32"):
         t0 = time.perf_counter()
         g = WignerMoyalHIP1D(precision=prec, **params(n))
@@ -107,7 +107,7 @@ def test_config3_full_pulse_fp64_fp32(gpu, tmp_path):
         else:
             g.propagate(steps)
         W = g.wignerfunction.get()
-        print("config 3: %s pulse (%d steps) in %.1f s" % (prec, steps, time.perf_counter() - t0), flush=True)
+        print("strong field: %s pulse (%d steps) in %.1f s" % (prec, steps, time.perf_counter() - t0), flush=True)
         out[prec] = (W, np.array(g.P_average_RHS), np.array(g.X_average))
         assert abs(W.sum() * g.dX * g.dP - 1.0) <= (1e-12 if prec == "fp64" else 1e-6)
     W64, d64, x64 = out["fp64"]
@@ -117,12 +117,12 @@ def test_config3_full_pulse_fp64_fp32(gpu, tmp_path):
     assert np.max(np.abs(x64[:20])) <= 1e-6
     werr = np.linalg.norm(W32 - W64) / np.linalg.norm(W64)
     derr = np.max(np.abs(d32 - d64)) / np.max(np.abs(d64))
-    print("config 3: %d steps, FP32 vs FP64: W rel L2 %.2e, dipole max rel %.2e" % (steps, werr, derr))
+    print("strong field: %d steps, FP32 vs FP64: W rel L2 %.2e, dipole max rel %.2e" % (steps, werr, derr))
     assert werr <= 1e-3
     assert derr <= 1e-3
     # the saved run reads back as written
     r = load_results(path)
-    print("config 3: results file %.1f MiB, 4 frames of 8 MiB written in %.2f s"
+    print("strong field: results file %.1f MiB, 4 frames of 8 MiB written in %.2f s"
           % (__import__("os").path.getsize(path) / 2 ** 20, tw))
     assert len(r["frames"]) == 4 and np.array_equal(r["frames"][-1]["wigner"], W64)
     assert abs(r["frames"][-1]["t"] - steps * 0.05) < 1e-9 and r["settings"]["X_gridDIM"] == n
@@ -131,7 +131,7 @@ def test_config3_full_pulse_fp64_fp32(gpu, tmp_path):
     om, spec = hhg_spectrum(d64, 0.05, OMEGA)
     pos = om > 0.5
     k = om[pos][np.argmax(spec[pos])]
-    print("config 3: HHG spectrum peak at %.3f omega" % k)
+    print("strong field: HHG spectrum peak at %.3f omega" % k)
     assert abs(k - 1.0) < 0.1
 
 
