@@ -26,7 +26,7 @@ const int kBT = 256;
 // host mirrors of fwk::BInst, fwk::BTask, fwk::BArgs (fw_batch.h)
 struct BInstH {
   long long off;
-  int n, ntile, part, dissip, nbloch, pad_;
+  int n, ntile, part, dissip, nbloch, swoff;
   double dX, dP, dLambda, dTheta;
   double w_coeff, dbeta;
   double prm[FW_BATCH_NPRM];
@@ -41,6 +41,8 @@ struct BArgsH {
   void* W;
   const void* W0s;
   void* G;
+  void* W0t;
+  const void* sw;
   const void* inst;
   const void* tasks;
   const int* qoff;
@@ -129,13 +131,16 @@ struct fw_batch {
   std::string arch, prelude;
   hipModule_t mod = nullptr;
   hipFunction_t f_step = nullptr, f_bloch = nullptr, f_norm = nullptr, f_vkmin = nullptr, f_zero = nullptr,
-                f_fill = nullptr, f_gibbs = nullptr;
+                f_fill = nullptr, f_gibbs = nullptr, f_sumw = nullptr;
   hipStream_t stream = nullptr;
 
   void* W = nullptr;
   void* W0s = nullptr;
   void* G = nullptr;          // Vopen(W0s) of the two-pass steps
+  void* W0t = nullptr;        // W0s on the theta spectrum (two-pass theta steps)
+  void* sw = nullptr;         // theta-sum weights of every instance (two-pass theta steps)
   bool two_pass = false;      // mode 1 real-time runs in two passes per step (V independent of t)
+  bool theta = false;         // two-pass steps with the sums on the theta spectrum (K, K' independent of t)
   void* inst_d = nullptr;
   void* qtasks_d = nullptr;   // mode 0 queues
   void* ptasks_d = nullptr;   // mode 1, pass-major, XCD-interleaved
@@ -217,6 +222,8 @@ std::string make_prelude(const fw_batch_desc* d, const fw_batch_instance* in, co
   s << "__device__ double fw_diff_K(double P, double t, const double* prm) {\n"
     << bind << "  return (" << or_default(x->diff_K, "0.") << ");\n}\n";
   s << "#undef sin\n#undef cos\n";
+  s << "#define FW_B_THETA "
+    << ((d->mode == 1 && !uses_t_token(x->V) && !uses_t_token(x->K) && !uses_t_token(x->diff_K)) ? 1 : 0) << "\n";
   s << "#include \"fw_batch.h\"\n";
   return s.str();
 }
@@ -248,6 +255,8 @@ BArgsH base_args(fw_batch* h) {
   a.W = h->W;
   a.W0s = h->W0s;
   a.G = h->G;
+  a.W0t = h->W0t;
+  a.sw = h->sw;
   a.inst = h->inst_d;
   a.tasks = h->d.mode == 0 ? h->qtasks_d : h->ptasks_d;
   a.qoff = h->qoff_d;
@@ -381,6 +390,7 @@ extern "C" int fw_batch_create(const fw_batch_desc* d, const fw_batch_instance* 
   h->d = *d;
   h->src.assign(in, in + d->n_instances);
   h->two_pass = d->mode == 1 && !uses_t_token(x->V);
+  h->theta = h->two_pass && !uses_t_token(x->K) && !uses_t_token(x->diff_K);
   h->fp32 = d->precision == FW_FP32;
   h->esz = h->fp32 ? 4 : 8;
   int ndev = 0;
@@ -398,6 +408,7 @@ extern "C" int fw_batch_create(const fw_batch_desc* d, const fw_batch_instance* 
   HIPCHK(hipModuleGetFunction(&h->f_zero, h->mod, "fw_batch_zero"));
   HIPCHK(hipModuleGetFunction(&h->f_fill, h->mod, "fw_batch_fill_int"));
   HIPCHK(hipModuleGetFunction(&h->f_gibbs, h->mod, "fw_batch_gibbs"));
+  HIPCHK(hipModuleGetFunction(&h->f_sumw, h->mod, "fw_batch_sumw"));
   HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
 
   // instances: offsets, tiles, partial rows
@@ -405,7 +416,7 @@ extern "C" int fw_batch_create(const fw_batch_desc* d, const fw_batch_instance* 
   h->inst.resize(ni);
   const double two_pi = 2.0 * 3.141592653589793;
   long long off = 0;
-  int part = 0;
+  int part = 0, swn = 0;
   for (int i = 0; i < ni; ++i) {
     BInstH& b = h->inst[i];
     std::memset(&b, 0, sizeof b);
@@ -420,6 +431,8 @@ extern "C" int fw_batch_create(const fw_batch_desc* d, const fw_batch_instance* 
     b.dTheta = 1.0 / ((double)n * (b.dP / two_pi));
     b.w_coeff = 1.0;
     for (int k = 0; k < FW_BATCH_NPRM; ++k) b.prm[k] = in[i].params[k];
+    b.swoff = swn;
+    swn += n / 2 + 1;
     off += (long long)n * n;
     part += b.ntile;
   }
@@ -484,6 +497,13 @@ extern "C" int fw_batch_create(const fw_batch_desc* d, const fw_batch_instance* 
   HIPCHK(hipMalloc(&h->err, sizeof(int)));
   HIPCHK(hipMalloc(&h->vkmin, sizeof(double) * 2 * ni));
   FWCHK(upload_instances(h));
+  if (h->theta) {   // theta-sum weights of every instance (K, K' independent of t)
+    HIPCHK(hipMalloc(&h->sw, (size_t)std::max(swn, 1) * 48));
+    BArgsH a = base_args(h);
+    void* sw = h->sw;
+    void* params[] = {&a, &sw};
+    HIPCHK(hipModuleLaunchKernel(h->f_sumw, ni, 1, 1, 256, 1, 1, 0, h->stream, params, nullptr));
+  }
   FWCHK(zero_counters(h));
   FWCHK(fill_int(h, h->fresh, ni, 1));
   FWCHK(fill_int(h, h->err, 1, 0));
@@ -504,7 +524,7 @@ extern "C" int fw_batch_destroy(fw_batch* h) {
   if (!h) return FW_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
-  void* bufs[] = {h->W, h->W0s, h->G, h->inst_d, h->qtasks_d, h->ptasks_d, h->qoff_d, h->tw, h->times, h->partials,
+  void* bufs[] = {h->W, h->W0s, h->G, h->W0t, h->sw, h->inst_d, h->qtasks_d, h->ptasks_d, h->qoff_d, h->tw, h->times, h->partials,
                   h->series, h->counters, h->fresh, h->err, h->vkmin};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
@@ -559,6 +579,10 @@ static int ensure_w0s(fw_batch* h) {
     HIPCHK(hipMalloc(&h->G, (size_t)h->points * h->esz));
     HIPCHK(hipMemsetAsync(h->G, 0, (size_t)h->points * h->esz, h->stream));
   }
+  if (h->theta) {
+    HIPCHK(hipMalloc(&h->W0t, (size_t)h->points * h->esz));
+    HIPCHK(hipMemsetAsync(h->W0t, 0, (size_t)h->points * h->esz, h->stream));
+  }
   if (h->gexec) {
     (void)hipGraphExecDestroy(h->gexec);
     h->gexec = nullptr;
@@ -573,6 +597,7 @@ static int update_g(fw_batch* h) {
   const double t0 = 0.0;
   HIPCHK(hipMemcpyAsync(h->times, &t0, sizeof(double), hipMemcpyHostToDevice, h->stream));
   FWCHK(enqueue_pass(h, 0, 2, 0, 0));
+  if (h->theta) FWCHK(enqueue_pass(h, 0, 3, 0, 0));   // W0t
   HIPCHK(hipStreamSynchronize(h->stream));
   return FW_OK;
 }

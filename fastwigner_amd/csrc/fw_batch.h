@@ -51,7 +51,7 @@ struct BInst {
   int part;               // first partial row of the instance
   int dissip;             // 1: random-collision dissipator on
   int nbloch;             // Bloch runs: steps of this instance
-  int pad_;
+  int swoff;              // first entry of this instance's theta-sum weights (BSumW)
   double dX, dP, dLambda, dTheta;
   double w_coeff;         // exp(-gamma dt / 2)
   double dbeta;           // Bloch step of this instance
@@ -63,10 +63,20 @@ struct BTask {
   int code;               // pass << 16 | tile; inst < 0: no work (padding)
 };
 
+// Weights of the theta-spectrum sums of one instance, entry k <= N/2 (the
+// layout of the single-grid SumW): 0 < k < N/2: a = Im 2F_k(P), b = Re 2F_k(P^2),
+// kk = 2F_k(K), dk = 2F_k(K'); k = 0, N/2: a = F_k(P), b = F_k(P^2), kk, dk = (F_k, 0).
+struct __attribute__((aligned(16))) BSumW {
+  double a, b;
+  double kk[2], dk[2];
+};
+
 struct BArgs {
   real_t* W;
   const real_t* W0s;
   real_t* G;              // Vopen(W0s) of the two-pass steps (dissipative instances), or null
+  real_t* W0t;            // W0s on the theta spectrum (two-pass theta steps), or null
+  const BSumW* sw;        // theta-sum weights of every instance (two-pass theta steps), or null
   const BInst* inst;
   const BTask* tasks;     // mode 0: queues concatenated; mode 1: passes concatenated
   const int* qoff;        // [9] queue offsets (mode 0)
@@ -86,7 +96,7 @@ struct BArgs {
   int task_base;          // mode 1: first task of this launch
   int spin_limit;
   int vv;                 // mode 1 Moyal: 0 three-pass step; 1 two-pass step (pass 0 tasks run
-                          // the fused column pass); 2: pass 0 tasks compute G from W0s
+                          // the fused column pass); 2: pass 0 tasks compute G (and W0t) from W0s
   int pad_;
   double cv, ck;          // Bloch exponent factors (1/4, 1/2 or 1/2, 1)
 };
@@ -390,6 +400,209 @@ FWD void b_vvpass(const BArgs& a, int i, int tile, int tid, cpx* lds, const tw_t
   for (int e = 0; e < E; ++e) *reinterpret_cast<cpx*>(W + (size_t)(t + TPL * e) * N + c) = v[e];
 }
 
+// ---------------------------------------------------------------------------
+// Two-pass steps on the theta spectrum (FW_B_THETA: K and K' independent of
+// t). The fused column pass never returns to x-p space between its halves:
+// with alpha_k = A_k f_k / N after the closing factor (A_k the FFT along p),
+// the dissipator is alpha <- w alpha + alpha0s (alpha0s the same transform of
+// W0s, stored in W0t), the nine sums are taken on alpha (Parseval, weights
+// BSumW; the single-grid vvpass_theta explains them), and the opening factor
+// multiplies alpha in registers: one FFT pair per pass instead of two.
+// W0t of column pair p of an instance: entries k < N/2 at off + 2 N p + 4 k as
+// (Re A0s_k, Im A0s_k, Re B0s_k, Im B0s_k); entry 0 carries the real DC and
+// Nyquist values as (A0s_0, A0s_N/2, B0s_0, B0s_N/2). Same size as W0s.
+// ---------------------------------------------------------------------------
+#ifndef FW_B_THETA
+#define FW_B_THETA 0
+#endif
+template <int N, int E, class L>
+struct BVVMul {
+  const cpx (&fa)[E / 2 + 1];
+  const cpx (&fb)[E / 2 + 1];
+  const real_t* __restrict__ w0t;   // this pair's W0t entries, or null (no dissipator)
+  const BSumW* __restrict__ sw;     // this instance's weights
+  real_t wc;
+  double dPN;                       // dP N
+  int t;
+  double acc[4];                    // P, P^2, K', K
+  double pur;
+  double a0, b0;                    // alpha_0 of columns a, b after the dissipator (thread 0)
+  FWD void pair(int i, cpx& zk, cpx& zm) {
+    const real_t h = (real_t)0.5 / (real_t)N;
+    const int k = spec_k<L>(t, i);
+    const cpx A = mk(h * (zk.x + zm.x), h * (zk.y - zm.y));
+    const cpx B = mk(h * (zk.y + zm.y), h * (zm.x - zk.x));
+    cpx A1 = cmul(A, fa[i]), B1 = cmul(B, fb[i]);
+    if (w0t) {
+      const real_t* e = w0t + 4 * k;
+      A1 = mk(wc * A1.x + e[0], wc * A1.y + e[1]);
+      B1 = mk(wc * B1.x + e[2], wc * B1.y + e[3]);
+    }
+    const double sx = (double)A1.x + (double)B1.x, sy = (double)A1.y + (double)B1.y;
+    const BSumW w = sw[k];
+    acc[0] += sy * w.a - sx * dPN;
+    acc[1] += sx * w.b;
+    acc[2] += sx * w.dk[0] + sy * w.dk[1];
+    acc[3] += sx * w.kk[0] + sy * w.kk[1];
+    const double ax = A1.x, ay = A1.y, bx = B1.x, by = B1.y;
+    pur += 2.0 * (ax * ax + ay * ay + bx * bx + by * by);
+    const cpx A2 = cmul(A1, fa[i]), B2 = cmul(B1, fb[i]);
+    zk = mk(A2.x - B2.y, A2.y + B2.x);
+    zm = mk(A2.x + B2.y, B2.x - A2.y);
+  }
+  FWD void self(cpx& z0, cpx& zn) {
+    const real_t s = (real_t)1 / (real_t)N;
+    real_t ra = z0.x * s * fa[0].x, rb = z0.y * s * fb[0].x;
+    real_t na = zn.x * s * fa[E / 2].x, nb = zn.y * s * fb[E / 2].x;
+    if (w0t) {
+      ra = wc * ra + w0t[0];
+      na = wc * na + w0t[1];
+      rb = wc * rb + w0t[2];
+      nb = wc * nb + w0t[3];
+    }
+    a0 = ra;
+    b0 = rb;
+    const double s0 = (double)ra + (double)rb, sn = (double)na + (double)nb;
+    const BSumW w0 = sw[0], wn = sw[N / 2];
+    acc[0] += s0 * w0.a + sn * wn.a;
+    acc[1] += s0 * w0.b + sn * wn.b;
+    acc[2] += s0 * w0.dk[0] + sn * wn.dk[0];
+    acc[3] += s0 * w0.kk[0] + sn * wn.kk[0];
+    pur += (double)ra * ra + (double)rb * rb + (double)na * na + (double)nb * nb;
+    z0 = mk(ra * fa[0].x, rb * fb[0].x);
+    zn = mk(na * fa[E / 2].x, nb * fb[E / 2].x);
+  }
+};
+
+template <int N, int E>
+FWD void b_vvpass_theta(const BArgs& a, int i, int tile, int tid, cpx* lds, const tw_t* twl, double* red) {
+  constexpr int TPL = N / E, P = BT / TPL;
+  using L = Line<N, E, P, false, FW_B_NMAX>;
+  const BInst* in = a.inst + i;
+  const int q = tid % P, t = tid / P;
+  const int c = (tile * P + q) * 2;
+  real_t* __restrict__ W = a.W + in->off;
+  const double* prm = in->prm;
+  const double tnow = a.times[a.step];
+  const double tau = tnow + FW_PHASE_SHIFT * ::dt;
+  const double dX = in->dX, dP = in->dP;
+  cpx v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = *reinterpret_cast<const cpx*>(W + (size_t)(t + TPL * e) * N + c);
+  cpx fa[E / 2 + 1], fb[E / 2 + 1];
+  b_vfactors<N, E, L>(in, c, t, tau, fa, fb);
+  __syncthreads();
+  const real_t* w0t = in->dissip ? a.W0t + in->off + (size_t)N * c : nullptr;   // pair c/2: 2 N entries
+  BVVMul<N, E, L> mul{fa, fb, w0t, a.sw + in->swoff, (real_t)in->w_coeff, dP * N, t, {0.0, 0.0, 0.0, 0.0},
+                      0.0, 0.0, 0.0};
+  spectral_step<L>(v, lds, q, t, twl, mul);
+#pragma unroll
+  for (int e = 0; e < E; ++e) *reinterpret_cast<cpx*>(W + (size_t)(t + TPL * e) * N + c) = v[e];
+  // the nine sums of step s (column sums N alpha_0 on thread 0 of each line)
+  const double Xa = dX * (c - 0.5 * N), Xb = dX * (c + 1 - 0.5 * N);
+  const double ca = (double)N * mul.a0, cb = (double)N * mul.b0;
+  double sums[NSER];
+  sums[0] = ca + cb;
+  sums[1] = Xa * ca + Xb * cb;
+  sums[2] = mul.acc[0];
+  sums[3] = Xa * Xa * ca + Xb * Xb * cb;
+  sums[4] = mul.acc[1];
+  sums[5] = mul.acc[2];
+  sums[6] = 0.0;
+  sums[7] = mul.acc[3];
+  if (t == 0) {
+    sums[6] = ::fw_diff_V(Xa, tnow, prm) * ca + ::fw_diff_V(Xb, tnow, prm) * cb;
+    sums[7] += ::V(Xa, tnow, prm) * ca + ::V(Xb, tnow, prm) * cb;
+  }
+  sums[8] = (double)N * mul.pur;
+  write_partial_rows<BT>(sums, red, a.partials + (size_t)in->part * NSER, tile, in->ntile);
+  // every K tile of this step has read `fresh`
+  if (tile == 0 && tid == 0 && a.fresh[i] != 0) a.fresh[i] = 0;
+}
+
+// W0t of a dissipative instance: the transform of W0s on the theta spectrum in
+// the convention of alpha (A_k / N), recorded by a spectral step whose multiply
+// stores the separated spectra and leaves them unchanged.
+template <int N, int E, class L>
+struct BStoreMul {
+  real_t* __restrict__ out;   // this pair's W0t entries
+  int t;
+  FWD void pair(int i, cpx& zk, cpx& zm) const {
+    const real_t h = (real_t)0.5 / (real_t)N;
+    const int k = spec_k<L>(t, i);
+    real_t* e = out + 4 * k;
+    e[0] = h * (zk.x + zm.x);
+    e[1] = h * (zk.y - zm.y);
+    e[2] = h * (zk.y + zm.y);
+    e[3] = h * (zm.x - zk.x);
+  }
+  FWD void self(cpx& z0, cpx& zn) const {
+    const real_t s = (real_t)1 / (real_t)N;
+    out[0] = z0.x * s;
+    out[1] = zn.x * s;
+    out[2] = z0.y * s;
+    out[3] = zn.y * s;
+  }
+};
+
+template <int N, int E>
+FWD void b_w0tpass(const BArgs& a, int i, int tile, int tid, cpx* lds, const tw_t* twl) {
+  constexpr int TPL = N / E, P = BT / TPL;
+  using L = Line<N, E, P, false, FW_B_NMAX>;
+  const BInst* in = a.inst + i;
+  if (!in->dissip) return;   // uniform: one instance per block
+  const int q = tid % P, t = tid / P;
+  const int c = (tile * P + q) * 2;
+  const real_t* __restrict__ S = a.W0s + in->off;
+  cpx v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = *reinterpret_cast<const cpx*>(S + (size_t)(t + TPL * e) * N + c);
+  __syncthreads();
+  BStoreMul<N, E, L> mul{a.W0t + in->off + (size_t)N * c, t};
+  spectral_step<L>(v, lds, q, t, twl, mul);
+}
+
+// Theta-sum weights of every instance: block i computes instance i's entries
+// k <= N/2 by direct summation in FP64 (exact angle index (n k) mod N).
+FWD void b_sumw(const BArgs& a, BSumW* sw) {
+  const int i = blockIdx.x;
+  const BInst* in = a.inst + i;
+  const int N = in->n;
+  const double tk = 0.0;   // K and K' do not depend on t here
+  for (int k = threadIdx.x; k <= N / 2; k += blockDim.x) {
+    double acc[4][2] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+    for (int n = 0; n < N; ++n) {
+      const int m = (int)(((long long)n * k) & (N - 1));
+      double sn, cs;
+      sincospi(-2.0 * (double)m / (double)N, &sn, &cs);
+      const double P = in->dP * (n - 0.5 * N);
+      const double f[4] = {P, P * P, ::fw_diff_K(P, tk, in->prm), ::K(P, tk, in->prm)};
+#pragma unroll
+      for (int o = 0; o < 4; ++o) {
+        acc[o][0] += f[o] * cs;
+        acc[o][1] += f[o] * sn;
+      }
+    }
+    BSumW w;
+    if (k == 0 || k == N / 2) {
+      w.a = acc[0][0];
+      w.b = acc[1][0];
+      w.dk[0] = acc[2][0];
+      w.dk[1] = 0.0;
+      w.kk[0] = acc[3][0];
+      w.kk[1] = 0.0;
+    } else {
+      w.a = 2.0 * acc[0][1];
+      w.b = 2.0 * acc[1][0];
+      w.dk[0] = 2.0 * acc[2][0];
+      w.dk[1] = 2.0 * acc[2][1];
+      w.kk[0] = 2.0 * acc[3][0];
+      w.kk[1] = 2.0 * acc[3][1];
+    }
+    sw[in->swoff + k] = w;
+  }
+}
+
 // G = Vopen(W0s) for a dissipative instance (the opening V/2 alone).
 template <int N, int E>
 FWD void b_gpass(const BArgs& a, int i, int tile, int tid, cpx* lds, const tw_t* twl) {
@@ -419,11 +632,16 @@ FWD void b_run_n(const BArgs& a, int i, int pass, int tile, int tid, cpx* lds, c
   if (pass == 0) {
     if constexpr (KIND == BK_MOYAL) {
       if (a.vv == 1) {
-        b_vvpass<N, E>(a, i, tile, tid, lds, twl, red);
+        if constexpr (FW_B_THETA) b_vvpass_theta<N, E>(a, i, tile, tid, lds, twl, red);
+        else b_vvpass<N, E>(a, i, tile, tid, lds, twl, red);
         return;
       }
       if (a.vv == 2) {
         b_gpass<N, E>(a, i, tile, tid, lds, twl);
+        return;
+      }
+      if (a.vv == 3) {
+        if constexpr (FW_B_THETA) b_w0tpass<N, E>(a, i, tile, tid, lds, twl);
         return;
       }
     }
@@ -609,6 +827,7 @@ extern "C" __global__ void __launch_bounds__(fwk::BT) fw_batch_bloch_step(fwk::B
   fwk::b_step<fwk::BK_BLOCH>(a);
 }
 extern "C" __global__ void __launch_bounds__(fwk::BT) fw_batch_norm(fwk::BArgs a, int slot) { fwk::b_norm(a, slot); }
+extern "C" __global__ void __launch_bounds__(256) fw_batch_sumw(fwk::BArgs a, fwk::BSumW* sw) { fwk::b_sumw(a, sw); }
 extern "C" __global__ void __launch_bounds__(fwk::BT) fw_batch_vkmin(fwk::BArgs a, double* out) {
   fwk::b_vkmin(a, out);
 }

@@ -104,8 +104,10 @@ def test_batch_fp32_form_and_modes_agree(gpu):
         print("fp32 instance %d N=%d: rel L2 vs fp64 oracle %.2e (two-pass %.2e; between them %.2e)"
               % (i, d["X_gridDIM"], err, err2, d02))
         assert err <= 1e-4 and err2 <= 1e-4, i
-        assert d02 <= 2e-5, i
-        assert np.max(np.abs(b2.series[:, i, 0] / b0.series[:, i, 0] - 1)) <= 1e-5, i
+        # the two-pass steps take fewer FP32 transforms (sums on the theta
+        # spectrum), so the two forms differ by about their own FP32 error
+        assert d02 <= 1e-4, i
+        assert np.max(np.abs(b2.series[:, i, 0] / b0.series[:, i, 0] - 1)) <= 1e-4, i
 
 
 def test_batched_256_instances_full(gpu):
@@ -155,3 +157,47 @@ def test_batched_256_instances_full(gpu):
         print("256 instances, instance %d (N=%d, gamma=%.3f, kT=%.2f): fp32 rel L2 vs fp64 oracle %.2e"
               % (i, d["X_gridDIM"], d["gamma"], d["kT"], err))
         assert err <= 1e-4
+
+
+def test_batch_two_pass_parametrised_nonpolynomial_k(gpu):
+    """Two-pass steps with the sums on the theta spectrum, whose weights are
+    built per instance: a K that depends on an instance parameter and is not a
+    polynomial, with and without the dissipator, FP64 against the oracle (W and
+    the logged sums, 100 steps)."""
+    from fastwigner_amd.batch import WignerBatchHIP1D, gaussian_state
+    inst = small_batch()[:4]
+    Ks, dKs = "0.5*P*P + 0.2*a4*cos(P)", "P - 0.2*a4*sin(P)"
+    b = WignerBatchHIP1D(inst, V=V, K=Ks, dt=0.01, param_names=("a2", "a3", "a4"), diff_V=DV, diff_K=dKs,
+                         precision="fp64", mode=1)
+    assert b.passes_per_step == 2
+    b.prepare_random_collision(dbeta=0.01)
+    for i, d in enumerate(inst):
+        b.set_wignerfunction(i, gaussian_state(d["X_gridDIM"], d["X_amplitude"], d["x0"], d["p0"]))
+    steps = 100
+    b.propagate(steps)
+    for i, d in enumerate(inst):
+        a2, a3, a4 = d["params"]
+        kw = dict(V=lambda X, t: a2 * X * X + a3 * X ** 3 + a4 * X ** 4,
+                  K=lambda P, t: 0.5 * P * P + 0.2 * a4 * np.cos(P),
+                  diff_V=lambda X, t: 2 * a2 * X + 3 * a3 * X * X + 4 * a4 * X ** 3,
+                  diff_K=lambda P, t: P - 0.2 * a4 * np.sin(P), phase_time_shift=0.0)
+        n, A = d["X_gridDIM"], d["X_amplitude"]
+        if d.get("gamma", 0) > 0:
+            o = fo.RandomCollisionOracle(n, A, n, A, 0.01, gamma=d["gamma"], kT=d["kT"], dbeta=0.01,
+                                         bloch_convention="fftw", **kw)
+        else:
+            o = fo.MoyalOracle(n, A, n, A, 0.01, **kw)
+        o.set_wignerfunction(gaussian_state(n, A, d["x0"], d["p0"]))
+        o.propagate(steps)
+        err = rel(b.get_wignerfunction(i), o.W)
+        obs = b.observables(i)
+        ho = np.array([s["hamiltonian_average"] for s in o.series])
+        po = np.array([s["P_average"] for s in o.series])
+        dko = np.array([s["X_average_RHS"] for s in o.series])
+        print("instance %d N=%d gamma=%.3f: rel L2 %.2e" % (i, n, d.get("gamma", 0), err))
+        assert err <= 1e-12, i
+        assert np.max(np.abs(obs["hamiltonian_average"] - ho)) <= 1e-11 * np.max(np.abs(ho)), i
+        assert np.max(np.abs(obs["P_average"] - po)) <= 1e-11, i
+        if d.get("gamma", 0) == 0:   # the collision model's RHS adds its relaxation term
+            assert np.max(np.abs(obs["X_average_RHS"] - dko)) <= 1e-11, i
+
