@@ -344,7 +344,7 @@ def main():
         print("[bench] surrogate: training on propagator trajectories", file=sys.stderr, flush=True)
         from fastwigner_amd.fno import surrogate_experiment
         var["surrogate_fno_256"] = surrogate_experiment(
-            20000, held_out=512, epochs=2.0, base_steps=2, out_scale=1e-2, zero_mass=True,
+            20000, held_out=512, epochs=3.0, base_steps=2, out_scale=1e-2, zero_mass=True,
             log=lambda m: print(m, file=sys.stderr, flush=True))
         var["batched_fp32_256_instances"] = batched_variant(o_workers=min(os.cpu_count() or 1, 16),
                                                       with_oracle=not args.no_baseline)
