@@ -278,7 +278,7 @@ def main():
         return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: a single Wigner grid is kept on one GPU (sharding it would be "
-              "transpose-bound over xGMI) and no multi-GPU run is part of this build; use --gpus 1.",
+              "transpose-bound over xGMI) and the benchmark measures one GPU; use --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
     n = args.n
