@@ -32,7 +32,6 @@ struct RArgs {
   const void* uv;
   const void* uk;
   const double* dw;
-  unsigned int* dcount;
   int step;
   int flags;
   int nbp;
@@ -86,9 +85,8 @@ struct fw_rho {
   hipFunction_t f_a = nullptr, f_b = nullptr, f_c = nullptr, f_ba = nullptr, f_bb = nullptr, f_bc = nullptr,
                 f_rows = nullptr, f_diag_p = nullptr, f_diag_f = nullptr, f_norm = nullptr, f_moments = nullptr,
                 f_shx = nullptr, f_shy = nullptr, f_flip = nullptr, f_wscale = nullptr, f_set = nullptr,
-                f_vmin = nullptr, f_dweights = nullptr;
+                f_vmin = nullptr, f_dweights = nullptr, f_diag_pub = nullptr;
   double* dw = nullptr;             // momentum weights (FW_DIAG_W), or null
-  unsigned int* dcount = nullptr;   // blocks of the diagonal pass done
   hipStream_t stream = nullptr;
 
   void* rho = nullptr;
@@ -241,7 +239,6 @@ RArgs base(fw_rho* h) {
   a.partials = h->partials;
   a.series = h->series;
   a.dw = h->dw;
-  a.dcount = h->dcount;
   a.uv = h->uv;
   a.uk = h->uk;
   a.nbp = h->row_nb;
@@ -261,6 +258,19 @@ int set_scalar(fw_rho* h, int slot, double v) {
 int args1(fw_rho* h, hipFunction_t f, int grid, int block, RArgs& a) {
   void* params[] = {&a};
   return launch(h, f, grid, block, params);
+}
+
+// Momentum family of the current rho into series[a.step]: the diagonal pass,
+// then the publish kernel (weights form) or the one-block inverse FFT.
+int momentum_sums(fw_rho* h, RArgs& a) {
+  const int nb = diag_blocks(h->N);
+  FWCHK(args1(h, h->f_diag_p, nb, 256, a));
+  if (h->dw) {
+    int n = nb;
+    void* params[] = {&a, &n};
+    return launch(h, h->f_diag_pub, 1, 256, params);
+  }
+  return args1(h, h->f_diag_f, 1, h->N / diag_e(h->N), a);
 }
 
 // rows + steps of one chunk; bloch selects the kernel set
@@ -283,8 +293,7 @@ int enqueue_chunk(fw_rho* h, bool bloch, int n, bool momentum, const RArgs& b0, 
     FWCHK(args1(h, bloch ? h->f_bc : h->f_c, h->row_nb, h->row_nt, a));
     if (ev) HIPCHK(hipEventRecord((*ev)[4 * s + 3], h->stream));
     if (momentum) {
-      FWCHK(args1(h, h->f_diag_p, diag_blocks(h->N), 256, a));
-      if (!h->dw) FWCHK(args1(h, h->f_diag_f, 1, h->N / diag_e(h->N), a));
+      FWCHK(momentum_sums(h, a));
     }
   }
   return FW_OK;
@@ -448,7 +457,8 @@ extern "C" int fw_rho_create(const fw_rho_desc* desc, const fw_rho_exprs* exprs,
              {"fw_rho_moments", &h->f_moments}, {"fw_rho_shear_x", &h->f_shx},
              {"fw_rho_shear_y", &h->f_shy}, {"fw_rho_flip_ifft", &h->f_flip},
              {"fw_rho_wigner_scale", &h->f_wscale}, {"fw_rho_set_scalar", &h->f_set},
-             {"fw_rho_vmin", &h->f_vmin}, {"fw_rho_diag_weights", &h->f_dweights}};
+             {"fw_rho_vmin", &h->f_vmin}, {"fw_rho_diag_weights", &h->f_dweights},
+             {"fw_rho_diag_publish", &h->f_diag_pub}};
   for (auto& f : fns)
     if (hipModuleGetFunction(f.f, h->mod, f.n) != hipSuccess)
       return bail(fw::fail(FW_ERR_HIP, std::string("missing kernel ") + f.n));
@@ -473,7 +483,6 @@ extern "C" int fw_rho_create(const fw_rho_desc* desc, const fw_rho_exprs* exprs,
   if (hipStreamCreate(&h->stream) != hipSuccess) return bail(fw::fail(FW_ERR_HIP, "hipStreamCreate failed"));
   if (diag_weights(exprs)) {
     if ((rc = alloc((void**)&h->dw, sizeof(double) * 8 * h->N)) != FW_OK) return bail(rc);
-    if ((rc = alloc((void**)&h->dcount, sizeof(unsigned int))) != FW_OK) return bail(rc);
     RArgs a = base(h);
     double* dw = h->dw;
     void* params[] = {&a, &dw};
@@ -493,7 +502,7 @@ extern "C" int fw_rho_destroy(fw_rho* h) {
       if (kv.second.mod) (void)hipModuleUnload(kv.second.mod);
   if (h->mod) (void)hipModuleUnload(h->mod);
   for (void* p : {h->rho, h->aux, h->wig, h->winit, h->snap, h->tw, h->uv, h->uk, (void*)h->times, (void*)h->scal,
-                  (void*)h->partials, (void*)h->series, (void*)h->dw, (void*)h->dcount})
+                  (void*)h->partials, (void*)h->series, (void*)h->dw})
     if (p) (void)hipFree(p);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -603,8 +612,7 @@ extern "C" int fw_rho_moments(fw_rho* h, double t, double sums[FW_RSER]) {
   // momentum family of the current rho through the diagonal-sum kernels, into series slot 0
   RArgs a = base(h);
   a.step = 0;
-  FWCHK(args1(h, h->f_diag_p, diag_blocks(h->N), 256, a));
-  if (!h->dw) FWCHK(args1(h, h->f_diag_f, 1, h->N / diag_e(h->N), a));
+  FWCHK(momentum_sums(h, a));
   HIPCHK(hipMemcpyAsync(sums, h->scal + RC_OUT, sizeof(double) * 11, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipMemcpyAsync(sums + 11, h->series + 11, sizeof(double) * 9, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));

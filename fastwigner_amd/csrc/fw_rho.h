@@ -47,7 +47,6 @@ struct RArgs {
   const cpx* uv;       // per-step rows u_j (unitary) or real Bloch rows
   const cpx* uk;       // per-step rows w_k
   const double* dw;    // momentum weights G_o[d] (re, im), o = P, P^2, K', K, or null
-  unsigned int* dcount;  // blocks of the diagonal pass done (the last one publishes; reset by it)
   int step;
   int flags;
   int nbp;             // number of partial rows written by pass C
@@ -337,8 +336,8 @@ FWD void rho_diag_partial(const RArgs& a) {
   }
 #if FW_DIAG_W
   // Tr(rho g(P)) = sum_d c_d G_d with G_d = (1/N) sum_k g(P_k) exp(2 pi i k d / N)
-  // (K and K' independent of t): this block's share over its diagonals, then
-  // the last block to finish adds the shares in block order and publishes them.
+  // (K and K' independent of t): this block's share over its diagonals; the
+  // next kernel (rho_diag_publish) adds the shares and publishes them.
   __syncthreads();
   double* dr = red;   // 8 rows of DDC lanes (8 DDC <= 2 DT)
   if (w == 0) {
@@ -355,26 +354,25 @@ FWD void rho_diag_partial(const RArgs& a) {
     double t = 0.0;
     for (int k = 0; k < DDC; ++k) t += dr[threadIdx.x * DDC + k];
     part[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = t;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-  }
-  __shared__ int last;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned int prev = __hip_atomic_fetch_add(a.dcount, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    last = prev == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (last) {
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    if (threadIdx.x < 8) {
-      double t = 0.0;
-      for (unsigned int b = 0; b < gridDim.x; ++b)
-        t += __hip_atomic_load(part + (size_t)threadIdx.x * gridDim.x + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      a.series[(size_t)a.step * RSER + R_P + threadIdx.x] = t;
-    }
-    if (threadIdx.x == 0) __hip_atomic_store(a.dcount, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 #endif
+}
+
+// The momentum sums of the weights form: one block adds the nb shares of each
+// of the 8 rows (every thread a strided subset, then the block; fixed order)
+// and publishes them to series[step].
+FWD void rho_diag_publish(const RArgs& a, int nb) {
+  __shared__ double red[8 * BlockRed<DT>::RN];
+  const double* part = reinterpret_cast<const double*>(a.aux);
+  double v[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < nb; b += DT) {
+#pragma unroll
+    for (int o = 0; o < 8; ++o) v[o] += part[(size_t)o * nb + b];
+  }
+#pragma unroll
+  for (int o = 0; o < 8; ++o) BlockRed<DT>::put(v[o], red, 8, o);
+  __syncthreads();
+  if (threadIdx.x < 8) a.series[(size_t)a.step * RSER + R_P + threadIdx.x] = BlockRed<DT>::total(red, 8, threadIdx.x);
 }
 
 // Momentum weights G_o[d] = (1/N) sum_k g_o(P_k) exp(2 pi i k d / N) for
@@ -600,6 +598,9 @@ extern "C" __global__ void __launch_bounds__(256) fw_rho_rows(fwk::RArgs a, fwk:
 }
 extern "C" __global__ void __launch_bounds__(fwk::DT) fw_rho_diag_partial(fwk::RArgs a) {
   fwk::rho_diag_partial(a);
+}
+extern "C" __global__ void __launch_bounds__(fwk::DT) fw_rho_diag_publish(fwk::RArgs a, int nb) {
+  fwk::rho_diag_publish(a, nb);
 }
 extern "C" __global__ void __launch_bounds__(256) fw_rho_diag_weights(fwk::RArgs a, double* dw) {
   fwk::rho_diag_weights(a, dw);
