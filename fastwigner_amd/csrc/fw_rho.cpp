@@ -144,7 +144,8 @@ int tiles(const fw_rho_desc* d, int& re, int& rp, int& ce, int& cp) {
   const int tpl = n / re;
   rp = d->row_p > 0 ? d->row_p : (tpl >= 256 ? 1 : 256 / tpl);
   while (rp > 1 && n % rp) rp /= 2;
-  const size_t tws = ((size_t)n * 17 / 16 + 1) * 16;
+  // FP32 stages its twiddle table in LDS; FP64 computes its twiddles (no table)
+  const size_t tws = d->precision == FW_FP32 ? ((size_t)n * 17 / 16 + 1) * 16 : 16;
   auto lds = [&](int lines) {
     const size_t l = (size_t)lines * n;
     return (l + l / 16 + 1) * esz + tws + 20 * 32 * 8;
