@@ -247,7 +247,7 @@ FWD bool previous_norm(const Args& a, int blk, double* red, double& nrm) {
 // V pass: columns (along p), lines = pairs of adjacent columns.
 // ---------------------------------------------------------------------------
 // Twiddles of the FP64 steps are computed, not staged (TwTab mode 2: one FP64
-// sincospi per butterfly, the powers by products): no table in LDS (32 KiB per
+// sine/cosine per butterfly, the powers by products): no table in LDS (32 KiB per
 // block at N = 2048), no table loads, twice the blocks per CU; measured 20-27%
 // faster per pass than the staged full table (mode 0) or the two-level table
 // (mode 1) on gfx950. The FP32 form keeps the full compensated table.

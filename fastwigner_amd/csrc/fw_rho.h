@@ -96,7 +96,7 @@ FWD cpx rowfac(const cpx* __restrict__ u, int s, int j) { return u[(size_t)s * F
 // Pass A / C: rows (along j). PASS_A: [norm], expV, forward FFT.
 // PASS_C: inverse FFT, expV, sums.
 // ---------------------------------------------------------------------------
-// FP64 steps compute their twiddles in the kernel (one sincospi per
+// FP64 steps compute their twiddles in the kernel (one sine/cosine per
 // butterfly, powers by products: no table staged in LDS, as the Wigner passes
 // do); the FP32 form keeps the staged compensated table.
 #ifndef FW_RHO_TW
