@@ -196,6 +196,8 @@ PROTOTYPES = {
     "fw_device_count": [_PI],
     "fw_wigner_grid_constants": [ctypes.POINTER(WignerDesc), _PD],
     "fw_wigner_check": [ctypes.POINTER(WignerDesc), ctypes.POINTER(WignerExprs)],
+    "fw_wigner_prelude": [ctypes.POINTER(WignerDesc), ctypes.POINTER(WignerExprs), ctypes.c_char_p, ctypes.c_size_t,
+                          ctypes.POINTER(ctypes.c_size_t)],
     "fw_wigner_create": [ctypes.POINTER(WignerDesc), ctypes.POINTER(WignerExprs), _PP],
     "fw_wigner_destroy": [_P],
     "fw_wigner_set_state": [_P, _PD],

@@ -113,6 +113,25 @@ FWD cpx vfactor(double X, double Theta, double tau, double tnow) {
   const double m = ::fw_mask_x_theta(X, Theta, tnow);
   return EXACT ? unit_phase_exact(ph, m) : unit_phase(ph, m);
 }
+// The V of the cached factor table: V itself, or its time-free part when V
+// splits as V0(X) + X F(t) (FW_V_SPLIT, set by the host), whose field factor
+// exp(+i dt/2 Theta F(tau)) the passes apply per step (v_field_phase).
+#ifndef FW_V_SPLIT
+#define FW_V_SPLIT 0
+#endif
+FWD double v_tab_fn(double X, double t) {
+#if FW_V_SPLIT
+  return ::fw_V_static(X, t);
+#else
+  return ::V(X, t);
+#endif
+}
+template <bool EXACT = false>
+FWD cpx vfactor_tab(double X, double Theta, double tau, double tnow) {
+  const double ph = -0.5 * ::dt * (v_tab_fn(X - 0.5 * Theta, tau) - v_tab_fn(X + 0.5 * Theta, tau));
+  const double m = ::fw_mask_x_theta(X, Theta, tnow);
+  return EXACT ? unit_phase_exact(ph, m) : unit_phase(ph, m);
+}
 // expK_bulk / expK_boundary, wigner_moyal_cuda_1d.py:592-622
 template <bool EXACT = false>
 FWD cpx kfactor(double Lambda, double P, double tau, double tnow) {
@@ -276,6 +295,26 @@ FWD void v_factors(const Args& a, int c, int t, cpx2 (&ftab)[FW_COL_E / 2 + 1]) 
   ftab[E / 2] = *reinterpret_cast<const cpx2*>(a.vtab + vtab_index(c, N / 2));
 }
 
+// The field factor exp(+i dt/2 Theta_k F(tau)) of split potentials on a
+// thread's slots (both columns of the pair share it); nothing otherwise.
+FWD void v_field_phase(int t, double tau, cpx2 (&ftab)[FW_COL_E / 2 + 1]) {
+#if FW_V_SPLIT
+  constexpr int N = FW_NP, E = FW_COL_E;
+  const double g = 0.5 * ::dt * ::fw_V_field(tau);
+#pragma unroll
+  for (int i = 0; i <= E / 2; ++i) {
+    const double Th = ::dTheta * (i < E / 2 ? spec_k<VL>(t, i) : N / 2);
+    const cpx e = unit_phase(g * Th, 1.0);
+    ftab[i].a = cmul(ftab[i].a, e);
+    ftab[i].b = cmul(ftab[i].b, e);
+  }
+#else
+  (void)t;
+  (void)tau;
+  (void)ftab;
+#endif
+}
+
 // The same factors evaluated at time tau (masks at tnow), for time-dependent V.
 // One straight-line block before the spectral step: the evaluations share
 // their time-only subexpressions (a field E(t) in V, say), which the compiler
@@ -416,8 +455,12 @@ FWD void vpass(const Args& a) {
   NormRow<NT> norm_row;
   if (deferred) norm_row.load(a.partials);
   cpx2 ftab[E / 2 + 1];
-  if constexpr (use_tab) v_factors(a, c, t, ftab);
-  else v_factors_at(c, t, tau, tnow, ftab);
+  if constexpr (use_tab) {
+    v_factors(a, c, t, ftab);
+    if constexpr (!(FL & F_BLOCH)) v_field_phase(t, tau, ftab);
+  } else {
+    v_factors_at(c, t, tau, tnow, ftab);
+  }
   tws.store(twl, tid);
   if (deferred) norm_row.reduce_waves(red);
   __syncthreads();
@@ -515,8 +558,12 @@ FWD void vvpass(const Args& a) {
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = *reinterpret_cast<const cpx*>(W + (size_t)(t + TPL * e) * FW_NX + c);
   cpx2 ftab[E / 2 + 1];
-  if constexpr (use_tab) v_factors(a, c, t, ftab);
-  else v_factors_at(c, t, t0 + FW_PHASE_SHIFT * ::dt, t0, ftab);
+  if constexpr (use_tab) {
+    v_factors(a, c, t, ftab);
+    v_field_phase(t, t0 + FW_PHASE_SHIFT * ::dt, ftab);
+  } else {
+    v_factors_at(c, t, t0 + FW_PHASE_SHIFT * ::dt, t0, ftab);
+  }
   tws.store(twl, tid);
   // the K pass of this step has read `fresh`; the next K pass uses the partials
   if (blk == 0 && tid == 0 && a.scal[SC_FRESH] != 0.0) a.scal[SC_FRESH] = 0.0;
@@ -559,7 +606,13 @@ FWD void vvpass(const Args& a) {
   // of keeping those of the closing step live across it (register pressure).
   int t2 = t;
   asm volatile("" : "+v"(t2));
-  if constexpr (!use_tab) v_factors_at(c, t2, t1 + FW_PHASE_SHIFT * ::dt, t1, ftab);
+  if constexpr (!use_tab) {
+    v_factors_at(c, t2, t1 + FW_PHASE_SHIFT * ::dt, t1, ftab);
+  } else if constexpr (FW_V_SPLIT) {
+    // the table again (L2) with the opening step's field factor
+    v_factors(a, c, t2, ftab);
+    v_field_phase(t2, t1 + FW_PHASE_SHIFT * ::dt, ftab);
+  }
   VMul mul1{ftab};
   spectral_step<L>(v, lds, q, t2, twl, mul1);
 #pragma unroll
@@ -595,6 +648,8 @@ FWD void vvpass_theta(const Args& a) {
     v_factors(a, c, t, f0);
 #pragma unroll
     for (int i = 0; i <= E / 2; ++i) f1[i] = f0[i];
+    v_field_phase(t, t0 + FW_PHASE_SHIFT * ::dt, f0);
+    v_field_phase(t, t1 + FW_PHASE_SHIFT * ::dt, f1);
   } else {
     v_factors_at(c, t, t0 + FW_PHASE_SHIFT * ::dt, t0, f0);
     v_factors_at(c, t, t1 + FW_PHASE_SHIFT * ::dt, t1, f1);
@@ -822,7 +877,7 @@ FWD void build_vtab(const Args& a, cpx* tab) {
   for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
     const int k = (int)(idx / FW_NX), j = (int)(idx % FW_NX);
     const double X = ::dX * (j - 0.5 * ::X_gridDIM), Th = ::dTheta * k;
-    tab[vtab_index(j, k)] = (a.flags & F_BLOCH) ? bloch_vfactor(a, X, Th) : vfactor<true>(X, Th, tau, tnow);
+    tab[vtab_index(j, k)] = (a.flags & F_BLOCH) ? bloch_vfactor(a, X, Th) : vfactor_tab<true>(X, Th, tau, tnow);
   }
 }
 FWD void build_ktab(const Args& a, cpx* tab) {

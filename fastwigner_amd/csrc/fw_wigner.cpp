@@ -4,6 +4,7 @@
 
 #include <cmath>
 #include <cctype>
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <sstream>
@@ -188,9 +189,101 @@ bool uses_t(const char* s) { return uses_token(s, "t"); }
 
 std::string or_default(const char* s, const char* d) { return (s && *s) ? std::string(s) : std::string(d); }
 
+// Splits V at the top level of its expression into time-free terms and terms
+// X * F(t) (a product with the single factor X, F free of X), as in the
+// strong-field potentials V0(X) + X * E(t): then V(x - th/2, t) - V(x + th/2, t)
+// = [V0(x - th/2) - V0(x + th/2)] - th F(t), and the V factor is a cached table
+// of V0 times exp(+i dt/2 th F(tau)) per theta. Returns false (no split) for
+// any other form; a term the scan cannot read is never split.
+bool split_field(const char* vexpr, std::string& vs, std::string& vf) {
+  if (!vexpr || !uses_t(vexpr)) return false;
+  const std::string v(vexpr);
+  auto ident = [](char c) { return isalnum((unsigned char)c) || c == '_' || c == '.'; };
+  auto trim = [](std::string t) {
+    const size_t a = t.find_first_not_of(" \t\n"), b = t.find_last_not_of(" \t\n");
+    return a == std::string::npos ? std::string() : t.substr(a, b - a + 1);
+  };
+  // additive terms at depth 0: (sign, text)
+  std::vector<std::pair<char, std::string>> terms;
+  int depth = 0;
+  char sign = '+';
+  std::string cur;
+  for (size_t i = 0; i < v.size(); ++i) {
+    const char ch = v[i];
+    if (ch == '(') ++depth;
+    if (ch == ')') --depth;
+    if (depth < 0) return false;
+    if (depth == 0 && (ch == '+' || ch == '-')) {
+      // binary only after an operand; not the sign of a numeric exponent (1e-5)
+      size_t j = i;
+      while (j > 0 && (v[j - 1] == ' ' || v[j - 1] == '\t')) --j;
+      const bool after_operand = j > 0 && (ident(v[j - 1]) || v[j - 1] == ')');
+      const bool exponent = j == i && i >= 2 && (v[i - 1] == 'e' || v[i - 1] == 'E') &&
+                            (isdigit((unsigned char)v[i - 2]) || v[i - 2] == '.');
+      if (after_operand && !exponent) {
+        terms.push_back({sign, trim(cur)});
+        sign = ch;
+        cur.clear();
+        continue;
+      }
+    }
+    cur += ch;
+  }
+  if (depth != 0) return false;
+  terms.push_back({sign, trim(cur)});
+  std::string st, fl;
+  for (auto& tm : terms) {
+    const std::string& t = tm.second;
+    if (t.empty()) return false;
+    if (!uses_t(t.c_str())) {
+      st += std::string(" ") + tm.first + " (" + t + ")";
+      continue;
+    }
+    // a product of factors at depth 0, exactly one of them the token X
+    std::vector<std::string> fac;
+    std::string f;
+    int dd = 0;
+    for (char ch : t) {
+      if (ch == '(') ++dd;
+      if (ch == ')') --dd;
+      if (dd == 0 && ch == '*') {
+        fac.push_back(trim(f));
+        f.clear();
+        continue;
+      }
+      if (dd == 0 && std::strchr("/%+-<>=!&|?:,^", ch)) return false;
+      f += ch;
+    }
+    fac.push_back(trim(f));
+    int nx = 0;
+    std::string rest;
+    for (auto& g : fac) {
+      if (g == "X" || g == "(X)") {
+        ++nx;
+        continue;
+      }
+      if (g.empty() || uses_token(g.c_str(), "X")) return false;
+      rest += (rest.empty() ? "(" : " * (") + g + ")";
+    }
+    if (nx != 1 || rest.empty()) return false;
+    fl += std::string(" ") + tm.first + " " + rest;
+  }
+  vs = st.empty() ? std::string("0.") : st;
+  vf = fl;
+  return !vf.empty();
+}
+
 // Constants and spliced expressions, wigner_moyal_cuda_1d.py:151-166,244-277.
 bool tables_v(const fw_wigner_desc* d, const fw_wigner_exprs* x) {
-  return d->phase_tables > 0 || (d->phase_tables == 0 && !uses_t(x->V) && !uses_t(x->abs_boundary_x_theta));
+  std::string vs, vf;
+  const bool t_free = !uses_t(x->V) || split_field(x->V, vs, vf);
+  return d->phase_tables > 0 || (d->phase_tables == 0 && t_free && !uses_t(x->abs_boundary_x_theta));
+}
+// The V factor table holds the time-free part and the field phase is applied
+// per step: V splits (split_field), the table is used, and the x-theta mask is
+// time-free.
+bool v_split(const fw_wigner_desc* d, const fw_wigner_exprs* x, std::string& vs, std::string& vf) {
+  return d->phase_tables >= 0 && tables_v(d, x) && !uses_t(x->abs_boundary_x_theta) && split_field(x->V, vs, vf);
 }
 bool tables_k(const fw_wigner_desc* d, const fw_wigner_exprs* x) {
   return d->phase_tables > 0 || (d->phase_tables == 0 && !uses_t(x->K) && !uses_t(x->abs_boundary_lambda_p));
@@ -234,6 +327,14 @@ std::string make_prelude(const fw_wigner_desc* d, const fw_wigner_exprs* x, cons
   if (x->consts) s << x->consts << "\n";
   s << "// ---- spliced expressions ----\n";
   s << "__device__ double V(double X, double t) { return (" << x->V << "); }\n";
+  {
+    std::string vs, vf;
+    if (v_split(d, x, vs, vf)) {
+      s << "#define FW_V_SPLIT 1\n";
+      s << "__device__ double fw_V_static(double X, double t) { (void)t; return (" << vs << "); }\n";
+      s << "__device__ double fw_V_field(double t) { return (" << vf << "); }\n";
+    }
+  }
   s << "__device__ double K(double P, double t) { return (" << x->K << "); }\n";
   s << "__device__ double fw_diff_V(double X, double t) { return (" << or_default(x->diff_V, "0.") << "); }\n";
   s << "__device__ double fw_diff_K(double P, double t) { return (" << or_default(x->diff_K, "0.") << "); }\n";
@@ -590,6 +691,22 @@ int build_expr_module(fw_wigner* h, const std::string& body, const char* fname, 
 extern "C" int fw_wigner_grid_constants(const fw_wigner_desc* desc, double out[4]) {
   if (!desc || !out) return fw::fail(FW_ERR_INVALID_ARG, "NULL argument");
   grid_constants(desc, &out[0], &out[1], &out[2], &out[3]);
+  return FW_OK;
+}
+
+extern "C" int fw_wigner_prelude(const fw_wigner_desc* desc, const fw_wigner_exprs* exprs, char* out, size_t cap,
+                                 size_t* len) {
+  fw::clear_error();
+  FWCHK(validate(desc, exprs));
+  Tiles t;
+  FWCHK(resolve_tiles(desc, t));
+  const std::string pre = make_prelude(desc, exprs, t);
+  if (len) *len = pre.size() + 1;
+  if (out && cap > 0) {
+    const size_t n = std::min(cap - 1, pre.size());
+    std::memcpy(out, pre.data(), n);
+    out[n] = '\0';
+  }
   return FW_OK;
 }
 

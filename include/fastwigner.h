@@ -96,6 +96,12 @@ typedef struct fw_wigner_exprs {
  * (wigner_moyal_cuda_1d.py:113-135), computed as numpy does. */
 int fw_wigner_grid_constants(const fw_wigner_desc* desc, double out[4]);
 
+/* The generated prelude of the kernel source (configuration macros, constants and the
+ * spliced expressions), without compiling: *len = its size + 1; up to cap - 1 characters
+ * and a terminating 0 are copied to out (out may be NULL). Replaces reading the source the
+ * reference assembles for SourceModule (wigner_moyal_cuda_1d.py:244-277). */
+int fw_wigner_prelude(const fw_wigner_desc* desc, const fw_wigner_exprs* exprs, char* out, size_t cap,
+                      size_t* len);
 /* Generate and compile the kernel source with hipRTC without touching a device.
  * On FW_ERR_COMPILE the compiler log is in fw_last_error(). */
 int fw_wigner_check(const fw_wigner_desc* desc, const fw_wigner_exprs* exprs);

@@ -131,3 +131,45 @@ def test_rho_record_matches_per_step_formula():
         for k in names:
             ref = o["purity" if k == "purity_series" else k]
             assert abs(getattr(h, k)[i] - ref) <= 1e-14 * max(1.0, abs(ref)), k
+
+
+def _prelude(V, consts=None, **kw):
+    d = _lib.wigner_desc(64, 8., 64, 8., 0.01, 0.5, 0, 0, None, **kw)
+    e = _lib.wigner_exprs(V=V, K="0.5*P*P", consts=consts)
+    n = ctypes.c_size_t()
+    _lib.call("fw_wigner_prelude", ctypes.byref(d), ctypes.byref(e), None, 0, ctypes.byref(n))
+    buf = ctypes.create_string_buffer(n.value)
+    _lib.call("fw_wigner_prelude", ctypes.byref(d), ctypes.byref(e), buf, n.value, ctypes.byref(n))
+    return buf.value.decode()
+
+
+@pytest.mark.parametrize("V,split", [
+    ("-1. / sqrt(X * X + 1.) + X * E(t)", True), ("0.5*X*X + X*E0*sin(om*t)", True),
+    ("E(t)*X - 0.5*X", True), ("1e-5*X - X*sin(t)", True),
+    ("0.5*X*X*(1. + 0.05*sin(0.3*t))", False), ("X*X*E(t)", False), ("0.5*X*X", False),
+    ("X*sin(t)/2", False), ("-X*sin(t)", False), ("X*sin(t) + E(t)", False)])
+def test_field_split_of_time_dependent_potentials(V, split):
+    """V = V0(X) + X F(t) at the top level of the expression gets a table of V0
+    and a per-step field factor (FW_V_SPLIT); every other time-dependent form
+    (and a time-independent V) is left whole. The split parts, evaluated with
+    numpy, add up to V."""
+    consts = "__device__ double E(double t) { return 0.05*sin(0.06*t); }\nconst double E0 = 0.05; const double om = 0.3;"
+    src = _prelude(V, consts)
+    assert ("#define FW_V_SPLIT 1" in src) == split
+    if not split:
+        return
+    import re as _re
+    vs = _re.search(r"fw_V_static\(double X, double t\) \{ \(void\)t; return \((.*)\); \}", src).group(1)
+    vf = _re.search(r"fw_V_field\(double t\) \{ return \((.*)\); \}", src).group(1)
+    env = dict(sqrt=np.sqrt, sin=np.sin, E=lambda t: 0.05 * np.sin(0.06 * t), E0=0.05, om=0.3)
+    for X in (-3.0, 0.0, 0.7, 5.0):
+        for t in (0.0, 1.3, 40.0):
+            full = eval(V, dict(env, X=X, t=t))
+            parts = eval(vs, dict(env, X=X, t=t)) + X * eval(vf, dict(env, t=t))
+            assert abs(full - parts) <= 1e-12 * max(1.0, abs(full))
+
+
+def test_field_split_needs_tables():
+    """With phase_tables = -1 (factors evaluated at every half-step) V is not split."""
+    src = _prelude("0.5*X*X + X*E0*sin(om*t)", "const double E0 = 0.05; const double om = 0.3;", phase_tables=-1)
+    assert "#define FW_V_SPLIT 1" not in src

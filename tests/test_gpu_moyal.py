@@ -100,13 +100,15 @@ def test_harmonic_closed_form_gpu(gpu):
     assert rel_l2(W, Wa) <= 1e-12
 
 
-@pytest.mark.parametrize("nx,npp,xp", [(128, 64, "x"), (64, 256, "x"), (512, 512, "x"), (128, 64, "xp"),
-                                       (256, 256, "xp")])
-def test_rectangular_masks_time_dependent(gpu, nx, npp, xp):
+@pytest.mark.parametrize("nx,npp,xp,passes", [(128, 64, "x", 0), (64, 256, "x", 0), (512, 512, "x", 0),
+                                              (128, 64, "xp", 0), (256, 256, "xp", 0), (128, 128, "x", 3)])
+def test_rectangular_masks_time_dependent(gpu, nx, npp, xp, passes):
     """Non-square grids, all three absorbing-boundary strings and a
-    time-dependent potential V(x, t) = x^2/2 + x E(t) (strong-field pattern).
+    time-dependent potential V(x, t) = x^2/2 + x E(t) (strong-field pattern;
+    a cached table of x^2/2 times a per-step field factor).
     An x-p mask of x alone keeps the fused column pass on the theta spectrum;
-    one that also depends on p sends it through x-p space between its halves."""
+    one that also depends on p sends it through x-p space between its halves;
+    three passes per step apply the field factor in the plain V pass."""
     consts = {"E0": 0.05, "om": 0.3}
     mxp = "pow(fabs(cos(0.5*M_PI*X/X_amplitude)), 0.125)"
     if xp == "xp":
@@ -115,7 +117,7 @@ def test_rectangular_masks_time_dependent(gpu, nx, npp, xp):
                  abs_boundary_lambda_p="exp(-1e-4*Lambda*Lambda)",
                  abs_boundary_x_p=mxp)
     g, o = make_pair(nx, 10., npp, 6., 0.02, "0.5*X*X + X*E0*sin(om*t)", "0.5*P*P",
-                     "X + E0*sin(om*t)", "P", consts=consts, masks=masks)
+                     "X + E0*sin(om*t)", "P", consts=consts, masks=masks, step_passes=passes)
     W0 = np.exp(-(o.X + 0.5) ** 2 - (o.P - 0.3) ** 2)
     g.set_wignerfunction(W0)
     o.set_wignerfunction(W0)
@@ -189,20 +191,28 @@ def test_tile_schedules_agree(gpu, tiles):
     check_series(g, o)
 
 
-@pytest.mark.parametrize("nx,npp,tables", [(128, 64, 0), (64, 256, 0), (256, 128, 1)])
-def test_theta_sums_time_dependent_v(gpu, nx, npp, tables):
+TD_V = {"field": ("0.5*X*X + X*E0*sin(om*t)", "X + E0*sin(om*t)"),
+        "scaled": ("0.5*X*X*(1. + E0*sin(om*t))", "X*(1. + E0*sin(om*t))"),
+        "static": (V_Q, DV_Q)}
+
+
+@pytest.mark.parametrize("nx,npp,tables,vkind", [(128, 64, 0, "field"), (64, 256, 0, "field"),
+                                                 (128, 128, 0, "scaled"), (256, 128, 1, "static")])
+def test_theta_sums_time_dependent_v(gpu, nx, npp, tables, vkind):
     """The sums on the theta spectrum with a time-dependent V (the closing and
     opening factors of the fused column pass evaluated at different times) or
     a tabulated time-independent one, a non-polynomial K whose weights are
     not those of P and P^2, and the x-theta and lambda-p masks (which sit in the
-    factors; no x-p mask, which would send the pass through x-p space)."""
+    factors; no x-p mask, which would send the pass through x-p space).
+    "field" is V0(X) + X F(t), split into a cached table of V0 and a per-step
+    field factor; "scaled" is a time-dependent V that does not split and is
+    evaluated at every half-step."""
     consts = {"E0": 0.05, "om": 0.3}
     masks = dict(abs_boundary_x_theta="pow(fabs(cos(0.5*M_PI*X_minus/X_amplitude)*cos(0.5*M_PI*X_plus/X_amplitude)), 0.01)",
                  abs_boundary_lambda_p="exp(-1e-4*Lambda*Lambda)")
     K = "0.5*P*P + 0.05*cos(P)"
     dK = "P - 0.05*sin(P)"
-    V = "0.5*X*X + X*E0*sin(om*t)" if tables == 0 else V_Q
-    dV = "X + E0*sin(om*t)" if tables == 0 else DV_Q
+    V, dV = TD_V[vkind]
     g, o = make_pair(nx, 10., npp, 6., 0.02, V, K, dV, dK, consts=consts, masks=masks, phase_tables=tables)
     W0 = np.exp(-(o.X + 0.5) ** 2 - (o.P - 0.3) ** 2)
     g.set_wignerfunction(W0)
