@@ -100,9 +100,10 @@ SF = dict(X_amplitude=60., P_amplitude=6., dt=0.05, t=0., E0=SF_E0, omega=SF_OME
 
 def strong_field_variant(n=1024, steps=2000, warmup=64):
     """Strong-field pulse at 1024^2, FP64: the field-free ground state (Bloch, get_ground_state),
-    then `steps` steps inside the pulse with the time-dependent V (factors evaluated
-    at every half-step, no table), the x-p mask after every step and the dipole
-    acceleration P_average_RHS recorded every step."""
+    then `steps` steps inside the pulse with the time-dependent V = V0(X) + X E(t)
+    (a cached table of V0's factors and the field phase exp(+i dt/2 Theta E) per
+    half-step), the x-p mask after every step and the dipole acceleration
+    P_average_RHS recorded every step."""
     from fastwigner_amd.wigner_bloch import WignerBlochHIP1D
     from fastwigner_amd.wigner_moyal import WignerMoyalHIP1D
     t0 = time.perf_counter()

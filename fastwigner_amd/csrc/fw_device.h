@@ -119,6 +119,9 @@ FWD cpx vfactor(double X, double Theta, double tau, double tnow) {
 #ifndef FW_V_SPLIT
 #define FW_V_SPLIT 0
 #endif
+#ifndef FW_DV_SPLIT
+#define FW_DV_SPLIT 0
+#endif
 FWD double v_tab_fn(double X, double t) {
 #if FW_V_SPLIT
   return ::fw_V_static(X, t);
@@ -297,23 +300,79 @@ FWD void v_factors(const Args& a, int c, int t, cpx2 (&ftab)[FW_COL_E / 2 + 1]) 
 
 // The field factor exp(+i dt/2 Theta_k F(tau)) of split potentials on a
 // thread's slots (both columns of the pair share it); nothing otherwise.
-FWD void v_field_phase(int t, double tau, cpx2 (&ftab)[FW_COL_E / 2 + 1]) {
+FWD void v_field_phase(int t, double F, cpx2 (&ftab)[FW_COL_E / 2 + 1]) {
 #if FW_V_SPLIT
   constexpr int N = FW_NP, E = FW_COL_E;
-  const double g = 0.5 * ::dt * ::fw_V_field(tau);
+  const double g = 0.5 * ::dt * F * ::dTheta;
+  if constexpr (VL::PAIRED && VL::RL == E / 2 && !FW_FP32) {
+    // FP64, one slot per thread: items m < RL/2 sit at k = t + m S, the rest at
+    // N - t - m S (thread 0: S/2 + (m - RL/2) S), so two runs of phases with
+    // step exp(+-i g S) from two directly evaluated starts (at most RL/2 - 1
+    // unit products each, a few ulp). The FP32 form evaluates every phase
+    // (its series is cheap and a float recurrence would drift).
+    constexpr int RL = VL::RL, S = VL::S;
+    const cpx st = unit_phase(g * S, 1.0);
+    const cpx stb = t == 0 ? st : mk(st.x, -st.y);
+    cpx ea = unit_phase(g * t, 1.0);
+    cpx eb = unit_phase(g * (t == 0 ? S / 2 : N - t - (RL / 2) * S), 1.0);
 #pragma unroll
-  for (int i = 0; i <= E / 2; ++i) {
-    const double Th = ::dTheta * (i < E / 2 ? spec_k<VL>(t, i) : N / 2);
-    const cpx e = unit_phase(g * Th, 1.0);
-    ftab[i].a = cmul(ftab[i].a, e);
-    ftab[i].b = cmul(ftab[i].b, e);
+    for (int m = 0; m < RL / 2; ++m) {
+      ftab[m].a = cmul(ftab[m].a, ea);
+      ftab[m].b = cmul(ftab[m].b, ea);
+      ftab[m + RL / 2].a = cmul(ftab[m + RL / 2].a, eb);
+      ftab[m + RL / 2].b = cmul(ftab[m + RL / 2].b, eb);
+      if (m + 1 < RL / 2) {
+        ea = cmul(ea, st);
+        eb = cmul(eb, stb);
+      }
+    }
+    const cpx en = unit_phase(g * (N / 2), 1.0);
+    ftab[E / 2].a = cmul(ftab[E / 2].a, en);
+    ftab[E / 2].b = cmul(ftab[E / 2].b, en);
+  } else {
+#pragma unroll
+    for (int i = 0; i <= E / 2; ++i) {
+      const cpx e = unit_phase(g * (i < E / 2 ? spec_k<VL>(t, i) : N / 2), 1.0);
+      ftab[i].a = cmul(ftab[i].a, e);
+      ftab[i].b = cmul(ftab[i].b, e);
+    }
   }
 #else
   (void)t;
-  (void)tau;
+  (void)F;
   (void)ftab;
 #endif
 }
+
+// Field values of split potentials, filled per chunk next to the step times
+// (fw_field): F(t_s + shift dt) for the factors, F(t_s) for the sums.
+#if FW_V_SPLIT
+FWD double field_tau(const Args& a, int s) { return a.times[FW_TCHUNK + s]; }
+FWD double field_now(const Args& a, int s) { return a.times[2 * FW_TCHUNK + s]; }
+FWD void field_fill(const Args& a) {
+  double* f = const_cast<double*>(a.times);
+  for (int s = threadIdx.x; s < a.step; s += blockDim.x) {
+    const double ts = a.times[s];
+    f[FW_TCHUNK + s] = ::fw_V_field(ts + FW_PHASE_SHIFT * ::dt);
+    f[2 * FW_TCHUNK + s] = ::fw_V_field(ts);
+#if FW_DV_SPLIT
+    f[3 * FW_TCHUNK + s] = ::fw_dV_field(ts);
+#endif
+  }
+}
+// V(X, t_s) in the sums: the table's V0 plus X F(t_s)
+FWD double v_now(const Args& a, double X, int s) { return ::fw_V_static(X, 0.0) + X * field_now(a, s); }
+#else
+FWD double field_tau(const Args&, int) { return 0.0; }
+FWD void field_fill(const Args&) {}
+FWD double v_now(const Args& a, double X, int s) { return ::V(X, a.times[s]); }
+#endif
+// V'(X, t_s) in the sums: dV0(X) plus the offset G(t_s) when V' splits
+#if FW_DV_SPLIT
+FWD double dv_now(const Args& a, double X, int s) { return ::fw_dV_static(X, 0.0) + a.times[3 * FW_TCHUNK + s]; }
+#else
+FWD double dv_now(const Args& a, double X, int s) { return ::fw_diff_V(X, a.times[s]); }
+#endif
 
 // The same factors evaluated at time tau (masks at tnow), for time-dependent V.
 // One straight-line block before the spectral step: the evaluations share
@@ -387,11 +446,28 @@ struct VVMul {
                                         // for P, P^2, K', K
   double pur;                           // sum c_k (|alpha_a|^2 + |alpha_b|^2)
   double a0, b0;                        // alpha_0 of columns a, b (thread 0 of a line)
+  double g0, g1;                        // split V: field phase per unit k at the two half-steps
+  // the factors of item i (k = spec_k): split potentials take the table of V0
+  // (f0 = f1) times exp(+i g k), evaluated here so that only one factor set
+  // stays in registers
+  FWD cpx2 fac(const cpx2 (&f)[FW_COL_E / 2 + 1], int i, double g) const {
+#if FW_V_SPLIT
+    const cpx e = unit_phase(g * (double)spec_k<VL>(t, i), 1.0);
+    cpx2 r;
+    r.a = cmul(f[i].a, e);
+    r.b = cmul(f[i].b, e);
+    return r;
+#else
+    (void)g;
+    return f[i];
+#endif
+  }
   FWD void pair(int i, cpx& zk, cpx& zm) {
     const real_t h = (real_t)0.5 / (real_t)FW_NP;
     const cpx A = mk(h * (zk.x + zm.x), h * (zk.y - zm.y));
     const cpx B = mk(h * (zk.y + zm.y), h * (zm.x - zk.x));
-    const cpx A1 = cscale(cmul(A, f0[i].a), ma), B1 = cscale(cmul(B, f0[i].b), mb);
+    const cpx2 F0 = fac(f0, i, g0);
+    const cpx A1 = cscale(cmul(A, F0.a), ma), B1 = cscale(cmul(B, F0.b), mb);
     const double sx = (double)A1.x + (double)B1.x, sy = (double)A1.y + (double)B1.y;
     const SumW w = sw[spec_k<VL>(t, i)];
     acc[0] += sy * w.a - sx * (::dP * FW_NP);
@@ -400,7 +476,8 @@ struct VVMul {
     acc[3] += sx * w.kk[0] + sy * w.kk[1];
     const double ax = A1.x, ay = A1.y, bx = B1.x, by = B1.y;
     pur += 2.0 * (ax * ax + ay * ay + bx * bx + by * by);
-    const cpx A2 = cmul(A1, f1[i].a), B2 = cmul(B1, f1[i].b);
+    const cpx2 F1 = fac(f1, i, g1);
+    const cpx A2 = cmul(A1, F1.a), B2 = cmul(B1, F1.b);
     zk = mk(A2.x - B2.y, A2.y + B2.x);
     zm = mk(A2.x + B2.y, B2.x - A2.y);
   }
@@ -408,8 +485,15 @@ struct VVMul {
   FWD void self(cpx& z0, cpx& zn) {
     constexpr int E2 = FW_COL_E / 2;
     const real_t s = (real_t)1 / (real_t)FW_NP;
+#if FW_V_SPLIT
+    // Nyquist: the real part of the phased factor (the DC phase is 1)
+    const cpx en0 = unit_phase(g0 * (FW_NP / 2), 1.0), en1 = unit_phase(g1 * (FW_NP / 2), 1.0);
+    const cpx2 n0 = {cmul(f0[E2].a, en0), cmul(f0[E2].b, en0)}, n1 = {cmul(f1[E2].a, en1), cmul(f1[E2].b, en1)};
+#else
+    const cpx2 n0 = f0[E2], n1 = f1[E2];
+#endif
     const real_t ra = z0.x * s * f0[0].a.x * ma, rb = z0.y * s * f0[0].b.x * mb;
-    const real_t na = zn.x * s * f0[E2].a.x * ma, nb = zn.y * s * f0[E2].b.x * mb;
+    const real_t na = zn.x * s * n0.a.x * ma, nb = zn.y * s * n0.b.x * mb;
     a0 = ra;
     b0 = rb;
     const double s0 = (double)ra + (double)rb, sn = (double)na + (double)nb;
@@ -420,7 +504,7 @@ struct VVMul {
     acc[3] += s0 * w0.kk[0] + sn * wn.kk[0];
     pur += (double)ra * ra + (double)rb * rb + (double)na * na + (double)nb * nb;
     z0 = mk(ra * f1[0].a.x, rb * f1[0].b.x);
-    zn = mk(na * f1[E2].a.x, nb * f1[E2].b.x);
+    zn = mk(na * n1.a.x, nb * n1.b.x);
   }
 };
 
@@ -457,7 +541,7 @@ FWD void vpass(const Args& a) {
   cpx2 ftab[E / 2 + 1];
   if constexpr (use_tab) {
     v_factors(a, c, t, ftab);
-    if constexpr (!(FL & F_BLOCH)) v_field_phase(t, tau, ftab);
+    if constexpr (!(FL & F_BLOCH)) v_field_phase(t, field_tau(a, a.step), ftab);
   } else {
     v_factors_at(c, t, tau, tnow, ftab);
   }
@@ -519,8 +603,8 @@ FWD void vpass(const Args& a) {
     sums[0] = ca + cb;
     sums[1] = Xa * ca + Xb * cb;
     sums[3] = Xa * Xa * ca + Xb * Xb * cb;
-    sums[6] = ::fw_diff_V(Xa, tnow) * ca + ::fw_diff_V(Xb, tnow) * cb;
-    sums[7] += ::V(Xa, tnow) * ca + ::V(Xb, tnow) * cb;
+    sums[6] = dv_now(a, Xa, a.step) * ca + dv_now(a, Xb, a.step) * cb;
+    sums[7] += v_now(a, Xa, a.step) * ca + v_now(a, Xb, a.step) * cb;
     // the previous step's sums were published by the K pass (three-pass steps
     // open with this pass only after a fresh start or a K pass)
     write_partial_rows<NT>(sums, red, a.partials, blk, nb);
@@ -560,7 +644,7 @@ FWD void vvpass(const Args& a) {
   cpx2 ftab[E / 2 + 1];
   if constexpr (use_tab) {
     v_factors(a, c, t, ftab);
-    v_field_phase(t, t0 + FW_PHASE_SHIFT * ::dt, ftab);
+    v_field_phase(t, field_tau(a, a.step), ftab);
   } else {
     v_factors_at(c, t, t0 + FW_PHASE_SHIFT * ::dt, t0, ftab);
   }
@@ -598,8 +682,8 @@ FWD void vvpass(const Args& a) {
   sums[0] = ca + cb;
   sums[1] = Xa * ca + Xb * cb;
   sums[3] = Xa * Xa * ca + Xb * Xb * cb;
-  sums[6] = ::fw_diff_V(Xa, t0) * ca + ::fw_diff_V(Xb, t0) * cb;
-  sums[7] += ::V(Xa, t0) * ca + ::V(Xb, t0) * cb;
+  sums[6] = dv_now(a, Xa, a.step) * ca + dv_now(a, Xb, a.step) * cb;
+  sums[7] += v_now(a, Xa, a.step) * ca + v_now(a, Xb, a.step) * cb;
   write_partial_rows<NT>(sums, red, a.partials, blk, nb);
   // opening V/2 of step s+1. The thread index passes through an empty asm so
   // that the compiler recomputes this step's twiddles and LDS indices instead
@@ -611,7 +695,7 @@ FWD void vvpass(const Args& a) {
   } else if constexpr (FW_V_SPLIT) {
     // the table again (L2) with the opening step's field factor
     v_factors(a, c, t2, ftab);
-    v_field_phase(t2, t1 + FW_PHASE_SHIFT * ::dt, ftab);
+    v_field_phase(t2, field_tau(a, a.step + 1), ftab);
   }
   VMul mul1{ftab};
   spectral_step<L>(v, lds, q, t2, twl, mul1);
@@ -643,13 +727,13 @@ FWD void vvpass_theta(const Args& a) {
   cpx v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = *reinterpret_cast<const cpx*>(W + (size_t)(t + TPL * e) * FW_NX + c);
+  static_assert(!FW_V_SPLIT || FW_VTAB, "split potentials use the table of V0");
   cpx2 f0[E / 2 + 1], f1[E / 2 + 1];
   if constexpr (use_tab) {
+    // split potentials: the multiply phases the one table per half-step
     v_factors(a, c, t, f0);
 #pragma unroll
     for (int i = 0; i <= E / 2; ++i) f1[i] = f0[i];
-    v_field_phase(t, t0 + FW_PHASE_SHIFT * ::dt, f0);
-    v_field_phase(t, t1 + FW_PHASE_SHIFT * ::dt, f1);
   } else {
     v_factors_at(c, t, t0 + FW_PHASE_SHIFT * ::dt, t0, f0);
     v_factors_at(c, t, t1 + FW_PHASE_SHIFT * ::dt, t1, f1);
@@ -664,7 +748,12 @@ FWD void vvpass_theta(const Args& a) {
   // the x-only mask at t0, once per column (MaskXP); 1 without a mask
   const MaskXP mask((a.flags & F_MASK_XP) != 0, Xa, Xb, t0);
   static_assert(!FW_THETA_SUMS || FW_MASK_XP_X_ONLY, "the theta-spectrum pass takes x-only masks");
-  VVMul mul{f0, f1, mask.ma, mask.mb, a.sw, t, {0.0, 0.0, 0.0, 0.0}, 0.0, 0.0, 0.0};
+  double g0 = 0.0, g1 = 0.0;
+#if FW_V_SPLIT
+  g0 = 0.5 * ::dt * field_tau(a, a.step) * ::dTheta;
+  g1 = 0.5 * ::dt * field_tau(a, a.step + 1) * ::dTheta;
+#endif
+  VVMul mul{f0, f1, mask.ma, mask.mb, a.sw, t, {0.0, 0.0, 0.0, 0.0}, 0.0, 0.0, 0.0, g0, g1};
   spectral_step<L>(v, lds, q, t, twl, mul);
 #pragma unroll
   for (int e = 0; e < E; ++e) *reinterpret_cast<cpx*>(W + (size_t)(t + TPL * e) * FW_NX + c) = v[e];
@@ -681,8 +770,8 @@ FWD void vvpass_theta(const Args& a) {
   sums[6] = 0.0;
   sums[7] = mul.acc[3];
   if (t == 0) {
-    sums[6] = ::fw_diff_V(Xa, t0) * ca + ::fw_diff_V(Xb, t0) * cb;
-    sums[7] += ::V(Xa, t0) * ca + ::V(Xb, t0) * cb;
+    sums[6] = dv_now(a, Xa, a.step) * ca + dv_now(a, Xb, a.step) * cb;
+    sums[7] += v_now(a, Xa, a.step) * ca + v_now(a, Xb, a.step) * cb;
   }
   sums[8] = (double)N * mul.pur;
   write_partial_rows<NT>(sums, red, a.partials, blk, nb);
@@ -947,6 +1036,7 @@ extern "C" __global__ void __launch_bounds__(fwk::VL::NT) fw_vv(fwk::Args a) {
   else fwk::vvpass(a);
 }
 extern "C" __global__ void __launch_bounds__(256) fw_build_sumw(fwk::Args a, fwk::SumW* sw) { fwk::build_sumw(a, sw); }
+extern "C" __global__ void __launch_bounds__(128) fw_field(fwk::Args a) { fwk::field_fill(a); }
 extern "C" __global__ void __launch_bounds__(fwk::HT) fw_moments(fwk::Args a) { fwk::moments(a); }
 extern "C" __global__ void __launch_bounds__(fwk::HT) fw_apply_norm(fwk::Args a) { fwk::apply_norm(a); }
 extern "C" __global__ void fw_set_scalar(double* p, double v) { *p = v; }

@@ -95,12 +95,13 @@ struct fw_wigner {
   bool bloch_ready = false;
 
   bool vtab_ok = false, ktab_ok = false;   // V (K) factors are time independent
+  bool vsplit = false;                      // V = V0(X) + X F(t): field values beside the step times
   void* vtab = nullptr;
   void* sw = nullptr;   // theta-spectrum sum weights (fused column pass), or null
   void* ktab = nullptr;
   int tab_kind = -1;          // what the tables hold: 0 Moyal, 1 Bloch
   double tab_dbeta = 0.0;
-  hipFunction_t f_build_vtab = nullptr, f_build_ktab = nullptr, f_build_sumw = nullptr;
+  hipFunction_t f_build_vtab = nullptr, f_build_ktab = nullptr, f_build_sumw = nullptr, f_field = nullptr;
 
   hipGraphExec_t gexec = nullptr;
   int gkind = -1;
@@ -188,26 +189,20 @@ bool uses_token(const char* s, const char* tok) {
 bool uses_t(const char* s) { return uses_token(s, "t"); }
 
 std::string or_default(const char* s, const char* d) { return (s && *s) ? std::string(s) : std::string(d); }
+std::string trim_ws(const std::string& t) {
+  const size_t a = t.find_first_not_of(" \t\n"), b = t.find_last_not_of(" \t\n");
+  return a == std::string::npos ? std::string() : t.substr(a, b - a + 1);
+}
 
-// Splits V at the top level of its expression into time-free terms and terms
-// X * F(t) (a product with the single factor X, F free of X), as in the
-// strong-field potentials V0(X) + X * E(t): then V(x - th/2, t) - V(x + th/2, t)
-// = [V0(x - th/2) - V0(x + th/2)] - th F(t), and the V factor is a cached table
-// of V0 times exp(+i dt/2 th F(tau)) per theta. Returns false (no split) for
-// any other form; a term the scan cannot read is never split.
-bool split_field(const char* vexpr, std::string& vs, std::string& vf) {
-  if (!vexpr || !uses_t(vexpr)) return false;
-  const std::string v(vexpr);
+// The additive terms at the top level of an expression as (sign, text); false
+// when the scan cannot read it (unbalanced parentheses, an empty term).
+bool top_terms(const char* expr, std::vector<std::pair<char, std::string>>& terms) {
+  const std::string v(expr);
   auto ident = [](char c) { return isalnum((unsigned char)c) || c == '_' || c == '.'; };
-  auto trim = [](std::string t) {
-    const size_t a = t.find_first_not_of(" \t\n"), b = t.find_last_not_of(" \t\n");
-    return a == std::string::npos ? std::string() : t.substr(a, b - a + 1);
-  };
-  // additive terms at depth 0: (sign, text)
-  std::vector<std::pair<char, std::string>> terms;
   int depth = 0;
   char sign = '+';
   std::string cur;
+  terms.clear();
   for (size_t i = 0; i < v.size(); ++i) {
     const char ch = v[i];
     if (ch == '(') ++depth;
@@ -221,7 +216,7 @@ bool split_field(const char* vexpr, std::string& vs, std::string& vf) {
       const bool exponent = j == i && i >= 2 && (v[i - 1] == 'e' || v[i - 1] == 'E') &&
                             (isdigit((unsigned char)v[i - 2]) || v[i - 2] == '.');
       if (after_operand && !exponent) {
-        terms.push_back({sign, trim(cur)});
+        terms.push_back({sign, trim_ws(cur)});
         sign = ch;
         cur.clear();
         continue;
@@ -230,11 +225,25 @@ bool split_field(const char* vexpr, std::string& vs, std::string& vf) {
     cur += ch;
   }
   if (depth != 0) return false;
-  terms.push_back({sign, trim(cur)});
+  terms.push_back({sign, trim_ws(cur)});
+  for (auto& tm : terms)
+    if (tm.second.empty()) return false;
+  return true;
+}
+
+// Splits V at the top level of its expression into time-free terms and terms
+// X * F(t) (a product with the single factor X, F free of X), as in the
+// strong-field potentials V0(X) + X * E(t): then V(x - th/2, t) - V(x + th/2, t)
+// = [V0(x - th/2) - V0(x + th/2)] - th F(t), and the V factor is a cached table
+// of V0 times exp(+i dt/2 th F(tau)) per theta. Returns false (no split) for
+// any other form; a term the scan cannot read is never split.
+bool split_field(const char* vexpr, std::string& vs, std::string& vf) {
+  if (!vexpr || !uses_t(vexpr)) return false;
+  std::vector<std::pair<char, std::string>> terms;
+  if (!top_terms(vexpr, terms)) return false;
   std::string st, fl;
   for (auto& tm : terms) {
     const std::string& t = tm.second;
-    if (t.empty()) return false;
     if (!uses_t(t.c_str())) {
       st += std::string(" ") + tm.first + " (" + t + ")";
       continue;
@@ -247,14 +256,14 @@ bool split_field(const char* vexpr, std::string& vs, std::string& vf) {
       if (ch == '(') ++dd;
       if (ch == ')') --dd;
       if (dd == 0 && ch == '*') {
-        fac.push_back(trim(f));
+        fac.push_back(trim_ws(f));
         f.clear();
         continue;
       }
       if (dd == 0 && std::strchr("/%+-<>=!&|?:,^", ch)) return false;
       f += ch;
     }
-    fac.push_back(trim(f));
+    fac.push_back(trim_ws(f));
     int nx = 0;
     std::string rest;
     for (auto& g : fac) {
@@ -271,6 +280,28 @@ bool split_field(const char* vexpr, std::string& vs, std::string& vf) {
   vs = st.empty() ? std::string("0.") : st;
   vf = fl;
   return !vf.empty();
+}
+
+// Splits V' = dV0(X) + G(t) (every time-dependent top-level term free of X,
+// as E(t) in the strong-field V' = X / (X^2 + 1)^1.5 + E(t)), so the sums of a
+// split potential read G per step instead of evaluating it per column.
+bool split_offset(const char* expr, std::string& ds, std::string& dg) {
+  if (!expr || !uses_t(expr)) return false;
+  std::vector<std::pair<char, std::string>> terms;
+  if (!top_terms(expr, terms)) return false;
+  std::string st, gt;
+  for (auto& tm : terms) {
+    const std::string& t = tm.second;
+    if (!uses_t(t.c_str())) {
+      st += std::string(" ") + tm.first + " (" + t + ")";
+    } else {
+      if (uses_token(t.c_str(), "X")) return false;
+      gt += std::string(" ") + tm.first + " (" + t + ")";
+    }
+  }
+  ds = st.empty() ? std::string("0.") : st;
+  dg = gt;
+  return !dg.empty();
 }
 
 // Constants and spliced expressions, wigner_moyal_cuda_1d.py:151-166,244-277.
@@ -330,9 +361,15 @@ std::string make_prelude(const fw_wigner_desc* d, const fw_wigner_exprs* x, cons
   {
     std::string vs, vf;
     if (v_split(d, x, vs, vf)) {
-      s << "#define FW_V_SPLIT 1\n";
+      s << "#define FW_V_SPLIT 1\n#define FW_TCHUNK " << (kChunk + 1) << "\n";
       s << "__device__ double fw_V_static(double X, double t) { (void)t; return (" << vs << "); }\n";
       s << "__device__ double fw_V_field(double t) { return (" << vf << "); }\n";
+      std::string ds, dg;
+      if (split_offset(x->diff_V, ds, dg)) {
+        s << "#define FW_DV_SPLIT 1\n";
+        s << "__device__ double fw_dV_static(double X, double t) { (void)t; return (" << ds << "); }\n";
+        s << "__device__ double fw_dV_field(double t) { return (" << dg << "); }\n";
+      }
     }
   }
   s << "__device__ double K(double P, double t) { return (" << x->K << "); }\n";
@@ -405,6 +442,20 @@ int set_scalar(fw_wigner* h, int slot, double v) {
 }
 
 int helper_blocks(fw_wigner* h) { return h->NP < 128 ? h->NP : 128; }
+
+// Upload the times of a chunk's steps (times[s], s < n). A split potential
+// also gets its field values F(t_s + shift dt) and F(t_s) beside them
+// (fw_field, one block), so the passes read one number per half-step instead
+// of evaluating the field in every thread.
+int upload_times(fw_wigner* h, const double* t, int n) {
+  HIPCHK(hipMemcpyAsync(h->times, t, sizeof(double) * n, hipMemcpyHostToDevice, h->stream));
+  if (h->vsplit) {
+    KArgs a = base_args(h);
+    a.step = n;
+    FWCHK(launch(h, h->f_field, 1, 128, a));
+  }
+  return FW_OK;
+}
 
 // Fill the factor tables for a run of `kind` (0 Moyal, 1 Bloch with a.dbeta)
 // and set the table flags in a. Tables are allocated once; contents follow kind.
@@ -501,7 +552,7 @@ int run_steps(fw_wigner* h, int kind, const KArgs& base0, const double* times, i
   int applied = steps;
   for (int off = 0; off < steps; off += kChunk) {
     const int n = std::min(kChunk, steps - off);
-    HIPCHK(hipMemcpyAsync(h->times, times + off, sizeof(double) * n, hipMemcpyHostToDevice, h->stream));
+    FWCHK(upload_times(h, times + off, n));
     if (use_graph && n == kChunk) {
       HIPCHK(hipGraphLaunch(h->gexec, h->stream));
     } else if (h->timing) {
@@ -600,7 +651,7 @@ int run_steps_fused(fw_wigner* h, const KArgs& base0, const double* times, int s
   for (int off = 0; off < steps; off += kChunk) {
     const int n = std::min(kChunk, steps - off);
     const int nt = std::min(kChunk + 1, steps - off);   // t of steps off .. off+n (the next chunk's first)
-    HIPCHK(hipMemcpyAsync(h->times, times + off, sizeof(double) * nt, hipMemcpyHostToDevice, h->stream));
+    FWCHK(upload_times(h, times + off, nt));
     if (off == 0) {
       KArgs a = base;
       a.step = 0;
@@ -743,6 +794,10 @@ extern "C" int fw_wigner_create(const fw_wigner_desc* desc, const fw_wigner_expr
   grid_constants(desc, &h->dX, &h->dP, &h->dLambda, &h->dTheta);
   h->has_mask_xp = exprs->abs_boundary_x_p && *exprs->abs_boundary_x_p;
   h->vtab_ok = tables_v(desc, exprs);
+  {
+    std::string vs, vf;
+    h->vsplit = v_split(desc, exprs, vs, vf);
+  }
   h->ktab_ok = tables_k(desc, exprs);
   h->prelude = make_prelude(desc, exprs, t);
   h->arch = fw::target_arch(desc->device);
@@ -767,7 +822,8 @@ extern "C" int fw_wigner_create(const fw_wigner_desc* desc, const fw_wigner_expr
              {"fw_apply_norm", &h->f_apply_norm}, {"fw_set_scalar", &h->f_set_scalar},
              {"fw_vk_min", &h->f_vk_min},   {"fw_build_vtab", &h->f_build_vtab},
              {"fw_build_ktab", &h->f_build_ktab}, {"fw_vv", &h->f_vv},
-             {"fw_k_norm", &h->f_k_norm},     {"fw_build_sumw", &h->f_build_sumw}};
+             {"fw_k_norm", &h->f_k_norm},     {"fw_build_sumw", &h->f_build_sumw},
+             {"fw_field", &h->f_field}};
   for (auto& f : fns)
     if ((rc = getfn(h->mod, f.n, f.f)) != FW_OK) return bail(rc);
 
@@ -779,7 +835,9 @@ extern "C" int fw_wigner_create(const fw_wigner_desc* desc, const fw_wigner_expr
   const size_t npts = (size_t)h->NX * h->NP;
   const int maxnb = std::max(std::max(h->col_nb, h->row_nb), 1024);
   if ((rc = alloc(&h->W, npts * h->esz)) != FW_OK) return bail(rc);
-  if ((rc = alloc((void**)&h->times, sizeof(double) * (kChunk + 1))) != FW_OK) return bail(rc);
+  // step times, then (split potentials) the field at t + shift dt and at t,
+  // and the time-dependent offset of V' at t
+  if ((rc = alloc((void**)&h->times, sizeof(double) * 4 * (kChunk + 1))) != FW_OK) return bail(rc);
   if ((rc = alloc((void**)&h->scal, sizeof(double) * fw::NSCAL)) != FW_OK) return bail(rc);
   if ((rc = alloc((void**)&h->partials, sizeof(double) * FW_NSER * maxnb)) != FW_OK) return bail(rc);
   if ((rc = alloc((void**)&h->counter, sizeof(unsigned int) * 4)) != FW_OK) return bail(rc);
@@ -1075,7 +1133,7 @@ extern "C" int fw_wigner_kernel_bench(fw_wigner* h, double t, int reps, double u
   HIPCHK(hipMemcpyAsync(save, h->W, bytes, hipMemcpyDeviceToDevice, h->stream));
   std::vector<double> tt(kChunk + 1);
   for (int s = 0; s <= kChunk; ++s) tt[s] = t + (s + 1) * h->d.dt;
-  HIPCHK(hipMemcpyAsync(h->times, tt.data(), sizeof(double) * (kChunk + 1), hipMemcpyHostToDevice, h->stream));
+  FWCHK(upload_times(h, tt.data(), kChunk + 1));
   FWCHK(set_scalar(h, fw::SC_STOP, 0.0));
   FWCHK(set_scalar(h, fw::SC_FRESH, 1.0));
   KArgs a = base;

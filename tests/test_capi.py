@@ -133,9 +133,9 @@ def test_rho_record_matches_per_step_formula():
             assert abs(getattr(h, k)[i] - ref) <= 1e-14 * max(1.0, abs(ref)), k
 
 
-def _prelude(V, consts=None, **kw):
+def _prelude(V, consts=None, diff_V=None, **kw):
     d = _lib.wigner_desc(64, 8., 64, 8., 0.01, 0.5, 0, 0, None, **kw)
-    e = _lib.wigner_exprs(V=V, K="0.5*P*P", consts=consts)
+    e = _lib.wigner_exprs(V=V, K="0.5*P*P", consts=consts, diff_V=diff_V, diff_K="P" if diff_V else None)
     n = ctypes.c_size_t()
     _lib.call("fw_wigner_prelude", ctypes.byref(d), ctypes.byref(e), None, 0, ctypes.byref(n))
     buf = ctypes.create_string_buffer(n.value)
@@ -158,9 +158,8 @@ def test_field_split_of_time_dependent_potentials(V, split):
     assert ("#define FW_V_SPLIT 1" in src) == split
     if not split:
         return
-    import re as _re
-    vs = _re.search(r"fw_V_static\(double X, double t\) \{ \(void\)t; return \((.*)\); \}", src).group(1)
-    vf = _re.search(r"fw_V_field\(double t\) \{ return \((.*)\); \}", src).group(1)
+    vs = re.search(r"fw_V_static\(double X, double t\) \{ \(void\)t; return \((.*)\); \}", src).group(1)
+    vf = re.search(r"fw_V_field\(double t\) \{ return \((.*)\); \}", src).group(1)
     env = dict(sqrt=np.sqrt, sin=np.sin, E=lambda t: 0.05 * np.sin(0.06 * t), E0=0.05, om=0.3)
     for X in (-3.0, 0.0, 0.7, 5.0):
         for t in (0.0, 1.3, 40.0):
@@ -173,3 +172,24 @@ def test_field_split_needs_tables():
     """With phase_tables = -1 (factors evaluated at every half-step) V is not split."""
     src = _prelude("0.5*X*X + X*E0*sin(om*t)", "const double E0 = 0.05; const double om = 0.3;", phase_tables=-1)
     assert "#define FW_V_SPLIT 1" not in src
+
+
+@pytest.mark.parametrize("dV,split", [("X / pow(X * X + 1., 1.5) + E(t)", True), ("X - 2.*E0*sin(om*t)", True),
+                                      ("X*(1. + E(t))", False), ("X + X*E(t)", False), ("X", False)])
+def test_field_split_of_the_force(dV, split):
+    """With a split V, V' = dV0(X) + G(t) (time-dependent terms free of X) is
+    split too (FW_DV_SPLIT) and the parts add up to V'."""
+    consts = "__device__ double E(double t) { return 0.05*sin(0.06*t); }\nconst double E0 = 0.05; const double om = 0.3;"
+    src = _prelude("-1. / sqrt(X * X + 1.) + X * E(t)", consts, diff_V=dV)
+    assert "#define FW_V_SPLIT 1" in src
+    assert ("#define FW_DV_SPLIT 1" in src) == split
+    if not split:
+        return
+    ds = re.search(r"fw_dV_static\(double X, double t\) \{ \(void\)t; return \((.*)\); \}", src).group(1)
+    dg = re.search(r"fw_dV_field\(double t\) \{ return \((.*)\); \}", src).group(1)
+    env = dict(sqrt=np.sqrt, sin=np.sin, pow=np.power, E=lambda t: 0.05 * np.sin(0.06 * t), E0=0.05, om=0.3)
+    for X in (-3.0, 0.0, 0.7, 5.0):
+        for t in (0.0, 1.3, 40.0):
+            full = eval(dV, dict(env, X=X, t=t))
+            parts = eval(ds, dict(env, X=X, t=t)) + eval(dg, dict(env, t=t))
+            assert abs(full - parts) <= 1e-12 * max(1.0, abs(full))
