@@ -603,8 +603,14 @@ FWD void vpass(const Args& a) {
     sums[0] = ca + cb;
     sums[1] = Xa * ca + Xb * cb;
     sums[3] = Xa * Xa * ca + Xb * Xb * cb;
-    sums[6] = dv_now(a, Xa, a.step) * ca + dv_now(a, Xb, a.step) * cb;
-    sums[7] += v_now(a, Xa, a.step) * ca + v_now(a, Xb, a.step) * cb;
+    if constexpr (FL & F_BLOCH) {
+      // imaginary time: the whole V at t_initial (no field values uploaded)
+      sums[6] = ::fw_diff_V(Xa, tnow) * ca + ::fw_diff_V(Xb, tnow) * cb;
+      sums[7] += ::V(Xa, tnow) * ca + ::V(Xb, tnow) * cb;
+    } else {
+      sums[6] = dv_now(a, Xa, a.step) * ca + dv_now(a, Xb, a.step) * cb;
+      sums[7] += v_now(a, Xa, a.step) * ca + v_now(a, Xb, a.step) * cb;
+    }
     // the previous step's sums were published by the K pass (three-pass steps
     // open with this pass only after a fresh start or a K pass)
     write_partial_rows<NT>(sums, red, a.partials, blk, nb);

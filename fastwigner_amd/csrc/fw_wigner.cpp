@@ -443,13 +443,14 @@ int set_scalar(fw_wigner* h, int slot, double v) {
 
 int helper_blocks(fw_wigner* h) { return h->NP < 128 ? h->NP : 128; }
 
-// Upload the times of a chunk's steps (times[s], s < n). A split potential
-// also gets its field values F(t_s + shift dt) and F(t_s) beside them
-// (fw_field, one block), so the passes read one number per half-step instead
-// of evaluating the field in every thread.
-int upload_times(fw_wigner* h, const double* t, int n) {
+// Upload the times of a chunk's steps (times[s], s < n). For real-time steps
+// (field) a split potential also gets its field values F(t_s + shift dt) and
+// F(t_s) beside them (fw_field, one block), so the passes read one number per
+// half-step instead of evaluating the field in every thread. Bloch steps use
+// the whole V at t_initial and need none.
+int upload_times(fw_wigner* h, const double* t, int n, bool field = true) {
   HIPCHK(hipMemcpyAsync(h->times, t, sizeof(double) * n, hipMemcpyHostToDevice, h->stream));
-  if (h->vsplit) {
+  if (h->vsplit && field) {
     KArgs a = base_args(h);
     a.step = n;
     FWCHK(launch(h, h->f_field, 1, 128, a));
@@ -552,7 +553,7 @@ int run_steps(fw_wigner* h, int kind, const KArgs& base0, const double* times, i
   int applied = steps;
   for (int off = 0; off < steps; off += kChunk) {
     const int n = std::min(kChunk, steps - off);
-    FWCHK(upload_times(h, times + off, n));
+    FWCHK(upload_times(h, times + off, n, kind == 0));
     if (use_graph && n == kChunk) {
       HIPCHK(hipGraphLaunch(h->gexec, h->stream));
     } else if (h->timing) {
