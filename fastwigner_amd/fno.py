@@ -1,7 +1,7 @@
 """Fourier-neural-operator surrogate of k split-operator steps, trained on the
 GPU by the library's own forward/backward kernels (csrc/fw_fno.cpp) on
 trajectories of the batched propagator, with the plain propagator as the
-judge and fallback.
+referee and fallback.
 
 The reference has no learned component: this is new work on top of the
 split-operator step it defines (wigner_moyal_cuda_1d.py:496-540 and the
@@ -222,20 +222,20 @@ class FNOSurrogate:
         return us.value
 
 
-def judged_jump(surrogate, W0, feat, judge_states, dXdP, l2_tol=1e-3, norm_tol=1e-4):
-    """One k-step jump of each sample with the surrogate, judged against the plain
+def checked_jump(surrogate, W0, feat, ref_states, dXdP, l2_tol=1e-3, norm_tol=1e-4):
+    """One k-step jump of each sample with the surrogate, checked against the plain
     propagator's result: accepted when the relative L2 error is <= l2_tol and the
-    norm drift |sum(W) dXdP - 1| <= norm_tol, otherwise replaced by the judge.
+    norm drift |sum(W) dXdP - 1| <= norm_tol, otherwise replaced by the referee.
     Returns (states, accepted mask, relative L2 errors, norm drifts). The norm
     drift is that of the raw network output; the accepted states are then
     renormalised like every propagator step (W /= sum(W) dXdP)."""
     pred = surrogate.predict(W0, feat).astype(np.float64)
-    err = np.linalg.norm((pred - judge_states).reshape(len(pred), -1), axis=1) / \
-        np.linalg.norm(judge_states.reshape(len(pred), -1), axis=1)
+    err = np.linalg.norm((pred - ref_states).reshape(len(pred), -1), axis=1) / \
+        np.linalg.norm(ref_states.reshape(len(pred), -1), axis=1)
     drift = np.abs(pred.reshape(len(pred), -1).sum(axis=1) * dXdP - 1.0)
     ok = (err <= l2_tol) & (drift <= norm_tol)
     pred = pred / (pred.reshape(len(pred), -1).sum(axis=1)[:, None, None] * dXdP)
-    out = np.where(ok[:, None, None], pred, judge_states)
+    out = np.where(ok[:, None, None], pred, ref_states)
     return out, ok, err, drift
 
 
@@ -245,7 +245,7 @@ def surrogate_experiment(n_pairs=20000, pairs_per_instance=20, held_out=512, epo
     256^2 Wigner functions jumping k = 20 steps of dt = 0.01, trained on about
     n_pairs trajectory pairs of the batched random-collision propagator (random
     quartic potentials, Gaussian initial states, seeded), then evaluated on
-    `held_out` new seeds against the FP64 split-operator judge (the batched
+    `held_out` new seeds against the FP64 split-operator referee (the batched
     propagator in FP64), with fallback where the relative L2 error exceeds 1e-3
     or the norm drift 1e-4. With base_steps = b > 0 the network corrects the
     coarse prediction of b propagator steps of k dt / b (FP32), whose own error
@@ -267,21 +267,21 @@ def surrogate_experiment(n_pairs=20000, pairs_per_instance=20, held_out=512, epo
     s.train(steps, lr=lr, batch=batch, log_every=max(steps // 10, 1))
     t_train = time.perf_counter() - t0
     say("[fno] %d Adam steps in %.1f s, loss %s" % (steps, t_train, [round(l, 5) for _, l in s.history]))
-    # held-out seeds and the FP64 judge
+    # held-out seeds and the FP64 referee
     inst, _ = random_quartic_instances(held_out, seed=held_seed, sizes=(256,), amplitude=8.)
-    judge = WignerBatchHIP1D(inst, V=QUARTIC_V, K="0.5*P*P", dt=0.01, param_names=("a2", "a3", "a4"),
+    referee = WignerBatchHIP1D(inst, V=QUARTIC_V, K="0.5*P*P", dt=0.01, param_names=("a2", "a3", "a4"),
                              diff_V=QUARTIC_DV, diff_K="P", precision="fp64")
-    judge.prepare_random_collision(dbeta=0.01)
+    referee.prepare_random_collision(dbeta=0.01)
     dXdP = (16. / 256) ** 2
     phys = [i for i in range(held_out)
-            if 2 * np.pi * np.sum(judge.get_wignerfunction(i) ** 2) * dXdP <= 1.0 + 1e-3]
+            if 2 * np.pi * np.sum(referee.get_wignerfunction(i) ** 2) * dXdP <= 1.0 + 1e-3]
     W0 = np.stack([gaussian_state(256, 8., d["x0"], d["p0"]) for d in inst])
     for i in range(held_out):
-        judge.set_wignerfunction(i, W0[i])
+        referee.set_wignerfunction(i, W0[i])
     t0 = time.perf_counter()
-    judge.propagate(20, record=False)
-    t_judge = time.perf_counter() - t0
-    WJ = np.stack([judge.get_wignerfunction(i) for i in phys])
+    referee.propagate(20, record=False)
+    t_referee = time.perf_counter() - t0
+    WJ = np.stack([referee.get_wignerfunction(i) for i in phys])
     feat = np.array([[inst[i]["params"][0], inst[i]["params"][1], inst[i]["params"][2], inst[i]["gamma"],
                       inst[i]["kT"]] for i in phys])
     W0p = W0[phys] / (W0[phys].sum(axis=(1, 2), keepdims=True) * dXdP)
@@ -297,7 +297,7 @@ def surrogate_experiment(n_pairs=20000, pairs_per_instance=20, held_out=512, epo
         del c
     else:
         Xin = W0p
-    _, ok, err, drift = judged_jump(s, Xin, feat, WJ, dXdP)
+    _, ok, err, drift = checked_jump(s, Xin, feat, WJ, dXdP)
     rel = lambda A: np.linalg.norm((A - WJ).reshape(len(phys), -1), axis=1) / np.linalg.norm(
         WJ.reshape(len(phys), -1), axis=1)
     ident = rel(W0p)
@@ -313,7 +313,7 @@ def surrogate_experiment(n_pairs=20000, pairs_per_instance=20, held_out=512, epo
                base_accepted=int(np.sum(base_err <= 1e-3)), norm_drift_max=float(np.max(drift)),
                accepted=int(ok.sum()), fallback=int((~ok).sum()),
                infer_us_per_batch=round(us, 1), base_us_per_batch=round(base_us, 1), value=rate,
-               unit="grid-point*steps/s", judge_fp64_s_for_held_out=round(t_judge, 3))
+               unit="grid-point*steps/s", referee_fp64_s_for_held_out=round(t_referee, 3))
     say("[fno] held-out: rel L2 median %.3e (base method %.3e, identity %.3e), accepted %d of %d; %.3e pt*steps/s"
         % (out["rel_l2_median"], out["base_rel_l2_median"], out["identity_rel_l2_mean"], out["accepted"],
            len(phys), rate))
@@ -324,7 +324,7 @@ if __name__ == "__main__":
     import argparse
     import json
     import sys
-    ap = argparse.ArgumentParser(description="train and judge the FNO surrogate on propagator trajectories")
+    ap = argparse.ArgumentParser(description="train and referee the FNO surrogate on propagator trajectories")
     ap.add_argument("--pairs", type=int, default=20000)
     ap.add_argument("--held-out", type=int, default=512)
     ap.add_argument("--epochs", type=float, default=2.0)

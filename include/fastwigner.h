@@ -330,7 +330,7 @@ int fw_batch_copy_states(fw_batch* dst, fw_batch* src);
 /* ------------------------------------------------------------------------ */
 /* Fourier-neural-operator surrogate of k propagator steps                   */
 /* No counterpart in the reference. Its network is trained on trajectories  */
-/* of the batched propagator and judged by the plain propagator (the        */
+/* of the batched propagator and checked by the plain propagator (the        */
 /* Moyal / random-collision step of wigner_moyal_cuda_1d.py:496-540 and     */
 /* wigner_random_collision_model_fftw_1d.py:56-76).                          */
 /* ------------------------------------------------------------------------ */

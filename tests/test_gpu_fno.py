@@ -1,6 +1,6 @@
 """FNO surrogate kernels against the numpy oracle (oracle/fno_oracle.py, which
 rounds to bf16 where the HIP channel-mixing GEMMs do), training on batched
-propagator trajectories, and the judged jump with fallback."""
+propagator trajectories, and the checked jump with fallback."""
 import numpy as np
 import pytest
 
@@ -130,19 +130,19 @@ def test_training_corrects_the_coarse_base(gpu):
     assert np.mean(e1) < 0.8 * np.mean(e0)
 
 
-def test_judged_jump_falls_back_to_the_propagator(gpu):
-    """The judge replaces every prediction whose error or norm drift exceeds the
+def test_checked_jump_falls_back_to_the_propagator(gpu):
+    """The referee replaces every prediction whose error or norm drift exceeds the
     bounds by the propagator's state; accepted ones are within the bounds."""
-    from fastwigner_amd.fno import FNOSurrogate, judged_jump
+    from fastwigner_amd.fno import FNOSurrogate, checked_jump
     from fastwigner_amd.batch import gaussian_state
     s = FNOSurrogate(grid_dim=256, max_batch=8, seed=0)
     rng = np.random.default_rng(5)
     W0 = np.stack([gaussian_state(256, 8., rng.uniform(-2, 2), rng.uniform(-2, 2)) for _ in range(4)])
     feat = rng.uniform(0, 1, (4, 5))
-    judge = W0.copy()
-    judge[2:] *= 1.01          # two samples the identity-like network cannot match to 1e-3
+    referee = W0.copy()
+    referee[2:] *= 1.01          # two samples the identity-like network cannot match to 1e-3
     dXdP = (16. / 256) ** 2
-    out, ok, err, drift = judged_jump(s, W0, feat, judge, dXdP)
+    out, ok, err, drift = checked_jump(s, W0, feat, referee, dXdP)
     assert ok.tolist() == [True, True, False, False]
-    assert np.array_equal(out[2:], judge[2:])
+    assert np.array_equal(out[2:], referee[2:])
     assert np.all(err[:2] <= 1e-3) and np.all(drift[:2] <= 1e-4)
