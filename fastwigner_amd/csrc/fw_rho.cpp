@@ -36,6 +36,7 @@ struct RArgs {
   int flags;
   int nbp;
   int nrows;
+  int pubnb;
   double dbeta, vmin, kmin, t_initial, guard;
 };
 
@@ -284,17 +285,24 @@ int enqueue_chunk(fw_rho* h, bool bloch, int n, bool momentum, const RArgs& b0, 
     void* params[] = {&a, &uv, &uk};
     FWCHK(launch(h, h->f_rows, 256, 256, params));
   }
+  // momentum weights without the purity guard: pass A of step s + 1 publishes
+  // step s's momentum sums (block 0); the chunk's last step publishes its own
+  const bool fold = momentum && !bloch && h->dw && a.guard <= 0;
+  const int nbd = diag_blocks(h->N);
   for (int s = 0; s < n; ++s) {
     a.step = s;
+    a.pubnb = fold && s > 0 ? nbd : 0;
     if (ev) HIPCHK(hipEventRecord((*ev)[4 * s], h->stream));
     FWCHK(args1(h, bloch ? h->f_ba : h->f_a, h->row_nb, h->row_nt, a));
+    a.pubnb = 0;
     if (ev) HIPCHK(hipEventRecord((*ev)[4 * s + 1], h->stream));
     FWCHK(args1(h, bloch ? h->f_bb : h->f_b, h->col_nb, h->col_nt, a));
     if (ev) HIPCHK(hipEventRecord((*ev)[4 * s + 2], h->stream));
     FWCHK(args1(h, bloch ? h->f_bc : h->f_c, h->row_nb, h->row_nt, a));
     if (ev) HIPCHK(hipEventRecord((*ev)[4 * s + 3], h->stream));
     if (momentum) {
-      FWCHK(momentum_sums(h, a));
+      if (fold && s + 1 < n) FWCHK(args1(h, h->f_diag_p, nbd, 256, a));
+      else FWCHK(momentum_sums(h, a));
     }
   }
   return FW_OK;

@@ -51,6 +51,7 @@ struct RArgs {
   int flags;
   int nbp;             // number of partial rows written by pass C
   int nrows;           // phase rows available (chunk)
+  int pubnb;           // > 0: block 0 of pass A publishes step - 1's momentum shares (pubnb blocks)
   double dbeta, vmin, kmin, t_initial, guard;
 };
 
@@ -105,6 +106,24 @@ FWD cpx rowfac(const cpx* __restrict__ u, int s, int j) { return u[(size_t)s * F
 using RL = Line<FW_N, FW_ROW_E, FW_ROW_P, true, FW_N, FW_RHO_TW>;
 using CL = Line<FW_N, FW_COL_E, FW_COL_P, false, FW_N, FW_RHO_TW>;
 
+// The 8 momentum sums of step s from the diagonal pass's shares (8 rows of nb
+// in aux), the work of fw_rho_diag_publish, done by block 0 of the next step's
+// pass A so that the step needs no launch for it.
+template <int NT>
+FWD void publish_shares(const RArgs& a, double* red, int nb, int s) {
+  const double* part = reinterpret_cast<const double*>(a.aux);
+  double v[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < nb; b += NT) {
+#pragma unroll
+    for (int o = 0; o < 8; ++o) v[o] += part[(size_t)o * nb + b];
+  }
+#pragma unroll
+  for (int o = 0; o < 8; ++o) BlockRed<NT>::put(v[o], red, 8, o);
+  __syncthreads();
+  if (threadIdx.x < 8) a.series[(size_t)s * RSER + R_P + threadIdx.x] = BlockRed<NT>::total(red, 8, threadIdx.x);
+  __syncthreads();   // red is reused by the pass
+}
+
 template <bool PASS_A, int FL>
 FWD void rho_row(const RArgs& a) {
   using L = RL;
@@ -114,7 +133,10 @@ FWD void rho_row(const RArgs& a) {
   __shared__ cpx lds[L::LDS];
   __shared__ tw_t twl[STAGE ? TwLds<N>::SIZE : 1];
   __shared__ double red[RSER * 32];
+  static_assert(8 * BlockRed<NT>::RN <= RSER * 32, "reduction buffer");
   if (a.scal[RC_STOP] != 0.0) return;
+  if constexpr (PASS_A)
+    if (a.pubnb > 0 && blockIdx.x == 0) publish_shares<NT>(a, red, a.pubnb, a.step - 1);
   const int tid = threadIdx.x;
   const int t = tid % TPL, q = tid / TPL;
   const int nb = gridDim.x;
