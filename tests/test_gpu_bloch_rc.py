@@ -137,3 +137,29 @@ def test_random_collision_gamma0_is_moyal(gpu):
     # separately compiled kernels (the dissipator set) may round one ulp differently
     a, b = g.propagate(100).get(), m.propagate(100).get()
     assert np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(b))
+
+
+def test_random_collision_driven_split_potential(gpu):
+    """A driven potential V0(X) + X E0 sin(om t), which splits into a table of V0
+    and a per-step field factor, under the dissipator (three-pass steps reading
+    the field values filled per chunk), 300 steps against the oracle."""
+    from fastwigner_amd.random_collision import WignerRandomCollisionHIP1D
+    consts = {"E0": 0.05, "om": 0.3}
+    V, K = "0.5*X*X + 0.05*X*X*X*X + X*E0*sin(om*t)", "0.5*P*P"
+    dV = "X + 0.2*X*X*X + E0*sin(om*t)"
+    n = 128
+    g = WignerRandomCollisionHIP1D(X_gridDIM=n, X_amplitude=8., P_gridDIM=n, P_amplitude=8., dt=0.01, t=0.,
+                                   gamma=0.05, kT=0.5, dbeta=0.01, V=V, K=K, diff_V=dV, diff_K="P", **consts)
+    o = fo.RandomCollisionOracle(n, 8., n, 8., 0.01, gamma=0.05, kT=0.5, dbeta=0.01, V=cexpr.make(V, consts),
+                                 K=cexpr.make(K, args=("P", "t")), diff_V=cexpr.make(dV, consts),
+                                 diff_K=cexpr.make("P", args=("P", "t")), phase_time_shift=0.0)
+    assert rel_l2(g.gibbs_state, o.gibbs) <= 1e-12
+    W0 = np.exp(-(o.X - 1.) ** 2 - (o.P - 0.5) ** 2)
+    g.set_wignerfunction(W0)
+    o.set_wignerfunction(W0)
+    Wg = g.propagate(300).get()
+    o.propagate(300)
+    assert rel_l2(Wg, o.W) <= 1e-12
+    for name in ("X_average", "P_average", "P_average_RHS", "hamiltonian_average"):
+        ref = np.array([s[name] for s in o.series])
+        assert np.max(np.abs(np.array(getattr(g, name)) - ref)) <= 1e-11, name
