@@ -105,6 +105,7 @@ struct fw_wigner {
 
   hipGraphExec_t gexec = nullptr;
   int gkind = -1;
+  bool gdissip = false;   // the three-pass graph was captured with the dissipator kernels
   hipGraphExec_t gexec2 = nullptr;   // two-pass Moyal steps
   int last_passes = 3;
 
@@ -195,7 +196,11 @@ std::string trim_ws(const std::string& t) {
 }
 
 // The additive terms at the top level of an expression as (sign, text); false
-// when the scan cannot read it (unbalanced parentheses, an empty term).
+// when the scan cannot read it (unbalanced parentheses or brackets, an empty
+// term) or when the top level holds an operator that binds more loosely than
+// + and - (?:, comparisons, shifts, bitwise and logical operators, assignment,
+// the comma): in "c ? a : b + X*F(t)" the last + belongs to one branch only,
+// so the top-level terms would not add up to the expression.
 bool top_terms(const char* expr, std::vector<std::pair<char, std::string>>& terms) {
   const std::string v(expr);
   auto ident = [](char c) { return isalnum((unsigned char)c) || c == '_' || c == '.'; };
@@ -205,9 +210,10 @@ bool top_terms(const char* expr, std::vector<std::pair<char, std::string>>& term
   terms.clear();
   for (size_t i = 0; i < v.size(); ++i) {
     const char ch = v[i];
-    if (ch == '(') ++depth;
-    if (ch == ')') --depth;
+    if (ch == '(' || ch == '[') ++depth;
+    if (ch == ')' || ch == ']') --depth;
     if (depth < 0) return false;
+    if (depth == 0 && std::strchr("?:<>=!&|^,;", ch)) return false;
     if (depth == 0 && (ch == '+' || ch == '-')) {
       // binary only after an operand; not the sign of a numeric exponent (1e-5)
       size_t j = i;
@@ -514,7 +520,7 @@ int enqueue_step(fw_wigner* h, int kind, int s, const KArgs& base, hipEvent_t* e
 }
 
 int ensure_graph(fw_wigner* h, int kind, const KArgs& base) {
-  if (h->gexec && h->gkind == kind) return FW_OK;
+  if (h->gexec && h->gkind == kind && h->gdissip == h->dissip) return FW_OK;
   if (h->gexec) {
     (void)hipGraphExecDestroy(h->gexec);
     h->gexec = nullptr;
@@ -530,6 +536,7 @@ int ensure_graph(fw_wigner* h, int kind, const KArgs& base) {
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) return fw::fail(FW_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
   h->gkind = kind;
+  h->gdissip = h->dissip;
   return FW_OK;
 }
 
@@ -1077,6 +1084,11 @@ extern "C" int fw_wigner_bloch_propagate(fw_wigner* h, double dbeta, int steps, 
 extern "C" int fw_wigner_set_dissipator(fw_wigner* h, const double* host_w0s, double w_coeff) {
   if (!h) return fw::fail(FW_ERR_INVALID_ARG, "NULL handle");
   if (!host_w0s) {
+    // the cached three-pass graph holds the dissipative kernels and the old coefficient
+    if (h->gexec) {
+      (void)hipGraphExecDestroy(h->gexec);
+      h->gexec = nullptr;
+    }
     h->dissip = false;
     h->w_coeff = 1.0;
     return FW_OK;

@@ -147,7 +147,11 @@ def _prelude(V, consts=None, diff_V=None, **kw):
     ("-1. / sqrt(X * X + 1.) + X * E(t)", True), ("0.5*X*X + X*E0*sin(om*t)", True),
     ("E(t)*X - 0.5*X", True), ("1e-5*X - X*sin(t)", True),
     ("0.5*X*X*(1. + 0.05*sin(0.3*t))", False), ("X*X*E(t)", False), ("0.5*X*X", False),
-    ("X*sin(t)/2", False), ("-X*sin(t)", False), ("X*sin(t) + E(t)", False)])
+    ("X*sin(t)/2", False), ("-X*sin(t)", False), ("X*sin(t) + E(t)", False),
+    # operators that bind more loosely than + and - at the top level: never split
+    ("X > 0. ? 0.5*X*X : 0.1*X*X + X*E(t)", False), ("(X > 0.) + X*E(t)", True),
+    ("X < 1. || t > 2. ? 0.5*X*X + X*E(t) : 0.", False), ("0.5*X*X + X*E(t) == 0.", False),
+    ("0.5*X*X + X*E(t), 0.", False), ("0.5*X*X + X*E(t) && 1", False)])
 def test_field_split_of_time_dependent_potentials(V, split):
     """V = V0(X) + X F(t) at the top level of the expression gets a table of V0
     and a per-step field factor (FW_V_SPLIT); every other time-dependent form
@@ -175,7 +179,8 @@ def test_field_split_needs_tables():
 
 
 @pytest.mark.parametrize("dV,split", [("X / pow(X * X + 1., 1.5) + E(t)", True), ("X - 2.*E0*sin(om*t)", True),
-                                      ("X*(1. + E(t))", False), ("X + X*E(t)", False), ("X", False)])
+                                      ("X*(1. + E(t))", False), ("X + X*E(t)", False), ("X", False),
+                                      ("X > 0. ? X : 0.2*X + E(t)", False), ("X != 0. ? X + E(t) : E(t)", False)])
 def test_field_split_of_the_force(dV, split):
     """With a split V, V' = dV0(X) + G(t) (time-dependent terms free of X) is
     split too (FW_DV_SPLIT) and the parts add up to V'."""

@@ -163,3 +163,34 @@ def test_random_collision_driven_split_potential(gpu):
     for name in ("X_average", "P_average", "P_average_RHS", "hamiltonian_average"):
         ref = np.array([s[name] for s in o.series])
         assert np.max(np.abs(np.array(getattr(g, name)) - ref)) <= 1e-11, name
+
+
+def test_dissipator_cleared_between_graph_runs(gpu):
+    """Three-pass steps with the dissipator for a full graph chunk (64 steps),
+    then fw_wigner_set_dissipator(NULL) and 64 more steps: the second run must
+    be plain Moyal (the graph captured with the dissipative kernels is not
+    replayed), against the oracle doing the same."""
+    from fastwigner_amd import _lib
+    from fastwigner_amd.wigner_moyal import WignerMoyalHIP1D
+    n, gamma = 128, 0.05
+    kw = dict(V="0.5*X*X + 0.05*X*X*X*X", K="0.5*P*P")
+    g = WignerMoyalHIP1D(X_gridDIM=n, X_amplitude=8., P_gridDIM=n, P_amplitude=8., dt=0.01, t=0.,
+                         phase_time_shift=0.0, step_passes=3, **kw)
+    o = fo.RandomCollisionOracle(n, 8., n, 8., 0.01, gamma=gamma, kT=0.5, dbeta=0.01, phase_time_shift=0.0,
+                                 **{k: cexpr.make(v, args=(("X", "t") if k == "V" else ("P", "t")))
+                                    for k, v in kw.items()})
+    w0s = np.ascontiguousarray(o.scaled_gibbs_state)
+    _lib.call("fw_wigner_set_dissipator", g._handle, _lib.dptr(w0s), float(o.w_coeff))
+    W0 = np.exp(-(o.X - 1.) ** 2 - (o.P - 0.5) ** 2)
+    g.set_wignerfunction(W0)
+    o.set_wignerfunction(W0)
+    g.propagate(64)
+    o.propagate(64)
+    assert rel_l2(g.wignerfunction.get(), o.W) <= 1e-12
+    _lib.call("fw_wigner_set_dissipator", g._handle, None, 1.0)
+    m = fo.MoyalOracle(n, 8., n, 8., 0.01, phase_time_shift=0.0, t=o.t,
+                       **{k: cexpr.make(v, args=(("X", "t") if k == "V" else ("P", "t"))) for k, v in kw.items()})
+    m.set_wignerfunction(o.W)
+    Wg = g.propagate(64).get()
+    m.propagate(64)
+    assert rel_l2(Wg, m.W) <= 1e-12

@@ -232,3 +232,204 @@ def test_planted_mistakes_are_caught(monkeypatch):
     # transposed kinetic operand (lambda and p swapped)
     monkeypatch.setattr(fo.MoyalOracle, "expK", lambda self, t: good_k(self, t) * np.exp(1e-2j * self.Lambda))
     assert run() > 1e-3
+
+
+# --------------------------------------------------------------------------- observables in closed form
+def leapfrog_moments(o, n, om2, x0=1.0, p0=0.5, var=0.5):
+    """Mean and covariance of the Gaussian under n exact leapfrog (kick-drift-kick) maps."""
+    a = 0.5 * o.dt * om2
+    kick = np.array([[1, 0], [-a, 1]])
+    M = kick @ np.array([[1, o.dt], [0, 1]]) @ kick
+    Mn = np.linalg.matrix_power(M, n)
+    return Mn @ np.array([x0, p0]), Mn @ (var * np.eye(2)) @ Mn.T
+
+
+def harmonic_run(n=400, om2=1.3):
+    o = fo.MoyalOracle(256, 8., 256, 8., 0.01, V=lambda X, t: 0.5 * om2 * X * X, K=lambda P, t: 0.5 * P * P,
+                       diff_V=lambda X, t: om2 * X, diff_K=lambda P, t: P)
+    o.set_wignerfunction(np.exp(-(o.X - 1) ** 2 - (o.P - 0.5) ** 2))
+    o.propagate(n)
+    return o
+
+
+def test_moyal_observables_of_the_leapfrog_gaussian():
+    """Every per-step observable of the harmonic run against the moments of the
+    closed-form Gaussian (get_Ehrenfest, get_purity, get_sigma_x_sigma_p,
+    wigner_moyal_cuda_1d.py:398-415,542-562): <x>, <p>, <x^2>, <p^2>, <K'> = <p>,
+    -<V'> = -om2 <x>, <H> = (<p^2> + om2 <x^2>)/2, purity 1 (pure state) and
+    sigma_x sigma_p = sqrt(S_xx S_pp) of the covariance M^n (I/2) M^nT."""
+    om2, n = 1.3, 400
+    o = harmonic_run(n, om2)
+    mu, S = leapfrog_moments(o, n, om2)
+    s = o.series[-1]
+    sm = o.sums(o.t)
+    x2, p2 = S[0, 0] + mu[0] ** 2, S[1, 1] + mu[1] ** 2
+    assert abs(s["X_average"] - mu[0]) < 1e-12
+    assert abs(s["P_average"] - mu[1]) < 1e-12
+    assert abs(sm[3] / sm[0] - x2) < 2e-12 * x2
+    assert abs(sm[4] / sm[0] - p2) < 2e-12 * p2
+    assert abs(s["X_average_RHS"] - mu[1]) < 1e-12
+    assert abs(s["P_average_RHS"] + om2 * mu[0]) < 1e-12
+    assert abs(s["hamiltonian_average"] - 0.5 * (p2 + om2 * x2)) < 2e-12
+    assert abs(s["purity"] - 1.0) < 1e-12
+    sxsp = np.sqrt(S[0, 0] * S[1, 1])
+    assert abs(s["sigma_x_sigma_p"] - sxsp) < 5e-12 * sxsp
+    # get_average of the unnormalised observable string's role: <x> and <x p>
+    assert abs(o.get_average(lambda X, P, t: X) - mu[0]) < 1e-12
+    assert abs(o.get_average(lambda X, P, t: X * P) - (S[0, 1] + mu[0] * mu[1])) < 2e-12
+
+
+@pytest.mark.parametrize("conv,kT_eff", [("cuda", 0.5), ("fftw", 0.25)])
+def test_gibbs_sigma_and_purity_closed_form(conv, kT_eff):
+    """Harmonic Gibbs state: sigma_x sigma_p = coth(om / 2kT)/2 and purity
+    tanh(om / 2kT) (om = 1), with the FFTW convention at kT/2; to the O(dbeta^2)
+    splitting error."""
+    b = fo.BlochOracle(256, 10., 256, 10., 0.01, kT=0.5, dbeta=0.005, convention=conv,
+                       V=lambda X, t: 0.5 * X * X, K=lambda P, t: 0.5 * P * P)
+    b.get_gibbs_state()
+    ob = b.observables(b.sums(b.t))
+    assert abs(ob["sigma_x_sigma_p"] - 0.5 / np.tanh(0.5 / kT_eff)) < 1e-6
+    assert abs(b.get_purity() - np.tanh(0.5 / kT_eff)) < 1e-6
+    assert abs(ob["purity"] - np.tanh(0.5 / kT_eff)) < 1e-6
+
+
+def test_random_collision_ehrenfest_with_dissipation():
+    """d<x>/dt and d<p>/dt of the dissipative run equal the amended right-hand
+    sides <K'> + gamma(<x>_G - <x>) and -<V'> + gamma(<p>_G - <p>)
+    (wigner_random_collision_model_fftw_1d.py:78-94), to the O(dt^2) of the
+    finite differences; without the gamma terms they differ by ~1e-1."""
+    rc = fo.RandomCollisionOracle(128, 8., 128, 8., 0.01, gamma=0.08, kT=0.5,
+                                  V=lambda X, t: 0.5 * X * X + 0.05 * X ** 4, K=lambda P, t: 0.5 * P * P,
+                                  diff_V=lambda X, t: X + 0.2 * X ** 3, diff_K=lambda P, t: P)
+    rc.set_wignerfunction(np.exp(-(rc.X - 1) ** 2 - (rc.P - 0.5) ** 2))
+    rc.propagate(300)
+    ser = {k: np.array([s[k] for s in rc.series]) for k in ("X_average", "X_average_RHS", "P_average",
+                                                            "P_average_RHS")}
+    dx = np.gradient(ser["X_average"], rc.dt)[2:-2]
+    dp = np.gradient(ser["P_average"], rc.dt)[2:-2]
+    assert np.max(np.abs(dx - ser["X_average_RHS"][2:-2])) < 1e-4
+    assert np.max(np.abs(dp - ser["P_average_RHS"][2:-2])) < 1e-3
+    assert np.max(np.abs(dx - ser["P_average"][2:-2])) > 1e-2
+
+
+@pytest.mark.parametrize("c", [0.0, 0.3])
+def test_rho_second_moments_closed_form(c):
+    """Chirped Gaussian psi = exp(-(x-x0)^2/(2 s^2) + i c (x-x0)^2 + i p0 x):
+    <x^2> = x0^2 + s^2/2, <p^2> = p0^2 + 1/(2 s^2) + 2 c^2 s^2 and
+    sigma_x sigma_p = sqrt(1 + 4 c^2 s^4)/2, through get_average tuples and
+    get_sigma_x_sigma_p (rho_vneumann_cuda_1d.py:431-500)."""
+    r = fo.RhoOracle(256, 10., 0.01, V=lambda X, t: 0.5 * X * X, K=lambda P, t: 0.5 * P * P)
+    x0, p0, s = 1.0, 0.5, 1.0
+    psi = np.exp(-(r.Xv - x0) ** 2 / (2 * s * s) + 1j * c * (r.Xv - x0) ** 2 + 1j * p0 * r.Xv)
+    r.set_rho(np.outer(psi.conj(), psi))
+    assert abs(r.get_average((lambda X, t: X,)) - x0) < 1e-12
+    assert abs(r.get_average((None, lambda P, t: P)) - p0) < 1e-12
+    assert abs(r.get_average((lambda X, t: X * X,)) - (x0 ** 2 + s * s / 2)) < 1e-12
+    assert abs(r.get_average((None, lambda P, t: P * P)) - (p0 ** 2 + 1 / (2 * s * s) + 2 * c * c * s * s)) < 1e-12
+    assert abs(r.get_sigma_x_sigma_p() - 0.5 * np.sqrt(1 + 4 * c * c * s ** 4)) < 1e-12
+
+
+def wigner_grid(r):
+    """The Wigner grid of the density-matrix classes (rho_vneumann_cuda_1d.py:103-110):
+    x_j = dX (j - N/2)/sqrt(2) along columns, p_i = dP (i - N/2)/sqrt(2) along rows."""
+    j = np.arange(r.N)
+    return ((r.dX * (j - 0.5 * r.N) / np.sqrt(2.0))[np.newaxis, :],
+            (r.dP * (j - 0.5 * r.N) / np.sqrt(2.0))[:, np.newaxis])
+
+
+def test_wigner_time_on_the_grid():
+    """get_wigner_time's indicator 0.5 (1 + signbit(H + Ip)) (rho_vneumann_cuda_1d.py:594-620,854)
+    exactly on the grid: 1 for a state well inside H < -Ip, the reference's 0.5
+    for one well outside, 0 when W = W_init, 0.5 everywhere for Ip > 0 (H >= 0)."""
+    r = fo.RhoOracle(256, 10., 0.01, V=lambda X, t: 0.5 * X * X, K=lambda P, t: 0.5 * P * P)
+    X, P = wigner_grid(r)
+    K, V = (lambda P, t: 0.5 * P * P), (lambda X, t: 0.5 * X * X)
+    inside = np.exp(-(X * X + P * P) / 0.18)
+    outside = np.exp(-((X - 6.0) ** 2 + P * P) / 0.18)
+    tot_in, tot_out = inside.sum() * r.wigner_dxdp, outside.sum() * r.wigner_dxdp
+    assert abs(r.get_wigner_time(inside, 0 * inside, 0.0, K, V, -8.0) - tot_in) < 1e-12 * tot_in
+    assert abs(r.get_wigner_time(outside, 0 * outside, 0.0, K, V, -1.0) - 0.5 * tot_out) < 1e-12 * tot_out
+    assert r.get_wigner_time(outside, outside, 0.0, K, V, -1.0) == 0.0
+    assert abs(r.get_wigner_time(inside, 0 * inside, 0.0, K, V, 0.5) - 0.5 * tot_in) < 1e-12 * tot_in
+
+
+def wigner_time_error(N, A, om=2.0):
+    """|get_wigner_time - (1 - exp(2 Ip/om)/2)| for W = exp(-(om x^2 + p^2/om))/pi,
+    W_init = 0, H = (p^2 + om^2 x^2)/2, Ip < 0: the integral of W over H < -Ip is
+    1 - exp(2 Ip/om) and the reference's indicator adds half the rest."""
+    r = fo.RhoOracle(N, A, 0.01, V=lambda X, t: 0.5 * om * om * X * X, K=lambda P, t: 0.5 * P * P)
+    X, P = wigner_grid(r)
+    W = np.exp(-(om * X * X + P * P / om)) / np.pi
+    errs = []
+    for Ip in (-0.5, -1.0, -2.0):
+        v = r.get_wigner_time(W, 0 * W, 0.0, lambda P, t: 0.5 * P * P, lambda X, t: 0.5 * om * om * X * X, Ip)
+        errs.append(abs(v - (1.0 - 0.5 * np.exp(2.0 * Ip / om))))
+    return max(errs)
+
+
+def test_wigner_time_closed_form_and_refinement():
+    """The closed form above to the quadrature error of the indicator's edge,
+    which shrinks as both grid spacings are refined (x_amp grows as sqrt(N));
+    a transposed state or a dropped 1/2 misses by 3e-2 or more."""
+    e256 = wigner_time_error(256, 10.)
+    e1024 = wigner_time_error(1024, 20.)
+    assert e256 < 5e-3
+    assert e1024 < 1.5e-3
+    assert e1024 < 0.5 * e256
+
+
+def test_planted_observable_mistakes_are_caught(monkeypatch):
+    """Slips planted in the oracle's observables fail the closed-form checks above."""
+    good_obs, good_sums = fo.MoyalOracle.observables, fo.MoyalOracle.sums
+
+    def sxsp_err():
+        om2, n = 1.3, 50
+        o = harmonic_run(n, om2)
+        mu, S = leapfrog_moments(o, n, om2)
+        return abs(o.series[-1]["sigma_x_sigma_p"] - np.sqrt(S[0, 0] * S[1, 1]))
+
+    assert sxsp_err() < 1e-11
+    # <p^2> sum weighted by P instead of P^2
+    def bad_sums(self, t=None):
+        s = good_sums(self, t)
+        s[4] = s[2]
+        return s
+    monkeypatch.setattr(fo.MoyalOracle, "sums", bad_sums)
+    assert not sxsp_err() < 1e-3          # a wrong value or a NaN both fail the bar
+    monkeypatch.setattr(fo.MoyalOracle, "sums", good_sums)
+
+    # variance without the squared mean
+    def bad_obs(self, s):
+        o = good_obs(self, s)
+        o["sigma_x_sigma_p"] = np.sqrt((s[3] / s[0]) * (s[4] / s[0]))
+        return o
+    monkeypatch.setattr(fo.MoyalOracle, "observables", bad_obs)
+    assert not sxsp_err() < 1e-3
+    monkeypatch.setattr(fo.MoyalOracle, "observables", good_obs)
+
+    # Wigner time: indicator 0 outside the region (the reference keeps 0.5), and
+    # the Wigner grid without its 1/sqrt(2)
+    good_wt = fo.RhoOracle.get_wigner_time
+
+    def no_quirk(self, wc, wi, t, K, V, Ip):
+        return good_wt(self, wc, wi, t, K, V, Ip) - 0.5 * good_wt(self, wc, wi, t, K, V, 1.0)
+    monkeypatch.setattr(fo.RhoOracle, "get_wigner_time", no_quirk)
+    assert wigner_time_error(256, 10.) > 3e-2
+    monkeypatch.setattr(fo.RhoOracle, "get_wigner_time", good_wt)
+
+    def no_sqrt2(self, wc, wi, t, K, V, Ip):
+        return good_wt(self, wc, wi, t, lambda P, t: K(P * np.sqrt(2.0), t), lambda X, t: V(X * np.sqrt(2.0), t), Ip)
+    monkeypatch.setattr(fo.RhoOracle, "get_wigner_time", no_sqrt2)
+    assert wigner_time_error(256, 10.) > 3e-2
+    monkeypatch.setattr(fo.RhoOracle, "get_wigner_time", good_wt)
+    # a second-moment slip of the density matrix: <p^2> through one FFT pair per P
+    good_avg = fo.RhoOracle.get_average
+
+    def bad_avg(self, obs):
+        if len(obs) == 2 and obs[0] is None:
+            g = obs[1]
+            return good_avg(self, (None, lambda P, t: g(P, t) / np.maximum(np.abs(P), 1e-300) * P))
+        return good_avg(self, obs)
+    monkeypatch.setattr(fo.RhoOracle, "get_average", bad_avg)
+    with pytest.raises(AssertionError):
+        test_rho_second_moments_closed_form(0.3)
