@@ -164,6 +164,26 @@ def timed_rate(s, n, steps, warmup):
     return n * n * steps / dt, dt * 1e3 / steps
 
 
+def host_info(workers):
+    """What the CPU baseline ran on: logical CPUs the OS reports, the CPU model,
+    the threads used and the library versions."""
+    import platform
+    import scipy
+    model = platform.processor() or ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model, "threads_used": workers,
+            "threads_note": "the host share of a one-GPU box is 16 threads; scipy.fft takes them "
+                            "(workers=), numpy's elementwise products run on one",
+            "numpy": np.__version__, "scipy": scipy.__version__, "python": platform.python_version()}
+
+
 def oracle_sample(n, max_seconds, max_steps, workers):
     """The oracle on the host cores: steps until max_seconds (bounded sample)."""
     import scipy  # noqa: F401
@@ -287,10 +307,13 @@ def main():
     n = args.n
     print("[bench] compiling and initialising %dx%d fp64" % (n, n), file=sys.stderr, flush=True)
     s = make(n, "fp64")
-    rate, ms = timed_rate(s, n, args.steps, args.warmup)
-    print("[bench] fp64 %dx%d: %.3e pt*steps/s, %.2f us/step" % (n, n, rate, ms * 1e3), file=sys.stderr, flush=True)
+    # the roofline's kernel timing first (640 steps of graph replays, the state restored
+    # afterwards): it also brings the GPU out of idle clocks before the warm-up and the
+    # timed steps, which are then the usual W + K steps of the propagator
     kt = kernel_times(s)
     rl = roofline(n, "fp64", kt)
+    rate, ms = timed_rate(s, n, args.steps, args.warmup)
+    print("[bench] fp64 %dx%d: %.3e pt*steps/s, %.2f us/step" % (n, n, rate, ms * 1e3), file=sys.stderr, flush=True)
 
     result = {
         "metric": "Wigner grid-point*steps/s at 1024^2 (plain FP64 Moyal step)",
@@ -304,13 +327,15 @@ def main():
     }
 
     if not args.no_baseline:
+        # the host's CPU share of a one-GPU box is 16 threads, whatever os.cpu_count() reports
         workers = min(os.cpu_count() or 1, 16)
         print("[bench] oracle sample on %d host threads" % workers, file=sys.stderr, flush=True)
         o, osteps, oel = oracle_sample(n, 15.0, 400, workers)
         result["cpu_baseline"] = {"value": n * n * osteps / oel, "unit": "grid-point*steps/s", "cores": workers,
                                   "kind": "port",
                                   "sample": "%d steps of the same %dx%d FP64 workload with the scipy.fft oracle "
-                                            "(oracle/fw_oracle.py), %.1f s" % (osteps, n, n, oel)}
+                                            "(oracle/fw_oracle.py), %.1f s" % (osteps, n, n, oel),
+                                  "host": host_info(workers)}
         _, osteps1, oel1 = oracle_sample(n, 6.0, 200, 1)
         result["cpu_baseline"]["single_thread"] = {
             "value": n * n * osteps1 / oel1, "unit": "grid-point*steps/s", "cores": 1,

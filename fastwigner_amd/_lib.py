@@ -53,6 +53,7 @@ class WignerDesc(ctypes.Structure):
         ("col_e", ctypes.c_int), ("col_p", ctypes.c_int),
         ("row_e", ctypes.c_int), ("row_p", ctypes.c_int),
         ("phase_tables", ctypes.c_int), ("step_passes", ctypes.c_int),
+        ("v_table", ctypes.c_int),
     ]
 
 
@@ -104,12 +105,19 @@ class FnoDesc(ctypes.Structure):
                 ("x_amplitude", ctypes.c_double), ("p_amplitude", ctypes.c_double)]
 
 
+def uses_token(expr, tok):
+    """Whether the C expression references the identifier token ``tok`` (the
+    library's own test for a time-dependent string, fw_wigner.cpp uses_token)."""
+    import re
+    return bool(expr) and re.search(r"(?<![A-Za-z0-9_])%s(?![A-Za-z0-9_])" % re.escape(tok), expr) is not None
+
+
 def _b(s):
     return None if s is None else s.encode()
 
 
 def wigner_desc(nx, xamp, npp, pamp, dt, shift=0.5, precision=0, device=0, tiles=None, phase_tables=0,
-                step_passes=0):
+                step_passes=0, v_table=0):
     d = WignerDesc()
     d.x_grid_dim, d.p_grid_dim = int(nx), int(npp)
     d.x_amplitude, d.p_amplitude = float(xamp), float(pamp)
@@ -119,6 +127,7 @@ def wigner_desc(nx, xamp, npp, pamp, dt, shift=0.5, precision=0, device=0, tiles
         d.col_e, d.col_p, d.row_e, d.row_p = [int(v) for v in tiles]
     d.phase_tables = int(phase_tables)
     d.step_passes = int(step_passes)
+    d.v_table = int(v_table)
     return d
 
 

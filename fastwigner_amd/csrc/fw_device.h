@@ -345,7 +345,7 @@ FWD void v_field_phase(int t, double F, cpx2 (&ftab)[FW_COL_E / 2 + 1]) {
 }
 
 // Field values of split potentials, filled per chunk next to the step times
-// (fw_field): F(t_s + shift dt) for the factors, F(t_s) for the sums.
+// (fw_begin): F(t_s + shift dt) for the factors, F(t_s) for the sums.
 #if FW_V_SPLIT
 FWD double field_tau(const Args& a, int s) { return a.times[FW_TCHUNK + s]; }
 FWD double field_now(const Args& a, int s) { return a.times[2 * FW_TCHUNK + s]; }
@@ -435,6 +435,9 @@ struct KMul {
 // ---------------------------------------------------------------------------
 #ifndef FW_THETA_SUMS
 #define FW_THETA_SUMS 0
+#endif
+#ifndef FW_V_STATIC
+#define FW_V_STATIC 0
 #endif
 struct VVMul {
   const cpx2 (&f0)[FW_COL_E / 2 + 1];   // closing V/2 (step s)
@@ -632,7 +635,8 @@ FWD void vvpass(const Args& a) {
   __shared__ cpx lds[L::LDS];
   __shared__ tw_t twl[L::TW::SIZE];
   __shared__ double red[NSER * BlockRed<NT>::RN];
-  if (a.scal[SC_STOP] != 0.0) return;
+  // two-pass steps are Moyal steps: the stop flag (purity guard of Bloch runs)
+  // is never raised, and not reading it lets the W loads issue at once
   const int tid = threadIdx.x;
   const int q = tid % P, t = tid / P;
   const int nb = gridDim.x;
@@ -718,7 +722,8 @@ FWD void vvpass_theta(const Args& a) {
   __shared__ cpx lds[L::LDS];
   __shared__ tw_t twl[L::TW::SIZE];
   __shared__ double red[NSER * BlockRed<NT>::RN];
-  if (a.scal[SC_STOP] != 0.0) return;
+  // two-pass steps are Moyal steps: the stop flag (purity guard of Bloch runs)
+  // is never raised, and not reading it lets the W loads issue at once
   const int tid = threadIdx.x;
   const int q = tid % P, t = tid / P;
   const int nb = gridDim.x;
@@ -738,6 +743,10 @@ FWD void vvpass_theta(const Args& a) {
   if constexpr (use_tab) {
     // split potentials: the multiply phases the one table per half-step
     v_factors(a, c, t, f0);
+#pragma unroll
+    for (int i = 0; i <= E / 2; ++i) f1[i] = f0[i];
+  } else if constexpr (FW_V_STATIC) {
+    v_factors_at(c, t, t0 + FW_PHASE_SHIFT * ::dt, t0, f0);
 #pragma unroll
     for (int i = 0; i <= E / 2; ++i) f1[i] = f0[i];
   } else {
@@ -835,7 +844,8 @@ FWD void kpass(const Args& a) {
   __shared__ cpx lds[L::LDS];
   __shared__ tw_t twl[L::TW::SIZE];
   __shared__ double red[(FL & F_KNORM) ? BlockRed<NT>::RN : 1];
-  if (a.scal[SC_STOP] != 0.0) return;
+  // F_KNORM: the row pass of two-pass Moyal steps, which never stop early
+  if (!(FL & F_KNORM) && a.scal[SC_STOP] != 0.0) return;
   const int tid = threadIdx.x;
   const int t = tid % TPL, q = tid / TPL;
   const int nb = gridDim.x;
@@ -941,6 +951,30 @@ FWD void moments(const Args& a) {
   }
 }
 
+// Start of a chunk of steps (one small block): the step times
+// times[s] = times[s-1] + inc from times[0] = t0 when n > 0, the same
+// sequence of IEEE additions the caller made (t += dt), so the same bits; with
+// reset, the run flags (no stop, W fresh); then the field values of split
+// potentials for steps s < nf. One launch replaces a host-to-device copy of
+// the times and the flag writes.
+FWD void begin_chunk(Args a, double t0, double inc, int n, int reset, int nf) {
+  if (threadIdx.x == 0) {
+    double* tm = const_cast<double*>(a.times);
+    double tt = t0;
+    for (int s = 0; s < n; ++s) {
+      if (s > 0) tt = tt + inc;
+      tm[s] = tt;
+    }
+    if (reset) {
+      a.scal[SC_STOP] = 0.0;
+      a.scal[SC_FRESH] = 1.0;
+    }
+  }
+  __syncthreads();
+  a.step = nf;
+  field_fill(a);
+}
+
 // W *= norm: the norm is scal[SC_NORM], or, at the end of a run
 // (F_FROM_PARTIALS), that of the last step, whose sums blocks 0..NSER-1
 // publish to series[step] (the grid has at least NSER blocks).
@@ -1042,7 +1076,10 @@ extern "C" __global__ void __launch_bounds__(fwk::VL::NT) fw_vv(fwk::Args a) {
   else fwk::vvpass(a);
 }
 extern "C" __global__ void __launch_bounds__(256) fw_build_sumw(fwk::Args a, fwk::SumW* sw) { fwk::build_sumw(a, sw); }
-extern "C" __global__ void __launch_bounds__(128) fw_field(fwk::Args a) { fwk::field_fill(a); }
+extern "C" __global__ void __launch_bounds__(128) fw_begin(fwk::Args a, double t0, double inc, int n, int reset,
+                                                          int nf) {
+  fwk::begin_chunk(a, t0, inc, n, reset, nf);
+}
 extern "C" __global__ void __launch_bounds__(fwk::HT) fw_moments(fwk::Args a) { fwk::moments(a); }
 extern "C" __global__ void __launch_bounds__(fwk::HT) fw_apply_norm(fwk::Args a) { fwk::apply_norm(a); }
 extern "C" __global__ void fw_set_scalar(double* p, double v) { *p = v; }

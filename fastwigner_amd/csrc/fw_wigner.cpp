@@ -18,7 +18,8 @@ using fw::KArgs;
 
 namespace {
 
-const int kChunk = 64;   // steps per captured graph / per times-series upload
+const int kChunk = 64;        // steps per captured graph
+const int kStageChunks = 32;  // chunks of per-step sums staged on the host before a wait
 
 struct Tiles {
   int col_e, col_p, row_e, row_p;
@@ -101,7 +102,9 @@ struct fw_wigner {
   void* ktab = nullptr;
   int tab_kind = -1;          // what the tables hold: 0 Moyal, 1 Bloch
   double tab_dbeta = 0.0;
-  hipFunction_t f_build_vtab = nullptr, f_build_ktab = nullptr, f_build_sumw = nullptr, f_field = nullptr;
+  hipFunction_t f_build_vtab = nullptr, f_build_ktab = nullptr, f_build_sumw = nullptr, f_begin = nullptr;
+  double* h_stage = nullptr;   // pinned: staged per-step sums (kStageChunks chunks + the closing slot)
+  double* h_stop = nullptr;    // pinned: the purity guard's stop flag
 
   hipGraphExec_t gexec = nullptr;
   int gkind = -1;
@@ -133,6 +136,7 @@ int validate(const fw_wigner_desc* d, const fw_wigner_exprs* x) {
     return fw::fail(FW_ERR_INVALID_ARG, "step_passes must be 0, 2 or 3");
   if (d->phase_tables < -1 || d->phase_tables > 1)
     return fw::fail(FW_ERR_INVALID_ARG, "phase_tables must be -1, 0 or 1");
+  if (d->v_table != 0 && d->v_table != -1) return fw::fail(FW_ERR_INVALID_ARG, "v_table must be 0 or -1");
   if (!x->V || !x->K) return fw::fail(FW_ERR_INVALID_ARG, "V and K expressions are required");
   return FW_OK;
 }
@@ -312,6 +316,7 @@ bool split_offset(const char* expr, std::string& ds, std::string& dg) {
 
 // Constants and spliced expressions, wigner_moyal_cuda_1d.py:151-166,244-277.
 bool tables_v(const fw_wigner_desc* d, const fw_wigner_exprs* x) {
+  if (d->v_table < 0) return false;
   std::string vs, vf;
   const bool t_free = !uses_t(x->V) || split_field(x->V, vs, vf);
   return d->phase_tables > 0 || (d->phase_tables == 0 && t_free && !uses_t(x->abs_boundary_x_theta));
@@ -358,6 +363,9 @@ std::string make_prelude(const fw_wigner_desc* d, const fw_wigner_exprs* x, cons
   s << "const double dTheta = " << fw::fmt_double(dT) << ";\n";
   s << "#define FW_MASK_XP_X_ONLY " << (uses_token(x->abs_boundary_x_p, "P") ? 0 : 1) << "\n";
   s << "#define FW_THETA_SUMS " << (theta_sums(x) ? 1 : 0) << "\n";
+  // V and its x-theta mask free of t: the factors of the two half-steps of the
+  // fused column pass are the same, evaluated once when there is no table
+  s << "#define FW_V_STATIC " << ((!uses_t(x->V) && !uses_t(x->abs_boundary_x_theta)) ? 1 : 0) << "\n";
   // the caller's sin and cos: the branch-free FP64 ones of fw_math.h (see there)
   s << "#include \"fw_math.h\"\n#define sin fw_user_sin\n#define cos fw_user_cos\n";
   s << "// ---- caller constants and functions ----\n";
@@ -449,20 +457,97 @@ int set_scalar(fw_wigner* h, int slot, double v) {
 
 int helper_blocks(fw_wigner* h) { return h->NP < 128 ? h->NP : 128; }
 
-// Upload the times of a chunk's steps (times[s], s < n). For real-time steps
-// (field) a split potential also gets its field values F(t_s + shift dt) and
-// F(t_s) beside them (fw_field, one block), so the passes read one number per
-// half-step instead of evaluating the field in every thread. Bloch steps use
-// the whole V at t_initial and need none.
-int upload_times(fw_wigner* h, const double* t, int n, bool field = true) {
-  HIPCHK(hipMemcpyAsync(h->times, t, sizeof(double) * n, hipMemcpyHostToDevice, h->stream));
-  if (h->vsplit && field) {
-    KArgs a = base_args(h);
-    a.step = n;
-    FWCHK(launch(h, h->f_field, 1, 128, a));
+// Start of a chunk of n steps with times t[0..n). When the times are the
+// progression t[s] = t[s-1] + inc for inc = dt (the propagators' t += dt) or
+// inc = 0 (Bloch runs at t_initial), checked here bit for bit, one small kernel
+// (fw_begin) regenerates them on the device by the same IEEE additions;
+// other times are copied. The same launch resets the run flags (reset) and,
+// for real-time steps of a split potential (field), writes the field values
+// F(t_s + shift dt), F(t_s) beside the times, so the passes read one number
+// per half-step instead of evaluating the field in every thread. Bloch steps
+// use the whole V at t_initial and need none.
+int begin_chunk(fw_wigner* h, const double* t, int n, bool reset, bool field = true) {
+  double inc = 0.0;
+  bool prog = false;
+  for (double c : {h->d.dt, 0.0}) {
+    prog = true;
+    for (int s = 1; s < n && prog; ++s) prog = (t[s] == t[s - 1] + c);
+    if (prog) {
+      inc = c;
+      break;
+    }
   }
+  int ngen = n;
+  if (!prog) {
+    // a pageable source: the copy is staged before hipMemcpyAsync returns
+    HIPCHK(hipMemcpyAsync(h->times, t, sizeof(double) * n, hipMemcpyHostToDevice, h->stream));
+    ngen = 0;
+  }
+  KArgs a = base_args(h);
+  double t0 = n > 0 ? t[0] : 0.0;
+  int rs = reset ? 1 : 0, nf = (h->vsplit && field) ? n : 0;
+  void* params[] = {&a, &t0, &inc, &ngen, &rs, &nf};
+  HIPCHK(hipModuleLaunchKernel(h->f_begin, 1, 1, 1, 128, 1, 1, 0, h->stream, params, nullptr));
   return FW_OK;
 }
+
+// Collects the per-step sums of a run on the host. The slots of each chunk go
+// device -> pinned staging memory asynchronously, in stream order after the
+// chunk's kernels; the host waits once at the end of the run (or when the
+// staging area is full) and scatters them: slot s of the chunk starting at
+// step off holds the sums of step off + s - 1, and the closing normalisation
+// publishes those of the last step into slot kChunk.
+struct SeriesSink {
+  fw_wigner* h;
+  double* out;
+  struct Rec {
+    int off, n, k;
+  };
+  std::vector<Rec> recs;
+  int last = -1;
+  SeriesSink(fw_wigner* h_, double* out_) : h(h_), out(out_) {}
+  int chunk(int off, int n) {
+    if (!out) return FW_OK;
+    if ((int)recs.size() == kStageChunks) FWCHK(flush());
+    const int k = (int)recs.size();
+    HIPCHK(hipMemcpyAsync(h->h_stage + (size_t)k * (kChunk + 1) * FW_NSER, h->series, sizeof(double) * FW_NSER * n,
+                          hipMemcpyDeviceToHost, h->stream));
+    recs.push_back({off, n, k});
+    return FW_OK;
+  }
+  int final_slot(int step) {
+    if (!out || step < 0) return FW_OK;
+    HIPCHK(hipMemcpyAsync(h->h_stage + (size_t)kStageChunks * (kChunk + 1) * FW_NSER,
+                          h->series + (size_t)kChunk * FW_NSER, sizeof(double) * FW_NSER, hipMemcpyDeviceToHost,
+                          h->stream));
+    last = step;
+    return FW_OK;
+  }
+  void scatter() {
+    for (const Rec& r : recs)
+      for (int s = 0; s < r.n; ++s) {
+        const int g = r.off + s - 1;
+        if (g >= 0)
+          std::memcpy(out + (size_t)g * FW_NSER, h->h_stage + ((size_t)r.k * (kChunk + 1) + s) * FW_NSER,
+                      sizeof(double) * FW_NSER);
+      }
+    recs.clear();
+  }
+  int flush() {
+    if (!out) return FW_OK;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    scatter();
+    return FW_OK;
+  }
+  // after the stream is synchronised
+  void finish() {
+    if (!out) return;
+    scatter();
+    if (last >= 0)
+      std::memcpy(out + (size_t)last * FW_NSER, h->h_stage + (size_t)kStageChunks * (kChunk + 1) * FW_NSER,
+                  sizeof(double) * FW_NSER);
+  }
+};
 
 // Fill the factor tables for a run of `kind` (0 Moyal, 1 Bloch with a.dbeta)
 // and set the table flags in a. Tables are allocated once; contents follow kind.
@@ -549,18 +634,16 @@ int run_steps(fw_wigner* h, int kind, const KArgs& base0, const double* times, i
               int* done) {
   KArgs base = base0;
   FWCHK(ensure_tables(h, kind, base));
-  FWCHK(set_scalar(h, fw::SC_STOP, 0.0));
-  FWCHK(set_scalar(h, fw::SC_FRESH, 1.0));
   const bool use_graph = !h->timing && kind == 0;
   if (use_graph && steps >= kChunk) FWCHK(ensure_graph(h, kind, base));
   std::vector<hipEvent_t> evs;
-  std::vector<double> slots((size_t)FW_NSER * (kChunk + 1));
+  SeriesSink sink(h, series);
   double acc[3] = {0, 0, 0};
   int nacc = 0;
   int applied = steps;
   for (int off = 0; off < steps; off += kChunk) {
     const int n = std::min(kChunk, steps - off);
-    FWCHK(upload_times(h, times + off, n, kind == 0));
+    FWCHK(begin_chunk(h, times + off, n, off == 0, kind == 0));
     if (use_graph && n == kChunk) {
       HIPCHK(hipGraphLaunch(h->gexec, h->stream));
     } else if (h->timing) {
@@ -580,22 +663,18 @@ int run_steps(fw_wigner* h, int kind, const KArgs& base0, const double* times, i
     } else {
       for (int s = 0; s < n; ++s) FWCHK(enqueue_step(h, kind, s, base, nullptr));
     }
-    double stop = 0;
-    if (series || base.guard > 0) {
-      HIPCHK(hipMemcpyAsync(slots.data(), h->series, sizeof(double) * FW_NSER * n, hipMemcpyDeviceToHost,
-                            h->stream));
-      HIPCHK(hipMemcpyAsync(&stop, h->scal + fw::SC_STOP, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    FWCHK(sink.chunk(off, n));
+    if (base.guard > 0) {
+      // the purity guard: whether a step of this chunk tripped it (stop = its step + 1)
+      HIPCHK(hipMemcpyAsync(h->h_stop, h->scal + fw::SC_STOP, sizeof(double), hipMemcpyDeviceToHost, h->stream));
       HIPCHK(hipStreamSynchronize(h->stream));
-      const int ns = stop != 0 ? (int)stop : n;   // slots written in this chunk
-      if (series)
-        for (int s = 0; s < ns; ++s) {
-          const int g = off + s - 1;
-          if (g >= 0) std::memcpy(series + (size_t)g * FW_NSER, &slots[(size_t)s * FW_NSER], sizeof(double) * FW_NSER);
-        }
-    }
-    if (stop != 0) {
-      applied = off + (int)stop - 1;
-      break;
+      const double stop = *h->h_stop;
+      if (stop != 0 && !sink.recs.empty()) sink.recs.back().n = (int)stop;   // slots written in this chunk
+      sink.scatter();
+      if (stop != 0) {
+        applied = off + (int)stop - 1;
+        break;
+      }
     }
   }
   if (nacc)
@@ -605,11 +684,9 @@ int run_steps(fw_wigner* h, int kind, const KArgs& base0, const double* times, i
   a.flags |= fw::F_FROM_PARTIALS;
   a.step = kChunk;
   FWCHK(launch(h, h->f_apply_norm, 1024, 256, a));
-  FWCHK(set_scalar(h, fw::SC_NORM, 1.0));
-  if (series && applied > 0)
-    HIPCHK(hipMemcpyAsync(series + (size_t)(applied - 1) * FW_NSER, h->series + (size_t)kChunk * FW_NSER,
-                          sizeof(double) * FW_NSER, hipMemcpyDeviceToHost, h->stream));
+  FWCHK(sink.final_slot(applied - 1));
   HIPCHK(hipStreamSynchronize(h->stream));
+  sink.finish();
   if (done) *done = applied;
   return FW_OK;
 }
@@ -648,18 +725,16 @@ int ensure_graph_fused(fw_wigner* h, const KArgs& base) {
 int run_steps_fused(fw_wigner* h, const KArgs& base0, const double* times, int steps, double* series) {
   KArgs base = base0;
   FWCHK(ensure_tables(h, 0, base));
-  FWCHK(set_scalar(h, fw::SC_STOP, 0.0));
-  FWCHK(set_scalar(h, fw::SC_FRESH, 1.0));
   const bool timing = h->timing;
   if (!timing && steps > kChunk) FWCHK(ensure_graph_fused(h, base));
-  std::vector<double> slots((size_t)FW_NSER * (kChunk + 1));
+  SeriesSink sink(h, series);
   std::vector<hipEvent_t> evs;
   double acc[2] = {0, 0};
   int nacc = 0;
   for (int off = 0; off < steps; off += kChunk) {
     const int n = std::min(kChunk, steps - off);
     const int nt = std::min(kChunk + 1, steps - off);   // t of steps off .. off+n (the next chunk's first)
-    FWCHK(upload_times(h, times + off, nt));
+    FWCHK(begin_chunk(h, times + off, nt, off == 0));
     if (off == 0) {
       KArgs a = base;
       a.step = 0;
@@ -688,15 +763,7 @@ int run_steps_fused(fw_wigner* h, const KArgs& base0, const double* times, int s
     } else {
       for (int s = 0; s < n; ++s) FWCHK(enqueue_fused(h, s, has_last && s == n - 1, base, nullptr));
     }
-    if (series) {
-      HIPCHK(hipMemcpyAsync(slots.data(), h->series, sizeof(double) * FW_NSER * n, hipMemcpyDeviceToHost,
-                            h->stream));
-      HIPCHK(hipStreamSynchronize(h->stream));
-      for (int s = 0; s < n; ++s) {
-        const int g = off + s - 1;
-        if (g >= 0) std::memcpy(series + (size_t)g * FW_NSER, &slots[(size_t)s * FW_NSER], sizeof(double) * FW_NSER);
-      }
-    }
+    FWCHK(sink.chunk(off, n));
   }
   if (nacc) {
     h->ktimes[0] = acc[0] / nacc;
@@ -707,11 +774,9 @@ int run_steps_fused(fw_wigner* h, const KArgs& base0, const double* times, int s
   a.flags |= fw::F_FROM_PARTIALS;
   a.step = kChunk;
   FWCHK(launch(h, h->f_apply_norm, 1024, 256, a));
-  FWCHK(set_scalar(h, fw::SC_NORM, 1.0));
-  if (series)
-    HIPCHK(hipMemcpyAsync(series + (size_t)(steps - 1) * FW_NSER, h->series + (size_t)kChunk * FW_NSER,
-                          sizeof(double) * FW_NSER, hipMemcpyDeviceToHost, h->stream));
+  FWCHK(sink.final_slot(steps - 1));
   HIPCHK(hipStreamSynchronize(h->stream));
+  sink.finish();
   h->last_passes = 2;
   return FW_OK;
 }
@@ -831,7 +896,7 @@ extern "C" int fw_wigner_create(const fw_wigner_desc* desc, const fw_wigner_expr
              {"fw_vk_min", &h->f_vk_min},   {"fw_build_vtab", &h->f_build_vtab},
              {"fw_build_ktab", &h->f_build_ktab}, {"fw_vv", &h->f_vv},
              {"fw_k_norm", &h->f_k_norm},     {"fw_build_sumw", &h->f_build_sumw},
-             {"fw_field", &h->f_field}};
+             {"fw_begin", &h->f_begin}};
   for (auto& f : fns)
     if ((rc = getfn(h->mod, f.n, f.f)) != FW_OK) return bail(rc);
 
@@ -853,6 +918,10 @@ extern "C" int fw_wigner_create(const fw_wigner_desc* desc, const fw_wigner_expr
   if ((rc = fw::upload_twiddles(h->NP, h->fp32, &h->tw_col)) != FW_OK) return bail(rc);
   if ((rc = fw::upload_twiddles(h->NX, h->fp32, &h->tw_row)) != FW_OK) return bail(rc);
   if (hipStreamCreate(&h->stream) != hipSuccess) return bail(fw::fail(FW_ERR_HIP, "hipStreamCreate failed"));
+  if (hipHostMalloc((void**)&h->h_stage, sizeof(double) * FW_NSER * ((size_t)kStageChunks * (kChunk + 1) + 1)) !=
+          hipSuccess ||
+      hipHostMalloc((void**)&h->h_stop, sizeof(double)) != hipSuccess)
+    return bail(fw::fail(FW_ERR_HIP, "hipHostMalloc of the staging buffers failed"));
   if ((rc = set_scalar(h, fw::SC_NORM, 1.0)) != FW_OK) return bail(rc);
   if (theta_sums(exprs)) {
     if ((rc = alloc(&h->sw, (size_t)(h->NP / 2 + 1) * 48)) != FW_OK) return bail(rc);
@@ -881,6 +950,8 @@ extern "C" int fw_wigner_destroy(fw_wigner* h) {
   for (void* p : {h->W, h->W0s, h->snap, h->tw_col, h->tw_row, h->vtab, h->ktab, h->sw, (void*)h->times, (void*)h->scal,
                   (void*)h->partials, (void*)h->counter, (void*)h->series})
     if (p) (void)hipFree(p);
+  for (double* p : {h->h_stage, h->h_stop})
+    if (p) (void)hipHostFree(p);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return FW_OK;
@@ -1146,9 +1217,7 @@ extern "C" int fw_wigner_kernel_bench(fw_wigner* h, double t, int reps, double u
   HIPCHK(hipMemcpyAsync(save, h->W, bytes, hipMemcpyDeviceToDevice, h->stream));
   std::vector<double> tt(kChunk + 1);
   for (int s = 0; s <= kChunk; ++s) tt[s] = t + (s + 1) * h->d.dt;
-  FWCHK(upload_times(h, tt.data(), kChunk + 1));
-  FWCHK(set_scalar(h, fw::SC_STOP, 0.0));
-  FWCHK(set_scalar(h, fw::SC_FRESH, 1.0));
+  FWCHK(begin_chunk(h, tt.data(), kChunk + 1, true));
   KArgs a = base;
   a.step = 0;
   a.flags |= fw::F_FIRST;

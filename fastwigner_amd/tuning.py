@@ -27,9 +27,14 @@ def key(kind, nx, npp, precision):
     return "%s:%dx%d:%s" % (kind, nx, npp, "fp32" if precision else "fp64")
 
 
+def entry(kind, nx, npp, precision):
+    """The tuned entry (a dict with "tiles" and, for Wigner grids, "v_table") or None."""
+    return _load().get(key(kind, nx, npp, precision))
+
+
 def lookup(kind, nx, npp, precision):
     """(col_e, col_p, row_e, row_p) or None."""
-    e = _load().get(key(kind, nx, npp, precision))
+    e = entry(kind, nx, npp, precision)
     return tuple(e["tiles"]) if e else None
 
 

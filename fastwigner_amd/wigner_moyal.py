@@ -23,6 +23,9 @@ Additional keyword arguments:
                       when the strings do not reference t (default) / never
     device, tiles     HIP device ordinal; (col_e, col_p, row_e, row_p) FFT tiles
                       (default: the autotuned entry of fastwigner_amd/tuning.json)
+    v_table           0: the V factors follow phase_tables; -1: a time-independent V is
+                      evaluated in the column pass instead of read from a table
+                      (default: the tuned entry, for a V free of t)
 """
 import ctypes
 
@@ -118,10 +121,16 @@ class WignerMoyalHIP1D:
         self.hamiltonian = self.K + ' + ' + self.V
 
         self._handle = None
-        tiles = getattr(self, "tiles", None) or tuning.lookup("wigner", self.X_gridDIM, self.P_gridDIM, self._prec)
+        tuned = tuning.entry("wigner", self.X_gridDIM, self.P_gridDIM, self._prec) or {}
+        tiles = getattr(self, "tiles", None) or (tuple(tuned["tiles"]) if "tiles" in tuned else None)
+        # the tuned V-factor policy was measured with a time-independent polynomial V; a V
+        # that depends on t keeps the default (its table of V0 when it splits)
+        static_v = not _lib.uses_token(self.V, "t")
+        v_table = int(getattr(self, "v_table", tuned.get("v_table", 0) if static_v else 0))
         desc = _lib.wigner_desc(self.X_gridDIM, self.X_amplitude, self.P_gridDIM, self.P_amplitude, self.dt,
                                 self.phase_time_shift, self._prec, self.device, tiles,
-                                int(getattr(self, "phase_tables", 0)), int(getattr(self, "step_passes", 0)))
+                                int(getattr(self, "phase_tables", 0)), int(getattr(self, "step_passes", 0)),
+                                v_table)
         ex = _lib.wigner_exprs(
             V=self.V, K=self.K, consts=self.hip_consts,
             diff_V=getattr(self, "diff_V", None) if isinstance(getattr(self, "diff_V", None), str) else None,

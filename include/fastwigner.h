@@ -79,6 +79,9 @@ typedef struct fw_wigner_desc {
                               mask do not contain the token t, 1 = always, -1 = never */
   int step_passes;         /* Moyal steps without the dissipator: 0 or 2 = two passes per step (the
                               closing V/2 of a step fused with the opening V/2 of the next), 3 = three */
+  int v_table;             /* 0 = the V factors follow phase_tables; -1 = evaluate them in the column
+                              pass even when they are time independent (no table read: faster where
+                              the pass is bound by the table's traffic, e.g. 2048^2 FP64) */
 } fw_wigner_desc;
 
 typedef struct fw_wigner_exprs {
