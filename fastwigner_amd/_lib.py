@@ -216,6 +216,7 @@ PROTOTYPES = {
     "fw_wigner_get_state": [_P, _PD],
     "fw_wigner_state_ptr": [_P, _PP, _PI],
     "fw_wigner_propagate": [_P, _PD, _I, _PD],
+    "fw_wigner_propagate_noalloc": [_P, _P, _PD, _I, _PD],
     "fw_wigner_moments": [_P, _D, _PD],
     "fw_wigner_get_average": [_P, ctypes.c_char_p, _D, _PD],
     "fw_wigner_bloch_setup": [_P, _I, _D, _PD],

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -16,7 +17,45 @@
 
 using fw::KArgs;
 
+// ---------------------------------------------------------------------------
+// Scratch of fw_wigner_propagate_noalloc: the run's device scalars, step
+// times, per-step sums and per-block partial sums in one file-scope array
+// instead of the handle's buffers. Each GPU holds its own copy until the
+// process ends. In host code `scratch` names a host-side stand-in; kernels get
+// the device address from hipGetSymbolAddress (g_scratch_dev, per device).
+// Layout in doubles (each part a multiple of 32 doubles, 256 bytes): scalars,
+// times (4 x 65: times and the field values of split potentials), series
+// slots (NSER x 65), the ticket counter, partial sums (NSER x 1024 blocks).
+// ---------------------------------------------------------------------------
 namespace {
+constexpr int kNoAllocMaxBlocks = 1024;   // blocks per pass whose partial sums the scratch holds
+constexpr size_t kScScal = 0, kScTimes = 32, kScSeries = kScTimes + 288, kScCounter = kScSeries + 608,
+                 kScPartials = kScCounter + 32, kScTotal = kScPartials + (size_t)FW_NSER * kNoAllocMaxBlocks;
+static_assert(kScTimes - kScScal >= fw::NSCAL, "scalar slots");
+static_assert(kScSeries - kScTimes >= 4 * 65 && kScCounter - kScSeries >= FW_NSER * 65, "times and series slots");
+}  // namespace
+
+static __device__ __align__(256) double scratch[kScTotal];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};
+
+// The device address of `scratch` on `device` (current), looked up once.
+int scratch_on(int device, double** out) {
+  if (device < 0 || device >= kMaxDevices) return fw::fail(FW_ERR_INVALID_ARG, "device ordinal out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fw::fail(FW_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_dev[device] = static_cast<double*>(p);
+  }
+  *out = g_scratch_dev[device];
+  return FW_OK;
+}
 
 const int kChunk = 64;        // steps per captured graph
 const int kStageChunks = 32;  // chunks of per-step sums staged on the host before a wait
@@ -466,7 +505,7 @@ int helper_blocks(fw_wigner* h) { return h->NP < 128 ? h->NP : 128; }
 // F(t_s + shift dt), F(t_s) beside the times, so the passes read one number
 // per half-step instead of evaluating the field in every thread. Bloch steps
 // use the whole V at t_initial and need none.
-int begin_chunk(fw_wigner* h, const double* t, int n, bool reset, bool field = true) {
+int begin_chunk(fw_wigner* h, const KArgs& run, const double* t, int n, bool reset, bool field = true) {
   double inc = 0.0;
   bool prog = false;
   for (double c : {h->d.dt, 0.0}) {
@@ -480,10 +519,10 @@ int begin_chunk(fw_wigner* h, const double* t, int n, bool reset, bool field = t
   int ngen = n;
   if (!prog) {
     // a pageable source: the copy is staged before hipMemcpyAsync returns
-    HIPCHK(hipMemcpyAsync(h->times, t, sizeof(double) * n, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync((void*)run.times, t, sizeof(double) * n, hipMemcpyHostToDevice, h->stream));
     ngen = 0;
   }
-  KArgs a = base_args(h);
+  KArgs a = run;
   double t0 = n > 0 ? t[0] : 0.0;
   int rs = reset ? 1 : 0, nf = (h->vsplit && field) ? n : 0;
   void* params[] = {&a, &t0, &inc, &ngen, &rs, &nf};
@@ -499,39 +538,42 @@ int begin_chunk(fw_wigner* h, const double* t, int n, bool reset, bool field = t
 // publishes those of the last step into slot kChunk.
 struct SeriesSink {
   fw_wigner* h;
+  const double* dser;   // the run's device series slots
   double* out;
   struct Rec {
     int off, n, k;
   };
-  std::vector<Rec> recs;
+  Rec recs[kStageChunks];
+  int nrec = 0;
   int last = -1;
-  SeriesSink(fw_wigner* h_, double* out_) : h(h_), out(out_) {}
+  SeriesSink(fw_wigner* h_, const double* dser_, double* out_) : h(h_), dser(dser_), out(out_) {}
   int chunk(int off, int n) {
     if (!out) return FW_OK;
-    if ((int)recs.size() == kStageChunks) FWCHK(flush());
-    const int k = (int)recs.size();
-    HIPCHK(hipMemcpyAsync(h->h_stage + (size_t)k * (kChunk + 1) * FW_NSER, h->series, sizeof(double) * FW_NSER * n,
+    if (nrec == kStageChunks) FWCHK(flush());
+    const int k = nrec;
+    HIPCHK(hipMemcpyAsync(h->h_stage + (size_t)k * (kChunk + 1) * FW_NSER, dser, sizeof(double) * FW_NSER * n,
                           hipMemcpyDeviceToHost, h->stream));
-    recs.push_back({off, n, k});
+    recs[nrec++] = {off, n, k};
     return FW_OK;
   }
   int final_slot(int step) {
     if (!out || step < 0) return FW_OK;
-    HIPCHK(hipMemcpyAsync(h->h_stage + (size_t)kStageChunks * (kChunk + 1) * FW_NSER,
-                          h->series + (size_t)kChunk * FW_NSER, sizeof(double) * FW_NSER, hipMemcpyDeviceToHost,
-                          h->stream));
+    HIPCHK(hipMemcpyAsync(h->h_stage + (size_t)kStageChunks * (kChunk + 1) * FW_NSER, dser + (size_t)kChunk * FW_NSER,
+                          sizeof(double) * FW_NSER, hipMemcpyDeviceToHost, h->stream));
     last = step;
     return FW_OK;
   }
   void scatter() {
-    for (const Rec& r : recs)
+    for (int i = 0; i < nrec; ++i) {
+      const Rec& r = recs[i];
       for (int s = 0; s < r.n; ++s) {
         const int g = r.off + s - 1;
         if (g >= 0)
           std::memcpy(out + (size_t)g * FW_NSER, h->h_stage + ((size_t)r.k * (kChunk + 1) + s) * FW_NSER,
                       sizeof(double) * FW_NSER);
       }
-    recs.clear();
+    }
+    nrec = 0;
   }
   int flush() {
     if (!out) return FW_OK;
@@ -637,13 +679,13 @@ int run_steps(fw_wigner* h, int kind, const KArgs& base0, const double* times, i
   const bool use_graph = !h->timing && kind == 0;
   if (use_graph && steps >= kChunk) FWCHK(ensure_graph(h, kind, base));
   std::vector<hipEvent_t> evs;
-  SeriesSink sink(h, series);
+  SeriesSink sink(h, base.series, series);
   double acc[3] = {0, 0, 0};
   int nacc = 0;
   int applied = steps;
   for (int off = 0; off < steps; off += kChunk) {
     const int n = std::min(kChunk, steps - off);
-    FWCHK(begin_chunk(h, times + off, n, off == 0, kind == 0));
+    FWCHK(begin_chunk(h, base, times + off, n, off == 0, kind == 0));
     if (use_graph && n == kChunk) {
       HIPCHK(hipGraphLaunch(h->gexec, h->stream));
     } else if (h->timing) {
@@ -669,7 +711,7 @@ int run_steps(fw_wigner* h, int kind, const KArgs& base0, const double* times, i
       HIPCHK(hipMemcpyAsync(h->h_stop, h->scal + fw::SC_STOP, sizeof(double), hipMemcpyDeviceToHost, h->stream));
       HIPCHK(hipStreamSynchronize(h->stream));
       const double stop = *h->h_stop;
-      if (stop != 0 && !sink.recs.empty()) sink.recs.back().n = (int)stop;   // slots written in this chunk
+      if (stop != 0 && sink.nrec > 0) sink.recs[sink.nrec - 1].n = (int)stop;   // slots written in this chunk
       sink.scatter();
       if (stop != 0) {
         applied = off + (int)stop - 1;
@@ -722,19 +764,24 @@ int ensure_graph_fused(fw_wigner* h, const KArgs& base) {
   return FW_OK;
 }
 
-int run_steps_fused(fw_wigner* h, const KArgs& base0, const double* times, int steps, double* series) {
+// graphs: replay the handle's captured 64-step graph for whole chunks (its
+// kernel arguments name the handle's own buffers, so a run on other buffers
+// launches every kernel itself)
+int run_steps_fused(fw_wigner* h, const KArgs& base0, const double* times, int steps, double* series,
+                    bool graphs = true) {
   KArgs base = base0;
   FWCHK(ensure_tables(h, 0, base));
   const bool timing = h->timing;
-  if (!timing && steps > kChunk) FWCHK(ensure_graph_fused(h, base));
-  SeriesSink sink(h, series);
+  graphs = graphs && !timing;
+  if (graphs && steps > kChunk) FWCHK(ensure_graph_fused(h, base));
+  SeriesSink sink(h, base.series, series);
   std::vector<hipEvent_t> evs;
   double acc[2] = {0, 0};
   int nacc = 0;
   for (int off = 0; off < steps; off += kChunk) {
     const int n = std::min(kChunk, steps - off);
     const int nt = std::min(kChunk + 1, steps - off);   // t of steps off .. off+n (the next chunk's first)
-    FWCHK(begin_chunk(h, times + off, nt, off == 0));
+    FWCHK(begin_chunk(h, base, times + off, nt, off == 0));
     if (off == 0) {
       KArgs a = base;
       a.step = 0;
@@ -742,7 +789,7 @@ int run_steps_fused(fw_wigner* h, const KArgs& base0, const double* times, int s
       FWCHK(launch(h, h->f_v_first, h->col_nb, h->col_nt, a));
     }
     const bool has_last = off + n == steps;
-    if (!timing && !has_last && n == kChunk) {
+    if (graphs && !has_last && n == kChunk) {
       HIPCHK(hipGraphLaunch(h->gexec2, h->stream));
     } else if (timing) {
       evs.resize(3 * n);
@@ -932,6 +979,13 @@ extern "C" int fw_wigner_create(const fw_wigner_desc* desc, const fw_wigner_expr
         hipSuccess)
       return bail(fw::fail(FW_ERR_HIP, "fw_build_sumw launch failed"));
   }
+  {
+    // the Moyal factor tables now, so that no run allocates (fw_wigner_propagate_noalloc)
+    KArgs a = base_args(h);
+    if ((rc = ensure_tables(h, 0, a)) != FW_OK) return bail(rc);
+    double* sp = nullptr;
+    if ((rc = scratch_on(desc->device, &sp)) != FW_OK) return bail(rc);
+  }
   if (hipStreamSynchronize(h->stream) != hipSuccess) return bail(fw::fail(FW_ERR_HIP, "initial sync failed"));
   *out = h;
   return FW_OK;
@@ -1059,6 +1113,35 @@ extern "C" int fw_wigner_propagate(fw_wigner* h, const double* times, int steps,
   if (!h->dissip && h->d.step_passes != 3) return run_steps_fused(h, a, times, steps, series);
   h->last_passes = 3;
   return run_steps(h, 0, a, times, steps, series, nullptr);
+}
+
+extern "C" int fw_wigner_propagate_noalloc(fw_wigner* h, void* w_dev, const double* times, int steps,
+                                           double* series) {
+  if (!h || !w_dev || (!times && steps > 0)) return fw::fail(FW_ERR_INVALID_ARG, "NULL argument");
+  if (steps < 0) return fw::fail(FW_ERR_INVALID_ARG, "steps must be >= 0");
+  if (h->dissip || h->d.step_passes == 3)
+    return fw::fail(FW_ERR_INVALID_ARG, "fw_wigner_propagate_noalloc runs two-pass Moyal steps only (no dissipator, "
+                                        "step_passes 0 or 2)");
+  const int nb = std::max(h->col_nb, h->row_nb);
+  if (nb > kNoAllocMaxBlocks)
+    return fw::fail(FW_ERR_INVALID_ARG, "fw_wigner_propagate_noalloc: the tiles need " + std::to_string(nb) +
+                                            " blocks per pass; its static scratch holds the partial sums of at most " +
+                                            std::to_string(kNoAllocMaxBlocks) + " (use larger col_p / row_p)");
+  double* sp = nullptr;
+  FWCHK(scratch_on(h->d.device, &sp));
+  if (steps == 0) return FW_OK;
+  HIPCHK(hipSetDevice(h->d.device));
+  KArgs a = base_args(h);
+  a.W = w_dev;
+  a.scal = sp + kScScal;
+  a.times = sp + kScTimes;
+  a.series = sp + kScSeries;
+  a.counter = reinterpret_cast<unsigned int*>(sp + kScCounter);
+  a.partials = sp + kScPartials;
+  a.nbp = h->col_nb;
+  // the tables were built at creation (the allocation happened there); a Bloch
+  // run may have refilled them, which ensure_tables undoes without allocating
+  return run_steps_fused(h, a, times, steps, series, false);
 }
 
 extern "C" int fw_wigner_moments(fw_wigner* h, double t, double sums[FW_NSER]) {
@@ -1217,7 +1300,7 @@ extern "C" int fw_wigner_kernel_bench(fw_wigner* h, double t, int reps, double u
   HIPCHK(hipMemcpyAsync(save, h->W, bytes, hipMemcpyDeviceToDevice, h->stream));
   std::vector<double> tt(kChunk + 1);
   for (int s = 0; s <= kChunk; ++s) tt[s] = t + (s + 1) * h->d.dt;
-  FWCHK(begin_chunk(h, tt.data(), kChunk + 1, true));
+  FWCHK(begin_chunk(h, base, tt.data(), kChunk + 1, true));
   KArgs a = base;
   a.step = 0;
   a.flags |= fw::F_FIRST;

@@ -133,6 +133,22 @@ int fw_wigner_state_ptr(fw_wigner* h, void** dev_ptr, int* elem_size);
  * No host synchronisation inside; steps are replayed from a captured hipGraph. */
 int fw_wigner_propagate(fw_wigner* h, const double* times, int steps, double* series);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): fw_wigner_propagate's two-pass Moyal steps on the
+ * caller's device buffer w_dev (p_grid_dim*x_grid_dim elements of the handle's
+ * precision, updated in place; the handle's own W is untouched), with the
+ * run's scalars, step times, step sums and per-block partial sums in one
+ * file-scope static device array of 81 KiB per GPU instead of handle buffers.
+ * No hipMalloc, hipFree or other allocating call: the factor tables are built
+ * by fw_wigner_create, the sums come back through the handle's pinned staging.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one GPU
+ * (two host threads, two handles) race. Errors: FW_ERR_INVALID_ARG for a handle
+ * with the dissipator or step_passes = 3, or tiles needing more than 1024
+ * blocks per pass (the partial sums the array holds). times and series as in
+ * fw_wigner_propagate. */
+int fw_wigner_propagate_noalloc(fw_wigner* h, void* w_dev, const double* times, int steps, double* series);
+
 /* The FW_NSER sums over the current W at time t (get_average building block). */
 int fw_wigner_moments(fw_wigner* h, double t, double sums[FW_NSER]);
 
