@@ -243,7 +243,9 @@ PROTOTYPES = {
     "fw_rho_bloch_propagate": [_P, _D, _I, _D, _PD, _PI],
     "fw_rho_snapshot": [_P, _I],
     "fw_rho_wigner": [_P, _PD],
-    "fw_rho_wigner_time": [_P, _PD, _D, _PD],
+    "fw_rho_set_wigner_initial": [_P, _PD],
+    "fw_rho_wigner_time": [_P, _D, _D, _PD],
+    "fw_rho_propagate_wigner_time": [_P, _PD, _I, _I, _D, _PD, _PD],
     "fw_rho_set_timing": [_P, _I],
     "fw_rho_kernel_times": [_P, _PD],
     "fw_batch_check": [ctypes.POINTER(BatchDesc), ctypes.POINTER(BatchInstance), ctypes.POINTER(BatchExprs)],

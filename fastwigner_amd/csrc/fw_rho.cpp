@@ -17,7 +17,7 @@ namespace {
 
 const int kChunk = 64;
 const int RSER = FW_RSER;
-enum { RC_NORM_RE = 0, RC_STOP = 1, RC_FRESH = 2, RC_NORM_IM = 3, RC_OUT = 8 };
+enum { RC_NORM_RE = 0, RC_STOP = 1, RC_FRESH = 2, RC_NORM_IM = 3, RC_OUT = 8, RC_IP = 48 };
 enum { RF_FIRST = 1, RF_BLOCH = 4 };
 
 // Host mirror of fwk::RArgs (fw_rho.h)
@@ -106,9 +106,17 @@ struct fw_rho {
   double t_initial = 0, vmin = 0, kmin = 0;
   bool bloch_ready = false;
 
+  // Wigner time (rho_vneumann_cuda_1d.py:594-620): W_init (uploaded once), the Wigner
+  // transform's own block sums, the time kernel's block sums, the per-step values
+  double* wpart = nullptr;
+  double* wtpart = nullptr;
+  double* wts = nullptr;
+  Mod wtm;
+  bool winit_set = false;
+
   hipGraphExec_t gexec = nullptr;
   int gkey = -1;
-  std::map<std::string, Mod> init_mods, obs_mods, wt_mods;
+  std::map<std::string, Mod> init_mods, obs_mods;
   bool timing = false;
   double ktimes[3] = {0, 0, 0};
 };
@@ -276,7 +284,56 @@ int momentum_sums(fw_rho* h, RArgs& a) {
 }
 
 // rows + steps of one chunk; bloch selects the kernel set
-int enqueue_chunk(fw_rho* h, bool bloch, int n, bool momentum, const RArgs& b0, std::vector<hipEvent_t>* ev) {
+constexpr int kWtBlocks = 256;   // blocks of the Wigner-time reduction (<= N rows each)
+
+// After step s: the Wigner function of rho (Larkin three shears and the
+// sign-flipped inverse FFT, rho_vneumann_cuda_1d.py:502-592) in the handle's
+// Wigner buffer, unnormalised, then the Wigner time of step s into wts[s].
+// The transform works in the Wigner buffer (aux holds the momentum shares that
+// the next step's first pass publishes) and keeps its block sums apart from the
+// step's partial sums, which the next step's first pass reads for the norm.
+int enqueue_wigner_time(fw_rho* h, const RArgs& a, int s) {
+  RArgs w = a;
+  w.partials = h->wpart;
+  w.step = s;
+  void* rho = h->rho;
+  void* wig = h->wig;
+  {
+    void* p[] = {&w, &rho, &wig};
+    FWCHK(launch(h, h->f_shx, h->row_nb, h->row_nt, p));
+  }
+  {
+    void* p[] = {&w, &wig, &wig};
+    FWCHK(launch(h, h->f_shy, h->col_nb, h->col_nt, p));
+  }
+  {
+    void* p[] = {&w, &wig, &wig};
+    FWCHK(launch(h, h->f_shx, h->row_nb, h->row_nt, p));
+  }
+  {
+    void* p[] = {&w, &wig, &wig};
+    FWCHK(launch(h, h->f_flip, h->col_nb, h->col_nt, p));
+  }
+  const void* w0 = h->winit;
+  const double* tm = h->times;
+  const double* sc = h->scal;
+  double* part = h->wtpart;
+  int step = s;
+  const int nbw = std::min(h->N, kWtBlocks);
+  {
+    void* p[] = {&wig, &w0, &tm, &step, &sc, &part};
+    HIPCHK(hipModuleLaunchKernel(h->wtm.f1, nbw, 1, 1, 256, 1, 1, 0, h->stream, p, nullptr));
+  }
+  double* out = h->wts;
+  double dxdp = (h->dX / std::sqrt(2.0)) * (h->dP / std::sqrt(2.0));
+  int nb = nbw;
+  void* p[] = {&part, &nb, &out, &step, &dxdp};
+  HIPCHK(hipModuleLaunchKernel(h->wtm.f2, 1, 1, 1, 256, 1, 1, 0, h->stream, p, nullptr));
+  return FW_OK;
+}
+
+int enqueue_chunk(fw_rho* h, bool bloch, int n, bool momentum, const RArgs& b0, std::vector<hipEvent_t>* ev,
+                  bool wtime = false) {
   RArgs a = b0;
   a.nrows = n;
   {
@@ -304,22 +361,25 @@ int enqueue_chunk(fw_rho* h, bool bloch, int n, bool momentum, const RArgs& b0, 
       if (fold && s + 1 < n) FWCHK(args1(h, h->f_diag_p, nbd, 256, a));
       else FWCHK(momentum_sums(h, a));
     }
+    if (wtime) FWCHK(enqueue_wigner_time(h, a, s));
   }
   return FW_OK;
 }
 
+// wtime (real-time steps only): the Wigner time of every step into wtime_out[s]
 int run(fw_rho* h, bool bloch, const RArgs& b0, const double* times, int steps, bool momentum, double* series,
-        int* done) {
+        int* done, double* wtime_out = nullptr) {
   FWCHK(set_scalar(h, RC_STOP, 0.0));
   FWCHK(set_scalar(h, RC_FRESH, 1.0));
+  const bool wtime = wtime_out != nullptr && !bloch;
   const bool use_graph = !h->timing && !bloch && b0.guard <= 0;
-  const int key = momentum ? 1 : 0;
+  const int key = (momentum ? 1 : 0) + (wtime ? 2 : 0);
   if (use_graph && steps >= kChunk && (!h->gexec || h->gkey != key)) {
     if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
     h->gexec = nullptr;
     hipGraph_t g;
     HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_chunk(h, false, kChunk, momentum, b0, nullptr);
+    int rc = enqueue_chunk(h, false, kChunk, momentum, b0, nullptr, wtime);
     hipError_t e = hipStreamEndCapture(h->stream, &g);
     if (rc != FW_OK) return rc;
     if (e != hipSuccess) return fw::fail(FW_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
@@ -341,7 +401,7 @@ int run(fw_rho* h, bool bloch, const RArgs& b0, const double* times, int steps, 
     } else if (h->timing) {
       std::vector<hipEvent_t> ev(4 * n);
       for (auto& e : ev) HIPCHK(hipEventCreate(&e));
-      FWCHK(enqueue_chunk(h, bloch, n, momentum, b0, &ev));
+      FWCHK(enqueue_chunk(h, bloch, n, momentum, b0, &ev, wtime));
       HIPCHK(hipStreamSynchronize(h->stream));
       for (int s = 0; s < n; ++s, ++nacc)
         for (int k = 0; k < 3; ++k) {
@@ -351,7 +411,11 @@ int run(fw_rho* h, bool bloch, const RArgs& b0, const double* times, int steps, 
         }
       for (auto& e : ev) (void)hipEventDestroy(e);
     } else {
-      FWCHK(enqueue_chunk(h, bloch, n, momentum, b0, nullptr));
+      FWCHK(enqueue_chunk(h, bloch, n, momentum, b0, nullptr, wtime));
+    }
+    if (wtime) {
+      HIPCHK(hipMemcpyAsync(wtime_out + off, h->wts, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+      if (!series && b0.guard <= 0) HIPCHK(hipStreamSynchronize(h->stream));   // wts is reused by the next chunk
     }
     double stop = 0;
     if (series || b0.guard > 0) {
@@ -506,12 +570,14 @@ extern "C" int fw_rho_destroy(fw_rho* h) {
   if (!h) return FW_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
-  for (auto* m : {&h->init_mods, &h->obs_mods, &h->wt_mods})
+  for (auto* m : {&h->init_mods, &h->obs_mods})
     for (auto& kv : *m)
       if (kv.second.mod) (void)hipModuleUnload(kv.second.mod);
+  if (h->wtm.mod) (void)hipModuleUnload(h->wtm.mod);
   if (h->mod) (void)hipModuleUnload(h->mod);
   for (void* p : {h->rho, h->aux, h->wig, h->winit, h->snap, h->tw, h->uv, h->uk, (void*)h->times, (void*)h->scal,
-                  (void*)h->partials, (void*)h->series, (void*)h->dw})
+                  (void*)h->partials, (void*)h->series, (void*)h->dw, (void*)h->wpart, (void*)h->wtpart,
+                  (void*)h->wts})
     if (p) (void)hipFree(p);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -787,51 +853,124 @@ extern "C" int fw_rho_wigner(fw_rho* h, double* host_w) {
   return FW_OK;
 }
 
-extern "C" int fw_rho_wigner_time(fw_rho* h, const double* host_w_init, double t, double* out) {
-  if (!h || !host_w_init || !out) return fw::fail(FW_ERR_INVALID_ARG, "NULL argument");
-  if (!h->wig) return fw::fail(FW_ERR_INVALID_ARG, "compute the Wigner function first (fw_rho_wigner)");
+// The Wigner-time kernels (wigner_time_mapper_cuda_code, rho_vneumann_cuda_1d.py:830-858):
+// the integral of ind (W - W_init) over the Wigner grid (X/sqrt2, P/sqrt2), with the
+// reference's indicator ind = 0.5 (1 + signbit(H + Ip)) (:854). W is the flip pass's
+// unnormalised result or the normalised one: with T1 = sum ind W, Sw = sum W (complex)
+// and T0 = sum ind W_init, the normalised W = W/(Sw dxdp) gives Re(T1/Sw) - T0 dxdp.
+// Ip comes from the scalar slot RC_IP, so it may be set after construction.
+static int ensure_wt_module(fw_rho* h) {
+  if (h->wtm.mod) return FW_OK;
+  const std::string body =
+      "extern \"C\" __global__ void __launch_bounds__(256) fw_wt_partial(const fwk::cpx* w, const double* w0, "
+      "const double* times, int step, const double* scal, double* part) {\n"
+      "  __shared__ double red[fwk::NSER * 4];\n"
+      "  const double t = times[step];\n"
+      "  const double Ip = scal[" + std::to_string((int)RC_IP) + "];\n"
+      "  double s[fwk::NSER];\n"
+      "  for (int k = 0; k < fwk::NSER; ++k) s[k] = 0.0;\n"
+      "  for (int i = blockIdx.x; i < FW_N; i += gridDim.x) {\n"
+      "    const double P = dP * (i - 0.5 * X_gridDIM) / sqrt(2.);\n"
+      "    for (int j = threadIdx.x; j < FW_N; j += 256) {\n"
+      "      const double X = dX * (j - 0.5 * X_gridDIM) / sqrt(2.);\n"
+      "      const double hamiltonian = K(P, t) + V(X, t);\n"
+      "      const double indicator_func = 0.5 * (1 + int(signbit(hamiltonian + Ip)));\n"
+      "      const size_t idx = (size_t)i * FW_N + j;\n"
+      "      const double wr = (double)w[idx].x, wi = (double)w[idx].y;\n"
+      "      s[0] += indicator_func * wr;\n"
+      "      s[1] += indicator_func * wi;\n"
+      "      s[2] += wr;\n"
+      "      s[3] += wi;\n"
+      "      s[4] += indicator_func * w0[idx];\n"
+      "    }\n"
+      "  }\n"
+      "  fwk::write_partials<256>(s, red, part, blockIdx.x);\n}\n"
+      "extern \"C\" __global__ void __launch_bounds__(256) fw_wt_finish(const double* part, int nb, double* out, "
+      "int step, double dxdp) {\n"
+      "  __shared__ double red[8];\n"
+      "  double v[5];\n"
+      "  for (int k = 0; k < 5; ++k) v[k] = fwk::sum_partials<256, fwk::NSER>(part, nb, k, red);\n"
+      "  if (threadIdx.x == 0) out[step] = (v[0] * v[2] + v[1] * v[3]) / (v[2] * v[2] + v[3] * v[3]) - v[4] * dxdp;\n"
+      "}\n";
+  return build_mod(h, body, "fw_wt_partial", "fw_wt_finish", h->wtm);
+}
+
+static int ensure_wigner_buffers(fw_rho* h) {
   const size_t nn = (size_t)h->N * h->N;
+  if (!h->wig) HIPCHK(hipMalloc(&h->wig, nn * h->csz));
+  if (!h->wpart) HIPCHK(hipMalloc(&h->wpart, sizeof(double) * RSER * std::max(h->col_nb, h->row_nb)));
+  if (!h->wtpart) HIPCHK(hipMalloc(&h->wtpart, sizeof(double) * FW_NSER * kWtBlocks));
+  if (!h->wts) HIPCHK(hipMalloc(&h->wts, sizeof(double) * (kChunk + 1)));
+  return FW_OK;
+}
+
+extern "C" int fw_rho_set_wigner_initial(fw_rho* h, const double* host_w_init) {
+  if (!h || !host_w_init) return fw::fail(FW_ERR_INVALID_ARG, "NULL argument");
+  const size_t nn = (size_t)h->N * h->N;
+  FWCHK(ensure_wigner_buffers(h));
   if (!h->winit) HIPCHK(hipMalloc(&h->winit, nn * 8));
-  HIPCHK(hipMemcpy(h->winit, host_w_init, nn * 8, hipMemcpyHostToDevice));
-  auto it = h->wt_mods.find("wt");
-  if (it == h->wt_mods.end()) {
-    // wigner_time_mapper_cuda_code, rho_vneumann_cuda_1d.py:830-858, with its indicator 0.5*(1+signbit(H+Ip))
-    std::string body =
-        "extern \"C\" __global__ void __launch_bounds__(256) fw_wigner_time(const fwk::cpx* w, const double* w0, "
-        "const double* times, double* partials) {\n"
-        "  __shared__ double red[fwk::NSER * 4];\n"
-        "  const double t = times[0];\n"
-        "  double s[fwk::NSER];\n"
-        "  for (int k = 0; k < fwk::NSER; ++k) s[k] = 0.0;\n"
-        "  for (int i = blockIdx.x; i < FW_N; i += gridDim.x) {\n"
-        "    const double P = dP * (i - 0.5 * X_gridDIM) / sqrt(2.);\n"
-        "    for (int j = threadIdx.x; j < FW_N; j += 256) {\n"
-        "      const double X = dX * (j - 0.5 * X_gridDIM) / sqrt(2.);\n"
-        "      const double hamiltonian = K(P, t) + V(X, t);\n"
-        "      const double indicator_func = 0.5 * (1 + int(signbit(hamiltonian + Ip)));\n"
-        "      const size_t idx = (size_t)i * FW_N + j;\n"
-        "      s[0] += indicator_func * ((double)w[idx].x - w0[idx]);\n"
-        "    }\n"
-        "  }\n"
-        "  fwk::write_partials<256>(s, red, partials, blockIdx.x);\n}\n";
-    Mod m;
-    FWCHK(build_mod(h, body, "fw_wigner_time", nullptr, m));
-    it = h->wt_mods.emplace("wt", m).first;
-  }
-  HIPCHK(hipMemcpyAsync(h->times, &t, sizeof(double), hipMemcpyHostToDevice, h->stream));
-  void* w = h->wig;
-  void* w0 = h->winit;
-  double* tm = h->times;
-  double* pp = h->partials;
-  void* params[] = {&w, &w0, &tm, &pp};
-  const int nb = std::min(h->N, 256);   // partials hold >= 256 * FW_RSER doubles
-  HIPCHK(hipModuleLaunchKernel(it->second.f1, nb, 1, 1, 256, 1, 1, 0, h->stream, params, nullptr));
-  std::vector<double> part((size_t)nb * FW_NSER);
-  HIPCHK(hipMemcpyAsync(part.data(), h->partials, part.size() * 8, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
-  double sum = 0.0;
-  for (int b = 0; b < nb; ++b) sum += part[(size_t)b * FW_NSER];
-  *out = sum * (h->dX / std::sqrt(2.0)) * (h->dP / std::sqrt(2.0));
+  HIPCHK(hipMemcpy(h->winit, host_w_init, nn * 8, hipMemcpyHostToDevice));
+  FWCHK(ensure_wt_module(h));
+  h->winit_set = true;
+  return FW_OK;
+}
+
+static int set_ip(fw_rho* h, double Ip) {
+  HIPCHK(hipMemcpyAsync(h->scal + RC_IP, &Ip, sizeof(double), hipMemcpyHostToDevice, h->stream));
+  return FW_OK;
+}
+
+extern "C" int fw_rho_wigner_time(fw_rho* h, double t, double Ip, double* out) {
+  if (!h || !out) return fw::fail(FW_ERR_INVALID_ARG, "NULL argument");
+  if (!h->wig) return fw::fail(FW_ERR_INVALID_ARG, "compute the Wigner function first (fw_rho_wigner)");
+  if (!h->winit_set) return fw::fail(FW_ERR_INVALID_ARG, "set W_init first (fw_rho_set_wigner_initial)");
+  FWCHK(ensure_wigner_buffers(h));
+  FWCHK(set_ip(h, Ip));
+  HIPCHK(hipMemcpyAsync(h->times, &t, sizeof(double), hipMemcpyHostToDevice, h->stream));
+  // the time kernel reads times[step]: step 0 here
+  RArgs a = base(h);
+  a.step = 0;
+  {
+    const void* w0 = h->winit;
+    const double* tm = h->times;
+    const double* sc = h->scal;
+    double* part = h->wtpart;
+    void* wig = h->wig;
+    int step = 0;
+    const int nbw = std::min(h->N, kWtBlocks);
+    void* p[] = {&wig, &w0, &tm, &step, &sc, &part};
+    HIPCHK(hipModuleLaunchKernel(h->wtm.f1, nbw, 1, 1, 256, 1, 1, 0, h->stream, p, nullptr));
+    double* o = h->wts;
+    double dxdp = (h->dX / std::sqrt(2.0)) * (h->dP / std::sqrt(2.0));
+    int nb = nbw;
+    void* q[] = {&part, &nb, &o, &step, &dxdp};
+    HIPCHK(hipModuleLaunchKernel(h->wtm.f2, 1, 1, 1, 256, 1, 1, 0, h->stream, q, nullptr));
+  }
+  HIPCHK(hipMemcpyAsync(out, h->wts, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return FW_OK;
+}
+
+extern "C" int fw_rho_propagate_wigner_time(fw_rho* h, const double* times, int steps, int momentum_sums, double Ip,
+                                            double* series, double* wigner_time) {
+  if (!h || !wigner_time || (!times && steps > 0)) return fw::fail(FW_ERR_INVALID_ARG, "NULL argument");
+  if (steps < 0) return fw::fail(FW_ERR_INVALID_ARG, "steps must be >= 0");
+  if (!h->winit_set) return fw::fail(FW_ERR_INVALID_ARG, "set W_init first (fw_rho_set_wigner_initial)");
+  if (steps == 0) return FW_OK;
+  FWCHK(ensure_wigner_buffers(h));
+  FWCHK(set_ip(h, Ip));
+  RArgs a = base(h);
+  FWCHK(run(h, false, a, times, steps, momentum_sums != 0, series, nullptr, wigner_time));
+  // the normalised Wigner function of the last step, as get_wignerfunction leaves it
+  RArgs b = base(h);
+  b.partials = h->wpart;
+  b.nbp = h->col_nb;
+  double dxdp = (h->dX / std::sqrt(2.0)) * (h->dP / std::sqrt(2.0));
+  void* wig = h->wig;
+  void* p[] = {&b, &wig, &dxdp};
+  FWCHK(launch(h, h->f_wscale, 1024, 256, p));
+  HIPCHK(hipStreamSynchronize(h->stream));
   return FW_OK;
 }
 

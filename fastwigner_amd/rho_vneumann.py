@@ -208,27 +208,41 @@ class RhoVNeumannHIP1D:
 
     def propagate(self, steps=1):
         """rho_vneumann_cuda_1d.py:355-374: steps x (t += dt, step, rho /= Tr(rho) dX,
-        Ehrenfest observables); returns ``self.rho``."""
+        Ehrenfest observables); returns ``self.rho``. With ``wigner_initial`` and ``Ip``
+        set, every step also computes the Wigner function and its Wigner time
+        (rho_vneumann_cuda_1d.py:408-412), on the device inside the step graph;
+        ``wigner_current`` is then the last step's Wigner function."""
         steps = int(steps)
         if steps <= 0:
             return self.rho
-        if self._wigner_time_enabled():
-            for _ in range(steps):       # the Wigner time needs the Wigner function every step
-                self._propagate_chunk(1)
-                self.wigner_current = self.get_wignerfunction()
-                self.wigner_time.append(self.get_wigner_time(self.wigner_current, self.wigner_initial, self.t))
-            return self.rho
-        self._propagate_chunk(steps)
-        return self.rho
-
-    def _propagate_chunk(self, steps):
         times = self._times(steps)
         series = np.empty((steps, _lib.FW_RSER))
-        _lib.call("fw_rho_propagate", self._handle, _lib.dptr(times), steps, 1 if self.isEhrenfest else 0,
-                  _lib.dptr(series))
+        mom = 1 if self.isEhrenfest else 0
+        if self._wigner_time_enabled():
+            self._upload_wigner_initial(self.wigner_initial)
+            wt = np.empty(steps)
+            _lib.call("fw_rho_propagate_wigner_time", self._handle, _lib.dptr(times), steps, mom, float(self.Ip),
+                      _lib.dptr(series), _lib.dptr(wt))
+            self.wigner_current = self._wigner
+            self.wigner_time.extend(wt.tolist())
+        else:
+            _lib.call("fw_rho_propagate", self._handle, _lib.dptr(times), steps, mom, _lib.dptr(series))
         self.last_series = series
         if self.isEhrenfest:
             self._record(series)
+        return self.rho
+
+    def _upload_wigner_initial(self, w_init):
+        """W_init goes to the device once per array object (re-assigning the attribute,
+        or passing another array, uploads the new one)."""
+        if getattr(self, "_winit_src", None) is w_init:
+            return
+        w0 = w_init.get() if isinstance(w_init, DeviceArray) else np.asarray(w_init)
+        n = self.X_gridDIM
+        assert w0.shape == (n, n), "wigner_initial must be N x N on the Wigner grid"
+        w0 = np.ascontiguousarray(np.real(w0), dtype=np.float64)
+        _lib.call("fw_rho_set_wigner_initial", self._handle, _lib.dptr(w0))
+        self._winit_src = w_init
 
     def single_step_propagation(self):
         """One step at the current t (renormalised, unlike the reference)."""
@@ -268,13 +282,17 @@ class RhoVNeumannHIP1D:
         return self._wigner
 
     def get_wigner_time(self, wigner_current, wigner_init, t):
-        """int_{H > -Ip} (W - W_init) dxdp with the reference's indicator
-        (rho_vneumann_cuda_1d.py:594-620,830-858); wigner_current must be the
-        Wigner function last computed by get_wignerfunction."""
-        w0 = wigner_init.get() if isinstance(wigner_init, DeviceArray) else np.asarray(wigner_init)
-        w0 = np.ascontiguousarray(np.real(w0), dtype=np.float64)
+        """int 0.5 (1 + signbit(H + Ip)) (W - W_init) dxdp, the reference's indicator
+        (rho_vneumann_cuda_1d.py:594-620,830-858), over the device's Wigner function:
+        wigner_current must be the array get_wignerfunction returned (the device
+        computes on its own copy; any other array is refused rather than silently
+        ignored). Ip is the attribute (or constructor keyword) ``Ip``."""
+        if wigner_current is not self._wigner:
+            raise ValueError("get_wigner_time: wigner_current must be the array returned by get_wignerfunction() "
+                             "(the Wigner function is kept on the device)")
+        self._upload_wigner_initial(wigner_init)
         out = ctypes.c_double()
-        _lib.call("fw_rho_wigner_time", self._handle, _lib.dptr(w0), float(t), ctypes.byref(out))
+        _lib.call("fw_rho_wigner_time", self._handle, float(t), float(self.Ip), ctypes.byref(out))
         return out.value
 
     @classmethod

@@ -255,9 +255,19 @@ int fw_rho_snapshot(fw_rho* h, int which);
 /* Wigner function of rho on the (X/sqrt2, P/sqrt2) grid (get_wignerfunction, rho_vneumann_cuda_1d.py:502-592),
  * complex N*N (re, im) pairs, rows in fftshift(P) order; kept on the device for fw_rho_wigner_time */
 int fw_rho_wigner(fw_rho* h, double* host_w);
-/* sum over the last Wigner function of 0.5(1 + signbit(H + Ip)) (Re W - W_init) dxdp
- * (get_wigner_time, rho_vneumann_cuda_1d.py:594-620,830-858); needs a constant Ip */
-int fw_rho_wigner_time(fw_rho* h, const double* host_w_init, double t, double* out);
+/* W_init of the Wigner time (the reference's wigner_initial attribute, rho_vneumann_cuda_1d.py:409-412):
+ * N*N doubles on the Wigner grid, uploaded once and kept on the device until replaced. */
+int fw_rho_set_wigner_initial(fw_rho* h, const double* host_w_init);
+/* get_wigner_time (rho_vneumann_cuda_1d.py:594-620,830-858): the integral over the Wigner grid of
+ * 0.5(1 + signbit(H(t) + Ip)) (Re W - W_init) dxdp, with the reference's indicator, for the Wigner
+ * function last computed (fw_rho_wigner) and the W_init set by fw_rho_set_wigner_initial. */
+int fw_rho_wigner_time(fw_rho* h, double t, double Ip, double* out);
+/* fw_rho_propagate with the reference's per-step Wigner time (get_Ehrenfest with wigner_initial and
+ * Ip, rho_vneumann_cuda_1d.py:408-412): after each step the Wigner function (three shears and the
+ * sign-flipped FFT) and its Wigner time, on the device, into wigner_time[s]; afterwards the
+ * normalised Wigner function of the last step is the one fw_rho_wigner would give. */
+int fw_rho_propagate_wigner_time(fw_rho* h, const double* times, int steps, int momentum_sums, double Ip,
+                                 double* series, double* wigner_time);
 int fw_rho_set_timing(fw_rho* h, int enable);
 int fw_rho_kernel_times(fw_rho* h, double us_out[3]);
 

@@ -115,15 +115,89 @@ def test_wigner_transform_and_time(gpu):
     assert rel_l2(W.real, Wa) <= 1e-12
     # Wigner time after some propagation, with the reference's indicator
     from fastwigner_amd.rho_vneumann import RhoVNeumannHIP1D
-    g2 = RhoVNeumannHIP1D(X_gridDIM=256, X_amplitude=10., dt=0.01, t=0., V=VQ, K=KQ, Ip=0.6)
+    # Ip < 0: the region H < -Ip = 0.8 cuts through the state (H >= 0 here, so Ip >= 0 would
+    # give the indicator 1/2 everywhere and a Wigner time of 0)
+    g2 = RhoVNeumannHIP1D(X_gridDIM=256, X_amplitude=10., dt=0.01, t=0., V=VQ, K=KQ, Ip=-0.8)
     g2.set_rho(r0)
     w_init = g2.get_wignerfunction().get().real
     g2.propagate(30)
     o.propagate(30)
     wc = g2.get_wignerfunction()
     got = g2.get_wigner_time(wc, w_init, g2.t)
-    ref = o.get_wigner_time(o.get_wignerfunction(), w_init, o.t, o.K, o.V, 0.6)
+    ref = o.get_wigner_time(o.get_wignerfunction(), w_init, o.t, o.K, o.V, -0.8)
+    print("Wigner time after 30 steps: GPU %.15f oracle %.15f" % (got, ref))
+    assert abs(ref) > 1e-3
     assert abs(got - ref) <= 1e-12 * max(1.0, abs(ref))
+
+
+@pytest.mark.parametrize("K,dK", [(KQ, "P"), ("0.5*P*P + 0.05*P*sin(0.3*t)", "P + 0.05*sin(0.3*t)")])
+def test_wigner_time_every_step_through_propagate(gpu, K, dK):
+    """rho-Ehrenfest with wigner_initial and Ip (rho_vneumann_cuda_1d.py:408-412):
+    propagate() computes the Wigner function and its Wigner time after every
+    step on the device, 70 steps (a graph-replayed chunk of 64, then 6 launched
+    steps; momentum weights or diagonal FFT sums), against the oracle stepped
+    one step at a time; Ip set after construction as an attribute."""
+    g, o = make(n=128, K=K, dK=dK)
+    r0 = gaussian_rho(o.Xv, 1.0, 0.5, 1.0)
+    g.set_rho(r0)
+    o.set_rho(r0)
+    w_init = np.ascontiguousarray(np.real(o.get_wignerfunction()))
+    g.wigner_initial = w_init
+    g.Ip = -0.8           # the region H < 0.8 cuts through the state (H >= 0 here)
+    steps = 70
+    g.propagate(steps)
+    ref, xs, ps = [], [], []
+    for _ in range(steps):
+        o.propagate(1)
+        ref.append(o.get_wigner_time(o.get_wignerfunction(), w_init, o.t, o.K, o.V, -0.8))
+        xs.append(o.get_average((lambda X, t: X,)))
+        ps.append(o.get_average((None, lambda P, t: P)))
+    ref = np.array(ref)
+    got = np.array(g.wigner_time)
+    assert got.shape == (steps,)
+    print("Wigner time max |GPU - oracle| = %.2e (range %.3f .. %.3f)" % (np.max(np.abs(got - ref)), ref.min(),
+                                                                        ref.max()))
+    assert np.ptp(ref) > 1e-3
+    assert np.max(np.abs(got - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref)))
+    assert rel_l2(g.wigner_current.get(), o.get_wignerfunction()) <= 1e-12
+    assert rel_l2(g.rho.get(), o.rho) <= 1e-12
+    assert np.max(np.abs(np.array(g.X_average) - xs)) <= 1e-11
+    assert np.max(np.abs(np.array(g.P_average) - ps)) <= 1e-11
+    # a second run continues the series, with W_init kept on the device
+    g.propagate(5)
+    for _ in range(5):
+        o.propagate(1)
+        ref = np.append(ref, o.get_wigner_time(o.get_wignerfunction(), w_init, o.t, o.K, o.V, -0.8))
+    assert np.max(np.abs(np.array(g.wigner_time) - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref)))
+    # only the handle's own Wigner function is accepted by get_wigner_time
+    with pytest.raises(ValueError):
+        g.get_wigner_time(o.get_wignerfunction(), w_init, g.t)
+
+
+def test_wigner_time_2048_properties(gpu):
+    """The strong-field density-matrix size (2048^2, strong_field_physics_cuda_1d.py:28):
+    100 steps with the per-step Wigner time on the device. Trace 1, Hermiticity,
+    purity kept by the unitary steps, the Wigner function normalised and real,
+    and the Wigner time of a state whose H stays above -Ip everywhere equal to
+    half the change of its norm, 0, at every step."""
+    n = 2048
+    g, _ = make(n=n, amp=40., dt=0.02)
+    r0 = gaussian_rho(g.X, 1.0, 0.5, 1.0)
+    g.set_rho(r0)
+    w0 = g.get_wignerfunction().get()
+    g.wigner_initial = np.ascontiguousarray(w0.real)
+    g.Ip = 0.5          # H >= 0 > -Ip: indicator 1/2 everywhere, Wigner time = (norm(W) - norm(W_init)) / 2 = 0
+    g.propagate(100)
+    wt = np.array(g.wigner_time)
+    assert wt.shape == (100,) and np.all(np.isfinite(wt))
+    assert np.max(np.abs(wt)) <= 1e-12
+    rho = g.rho.get()
+    assert abs(np.trace(rho) * g.dX - 1.0) <= 1e-12
+    assert np.max(np.abs(rho - rho.conj().T)) <= 1e-12 * np.max(np.abs(rho))
+    assert abs(g.purity_series[-1] - 1.0) <= 1e-10
+    W = g.wigner_current.get()
+    assert abs(W.real.sum() * g.wigner_dxdp - 1.0) <= 1e-12
+    assert np.max(np.abs(W.imag)) <= 1e-10 * np.max(np.abs(W.real))
 
 
 def test_rho_bloch_gibbs(gpu):
@@ -187,3 +261,77 @@ def test_fp32_density_matrix(gpu):
     err = rel_l2(g.rho.get(), o.rho)
     print("vN fp32 256^2 500 steps rel L2 vs fp64 oracle:", err)
     assert err <= 1e-4
+
+
+DW_V, DW_K, DW_DV = "-0.5*X*X + X*X*X*X/20.", "0.5*P*P", "-X + X*X*X/5."
+
+
+def _double_well_thermal(n=1024):
+    from fastwigner_amd.rho_bloch import RhoBlochHIP1D
+    b = RhoBlochHIP1D(X_gridDIM=n, X_amplitude=10., kT=0.2, dbeta=0.005, t=0., V=DW_V, K=DW_K)
+    return b, b.get_gibbs_state().get()
+
+
+@pytest.mark.parametrize("kick", [0.0, 1.0])
+def test_double_well_fp32_against_fp64_1024(gpu, kick):
+    """Config 2's von Neumann part in both forms: the double-well thermal state
+    at 1024^2 (x in [-10, 10), RhoBloch from rho = I, kT = 0.2), as prescribed
+    (stationary) and with a momentum kick exp(i p0 (x - x')) so that the series
+    move; 2000 steps of dt = 0.005 in FP64 and in FP32 (complex64 state and
+    FFTs, FP64 phases and sums); the FP32 state and series against the FP64 run
+    (itself at 1e-12 of the oracle on the smaller parity cases)."""
+    from fastwigner_amd.rho_vneumann import RhoVNeumannHIP1D
+    b, rho_th = _double_well_thermal()
+    X = b.X
+    rho0 = rho_th * np.exp(1j * kick * (X[np.newaxis, :] - X[:, np.newaxis]))
+    runs = {}
+    for prec in ("fp64", "fp32"):
+        vn = RhoVNeumannHIP1D(X_gridDIM=1024, X_amplitude=10., dt=0.005, t=0., V=DW_V, K=DW_K, diff_V=DW_DV,
+                              diff_K="P", precision=prec)
+        vn.set_rho(rho0)
+        vn.propagate(2000)
+        runs[prec] = (vn.rho.get(), np.array(vn.X_average), np.array(vn.P_average), np.array(vn.hamiltonian_average))
+        del vn
+    r64, x64, p64, h64 = runs["fp64"]
+    r32, x32, p32, h32 = runs["fp32"]
+    err = rel_l2(r32, r64)
+    ex, ep, eh = np.max(np.abs(x32 - x64)), np.max(np.abs(p32 - p64)), np.max(np.abs(h32 - h64))
+    print("double well rho 1024^2, kick %.1f, 2000 steps: FP32 vs FP64 rel L2 %.2e; max |d<x>| %.2e, "
+          "|d<p>| %.2e, |d<H>| %.2e (<x> range %.3f, <H> %.4f)" % (kick, err, ex, ep, eh, np.ptp(x64), h64[0]))
+    assert err <= 1e-4
+    assert max(ex, ep) <= 2e-4 * max(1.0, np.max(np.abs(p64)), np.max(np.abs(x64)))
+    # <H> = <K> + <V>: the momentum grid of the density matrix reaches |P| = pi/dX = 161 here,
+    # and <K> weights the FP32 rounding noise of the state there by P^2/2 (up to 1.3e4), so the
+    # FP32 energy is far less accurate than the state (the FP64 run sees the same effect at
+    # 1e-9, its energy drift). The bound records the measured size, it is no accuracy claim.
+    assert eh <= 0.3
+    if kick:
+        assert np.ptp(x64) > 0.1          # the kicked state moves
+
+
+def test_thermal_density_matrix_against_wigner_bloch(gpu):
+    """The two thermal states of config 2 agree: the density-matrix Bloch state
+    (1024^2, x in [-10, 10), rho_bloch_cuda_1d.py:124-160) and the Wigner Bloch
+    state (512^2, x, p in [-10, 10), wigner_bloch_cuda_1d.py:129-165), both in
+    the CUDA convention, which gives exp(-H/kT) at kT = 0.2 for each (exponents
+    dbeta/4 on V and dbeta/2 on K per half-step). The two are the same
+    band-limited operator splitting in two representations: <x^2>, <p^2>, <H>
+    and the purity agree to 1e-10 (1e-14 measured on the CPU oracle at 128^2
+    against 256^2)."""
+    from fastwigner_amd.wigner_bloch import WignerBlochHIP1D
+    b, _ = _double_well_thermal()
+    rx2 = b.get_average(("X * X",))
+    rp2 = b.get_average((None, "P * P"))
+    rH = b.get_average(("-0.5*X*X + X*X*X*X/20.",)) + b.get_average((None, "0.5*P*P"))
+    rpur = b.get_purity()
+    w = WignerBlochHIP1D(X_gridDIM=512, X_amplitude=10., P_gridDIM=512, P_amplitude=10., kT=0.2, dbeta=0.005,
+                         t=0., V=DW_V, K=DW_K, diff_V=DW_DV, diff_K="P")
+    w.get_gibbs_state()
+    wx2 = w.get_average("X * X")
+    wp2 = w.get_average("P * P")
+    wH = w.get_average("0.5*P*P - 0.5*X*X + X*X*X*X/20.")
+    wpur = w.get_purity()
+    rel = [abs(a - c) / abs(c) for a, c in ((wx2, rx2), (wp2, rp2), (wH, rH), (wpur, rpur))]
+    print("thermal rho 1024^2 vs Wigner Bloch 512^2: <x^2> %.10f/%.10f <p^2> %.10f/%.10f <H> %.10f/%.10f "
+          "purity %.10f/%.10f, max rel %.1e" % (rx2, wx2, rp2, wp2, rH, wH, rpur, wpur, max(rel)))
+    assert max(rel) <= 1e-10
