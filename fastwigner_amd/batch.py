@@ -20,7 +20,7 @@ import ctypes
 
 import numpy as np
 
-from . import _lib
+from . import _lib, tuning
 from ._common import precision_code
 
 SIZES = (64, 128, 256, 512, 1024)
@@ -63,6 +63,10 @@ class WignerBatchHIP1D:
                 raise ValueError("X_gridDIM must be one of %s" % (SIZES,))
             d.setdefault("P_amplitude", d.get("X_amplitude"))
             specs.append((n, d["X_amplitude"], d["P_amplitude"], d.get("params", ())))
+        if e_for_n is None:
+            tuned = tuning.batch_entry(precision_code(precision))
+            e_for_n = tuned["e_for_n"] if tuned else None
+        self.e_for_n = e_for_n
         self._desc = _lib.batch_desc(self.n, dt, precision_code(precision), device, self.mode, phase_time_shift,
                                      e_for_n, blocks_per_cu)
         self._inst = _lib.batch_instances(specs)

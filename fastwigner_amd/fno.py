@@ -239,6 +239,65 @@ def checked_jump(surrogate, W0, feat, ref_states, dXdP, l2_tol=1e-3, norm_tol=1e
     return out, ok, err, drift
 
 
+def forward_flops(batch, n=256, width=32, modes=16, layers=4):
+    """FLOPs (2 M N K per GEMM) of one forward over `batch` inputs, stage by stage
+    as csrc/fw_fno.cpp runs them: per layer the x-DFT (2M x N times N x C per
+    row), the p-DFT, the per-mode channel mixing, the inverse p- and x-DFTs and
+    the pointwise C x C mixing. Returns (total, FP32-MFMA part): the DFT stages
+    run on FP32 MFMA, the two mixings on BF16 MFMA."""
+    C, M, N, B = width, modes, n, batch
+    KP, Q, QO, nm = 2 * M, 2 * M * width, M * width, 2 * M * M
+    s1 = 2 * (2 * M) * C * N * (B * N)
+    s2 = 2 * (2 * KP) * Q * N * B
+    s3 = 2 * B * (2 * C) * (2 * C) * nm
+    s4 = 2 * (2 * N) * QO * (2 * KP) * B
+    s5 = 2 * (2 * N * C * M * (B * N))
+    pw = 2 * (B * N * N) * C * C
+    return layers * (s1 + s2 + s3 + s4 + s5 + pw), layers * (s1 + s2 + s4 + s5)
+
+
+def hamiltonian_grid(n, amplitude, a2, a3, a4):
+    """H(x, p) = p^2/2 + a2 x^2 + a3 x^3 + a4 x^4 on the n x n grid (rows p, columns x)."""
+    d = 2. * amplitude / n
+    x = d * (np.arange(n) - n / 2)
+    return (a2 * x ** 2 + a3 * x ** 3 + a4 * x ** 4)[np.newaxis, :] + (0.5 * x ** 2)[:, np.newaxis]
+
+
+def energy_law(e_in, e_gibbs, gamma, k, dt):
+    """<H> after k random-collision steps of the split operator: each step is
+    dissipator, Moyal step, dissipator, the dissipator W <- w W + (1 - w) W_G
+    (w = exp(-gamma dt/2), wigner_random_collision_model_fftw_1d.py:32,63-74)
+    moves <H> to w <H> + (1 - w) <H>_G and the Moyal step keeps <H> (to the
+    splitting error), so <H>_k = <H>_G + w^(2k) (<H>_0 - <H>_G)."""
+    w = np.exp(-np.asarray(gamma) * dt / 2.)
+    return e_gibbs + w ** (2 * k) * (e_in - e_gibbs)
+
+
+def physical_referee(pred, w_in, feat, e_gibbs, k, dt, amplitude, norm_tol=1e-4, energy_tol=1e-3):
+    """Referee of k-step jumps that needs no reference propagation: a prediction
+    is accepted when its norm drift |sum(W) dXdP - 1| <= norm_tol, its energy
+    drift |<H>_pred - <H>_law| / max(|<H>_0|, 1) <= energy_tol against the
+    relaxation law of the random-collision step (energy_law), and its purity
+    2 pi sum(W^2) dXdP / norm^2 <= 1. w_in: the states the jumps start from.
+    Returns (ok, norm drift, energy drift, purity)."""
+    pred = np.asarray(pred, dtype=np.float64)
+    n = pred.shape[-1]
+    dXdP = (2. * amplitude / n) ** 2
+    drift, edrift, pur = np.empty(len(pred)), np.empty(len(pred)), np.empty(len(pred))
+    for i in range(len(pred)):
+        a2, a3, a4, gamma = (float(v) for v in feat[i][:4])
+        H = hamiltonian_grid(n, amplitude, a2, a3, a4)
+        s = pred[i].sum() * dXdP
+        s0 = w_in[i].sum() * dXdP
+        e_pred = (pred[i] * H).sum() * dXdP / s
+        e_in = (w_in[i] * H).sum() * dXdP / s0
+        drift[i] = abs(s - 1.0)
+        edrift[i] = abs(e_pred - energy_law(e_in, e_gibbs[i], gamma, k, dt)) / max(abs(e_in), 1.0)
+        pur[i] = 2 * np.pi * (pred[i] ** 2).sum() * dXdP / (s * s)
+    ok = (drift <= norm_tol) & (edrift <= energy_tol) & (pur <= 1.0)
+    return ok, drift, edrift, pur
+
+
 def surrogate_experiment(n_pairs=20000, pairs_per_instance=20, held_out=512, epochs=2.0, batch=8, lr=2e-3,
                          seed=4, held_seed=5, base_steps=2, out_scale=1e-2, zero_mass=True, log=None):
     """The learned-surrogate workload: a 4-layer, 32-channel, 16 x 16-mode FNO on
@@ -273,8 +332,12 @@ def surrogate_experiment(n_pairs=20000, pairs_per_instance=20, held_out=512, epo
                              diff_V=QUARTIC_DV, diff_K="P", precision="fp64")
     referee.prepare_random_collision(dbeta=0.01)
     dXdP = (16. / 256) ** 2
-    phys = [i for i in range(held_out)
-            if 2 * np.pi * np.sum(referee.get_wignerfunction(i) ** 2) * dXdP <= 1.0 + 1e-3]
+    gibbs = [referee.get_wignerfunction(i) for i in range(held_out)]
+    phys = [i for i in range(held_out) if 2 * np.pi * np.sum(gibbs[i] ** 2) * dXdP <= 1.0 + 1e-3]
+    # <H> of each Gibbs state (the dissipator's target), for the referee's energy law
+    e_gibbs = np.array([(gibbs[i] * hamiltonian_grid(256, 8., *inst[i]["params"])).sum() * dXdP /
+                        (gibbs[i].sum() * dXdP) for i in phys])
+    del gibbs
     W0 = np.stack([gaussian_state(256, 8., d["x0"], d["p0"]) for d in inst])
     for i in range(held_out):
         referee.set_wignerfunction(i, W0[i])
@@ -302,8 +365,28 @@ def surrogate_experiment(n_pairs=20000, pairs_per_instance=20, held_out=512, epo
         WJ.reshape(len(phys), -1), axis=1)
     ident = rel(W0p)
     base_err = rel(Xin)
+    # the referee without the FP64 answer, on the same predictions; its cost per network batch
+    pred = s.predict(Xin, feat)
+    t0 = time.perf_counter()
+    pok, pdrift, pedrift, ppur = physical_referee(pred, W0p, feat, e_gibbs, 20, 0.01, 8.)
+    phys_us = (time.perf_counter() - t0) * 1e6 * batch / len(phys)
+    # the same law on the FP64 answers: the splitting's own energy drift, the floor of the criterion
+    _, _, ref_edrift, _ = physical_referee(WJ, W0p, feat, e_gibbs, 20, 0.01, 8.)
+    # a sampled FP64 audit of one jump in audit_every: its share of the FP64 referee's cost
+    audit_every = 8
+    rng = np.random.default_rng(held_seed)
+    audited = rng.choice(len(phys), size=max(1, len(phys) // audit_every), replace=False)
+    fp64_us_per_jump = t_referee * 1e6 / held_out
+    audit_us = fp64_us_per_jump * batch / audit_every
     us = s.infer_time_us(batch, reps=20)
     rate = batch * 256 * 256 * 20 / ((us + base_us) * 1e-6)
+    rate_refereed = batch * 256 * 256 * 20 / ((us + base_us + phys_us + audit_us) * 1e-6)
+    flops, flops32 = forward_flops(batch)
+    roof = dict(bound="mfma", kernel="FNO forward (truncated-DFT GEMMs on FP32 MFMA, mixings on BF16 MFMA)",
+                achieved=round(flops / (us * 1e-6) / 1e12, 2), peak=157.3, unit="TFLOP/s",
+                frac=round(flops / (us * 1e-6) / 1e12 / 157.3, 4), traffic=None, flops_per_forward=flops,
+                fp32_mfma_flops_per_forward=flops32,
+                note="peak: FP32 MFMA, which the DFT stages (70% of the FLOPs) use; the mixings' BF16 peak is 2.5 PFLOP/s")
     out = dict(pairs=s.npairs, train_instances=len(kept), adam_steps=steps, batch=batch, train_s=round(t_train, 1),
                data_s=round(t_data, 1), held_out=len(phys), held_out_unphysical=held_out - len(phys),
                rel_l2_median=float(np.median(err)), rel_l2_mean=float(np.mean(err)), rel_l2_max=float(np.max(err)),
@@ -312,11 +395,24 @@ def surrogate_experiment(n_pairs=20000, pairs_per_instance=20, held_out=512, epo
                base_rel_l2_median=float(np.median(base_err)), base_rel_l2_mean=float(np.mean(base_err)),
                base_accepted=int(np.sum(base_err <= 1e-3)), norm_drift_max=float(np.max(drift)),
                accepted=int(ok.sum()), fallback=int((~ok).sum()),
-               infer_us_per_batch=round(us, 1), base_us_per_batch=round(base_us, 1), value=rate,
-               unit="grid-point*steps/s", referee_fp64_s_for_held_out=round(t_referee, 3))
-    say("[fno] held-out: rel L2 median %.3e (base method %.3e, identity %.3e), accepted %d of %d; %.3e pt*steps/s"
-        % (out["rel_l2_median"], out["base_rel_l2_median"], out["identity_rel_l2_mean"], out["accepted"],
-           len(phys), rate))
+               infer_us_per_batch=round(us, 1), base_us_per_batch=round(base_us, 1),
+               value=rate_refereed, value_without_referee=rate, roofline=roof,
+               unit="grid-point*steps/s", referee_fp64_s_for_held_out=round(t_referee, 3),
+               physical_referee=dict(
+                   criteria="norm drift <= 1e-4, energy drift against the relaxation law <= 1e-3, purity <= 1",
+                   accepted=int(pok.sum()), false_accepts_l2_above_1e3=int(np.sum(pok & (err > 1e-3))),
+                   accepted_and_l2_ok=int(np.sum(pok & (err <= 1e-3))),
+                   energy_drift_median=float(np.median(pedrift)), energy_drift_max=float(np.max(pedrift)),
+                   fp64_energy_drift_median=float(np.median(ref_edrift)), fp64_energy_drift_max=float(np.max(ref_edrift)),
+                   purity_max=float(np.max(ppur)), us_per_batch=round(phys_us, 1),
+                   audit_every=audit_every, audited=int(len(audited)),
+                   audit_failures=int(np.sum(pok[audited] & (err[audited] > 1e-3))),
+                   audit_us_per_batch=round(audit_us, 1)))
+    say("[fno] held-out: rel L2 median %.3e (base method %.3e, identity %.3e), accepted %d of %d by the FP64 "
+        "referee; %d by the physical referee (%d of them above 1e-3); %.3e pt*steps/s with the referee "
+        "(%.3e without)" % (out["rel_l2_median"], out["base_rel_l2_median"], out["identity_rel_l2_mean"],
+                            out["accepted"], len(phys), int(pok.sum()), int(np.sum(pok & (err > 1e-3))),
+                            rate_refereed, rate))
     return out
 
 

@@ -26,7 +26,7 @@ import ctypes
 
 import numpy as np
 
-from . import _lib
+from . import _lib, tuning
 from ._common import DeviceArray, bind_kwargs, consts_source, precision_code
 
 _GRID_NAMES = ("X_gridDIM", "X_amplitude", "dt", "dX", "dP")
@@ -80,8 +80,9 @@ class RhoVNeumannHIP1D:
         self.abs_boundary_p = mp if isinstance(mp, str) else "1."
 
         self._handle = None
+        tiles = getattr(self, "tiles", None) or tuning.lookup("rho", self.X_gridDIM, self.X_gridDIM, self._prec)
         desc = _lib.rho_desc(self.X_gridDIM, self.X_amplitude, self.dt, self.phase_time_shift, self._prec,
-                             self.device, getattr(self, "tiles", None))
+                             self.device, tiles)
         dv = getattr(self, "diff_V", None)
         dk = getattr(self, "diff_K", None)
         ex = _lib.rho_exprs(V=self.V, K=self.K, consts=self.hip_consts,

@@ -38,13 +38,26 @@ def lookup(kind, nx, npp, precision):
     return tuple(e["tiles"]) if e else None
 
 
-def store(kind, nx, npp, precision, tiles, info=None, path=PATH):
-    """Add or replace one entry of the tuning file at ``path`` (merging with what it holds)."""
+def batch_entry(precision):
+    """The tuned batch schedule ({"e_for_n": [E for N = 64 .. 1024], ...}) or None."""
+    return _load().get("batch:%s" % ("fp32" if precision else "fp64"))
+
+
+def store_entry(name, value, path=PATH):
+    """Add or replace the entry `name` of the tuning file at ``path`` (merging with what it holds)."""
+    global _cache
     try:
         with open(path) as f:
             data = json.load(f)
     except (OSError, ValueError):
         data = {}
-    data[key(kind, nx, npp, precision)] = {"tiles": list(tiles), **(info or {})}
+    data[name] = value
     with open(path, "w") as f:
         json.dump(data, f, indent=1, sort_keys=True)
+    if path == PATH:
+        _cache = None
+
+
+def store(kind, nx, npp, precision, tiles, info=None, path=PATH):
+    """Add or replace one entry of the tuning file at ``path`` (merging with what it holds)."""
+    store_entry(key(kind, nx, npp, precision), {"tiles": list(tiles), **(info or {})}, path=path)
