@@ -126,32 +126,53 @@ def strong_field_variant(n=1024, steps=2000, warmup=64):
     return out
 
 
-def rho_variant(n=1024, steps=1000, warmup=64):
-    """Double-well density matrix: the thermal state at 1024^2
-    (x in [-10, 10), kT = 0.2, RhoBloch from rho = I), then von Neumann steps
-    (dt = 0.005) with the x-family observables every step; complex128, three
-    passes per step of 32 B per matrix element each (96 B per element per step)."""
+def rho_variant(n=1024, steps=2000, warmup=64):
+    """Double-well density matrix (config 2): the thermal state at 1024^2
+    (x in [-10, 10), kT = 0.2, RhoBloch from rho = I), kicked by exp(i (x - x')),
+    then 2000 von Neumann steps (dt = 0.005) with the Ehrenfest series every
+    step, in complex128 (three passes of 32 B per element: 96 B per element per
+    step) and in complex64 with FP64 phases and sums (48 B), the latter's error
+    against the FP64 run stated."""
     from fastwigner_amd.rho_bloch import RhoBlochHIP1D
     from fastwigner_amd.rho_vneumann import RhoVNeumannHIP1D
     V, K = "-0.5*X*X + X*X*X*X/20.", "0.5*P*P"
     b = RhoBlochHIP1D(X_gridDIM=n, X_amplitude=10., kT=0.2, dbeta=0.005, t=0., V=V, K=K)
     rho_th = b.get_gibbs_state().get()
+    X = b.X
     del b
-    vn = RhoVNeumannHIP1D(X_gridDIM=n, X_amplitude=10., dt=0.005, t=0., V=V, K=K, diff_V="-X + X*X*X/5.",
-                          diff_K="P")
-    vn.set_rho(rho_th)
-    vn.propagate(warmup)
-    t0 = time.perf_counter()
-    vn.propagate(steps)
-    el = time.perf_counter() - t0
-    us = el * 1e6 / steps
-    achieved = 96.0 * n * n / (us * 1e-6) / 1e9
-    out = {"value": n * n * steps / el, "unit": "matrix-element*steps/s", "ms_per_step": el * 1e3 / steps,
-           "steps": steps, "dtype": "c128",
-           "roofline": {"bound": "hbm", "kernel": "von Neumann step (three passes)", "achieved": round(achieved, 1),
-                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                        "traffic": None, "bytes_per_step": 96 * n * n}}
-    print("[bench] von Neumann c128 %dx%d: %.1f us/step" % (n, n, us), file=sys.stderr, flush=True)
+    # the thermal state with a momentum kick exp(i p0 (x - x')), so that the observables move
+    rho0 = rho_th * np.exp(1j * 1.0 * (X[np.newaxis, :] - X[:, np.newaxis]))
+    res = {}
+    for prec, esz in (("fp64", 16), ("fp32", 8)):
+        vn = RhoVNeumannHIP1D(X_gridDIM=n, X_amplitude=10., dt=0.005, t=0., V=V, K=K, diff_V="-X + X*X*X/5.",
+                              diff_K="P", precision=prec)
+        vn.set_rho(rho0)
+        vn.propagate(warmup)
+        t0 = time.perf_counter()
+        vn.propagate(steps)
+        el = time.perf_counter() - t0
+        us = el * 1e6 / steps
+        achieved = 6.0 * esz * n * n / (us * 1e-6) / 1e9
+        res[prec] = dict(value=n * n * steps / el, ms=el * 1e3 / steps, achieved=achieved, esz=esz,
+                         rho=vn.rho.get(), x=np.array(vn.X_average), p=np.array(vn.P_average),
+                         h=np.array(vn.hamiltonian_average))
+        print("[bench] von Neumann %s %dx%d: %.1f us/step" % (prec, n, n, us), file=sys.stderr, flush=True)
+        del vn
+    r64, r32 = res["fp64"], res["fp32"]
+    out = {"value": r64["value"], "unit": "matrix-element*steps/s", "ms_per_step": r64["ms"], "steps": steps,
+           "dtype": "c128", "state": "double-well thermal state (kT = 0.2) kicked by exp(i (x - x'))",
+           "roofline": {"bound": "hbm", "kernel": "von Neumann step (three passes)",
+                        "achieved": round(r64["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(r64["achieved"] / HBM_PEAK_GBS, 4), "traffic": None,
+                        "bytes_per_step": 96 * n * n},
+           "fp32": {"value": r32["value"], "ms_per_step": r32["ms"], "dtype": "c64 (FP64 phases and sums)",
+                    "roofline_frac": round(r32["achieved"] / HBM_PEAK_GBS, 4),
+                    "rel_l2_vs_fp64": float(np.linalg.norm(r32["rho"] - r64["rho"]) / np.linalg.norm(r64["rho"])),
+                    "max_abs_err_x_average": float(np.max(np.abs(r32["x"] - r64["x"]))),
+                    "max_abs_err_p_average": float(np.max(np.abs(r32["p"] - r64["p"]))),
+                    "max_abs_err_hamiltonian_average": float(np.max(np.abs(r32["h"] - r64["h"]))),
+                    "note": "<H> in FP32 carries the state's rounding noise at |P| up to pi/dX = 161, weighted by "
+                            "P^2/2"}}
     return out
 
 

@@ -269,6 +269,7 @@ PROTOTYPES = {
     "fw_fno_get_params": [_P, _PF],
     "fw_fno_set_output": [_P, ctypes.c_float, _I],
     "fw_fno_forward": [_P, _PF, _PF, _I, _PF],
+    "fw_fno_moments": [_P, _I, _PD],
     "fw_fno_loss_grad": [_P, _PF, _PF, _PF, _I, _PD, _PF],
     "fw_fno_dataset_reserve": [_P, _I],
     "fw_fno_dataset_put": [_P, _I, _P, _P, _PF],

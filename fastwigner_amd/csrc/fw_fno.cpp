@@ -381,6 +381,44 @@ __global__ void __launch_bounds__(256) fold_modes(const float* dWm, float* dR, i
   }
 }
 
+// Moments of predicted Wigner functions for the physical referee, in FP64:
+// per sample b, mom[b][0 .. N) = sum over p of W (the x marginal),
+// mom[b][N .. 2N) = sum over x (the p marginal), mom[b][2N .. 3N) = sum over x
+// of W^2 per row. Rows: block (p, b); columns: block (x chunk, b). Fixed order.
+__global__ void __launch_bounds__(256) moments_rows(const float* w, int N, double* mom) {
+  __shared__ double rs[256], rq[256];
+  const int p = blockIdx.x, b = blockIdx.y;
+  const float* row = w + ((size_t)b * N + p) * N;
+  double sv = 0.0, sq = 0.0;
+  for (int x = threadIdx.x; x < N; x += 256) {
+    const double v = row[x];
+    sv += v;
+    sq += v * v;
+  }
+  rs[threadIdx.x] = sv;
+  rq[threadIdx.x] = sq;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      rs[threadIdx.x] += rs[threadIdx.x + o];
+      rq[threadIdx.x] += rq[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    mom[(size_t)b * 3 * N + N + p] = rs[0];
+    mom[(size_t)b * 3 * N + 2 * N + p] = rq[0];
+  }
+}
+__global__ void __launch_bounds__(256) moments_cols(const float* w, int N, double* mom) {
+  const int x = blockIdx.x * 256 + threadIdx.x, b = blockIdx.y;
+  if (x >= N) return;
+  const float* col = w + (size_t)b * N * N + x;
+  double sv = 0.0;
+  for (int p = 0; p < N; ++p) sv += col[(size_t)p * N];
+  mom[(size_t)b * 3 * N + x] = sv;
+}
+
 __global__ void __launch_bounds__(256) to_bf16_kernel(const float* s, bf16_t* d, long long n) {
   for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) d[i] = to_bf16(s[i]);
 }
@@ -494,6 +532,7 @@ struct fw_fno {
   float *bin = nullptr, *btg = nullptr, *bft = nullptr;
   int* bidx = nullptr;
   double* persample = nullptr;
+  double* mom = nullptr;   // moments of the last forward's outputs (max_batch x 3N)
   // dataset
   float *din = nullptr, *dtg = nullptr, *dft = nullptr;
   int npairs = 0;
@@ -825,6 +864,7 @@ extern "C" int fw_fno_create(const fw_fno_desc* d, fw_fno** out) {
   FWCHK(dalloc(&h->bft, (size_t)B * 5));
   FWCHK(dalloc(&h->bidx, (size_t)B));
   FWCHK(dalloc(&h->persample, (size_t)B));
+  FWCHK(dalloc(&h->mom, (size_t)B * 3 * N));
   FWCHK(refresh_weights(h));
   HIPCHK(hipStreamSynchronize(h->stream));
   *out = guard.release();
@@ -836,7 +876,7 @@ extern "C" int fw_fno_destroy(fw_fno* h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   std::vector<void*> bufs = {h->P, h->G, h->Ma, h->Va, h->Wm, h->Wpb, h->Fx, h->Fp, h->Gp, h->Gxr, h->Gxi, h->T1,
                              h->Gb, h->Y, h->Yst, h->H, h->dv, h->dv2, h->dWm, h->part, h->out, h->dout, h->bin,
-                             h->btg, h->bft, h->bidx, h->persample, h->din, h->dtg, h->dft};
+                             h->btg, h->bft, h->bidx, h->persample, h->mom, h->din, h->dtg, h->dft};
   for (auto p : h->v) bufs.push_back(p);
   for (auto p : h->pre) bufs.push_back(p);
   for (auto p : h->Zs) bufs.push_back(p);
@@ -897,6 +937,19 @@ extern "C" int fw_fno_forward(fw_fno* h, const float* host_in, const float* host
   FWCHK(upload_batch(h, host_in, host_feat, nullptr, batch));
   FWCHK(forward(h, batch));
   HIPCHK(hipMemcpyAsync(host_out, h->out, sizeof(float) * batch * h->N * h->N, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return FW_OK;
+}
+
+extern "C" int fw_fno_moments(fw_fno* h, int batch, double* host_mom) {
+  fw::clear_error();
+  if (!h || !host_mom) return fw::fail(FW_ERR_INVALID_ARG, "NULL argument");
+  FWCHK(ensure_batch(h, batch));
+  const int N = h->N;
+  hipLaunchKernelGGL(fno::moments_rows, dim3(N, batch), dim3(256), 0, h->stream, h->out, N, h->mom);
+  hipLaunchKernelGGL(fno::moments_cols, dim3((N + 255) / 256, batch), dim3(256), 0, h->stream, h->out, N, h->mom);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(host_mom, h->mom, sizeof(double) * 3 * N * batch, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
   return FW_OK;
 }

@@ -103,18 +103,36 @@ class FNOSurrogate:
         return P
 
     # -- inference and gradients (host arrays) ---------------------------------------------
-    def predict(self, W, feat):
-        """W(t + k dt) for a batch: W [B, N, N], feat [B, 5] = (a2, a3, a4, gamma, kT)."""
+    def predict(self, W, feat, moments=False):
+        """W(t + k dt) for a batch: W [B, N, N], feat [B, 5] = (a2, a3, a4, gamma, kT).
+        moments=True also returns the predictions' moments computed on the device
+        (fw_fno_moments: x marginal, p marginal, row sums of W^2; [B, 3N] FP64)."""
         W = np.ascontiguousarray(W, dtype=np.float32)
         feat = np.ascontiguousarray(feat, dtype=np.float32)
         out = np.empty_like(W)
+        mom = np.empty((W.shape[0], 3 * self.N)) if moments else None
         for s in range(0, W.shape[0], self.max_batch):
             e = min(s + self.max_batch, W.shape[0])
             o = np.empty((e - s,) + W.shape[1:], dtype=np.float32)
             _lib.call("fw_fno_forward", self._handle, _lib.fptr(np.ascontiguousarray(W[s:e])),
                       _lib.fptr(np.ascontiguousarray(feat[s:e])), e - s, _lib.fptr(o))
             out[s:e] = o
-        return out
+            if moments:
+                m = np.empty((e - s, 3 * self.N))
+                _lib.call("fw_fno_moments", self._handle, e - s, _lib.dptr(m))
+                mom[s:e] = m
+        return (out, mom) if moments else out
+
+    def moments_time_us(self, batch=None, reps=20):
+        """Wall time of fw_fno_moments for one batch (after a forward), copy to the host included."""
+        import time
+        batch = batch or self.max_batch
+        m = np.empty((batch, 3 * self.N))
+        _lib.call("fw_fno_moments", self._handle, batch, _lib.dptr(m))
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            _lib.call("fw_fno_moments", self._handle, batch, _lib.dptr(m))
+        return (time.perf_counter() - t0) * 1e6 / reps
 
     def loss_grad(self, W, feat, target, want_grad=True):
         W = np.ascontiguousarray(W, dtype=np.float32)
@@ -273,29 +291,52 @@ def energy_law(e_in, e_gibbs, gamma, k, dt):
     return e_gibbs + w ** (2 * k) * (e_in - e_gibbs)
 
 
-def physical_referee(pred, w_in, feat, e_gibbs, k, dt, amplitude, norm_tol=1e-4, energy_tol=1e-3):
-    """Referee of k-step jumps that needs no reference propagation: a prediction
-    is accepted when its norm drift |sum(W) dXdP - 1| <= norm_tol, its energy
-    drift |<H>_pred - <H>_law| / max(|<H>_0|, 1) <= energy_tol against the
-    relaxation law of the random-collision step (energy_law), and its purity
-    2 pi sum(W^2) dXdP / norm^2 <= 1. w_in: the states the jumps start from.
-    Returns (ok, norm drift, energy drift, purity)."""
-    pred = np.asarray(pred, dtype=np.float64)
-    n = pred.shape[-1]
-    dXdP = (2. * amplitude / n) ** 2
-    drift, edrift, pur = np.empty(len(pred)), np.empty(len(pred)), np.empty(len(pred))
-    for i in range(len(pred)):
-        a2, a3, a4, gamma = (float(v) for v in feat[i][:4])
-        H = hamiltonian_grid(n, amplitude, a2, a3, a4)
-        s = pred[i].sum() * dXdP
-        s0 = w_in[i].sum() * dXdP
-        e_pred = (pred[i] * H).sum() * dXdP / s
-        e_in = (w_in[i] * H).sum() * dXdP / s0
-        drift[i] = abs(s - 1.0)
-        edrift[i] = abs(e_pred - energy_law(e_in, e_gibbs[i], gamma, k, dt)) / max(abs(e_in), 1.0)
-        pur[i] = 2 * np.pi * (pred[i] ** 2).sum() * dXdP / (s * s)
+def state_moments(W):
+    """The moments fw_fno_moments computes on the device, here from host arrays:
+    [B, 3N] = [x marginal | p marginal | row sums of W^2]."""
+    W = np.asarray(W, dtype=np.float64)
+    return np.concatenate([W.sum(axis=1), W.sum(axis=2), (W * W).sum(axis=2)], axis=1)
+
+
+def physical_referee(mom, e_in, feat, e_gibbs, k, dt, amplitude, norm_tol=1e-4, energy_tol=1e-3):
+    """Referee of k-step jumps that needs no reference propagation, from the
+    predictions' moments (state_moments / fw_fno_moments, [B, 3N]): a
+    prediction is accepted when its norm drift |sum(W) dXdP - 1| <= norm_tol,
+    its energy drift |<H>_pred - <H>_law| / max(|<H>_0|, 1) <= energy_tol
+    against the relaxation law of the random-collision step (energy_law, from
+    e_in = <H> of the states the jumps start from), and its purity
+    2 pi sum(W^2) dXdP / norm^2 <= 1. The energies come from the marginals
+    (H = K(p) + V(x)). Returns (ok, norm drift, energy drift, purity)."""
+    mom = np.asarray(mom, dtype=np.float64)
+    feat = np.asarray(feat, dtype=np.float64)
+    n = mom.shape[1] // 3
+    d = 2. * amplitude / n
+    dXdP = d * d
+    x = d * (np.arange(n) - n / 2)
+    V = feat[:, 0:1] * x ** 2 + feat[:, 1:2] * x ** 3 + feat[:, 2:3] * x ** 4       # (B, N), along x
+    K = 0.5 * x ** 2                                                                  # along p
+    mx, mp, sq = mom[:, :n], mom[:, n:2 * n], mom[:, 2 * n:]
+    tot = mx.sum(axis=1)
+    e_pred = ((mx * V).sum(axis=1) + mp @ K) / tot
+    s = tot * dXdP
+    drift = np.abs(s - 1.0)
+    e_in = np.asarray(e_in, dtype=np.float64)
+    edrift = np.abs(e_pred - energy_law(e_in, np.asarray(e_gibbs), feat[:, 3], k, dt)) / np.maximum(np.abs(e_in), 1.0)
+    pur = 2 * np.pi * sq.sum(axis=1) * dXdP / (s * s)
     ok = (drift <= norm_tol) & (edrift <= energy_tol) & (pur <= 1.0)
     return ok, drift, edrift, pur
+
+
+def state_energy(W, feat, amplitude):
+    """<H> of host states [B, N, N] for the potentials in feat (H = p^2/2 + V)."""
+    mom = state_moments(W)
+    feat = np.asarray(feat, dtype=np.float64)
+    n = mom.shape[1] // 3
+    d = 2. * amplitude / n
+    x = d * (np.arange(n) - n / 2)
+    V = feat[:, 0:1] * x ** 2 + feat[:, 1:2] * x ** 3 + feat[:, 2:3] * x ** 4
+    mx, mp = mom[:, :n], mom[:, n:2 * n]
+    return ((mx * V).sum(axis=1) + mp @ (0.5 * x ** 2)) / mx.sum(axis=1)
 
 
 def surrogate_experiment(n_pairs=20000, pairs_per_instance=20, held_out=512, epochs=2.0, batch=8, lr=2e-3,
@@ -365,13 +406,15 @@ def surrogate_experiment(n_pairs=20000, pairs_per_instance=20, held_out=512, epo
         WJ.reshape(len(phys), -1), axis=1)
     ident = rel(W0p)
     base_err = rel(Xin)
-    # the referee without the FP64 answer, on the same predictions; its cost per network batch
-    pred = s.predict(Xin, feat)
+    # the referee without the FP64 answer, on the same predictions, from their moments computed on
+    # the device; its cost per network batch: the moments (device + copy) and the host's O(N) checks
+    e_in = state_energy(W0p, feat, 8.)
+    _, mom = s.predict(Xin, feat, moments=True)
     t0 = time.perf_counter()
-    pok, pdrift, pedrift, ppur = physical_referee(pred, W0p, feat, e_gibbs, 20, 0.01, 8.)
-    phys_us = (time.perf_counter() - t0) * 1e6 * batch / len(phys)
+    pok, pdrift, pedrift, ppur = physical_referee(mom, e_in, feat, e_gibbs, 20, 0.01, 8.)
+    phys_us = (time.perf_counter() - t0) * 1e6 * batch / len(phys) + s.moments_time_us(batch)
     # the same law on the FP64 answers: the splitting's own energy drift, the floor of the criterion
-    _, _, ref_edrift, _ = physical_referee(WJ, W0p, feat, e_gibbs, 20, 0.01, 8.)
+    _, _, ref_edrift, _ = physical_referee(state_moments(WJ), e_in, feat, e_gibbs, 20, 0.01, 8.)
     # a sampled FP64 audit of one jump in audit_every: its share of the FP64 referee's cost
     audit_every = 8
     rng = np.random.default_rng(held_seed)

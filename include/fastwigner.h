@@ -390,6 +390,10 @@ int fw_fno_get_params(fw_fno* h, float* host_params);
 int fw_fno_set_output(fw_fno* h, float scale, int zero_mass);
 /* prediction W(t + k dt) for `batch` samples (host arrays: batch*N*N inputs, batch*5 features) */
 int fw_fno_forward(fw_fno* h, const float* host_in, const float* host_feat, int batch, float* host_out);
+/* Moments of the last fw_fno_forward's `batch` outputs for the physical referee, computed on the
+ * device in FP64: per sample 3N doubles, [x marginal (sum over p) | p marginal (sum over x) |
+ * per-row sum of W^2]. No counterpart in the reference. */
+int fw_fno_moments(fw_fno* h, int batch, double* host_mom);
 /* loss L = mean_b |pred_b - target_b|^2 / |target_b|^2 and, if host_grad != NULL, dL/dparams */
 int fw_fno_loss_grad(fw_fno* h, const float* host_in, const float* host_feat, const float* host_target, int batch,
                      double* loss, float* host_grad);

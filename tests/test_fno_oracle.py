@@ -103,7 +103,7 @@ def test_energy_law_of_the_physical_referee():
     splitting error (below 1e-4 of the energy scale after 20 and 60 steps),
     well inside the referee's 1e-3; the slip w^k for w^(2k) misses by > 5e-3."""
     from oracle import fw_oracle as fo
-    from fastwigner_amd.fno import energy_law, hamiltonian_grid, physical_referee
+    from fastwigner_amd.fno import energy_law, hamiltonian_grid, physical_referee, state_energy, state_moments
     a2, a3, a4, gamma, kT = 0.6, 0.05, 0.1, 0.08, 0.5
     n, amp, dt = 128, 8., 0.01
     V = lambda X, t: a2 * X * X + a3 * X ** 3 + a4 * X ** 4
@@ -123,10 +123,12 @@ def test_energy_law_of_the_physical_referee():
         assert abs(E(r.W) - wrong) > 5e-3 * max(abs(e0), 1.0)
         # the referee accepts the propagator's own answer and refuses a perturbed one
         feat = np.array([[a2, a3, a4, gamma, kT]])
-        ok, drift, edrift, pur = physical_referee(r.W[None], W0[None], feat, [eg], k, dt, amp)
+        e_in = state_energy(W0[None], feat, amp)
+        assert abs(e_in[0] - e0) <= 1e-12 * abs(e0)
+        ok, drift, edrift, pur = physical_referee(state_moments(r.W[None]), e_in, feat, [eg], k, dt, amp)
         assert ok[0] and drift[0] < 1e-12 and edrift[0] < 1e-4 and pur[0] <= 1.0
         # the state moved by three momentum cells: norm and purity unchanged, the energy not
         bad = np.roll(r.W, 3, axis=0)
-        okb, driftb, edriftb, purb = physical_referee(bad[None], W0[None], feat, [eg], k, dt, amp)
+        okb, driftb, edriftb, purb = physical_referee(state_moments(bad[None]), e_in, feat, [eg], k, dt, amp)
         assert driftb[0] < 1e-12 and abs(purb[0] - pur[0]) < 1e-12
         assert edriftb[0] > 1e-3 and not okb[0]

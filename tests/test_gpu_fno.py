@@ -146,3 +146,26 @@ def test_checked_jump_falls_back_to_the_propagator(gpu):
     assert ok.tolist() == [True, True, False, False]
     assert np.array_equal(out[2:], referee[2:])
     assert np.all(err[:2] <= 1e-3) and np.all(drift[:2] <= 1e-4)
+
+
+def test_device_moments_of_predictions(gpu):
+    """fw_fno_moments (x and p marginals, row sums of W^2 of the last forward's
+    outputs, FP64 on the device) against the same moments of the downloaded
+    predictions, and the physical referee's decisions from either."""
+    from fastwigner_amd.batch import gaussian_state
+    from fastwigner_amd.fno import FNOSurrogate, physical_referee, state_energy, state_moments
+    s = FNOSurrogate(grid_dim=256, width=32, modes=16, layers=4, max_batch=8, seed=1)
+    rng = np.random.default_rng(3)
+    W = np.stack([gaussian_state(256, 8., rng.uniform(-2, 2), rng.uniform(-2, 2)) for _ in range(11)])
+    feat = np.column_stack([rng.uniform(0.2, 1, 11), rng.uniform(-0.1, 0.1, 11), rng.uniform(0, 0.2, 11),
+                            rng.uniform(0, 0.1, 11), rng.uniform(0.1, 1, 11)])
+    pred, mom = s.predict(W, feat, moments=True)
+    ref = state_moments(pred)
+    assert mom.shape == ref.shape == (11, 3 * 256)
+    assert np.max(np.abs(mom - ref)) <= 1e-12 * np.max(np.abs(ref))
+    e_in = state_energy(W, feat, 8.)
+    a = physical_referee(mom, e_in, feat, np.zeros(11), 20, 0.01, 8.)
+    b = physical_referee(ref, e_in, feat, np.zeros(11), 20, 0.01, 8.)
+    assert np.array_equal(a[0], b[0])
+    for x, y in zip(a[1:], b[1:]):
+        assert np.max(np.abs(x - y)) <= 1e-10 * max(1.0, np.max(np.abs(y)))
