@@ -306,7 +306,14 @@ def main():
                          "known-size reference that scales the FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--batched-only", action="store_true",
                     help="only the batched FP32 random-collision workload (for profiling)")
+    ap.add_argument("--rho-only", action="store_true",
+                    help="only the double-well density-matrix workload, FP64 and FP32 (for profiling)")
     args = ap.parse_args()
+    if args.rho_only:
+        r = rho_variant(steps=args.steps)
+        r.pop("rho", None)
+        print(json.dumps(r), flush=True)
+        return
     if args.batched_only:
         print(json.dumps(batched_variant(steps=args.steps, with_oracle=False)), flush=True)
         return
