@@ -157,11 +157,14 @@ class MoyalOracle(Grid):
         get_sigma_x_sigma_p :405-415."""
         S0 = s[0]
         x, p = s[1] / S0, s[2] / S0
+        with np.errstate(invalid="ignore"):
+            # a negative variance product (coarse grids) gives NaN, as np.sqrt does in the reference
+            sxsp = np.sqrt((s[3] / S0 - x * x) * (s[4] / S0 - p * p))
         return dict(
             X_average=x, X_average_RHS=s[5] / S0, P_average=p, P_average_RHS=-s[6] / S0,
             hamiltonian_average=s[7] / S0,
             purity=2.0 * np.pi * s[8] / (S0 * S0 * self.dXdP),
-            sigma_x_sigma_p=np.sqrt((s[3] / S0 - x * x) * (s[4] / S0 - p * p)),
+            sigma_x_sigma_p=sxsp,
         )
 
     def propagate(self, steps=1):
