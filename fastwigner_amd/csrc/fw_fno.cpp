@@ -2,7 +2,7 @@
 // forward, backward and Adam on gfx950, compiled ahead of time into
 // libfastwigner.so. Contract: include/fastwigner.h (fw_fno_*).
 //
-// Network (N x N grid, width C, retained modes M x M, L layers):
+// Network (N x N grid, width C = 32, retained modes M x M, L layers):
 //   features f(p, x) = [W, x/X_amplitude, p/P_amplitude, a2, a3, a4, gamma, kT]
 //   v0 = f Pw + pb                                   (lift, 8 -> C)
 //   v_{l+1} = act_l(S_l(v_l) + v_l Wp_l + b_l)       (act = GELU for l < L-1, identity for the last)
@@ -14,26 +14,33 @@
 // kp in [0, M) U [N-M, N) (along p), mixed per mode by complex C x C weights
 // R_l[m], and transformed back as numpy's irfft2 of that truncated spectrum.
 //
-// With only M of N modes kept, each truncated transform is a dense product
-// with a (2M x N) or (N x 2M) cosine/sine matrix: every stage is a GEMM. DFT
-// stages run on FP32 MFMA (v_mfma_f32_16x16x4f32); the channel mixing (per-mode
-// complex weights, pointwise C x C) runs on BF16 MFMA
-// (v_mfma_f32_16x16x16bf16_1k) with BF16 weights and BF16-rounded activations,
-// FP32 accumulation. Activations are channels-last, [b][p][x][c], FP32.
+// With only M of N modes kept each truncated transform is a product with a
+// cosine/sine matrix, so the whole layer is three fused kernels on BF16 MFMA
+// (v_mfma_f32_16x16x32_bf16, FP32 accumulation): the forward DFT of each
+// channel image (x then p, the intermediate kept in registers), the per-mode
+// channel mixing, and the inverse (p then x, through LDS) fused with the
+// pointwise mixing, bias and GELU (or, in the last layer, the projection).
+// Activations are stored channel-first as bf16, v[b][c][p][x]; pre-activations
+// for the backward in FP32. The backward reuses the same three kernels: the
+// adjoint of the inverse is the forward DFT (scaled by c_ky / N^2), the adjoint
+// of the forward DFT is the inverse's structure with unit weights.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
-#include <type_traits>
 #include <cstring>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "fastwigner.h"
 #include "fw_internal.h"
 
 typedef unsigned short bf16_t;
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x8_t __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x4_t __attribute__((ext_vector_type(4)));
 typedef short short4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
@@ -50,15 +57,13 @@ __device__ __forceinline__ bf16_t to_bf16(float f) {
 __device__ __forceinline__ float from_bf16(bf16_t h) { return __uint_as_float((unsigned int)h << 16); }
 
 // --------------------------------------------------------------------------
-// Strided, batched GEMM  C[z] = alpha A[z] B[z] + beta C[z]
+// Strided, batched GEMM  C[z] = alpha A[z] B[z] + beta C[z]  (the pointwise
+// weights' gradient, a reduction over points; and the GEMM kernel test)
 //   A(m, k) at A + z1 sA1 + z2 sA2 + m sAm + k sAk, likewise B(k, n), C(m, n);
 //   z = z1 * nb2 + z2 = blockIdx.z.
 // 64 x 64 tile per 256-thread block, each wave a 32 x 32 quadrant of 2 x 2
 // MFMA 16 x 16 blocks; K in steps of 32 staged through LDS. Out-of-range
 // elements load as zero; stores are masked.
-// MFMA operand layout (lane l, 16 x 16 blocks): A rows i = l % 16 with
-// k = 4 (l / 16) + 0..3 (bf16) or k = l / 16 (f32); B columns j = l % 16 with
-// the same k; the accumulator holds rows 4 (l / 16) + 0..3 of column l % 16.
 // --------------------------------------------------------------------------
 struct GemmArgs {
   const void* A;
@@ -166,66 +171,577 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
 }
 
 // --------------------------------------------------------------------------
-// Elementwise and reduction kernels
+// Fused layer kernels (BF16 MFMA, FP32 accumulation). Activations are
+// channel-first, v[b][c][p][x], stored as bf16. Lane layouts of the MFMA
+// operands (lane l, 16 x 16 blocks, lr = l % 16, lg = l / 16):
+//   v_mfma_f32_16x16x32_bf16:   A row lr, k = 8 lg + 0..7; B column lr, same k
+//   v_mfma_f32_16x16x16bf16_1k: A row lr, k = 4 lg + 0..3; B column lr, same k
+//   accumulator: column lr, rows 4 lg + 0..3
 // --------------------------------------------------------------------------
+constexpr int C = 32;    // channels: the kernels are written for 32
+constexpr int XK = 32;   // contraction of the inverse's x stage (2M, zero-padded to 32)
+constexpr int PB = 8;    // rows of W per block of the inverse (one per wave)
+
+__device__ __forceinline__ float4_t mma32(u16x8_t a, u16x8_t b, float4_t c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a), __builtin_bit_cast(bf16x8_t, b), c,
+                                                 0, 0, 0);
+}
+__device__ __forceinline__ float4_t mma16(u16x4_t a, u16x4_t b, float4_t c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(short4_t, a), __builtin_bit_cast(short4_t, b), c,
+                                                   0, 0, 0);
+}
+__device__ __forceinline__ u16x8_t ld8(const bf16_t* p) { return *reinterpret_cast<const u16x8_t*>(p); }
+__device__ __forceinline__ void st8(bf16_t* p, u16x8_t v) { *reinterpret_cast<u16x8_t*>(p) = v; }
+
 __device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.f + erff(x * 0.70710678118654752f)); }
 __device__ __forceinline__ float gelu_d(float x) {
   return 0.5f * (1.f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * __expf(-0.5f * x * x);
 }
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int s = 32; s > 0; s >>= 1) v += __shfl_xor(v, s);
+  return v;
+}
 
-// v0[n][c] = sum_j Pw[j][c] f_j(n) + pb[c]; f = [W, x/Xa, p/Pa, feat[b][0..4]].
-// One thread per output (n, c): consecutive threads write consecutive channels.
-__global__ void __launch_bounds__(256) lift_fwd(const float* W, const float* feat, const float* Pw, const float* pb,
-                                                float* v0, int B, int N, int C, float xs, float ps) {
-  const long long total = (long long)B * N * N * C;
-  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < total; i += (long long)gridDim.x * 256) {
-    const long long n = i / C;
-    const int c = (int)(i % C);
-    const int b = (int)(n / ((long long)N * N));
-    const int rem = (int)(n % ((long long)N * N));
-    const int p = rem / N, x = rem % N;
-    float f[8];
-    f[0] = W[n];
-    f[1] = xs * (x - 0.5f * N);
-    f[2] = ps * (p - 0.5f * N);
+// ---- forward truncated DFT of channel images ------------------------------
+// One block per image (b, c). Wave w takes row groups of 16 rows p; per group:
+//   stage A (x -> ky):  T[p][r] = sum_x v[p][x] Fx[r][x], r < 2M       (16x16x32, K = x)
+//   stage B (p -> kp):  O[s][r] += sum_p Fp[s][p] bf16(T[p][r]), s < 4M (16x16x16, K = 16 rows)
+// T leaves stage A in the accumulator layout (rows p = 4 lg + j of column r),
+// which is exactly the B operand layout of the 16x16x16 product: no LDS trip.
+// The waves' O are added in LDS and assembled into the retained modes
+//   Z[i][ky] = (O[i][ky] + O[2M+i][M+ky]) + i (O[i][M+ky] - O[2M+i][ky]),
+// scaled by zscale[ky], written as Z[b][m = i M + ky][re, im][c].
+// Input modes: IN_V reads bf16 activations; IN_LIFT evaluates the lift
+//   v0 = pb + sum_j Pw[j] f_j from W and the features and stores v0; IN_GRAD /
+//   IN_GRAD_LIN form dpre = dv gelu'(pre) (or dv), store it as bf16 and sum it
+//   per image (the bias gradient).
+enum { IN_V = 0, IN_LIFT = 1, IN_GRAD = 2, IN_GRAD_LIN = 3 };
+
+struct DftArgs {
+  const bf16_t* v;        // IN_V: activations [B][C][N][N]
+  const float* W;         // IN_LIFT: inputs [B][N][N]
+  const float* feat;      // IN_LIFT: features [B][5]
+  const float* Pw;        // IN_LIFT: lift weights [8][C]
+  const float* pb;        // IN_LIFT: lift bias [C]
+  const float* dv;        // IN_GRAD*: gradient w.r.t. the layer's output [B][C][N][N]
+  const float* pre;       // IN_GRAD: pre-activations
+  bf16_t* vout;           // IN_LIFT: v0; IN_GRAD*: dpre
+  float* bias_part;       // IN_GRAD*: per-image sums of dpre [B][C]
+  const bf16_t* Fx;       // [2M][N]
+  const bf16_t* Fp;       // [4M][N]
+  const float* zscale;    // [M]
+  float* Z;               // [B][2 M M][2][C]
+  float xs, ps;           // x/X_amplitude = xs (x - N/2), likewise p
+};
+
+template <int N>
+struct DftShape {
+  static constexpr int WAVES = N >= 128 ? 8 : 4;
+  static constexpr int RG = N / (16 * WAVES);   // row groups per wave
+};
+
+template <int N, int MM, int MODE>
+__global__ void __launch_bounds__(DftShape<N>::WAVES * 64) dft_fwd(DftArgs a) {
+  constexpr int WAVES = DftShape<N>::WAVES, RG = DftShape<N>::RG, NT = WAVES * 64;
+  constexpr int R2 = 2 * MM, RT = R2 / 16, S4 = 4 * MM, ST = S4 / 16, KS = N / 32, NM = 2 * MM * MM;
+  __shared__ float ored[WAVES][S4 * R2];
+  __shared__ float bred[WAVES];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, lr = lane & 15, lg = lane >> 4;
+  const int b = blockIdx.x / C, c = blockIdx.x % C;
+  const size_t img = (size_t)blockIdx.x * N * N;
+
+  u16x8_t bfx[RT][KS];
 #pragma unroll
-    for (int j = 0; j < 5; ++j) f[3 + j] = feat[b * 5 + j];
-    float s = pb[c];
+  for (int t = 0; t < RT; ++t)
 #pragma unroll
-    for (int j = 0; j < 8; ++j) s += Pw[j * C + c] * f[j];
-    v0[i] = s;
+    for (int s = 0; s < KS; ++s) bfx[t][s] = ld8(a.Fx + (size_t)(16 * t + lr) * N + 32 * s + 8 * lg);
+  float pw0 = 0.f, pw1 = 0.f, pw2 = 0.f, kf = 0.f;
+  if constexpr (MODE == IN_LIFT) {
+    pw0 = a.Pw[c];
+    pw1 = a.Pw[C + c];
+    pw2 = a.Pw[2 * C + c];
+    kf = a.pb[c];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) kf += a.Pw[(3 + j) * C + c] * a.feat[b * 5 + j];
+  }
+  float4_t O[ST][RT];
+#pragma unroll
+  for (int u = 0; u < ST; ++u)
+#pragma unroll
+    for (int t = 0; t < RT; ++t) O[u][t] = (float4_t){0.f, 0.f, 0.f, 0.f};
+  float bsum = 0.f;
+
+  // the plain input mode issues the loads of all its row groups together; the
+  // others, with more live values per element, take one group at a time
+  constexpr int UR = MODE == IN_V ? RG : 1;
+#pragma unroll UR
+  for (int g = 0; g < RG; ++g) {
+    const int p0 = 16 * (w * RG + g), p = p0 + lr;
+    u16x8_t av[KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const int x = 32 * s + 8 * lg;
+      const size_t e = img + (size_t)p * N + x;
+      if constexpr (MODE == IN_V) {
+        av[s] = ld8(a.v + e);
+      } else if constexpr (MODE == IN_LIFT) {
+        const float4_t* wp = reinterpret_cast<const float4_t*>(a.W + ((size_t)b * N + p) * N + x);
+        const float4_t w0 = wp[0], w1 = wp[1];
+        const float base = kf + pw2 * (a.ps * ((float)p - 0.5f * N));
+        u16x8_t r;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const float wv = j < 4 ? w0[j] : w1[j - 4];
+          r[j] = to_bf16(base + pw0 * wv + pw1 * (a.xs * ((float)(x + j) - 0.5f * N)));
+        }
+        st8(a.vout + e, r);
+        av[s] = r;
+      } else {
+        const float4_t* dp = reinterpret_cast<const float4_t*>(a.dv + e);
+        float4_t d0 = dp[0], d1 = dp[1];
+        if constexpr (MODE == IN_GRAD) {
+          const float4_t* pp = reinterpret_cast<const float4_t*>(a.pre + e);
+          const float4_t q0 = pp[0], q1 = pp[1];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            d0[j] *= gelu_d(q0[j]);
+            d1[j] *= gelu_d(q1[j]);
+          }
+        }
+        u16x8_t r;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          bsum += d0[j] + d1[j];
+          r[j] = to_bf16(d0[j]);
+          r[4 + j] = to_bf16(d1[j]);
+        }
+        st8(a.vout + e, r);
+        av[s] = r;
+      }
+    }
+    float4_t T[RT];
+#pragma unroll
+    for (int t = 0; t < RT; ++t) T[t] = (float4_t){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < KS; ++s)
+#pragma unroll
+      for (int t = 0; t < RT; ++t) T[t] = mma32(av[s], bfx[t][s], T[t]);
+    u16x4_t tb[RT];
+#pragma unroll
+    for (int t = 0; t < RT; ++t)
+      tb[t] = (u16x4_t){to_bf16(T[t][0]), to_bf16(T[t][1]), to_bf16(T[t][2]), to_bf16(T[t][3])};
+#pragma unroll
+    for (int u = 0; u < ST; ++u) {
+      const u16x4_t fa = *reinterpret_cast<const u16x4_t*>(a.Fp + (size_t)(16 * u + lr) * N + p0 + 4 * lg);
+#pragma unroll
+      for (int t = 0; t < RT; ++t) O[u][t] = mma16(fa, tb[t], O[u][t]);
+    }
+  }
+
+#pragma unroll
+  for (int u = 0; u < ST; ++u)
+#pragma unroll
+    for (int t = 0; t < RT; ++t)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) ored[w][(16 * u + 4 * lg + j) * R2 + 16 * t + lr] = O[u][t][j];
+  if constexpr (MODE == IN_GRAD || MODE == IN_GRAD_LIN) {
+    bsum = wave_sum(bsum);
+    if (lane == 0) bred[w] = bsum;
+  }
+  __syncthreads();
+  for (int e = tid; e < S4 * R2; e += NT) {
+    float s = 0.f;
+#pragma unroll
+    for (int q = 0; q < WAVES; ++q) s += ored[q][e];
+    ored[0][e] = s;
+  }
+  if constexpr (MODE == IN_GRAD || MODE == IN_GRAD_LIN) {
+    if (tid == 0) {
+      float s = 0.f;
+#pragma unroll
+      for (int q = 0; q < WAVES; ++q) s += bred[q];
+      a.bias_part[blockIdx.x] = s;
+    }
+  }
+  __syncthreads();
+  for (int m = tid; m < NM; m += NT) {
+    const int i = m / MM, ky = m % MM;
+    const float* o0 = ored[0];
+    const float zr = o0[i * R2 + ky] + o0[(2 * MM + i) * R2 + MM + ky];
+    const float zi = o0[i * R2 + MM + ky] - o0[(2 * MM + i) * R2 + ky];
+    const float sc = a.zscale[ky];
+    float* z = a.Z + ((size_t)b * NM + m) * 2 * C + c;
+    z[0] = sc * zr;
+    z[C] = sc * zi;
   }
 }
 
-// per-block partial sums of f_j(n) dv0[n][c] (j < 8) and dv0[n][c]: 9 C values per block
-__global__ void __launch_bounds__(256) lift_bwd_partial(const float* W, const float* feat, const float* dv0, float* part,
-                                                        int B, int N, int C, float xs, float ps) {
-  __shared__ float red[256];
-  const long long total = (long long)B * N * N;
-  // thread t accumulates for channel c = t % C over points n = t / C + k (256 / C)
-  const int c = threadIdx.x % C, lanes = 256 / C, li = threadIdx.x / C;
-  float acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (long long n = (long long)blockIdx.x * lanes + li; n < total; n += (long long)gridDim.x * lanes) {
-    const int b = (int)(n / ((long long)N * N));
-    const int rem = (int)(n % ((long long)N * N));
-    const int p = rem / N, x = rem % N;
-    const float d = dv0[n * C + c];
-    acc[0] += W[n] * d;
-    acc[1] += xs * (x - 0.5f * N) * d;
-    acc[2] += ps * (p - 0.5f * N) * d;
+// ---- mode mixing ---------------------------------------------------------
+// One wave per mode m (4 per block): [Yr | Yi] (B x 2C) = bf16([Zr | Zi]) Wm[m]
+// with the expanded bf16 weights Wx[m][n][k] (n = output column, k contiguous):
+// forward Wm^T of [[Rr, Ri], [-Ri, Rr]], backward (dZ = dY Wm^T) Wm itself. The
+// result goes straight into the B operand of the inverse's p stage,
+//   BT[b][n' = (ri M + ky) C + o][k]  (k = i for cos kp, 2M + i for sin kp):
+//   Hr = sum_i cos Yr - sin Yi,  Hi = sum_i sin Yr + cos Yi.
+struct MixArgs {
+  const float* Zin;   // [B][2 M M][2][C]
+  const bf16_t* Wx;   // [2 M M][2C][2C]
+  bf16_t* BT;         // [B][2 M C][4M]
+  int B;              // <= 16
+};
+
+template <int MM>
+__global__ void __launch_bounds__(256) mix_modes(MixArgs a) {
+  constexpr int NM = 2 * MM * MM, NP = 2 * MM * C, K4 = 4 * MM;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, lr = lane & 15, lg = lane >> 4;
+  const int m = blockIdx.x * 4 + w;
+  const int i = m / MM, ky = m % MM;
+  u16x8_t za[2];
 #pragma unroll
-    for (int j = 0; j < 5; ++j) acc[3 + j] += feat[b * 5 + j] * d;
-    acc[8] += d;
+  for (int ks = 0; ks < 2; ++ks) {
+    za[ks] = (u16x8_t){0, 0, 0, 0, 0, 0, 0, 0};
+    if (lr < a.B) {
+      const float4_t* zp = reinterpret_cast<const float4_t*>(a.Zin + ((size_t)lr * NM + m) * 2 * C + 32 * ks + 8 * lg);
+      const float4_t z0 = zp[0], z1 = zp[1];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        za[ks][j] = to_bf16(z0[j]);
+        za[ks][4 + j] = to_bf16(z1[j]);
+      }
+    }
   }
-  for (int j = 0; j < 9; ++j) {
-    red[threadIdx.x] = acc[j];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    float4_t acc = (float4_t){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks)
+      acc = mma32(za[ks], ld8(a.Wx + ((size_t)m * 2 * C + 16 * t + lr) * 2 * C + 32 * ks + 8 * lg), acc);
+    const int n = 16 * t + lr, o = n % C;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int bb = 4 * lg + j;
+      if (bb < a.B) {
+        bf16_t* bt = a.BT + (size_t)bb * NP * K4;
+        const float y = acc[j];
+        if (t < 2) {   // real part
+          bt[((size_t)ky * C + o) * K4 + i] = to_bf16(y);
+          bt[((size_t)(MM + ky) * C + o) * K4 + 2 * MM + i] = to_bf16(y);
+        } else {       // imaginary part
+          bt[((size_t)(MM + ky) * C + o) * K4 + i] = to_bf16(y);
+          bt[((size_t)ky * C + o) * K4 + 2 * MM + i] = to_bf16(-y);
+        }
+      }
+    }
+  }
+}
+
+// gradient of the mode weights, one block per mode: with bf16-rounded Z and dY,
+// dRr[c][o] = sum_b Zr dYr + Zi dYi, dRi[c][o] = sum_b Zr dYi - Zi dYr
+template <int MM>
+__global__ void __launch_bounds__(256) mode_grad(const float* Z, const float* dY, float* gR, int B) {
+  constexpr int NM = 2 * MM * MM;
+  __shared__ float zs[16][2 * C], ys[16][2 * C];
+  const int m = blockIdx.x, tid = threadIdx.x;
+  for (int e = tid; e < B * 2 * C; e += 256) {
+    const int bb = e / (2 * C), k = e % (2 * C);
+    const size_t src = ((size_t)bb * NM + m) * 2 * C + k;
+    zs[bb][k] = from_bf16(to_bf16(Z[src]));
+    ys[bb][k] = from_bf16(to_bf16(dY[src]));
+  }
+  __syncthreads();
+  for (int e = tid; e < C * C; e += 256) {
+    const int c = e / C, o = e % C;
+    float rr = 0.f, ri = 0.f;
+    for (int bb = 0; bb < B; ++bb) {
+      const float zr = zs[bb][c], zi = zs[bb][C + c], yr = ys[bb][o], yi = ys[bb][C + o];
+      rr += zr * yr + zi * yi;
+      ri += zr * yi - zi * yr;
+    }
+    gR[((size_t)m * C * C + e) * 2] = rr;
+    gR[((size_t)m * C * C + e) * 2 + 1] = ri;
+  }
+}
+
+// ---- inverse transform fused with the pointwise mixing ---------------------
+// One block per (b, 8 rows p0..p0+7), 8 waves.
+//   p stage (kp -> p): H[r][n'] = sum_k GpT[p0+r][k] BT[b][n'][k]   (16x16x32; rows r >= 8 zero)
+//     -> bf16 H in LDS as hs[r][o][k' = ri M + ky]
+//   x stage + pointwise, wave w on row p = p0 + w, tiles of 16 x by 16 o:
+//     acc = sum_k' Gx[x][k'] H[o][k']  +  sum_c vin[c][x] Wpt[o][c]
+//   (two 16x16x32 products into one accumulator). vin's row is staged in LDS
+//   (channel rows rotated by 16 (c / 8) elements so that the transposed reads of
+//   8 channels at one x by the 4 lane groups fall in distinct banks).
+// Epilogues: OUT_HIDDEN pre = acc + bias -> bf16 gelu (and FP32 pre for training);
+// OUT_LAST pre = acc + bias -> bf16 v_L, projection c = Q . v_L + qb over the 32
+// channels (a shuffle sum across the 16 lanes of a row), out = W + scale c and
+// per-block sums of c (zero-mass output); OUT_GRAD stores acc (FP32).
+enum { OUT_HIDDEN = 0, OUT_LAST = 1, OUT_GRAD = 2 };
+
+struct InvArgs {
+  const bf16_t* BT;     // [B][2 M C][4M]
+  const bf16_t* GpT;    // [N][4M]
+  const bf16_t* Gx;     // [N][32]
+  const bf16_t* vin;    // [B][C][N][N]
+  const bf16_t* Wpt;    // [C][C]: B operand read as [n][k]
+  const float* bias;    // [C]
+  bf16_t* vout;         // next activations, or NULL
+  float* pre;           // FP32 pre-activations, or NULL
+  float* dvout;         // OUT_GRAD
+  const float* W;       // OUT_LAST: inputs [B][N][N]
+  const float* Q;
+  const float* qb;
+  float oscale;
+  float* out;           // OUT_LAST: [B][N][N]
+  float* cpart;         // OUT_LAST: [B * N / 8]
+};
+
+template <int N, int MM, int MODE>
+__global__ void __launch_bounds__(512) inv_point(InvArgs a) {
+  constexpr int NP = 2 * MM * C, K4 = 4 * MM, KSB = K4 / 32, NTW = NP / 16 / 8;
+  constexpr int XC = N < 128 ? N : 128, NCH = N / XC, HS = XK + 8, NBR = N / PB;
+  __shared__ bf16_t hs[PB][C][HS];
+  __shared__ bf16_t vs[8][C][XC];
+  __shared__ float cred[8];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, lr = lane & 15, lg = lane >> 4;
+  const int b = blockIdx.x / NBR, p0 = (blockIdx.x % NBR) * PB;
+  if constexpr (2 * MM < XK) {
+    for (int e = tid; e < PB * C * HS; e += 512) (&hs[0][0][0])[e] = 0;
     __syncthreads();
-    if (threadIdx.x < C) {
-      float s = 0.f;
-      for (int q = 0; q < lanes; ++q) s += red[q * C + threadIdx.x];
-      part[((long long)blockIdx.x * 9 + j) * C + threadIdx.x] = s;
+  }
+  // p stage
+  u16x8_t ga[KSB];
+#pragma unroll
+  for (int ks = 0; ks < KSB; ++ks) {
+    ga[ks] = (u16x8_t){0, 0, 0, 0, 0, 0, 0, 0};
+    if (lr < PB) ga[ks] = ld8(a.GpT + (size_t)(p0 + lr) * K4 + 32 * ks + 8 * lg);
+  }
+  const bf16_t* bt = a.BT + (size_t)b * NP * K4;
+#pragma unroll
+  for (int q = 0; q < NTW; ++q) {
+    const int np = 16 * (w * NTW + q) + lr;
+    float4_t acc = (float4_t){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < KSB; ++ks) acc = mma32(ga[ks], ld8(bt + (size_t)np * K4 + 32 * ks + 8 * lg), acc);
+    if (lg < 2) {
+      const int o = np % C, kq = np / C;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) hs[4 * lg + j][o][kq] = to_bf16(acc[j]);
+    }
+  }
+  __syncthreads();
+
+  // x stage and pointwise mixing on row p
+  const int p = p0 + w;
+  u16x8_t bh[2], bv[2];
+#pragma unroll
+  for (int ot = 0; ot < 2; ++ot) {
+    bh[ot] = *reinterpret_cast<const u16x8_t*>(&hs[w][16 * ot + lr][8 * lg]);
+    bv[ot] = ld8(a.Wpt + (16 * ot + lr) * C + 8 * lg);
+  }
+  float csum = 0.f;
+  for (int ch = 0; ch < NCH; ++ch) {
+    const int x0 = ch * XC;
+#pragma unroll
+    for (int e = 0; e < XC / 16; ++e) {
+      const int f = e * 64 + lane, cc = f / (XC / 8), xg = f % (XC / 8);
+      const u16x8_t val = ld8(a.vin + (((size_t)b * C + cc) * N + p) * N + x0 + 8 * xg);
+      *reinterpret_cast<u16x8_t*>(&vs[w][cc][(8 * xg + 16 * (cc >> 3)) & (XC - 1)]) = val;
     }
     __syncthreads();
+#pragma unroll 2
+    for (int xt = 0; xt < XC / 16; ++xt) {
+      const int xl = 16 * xt + lr;
+      const u16x8_t ah = ld8(a.Gx + (size_t)(x0 + xl) * XK + 8 * lg);
+      u16x8_t av;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) av[j] = vs[w][8 * lg + j][(xl + 16 * lg) & (XC - 1)];
+      float4_t acc[2];
+#pragma unroll
+      for (int ot = 0; ot < 2; ++ot) {
+        acc[ot] = mma32(ah, bh[ot], (float4_t){0.f, 0.f, 0.f, 0.f});
+        acc[ot] = mma32(av, bv[ot], acc[ot]);
+      }
+      const int xb = x0 + 16 * xt + 4 * lg;
+      if constexpr (MODE == OUT_HIDDEN) {
+#pragma unroll
+        for (int ot = 0; ot < 2; ++ot) {
+          const int o = 16 * ot + lr;
+          const float bo = a.bias[o];
+          const size_t e = (((size_t)b * C + o) * N + p) * N + xb;
+          float4_t pr;
+          u16x4_t vv;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            pr[j] = acc[ot][j] + bo;
+            vv[j] = to_bf16(gelu(pr[j]));
+          }
+          if (a.pre) *reinterpret_cast<float4_t*>(a.pre + e) = pr;
+          *reinterpret_cast<u16x4_t*>(a.vout + e) = vv;
+        }
+      } else if constexpr (MODE == OUT_LAST) {
+        float cp[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ot = 0; ot < 2; ++ot) {
+          const int o = 16 * ot + lr;
+          const float bo = a.bias[o], qo = a.Q[o];
+          const size_t e = (((size_t)b * C + o) * N + p) * N + xb;
+          u16x4_t vv;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            vv[j] = to_bf16(acc[ot][j] + bo);
+            cp[j] += qo * from_bf16(vv[j]);
+          }
+          if (a.vout) *reinterpret_cast<u16x4_t*>(a.vout + e) = vv;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+          for (int s = 1; s < 16; s <<= 1) cp[j] += __shfl_xor(cp[j], s, 16);
+        if (lr == 0) {
+          const size_t e = ((size_t)b * N + p) * N + xb;
+          const float4_t wv = *reinterpret_cast<const float4_t*>(a.W + e);
+          float4_t ov;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const float cj = cp[j] + a.qb[0];
+            ov[j] = wv[j] + a.oscale * cj;
+            csum += cj;
+          }
+          *reinterpret_cast<float4_t*>(a.out + e) = ov;
+        }
+      } else {
+#pragma unroll
+        for (int ot = 0; ot < 2; ++ot) {
+          const int o = 16 * ot + lr;
+          *reinterpret_cast<float4_t*>(a.dvout + (((size_t)b * C + o) * N + p) * N + xb) = acc[ot];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if constexpr (MODE == OUT_LAST) {
+    csum = wave_sum(csum);
+    if (lane == 0) cred[w] = csum;
+    __syncthreads();
+    if (tid == 0) {
+      float s = 0.f;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) s += cred[q];
+      a.cpart[blockIdx.x] = s;
+    }
+  }
+}
+
+// zero-mass output: out[b] -= scale mean(c) with c summed per block of the last layer
+__global__ void __launch_bounds__(256) center_out(float* out, const float* cpart, int nbr, long long nn, float oscale) {
+  const int b = blockIdx.y;
+  float s = 0.f;
+  for (int k = 0; k < nbr; ++k) s += cpart[b * nbr + k];
+  const float m = oscale * (s / (float)nn);
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < nn; i += (long long)gridDim.x * 256)
+    out[(long long)b * nn + i] -= m;
+}
+
+// projection backward, one block per row (b, p): dv[b][o][p][x] = Q[o] d with
+// d = scale dout[b][p][x]; per-row sums of v_L[o] d (C) and d (1)
+__global__ void __launch_bounds__(256) proj_bwd(const bf16_t* vL, const float* dout, const float* Q, float* dv,
+                                                float* part, int N, float oscale) {
+  __shared__ float red[4][C + 1];
+  const int row = blockIdx.x, b = row / N, p = row % N, x = threadIdx.x;
+  const int lane = x & 63, w = x >> 6;
+  float acc[C + 1];
+#pragma unroll
+  for (int k = 0; k <= C; ++k) acc[k] = 0.f;
+  if (x < N) {
+    const float d = oscale * dout[(size_t)row * N + x];
+#pragma unroll
+    for (int o = 0; o < C; ++o) {
+      const size_t e = (((size_t)b * C + o) * N + p) * N + x;
+      dv[e] = Q[o] * d;
+      acc[o] = from_bf16(vL[e]) * d;
+    }
+    acc[C] = d;
+  }
+#pragma unroll
+  for (int k = 0; k <= C; ++k) {
+    const float s = wave_sum(acc[k]);
+    if (lane == 0) red[w][k] = s;
+  }
+  __syncthreads();
+  if (x <= C) part[(size_t)row * (C + 1) + x] = red[0][x] + red[1][x] + red[2][x] + red[3][x];
+}
+
+// lift backward: per image (b, c) the sums of dv0, W dv0, x dv0, p dv0; then
+// gPw[j][c] and gpb[c] over the batch (features constant per sample)
+__global__ void __launch_bounds__(256) lift_bwd_img(const float* dv0, const float* W, float* part, int N, float xs,
+                                                    float ps) {
+  __shared__ float red[4][4];
+  const int img = blockIdx.x, b = img / C, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const size_t nn = (size_t)N * N;
+  float s[4] = {0.f, 0.f, 0.f, 0.f};
+  for (int e = threadIdx.x; e < N * N; e += 256) {
+    const int p = e / N, x = e % N;
+    const float d = dv0[(size_t)img * nn + e];
+    s[0] += d;
+    s[1] += W[(size_t)b * nn + e] * d;
+    s[2] += xs * ((float)x - 0.5f * N) * d;
+    s[3] += ps * ((float)p - 0.5f * N) * d;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const float t = wave_sum(s[k]);
+    if (lane == 0) red[w][k] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int k = threadIdx.x;
+    part[(size_t)img * 4 + k] = red[0][k] + red[1][k] + red[2][k] + red[3][k];
+  }
+}
+__global__ void __launch_bounds__(64) lift_bwd_fin(const float* part, const float* feat, float* gPw, float* gpb, int B) {
+  const int c = threadIdx.x;
+  if (c >= C) return;
+  float s[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  for (int b = 0; b < B; ++b) {
+    const float* q = part + ((size_t)b * C + c) * 4;
+    s[0] += q[1];
+    s[1] += q[2];
+    s[2] += q[3];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) s[3 + j] += feat[b * 5 + j] * q[0];
+    s[8] += q[0];
+  }
+#pragma unroll
+  for (int j = 0; j < 8; ++j) gPw[j * C + c] = s[j];
+  gpb[c] = s[8];
+}
+
+// bf16 weights: the expanded mode blocks Wm[m][k][n] = [[Rr, Ri], [-Ri, Rr]] and
+// their transpose WmT[m][n][k]; the pointwise Wp[c][o] and its transpose
+__global__ void __launch_bounds__(256) expand_modes(const float* R, bf16_t* Wm, bf16_t* WmT, int nm) {
+  const long long total = (long long)nm * C * C;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < total; i += (long long)gridDim.x * 256) {
+    const int o = (int)(i % C), c = (int)((i / C) % C), m = (int)(i / (C * C));
+    const bf16_t rr = to_bf16(R[i * 2]), ri = to_bf16(R[i * 2 + 1]), nri = to_bf16(-R[i * 2 + 1]);
+    bf16_t* w = Wm + (long long)m * 4 * C * C;
+    bf16_t* wt = WmT + (long long)m * 4 * C * C;
+    w[c * 2 * C + o] = rr;
+    w[c * 2 * C + C + o] = ri;
+    w[(C + c) * 2 * C + o] = nri;
+    w[(C + c) * 2 * C + C + o] = rr;
+    wt[o * 2 * C + c] = rr;
+    wt[(C + o) * 2 * C + c] = ri;
+    wt[o * 2 * C + C + c] = nri;
+    wt[(C + o) * 2 * C + C + c] = rr;
+  }
+}
+__global__ void __launch_bounds__(256) pointwise_weights(const float* Wp, bf16_t* Wpb, bf16_t* WpT) {
+  for (int e = blockIdx.x * 256 + threadIdx.x; e < C * C; e += gridDim.x * 256) {
+    const int c = e / C, o = e % C;
+    const bf16_t v = to_bf16(Wp[e]);
+    Wpb[e] = v;
+    WpT[o * C + c] = v;
   }
 }
 
@@ -237,148 +753,6 @@ __global__ void __launch_bounds__(256) reduce_rows(const float* part, float* out
   float s = 0.f;
   for (int z = 0; z < nz; ++z) s += part[(long long)z * nr + r];
   out[r] = accumulate ? out[r] + scale * s : scale * s;
-}
-
-// out[n] = W[n] + scale (sum_c Q[c] v[n][c] + qb)
-__global__ void __launch_bounds__(256) proj_fwd(const float* W, const float* v, const float* Q, const float* qb,
-                                                float* out, long long total, int C, float scale) {
-  for (long long n = (long long)blockIdx.x * 256 + threadIdx.x; n < total; n += (long long)gridDim.x * 256) {
-    float s = qb[0];
-    for (int c = 0; c < C; ++c) s += Q[c] * v[n * C + c];
-    out[n] = W[n] + scale * s;
-  }
-}
-
-// with d = scale dout[n]: dv[n][c] = Q[c] d; partials of sum_n v[n][c] d (C) and sum d (1) per block
-__global__ void __launch_bounds__(256) proj_bwd(const float* v, const float* dout, const float* Q, float* dv,
-                                                float* part, long long total, int C, float scale) {
-  __shared__ float red[256];
-  const int c = threadIdx.x % C, lanes = 256 / C, li = threadIdx.x / C;
-  float aq = 0.f, ab = 0.f;
-  for (long long n = (long long)blockIdx.x * lanes + li; n < total; n += (long long)gridDim.x * lanes) {
-    const float d = scale * dout[n];
-    dv[n * C + c] = Q[c] * d;
-    aq += v[n * C + c] * d;
-    if (c == 0) ab += d;
-  }
-  red[threadIdx.x] = aq;
-  __syncthreads();
-  if (threadIdx.x < C) {
-    float s = 0.f;
-    for (int q = 0; q < lanes; ++q) s += red[q * C + threadIdx.x];
-    part[(long long)blockIdx.x * (C + 1) + threadIdx.x] = s;
-  }
-  __syncthreads();
-  red[threadIdx.x] = ab;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    float s = 0.f;
-    for (int q = 0; q < lanes; ++q) s += red[q * C];
-    part[(long long)blockIdx.x * (C + 1) + C] = s;
-  }
-}
-
-// pre = a + b[c] (kept), v = gelu(pre) or pre (last layer)
-__global__ void __launch_bounds__(256) bias_act(float* a, const float* bias, float* v, long long total, int C, int act) {
-  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < total; i += (long long)gridDim.x * 256) {
-    const float pre = a[i] + bias[i % C];
-    a[i] = pre;
-    v[i] = act ? gelu(pre) : pre;
-  }
-}
-
-// dpre = dv * gelu'(pre) in place (act) ; partial bias sums per block
-__global__ void __launch_bounds__(256) act_bwd(const float* pre, float* dv, float* part, long long total, int C, int act) {
-  __shared__ float red[256];
-  const int c = threadIdx.x % C, lanes = 256 / C, li = threadIdx.x / C;
-  const long long npts = total / C;
-  float s = 0.f;
-  for (long long n = (long long)blockIdx.x * lanes + li; n < npts; n += (long long)gridDim.x * lanes) {
-    const long long i = n * C + c;
-    float d = dv[i];
-    if (act) d *= gelu_d(pre[i]);
-    dv[i] = d;
-    s += d;
-  }
-  red[threadIdx.x] = s;
-  __syncthreads();
-  if (threadIdx.x < C) {
-    float t = 0.f;
-    for (int q = 0; q < lanes; ++q) t += red[q * C + threadIdx.x];
-    part[(long long)blockIdx.x * C + threadIdx.x] = t;
-  }
-}
-
-// Z[b][m][ri][c] from G[b][r(64)][(r1 = ri_in*M + ky)*C + c], m = kpi*M + ky
-// (Zr = G[cos kp, Tr] + G[sin kp, Ti], Zi = G[cos kp, Ti] - G[sin kp, Tr]); inverse=1: the transpose.
-__global__ void __launch_bounds__(256) assemble(float* G, float* Z, int B, int Mm, int C, int inverse) {
-  const int KP = 2 * Mm;                       // kp indices
-  const long long total = (long long)B * KP * Mm * C;
-  const long long gq = (long long)2 * Mm * C;  // row length of G
-  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < total; i += (long long)gridDim.x * 256) {
-    const int c = (int)(i % C);
-    const int ky = (int)((i / C) % Mm);
-    const int kpi = (int)((i / ((long long)C * Mm)) % KP);
-    const int b = (int)(i / ((long long)C * Mm * KP));
-    float* Gb = G + (long long)b * (2 * KP) * gq;
-    const long long cos_tr = (long long)kpi * gq + (long long)ky * C + c;
-    const long long cos_ti = (long long)kpi * gq + (long long)(Mm + ky) * C + c;
-    const long long sin_tr = (long long)(KP + kpi) * gq + (long long)ky * C + c;
-    const long long sin_ti = (long long)(KP + kpi) * gq + (long long)(Mm + ky) * C + c;
-    const int m = kpi * Mm + ky;
-    float* zr = Z + ((long long)b * KP * Mm + m) * 2 * C + c;
-    float* zi = zr + C;
-    if (!inverse) {
-      *zr = Gb[cos_tr] + Gb[sin_ti];
-      *zi = Gb[cos_ti] - Gb[sin_tr];
-    } else {
-      Gb[cos_tr] = *zr;
-      Gb[sin_ti] = *zr;
-      Gb[cos_ti] = *zi;
-      Gb[sin_tr] = -*zi;
-    }
-  }
-}
-
-// Yst[b][ri][kpi][ky][o] <-> Y[b][m = kpi*M + ky][ri*O + o]
-__global__ void __launch_bounds__(256) disassemble(float* Y, float* Yst, int B, int Mm, int O, int inverse) {
-  const int KP = 2 * Mm;
-  const long long total = (long long)B * 2 * KP * Mm * O;
-  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < total; i += (long long)gridDim.x * 256) {
-    const int o = (int)(i % O);
-    const int ky = (int)((i / O) % Mm);
-    const int kpi = (int)((i / ((long long)O * Mm)) % KP);
-    const int ri = (int)((i / ((long long)O * Mm * KP)) % 2);
-    const int b = (int)(i / ((long long)O * Mm * KP * 2));
-    const long long yi = ((long long)b * KP * Mm + (long long)kpi * Mm + ky) * 2 * O + (long long)ri * O + o;
-    if (!inverse) Yst[i] = Y[yi];
-    else Y[yi] = Yst[i];
-  }
-}
-
-// expanded real weights per mode: Wm[m][2C][2O] = [[Rr, Ri], [-Ri, Rr]] as bf16
-__global__ void __launch_bounds__(256) expand_modes(const float* R, bf16_t* Wm, int nm, int C, int O) {
-  const long long total = (long long)nm * C * O;
-  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < total; i += (long long)gridDim.x * 256) {
-    const int o = (int)(i % O), c = (int)((i / O) % C), m = (int)(i / ((long long)O * C));
-    const float rr = R[i * 2], ri = R[i * 2 + 1];
-    bf16_t* w = Wm + (long long)m * 4 * C * O;
-    w[(long long)c * 2 * O + o] = to_bf16(rr);
-    w[(long long)c * 2 * O + O + o] = to_bf16(ri);
-    w[(long long)(C + c) * 2 * O + o] = to_bf16(-ri);
-    w[(long long)(C + c) * 2 * O + O + o] = to_bf16(rr);
-  }
-}
-
-// dR[m][c][o][re, im] from dWm[m][2C][2O]: dRr = dW11 + dW22, dRi = dW12 - dW21
-__global__ void __launch_bounds__(256) fold_modes(const float* dWm, float* dR, int nm, int C, int O) {
-  const long long total = (long long)nm * C * O;
-  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < total; i += (long long)gridDim.x * 256) {
-    const int o = (int)(i % O), c = (int)((i / O) % C), m = (int)(i / ((long long)O * C));
-    const float* w = dWm + (long long)m * 4 * C * O;
-    dR[i * 2] = w[(long long)c * 2 * O + o] + w[(long long)(C + c) * 2 * O + O + o];
-    dR[i * 2 + 1] = w[(long long)c * 2 * O + O + o] - w[(long long)(C + c) * 2 * O + o];
-  }
 }
 
 // Moments of predicted Wigner functions for the physical referee, in FP64:
@@ -417,10 +791,6 @@ __global__ void __launch_bounds__(256) moments_cols(const float* w, int N, doubl
   double sv = 0.0;
   for (int p = 0; p < N; ++p) sv += col[(size_t)p * N];
   mom[(size_t)b * 3 * N + x] = sv;
-}
-
-__global__ void __launch_bounds__(256) to_bf16_kernel(const float* s, bf16_t* d, long long n) {
-  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) d[i] = to_bf16(s[i]);
 }
 
 // Per sample b (one block of 1024 threads each): x[b][n] -= mean_n (x[b][n] - y[b][n])
@@ -505,7 +875,7 @@ using fno::GemmArgs;
 
 struct fw_fno {
   fw_fno_desc d{};
-  int N = 256, C = 32, Mm = 16, L = 4, Bmax = 8;
+  int N = 256, Mm = 16, L = 4, Bmax = 8;
   int nm = 0;              // retained modes 2 Mm * Mm
   hipStream_t stream = nullptr;
   // parameters: flat FP32 vector and offsets
@@ -519,16 +889,22 @@ struct fw_fno {
   float* Ma = nullptr;     // Adam moments
   float* Va = nullptr;
   long long t_adam = 0;
-  bf16_t* Wm = nullptr;    // expanded spectral weights, L * nm * 4 C C
-  bf16_t* Wpb = nullptr;   // pointwise weights bf16, L * C * C
-  // DFT matrices (FP32)
-  float *Fx = nullptr, *Fp = nullptr, *Gp = nullptr, *Gxr = nullptr, *Gxi = nullptr;
-  // activations for batch Bmax
-  std::vector<float*> v;   // L+1 tensors [B][N][N][C]
-  std::vector<float*> pre; // L tensors
-  std::vector<float*> Zs;  // L tensors [B][nm][2C] (kept for the weight gradient)
-  float *T1 = nullptr, *Gb = nullptr, *Y = nullptr, *Yst = nullptr, *H = nullptr, *dv = nullptr, *dv2 = nullptr;
-  float *dWm = nullptr, *part = nullptr, *out = nullptr, *dout = nullptr;
+  bf16_t* Wm = nullptr;    // expanded mode weights L x nm x [2C][2C], and transposed
+  bf16_t* WmT = nullptr;
+  bf16_t* Wpb = nullptr;   // pointwise weights L x [C][C], and transposed
+  bf16_t* WpT = nullptr;
+  // transform tables (bf16) and output scales of the forward transform
+  bf16_t *Fx = nullptr, *Fp = nullptr, *GpT = nullptr, *Gxf = nullptr, *Gxu = nullptr;
+  float *zs_f = nullptr, *zs_b = nullptr;
+  // activations for batch Bmax, channel-first [B][C][N][N]
+  std::vector<bf16_t*> v;  // L+1 (v[L] only in training)
+  std::vector<float*> pre; // L-1 pre-activations of the GELU layers (training)
+  std::vector<float*> Zs;  // L spectra [B][nm][2][C] (kept for the weight gradient)
+  bf16_t* BT = nullptr;    // B operand of the inverse's p stage [B][2 M C][4M]
+  float* dY = nullptr;     // [B][nm][2][C]
+  bf16_t* dpre = nullptr;  // [B][C][N][N]
+  float *dv = nullptr, *dv2 = nullptr;
+  float *part = nullptr, *cpart = nullptr, *out = nullptr, *dout = nullptr;
   float *bin = nullptr, *btg = nullptr, *bft = nullptr;
   int* bidx = nullptr;
   double* persample = nullptr;
@@ -536,26 +912,26 @@ struct fw_fno {
   // dataset
   float *din = nullptr, *dtg = nullptr, *dft = nullptr;
   int npairs = 0;
-  int nblk_red = 512;      // blocks of the reduction kernels
 };
 
 namespace {
 
-int gemm(fw_fno* h, bool bf, bool b_is_bf16, const void* A, const void* Bm, float* Cm, int M, int N, int K, int nb1,
-         int nb2, long long sAm, long long sAk, long long sBk, long long sBn, long long sCm, long long sCn,
-         long long sA1, long long sA2, long long sB1, long long sB2, long long sC1, long long sC2, float alpha,
-         float beta) {
+enum { G_F32 = 0, G_BF = 1, G_BF_B16 = 2, G_BF_AB16 = 3 };
+
+int gemm(fw_fno* h, int kind, const void* A, const void* Bm, float* Cm, int M, int N, int K, int nb1, int nb2,
+         long long sAm, long long sAk, long long sBk, long long sBn, long long sCm, long long sCn, long long sA1,
+         long long sA2, long long sB1, long long sB2, long long sC1, long long sC2, float alpha, float beta) {
   GemmArgs g;
   g.A = A; g.B = Bm; g.C = Cm; g.M = M; g.N = N; g.K = K; g.nb2 = nb2;
   g.sAm = sAm; g.sAk = sAk; g.sBk = sBk; g.sBn = sBn; g.sCm = sCm; g.sCn = sCn;
   g.sA1 = sA1; g.sA2 = sA2; g.sB1 = sB1; g.sB2 = sB2; g.sC1 = sC1; g.sC2 = sC2;
   g.alpha = alpha; g.beta = beta;
   dim3 grid((N + fno::GT - 1) / fno::GT, (M + fno::GT - 1) / fno::GT, nb1 * nb2);
-  if (bf) {
-    if (b_is_bf16) hipLaunchKernelGGL((fno::gemm_kernel<float, bf16_t, true>), grid, dim3(256), 0, h->stream, g);
-    else hipLaunchKernelGGL((fno::gemm_kernel<float, float, true>), grid, dim3(256), 0, h->stream, g);
-  } else {
-    hipLaunchKernelGGL((fno::gemm_kernel<float, float, false>), grid, dim3(256), 0, h->stream, g);
+  switch (kind) {
+    case G_F32: hipLaunchKernelGGL((fno::gemm_kernel<float, float, false>), grid, dim3(256), 0, h->stream, g); break;
+    case G_BF: hipLaunchKernelGGL((fno::gemm_kernel<float, float, true>), grid, dim3(256), 0, h->stream, g); break;
+    case G_BF_B16: hipLaunchKernelGGL((fno::gemm_kernel<float, bf16_t, true>), grid, dim3(256), 0, h->stream, g); break;
+    default: hipLaunchKernelGGL((fno::gemm_kernel<bf16_t, bf16_t, true>), grid, dim3(256), 0, h->stream, g); break;
   }
   HIPCHK(hipGetLastError());
   return FW_OK;
@@ -563,144 +939,176 @@ int gemm(fw_fno* h, bool bf, bool b_is_bf16, const void* A, const void* Bm, floa
 
 int grid_for(long long n) { return (int)std::max(1LL, std::min(4096LL, (n + 255) / 256)); }
 
+// ---- kernel dispatch over the compiled shapes (N in {64, 128, 256}, M in {8, 16}) ----
+template <int N, int M>
+void launch_dft(fw_fno* h, int mode, const fno::DftArgs& a, int B) {
+  const dim3 grid(B * fno::C), blk(fno::DftShape<N>::WAVES * 64);
+  switch (mode) {
+    case fno::IN_V: hipLaunchKernelGGL((fno::dft_fwd<N, M, fno::IN_V>), grid, blk, 0, h->stream, a); break;
+    case fno::IN_LIFT: hipLaunchKernelGGL((fno::dft_fwd<N, M, fno::IN_LIFT>), grid, blk, 0, h->stream, a); break;
+    case fno::IN_GRAD: hipLaunchKernelGGL((fno::dft_fwd<N, M, fno::IN_GRAD>), grid, blk, 0, h->stream, a); break;
+    default: hipLaunchKernelGGL((fno::dft_fwd<N, M, fno::IN_GRAD_LIN>), grid, blk, 0, h->stream, a); break;
+  }
+}
+template <int N, int M>
+void launch_inv(fw_fno* h, int mode, const fno::InvArgs& a, int B) {
+  const dim3 grid(B * N / fno::PB), blk(512);
+  switch (mode) {
+    case fno::OUT_HIDDEN: hipLaunchKernelGGL((fno::inv_point<N, M, fno::OUT_HIDDEN>), grid, blk, 0, h->stream, a); break;
+    case fno::OUT_LAST: hipLaunchKernelGGL((fno::inv_point<N, M, fno::OUT_LAST>), grid, blk, 0, h->stream, a); break;
+    default: hipLaunchKernelGGL((fno::inv_point<N, M, fno::OUT_GRAD>), grid, blk, 0, h->stream, a); break;
+  }
+}
+template <int N>
+void launch_shape(fw_fno* h, bool dft, int mode, const void* a, int B) {
+  if (h->Mm == 8) {
+    if (dft) launch_dft<N, 8>(h, mode, *(const fno::DftArgs*)a, B);
+    else launch_inv<N, 8>(h, mode, *(const fno::InvArgs*)a, B);
+  } else {
+    if (dft) launch_dft<N, 16>(h, mode, *(const fno::DftArgs*)a, B);
+    else launch_inv<N, 16>(h, mode, *(const fno::InvArgs*)a, B);
+  }
+}
+int launch_layer_kernel(fw_fno* h, bool dft, int mode, const void* a, int B) {
+  switch (h->N) {
+    case 64: launch_shape<64>(h, dft, mode, a, B); break;
+    case 128: launch_shape<128>(h, dft, mode, a, B); break;
+    default: launch_shape<256>(h, dft, mode, a, B); break;
+  }
+  HIPCHK(hipGetLastError());
+  return FW_OK;
+}
+int dft(fw_fno* h, int mode, const fno::DftArgs& a, int B) { return launch_layer_kernel(h, true, mode, &a, B); }
+int inv(fw_fno* h, int mode, const fno::InvArgs& a, int B) { return launch_layer_kernel(h, false, mode, &a, B); }
+int mix(fw_fno* h, const fno::MixArgs& a, bool grad_R, const float* Z, float* gR) {
+  if (h->Mm == 8) {
+    hipLaunchKernelGGL(fno::mix_modes<8>, dim3(h->nm / 4), dim3(256), 0, h->stream, a);
+    if (grad_R) hipLaunchKernelGGL(fno::mode_grad<8>, dim3(h->nm), dim3(256), 0, h->stream, Z, a.Zin, gR, a.B);
+  } else {
+    hipLaunchKernelGGL(fno::mix_modes<16>, dim3(h->nm / 4), dim3(256), 0, h->stream, a);
+    if (grad_R) hipLaunchKernelGGL(fno::mode_grad<16>, dim3(h->nm), dim3(256), 0, h->stream, Z, a.Zin, gR, a.B);
+  }
+  HIPCHK(hipGetLastError());
+  return FW_OK;
+}
+
 int refresh_weights(fw_fno* h) {
-  const int C = h->C;
+  const int C = fno::C;
   for (int l = 0; l < h->L; ++l) {
+    const size_t wo = (size_t)l * h->nm * 4 * C * C;
     hipLaunchKernelGGL(fno::expand_modes, dim3(grid_for((long long)h->nm * C * C)), dim3(256), 0, h->stream,
-                       h->P + h->o_R[l], h->Wm + (long long)l * h->nm * 4 * C * C, h->nm, C, C);
-    hipLaunchKernelGGL(fno::to_bf16_kernel, dim3(grid_for((long long)C * C)), dim3(256), 0, h->stream,
-                       h->P + h->o_Wp[l], h->Wpb + (long long)l * C * C, (long long)C * C);
+                       h->P + h->o_R[l], h->Wm + wo, h->WmT + wo, h->nm);
+    hipLaunchKernelGGL(fno::pointwise_weights, dim3(grid_for((long long)C * C)), dim3(256), 0, h->stream,
+                       h->P + h->o_Wp[l], h->Wpb + (size_t)l * C * C, h->WpT + (size_t)l * C * C);
   }
   HIPCHK(hipGetLastError());
   return FW_OK;
 }
 
-// spectral convolution of layer l: out (+)= S_l(vin) ; Z kept in h->Zs[l]
-int spectral_fwd(fw_fno* h, int l, const float* vin, float* outp, int B, float beta) {
-  const int N = h->N, C = h->C, Mm = h->Mm, KP = 2 * Mm, nm = h->nm;
-  const long long NN = (long long)N * N;
-  // S1: T1[b][p][r(2M)][c] = sum_x Fx[r][x] v[b][p][x][c]
-  FWCHK(gemm(h, false, false, h->Fx, vin, h->T1, 2 * Mm, C, N, B, N, N, 1, C, 1, C, 1, 0, 0, NN * C, (long long)N * C,
-             (long long)N * 2 * Mm * C, (long long)2 * Mm * C, 1.f, 0.f));
-  // S2: G[b][r(2KP)][q] = sum_p Fp[r][p] T1[b][p][q], q < 2M C
-  const long long Q = (long long)2 * Mm * C;
-  FWCHK(gemm(h, false, false, h->Fp, h->T1, h->Gb, 2 * KP, (int)Q, N, B, 1, N, 1, Q, 1, Q, 1, 0, 0, (long long)N * Q, 0,
-             (long long)2 * KP * Q, 0, 1.f, 0.f));
-  float* Z = h->Zs[l];
-  hipLaunchKernelGGL(fno::assemble, dim3(grid_for((long long)B * KP * Mm * C)), dim3(256), 0, h->stream, h->Gb, Z, B,
-                     Mm, C, 0);
-  // S3: Y[b][m][2O] = Z[b][m][2C] Wm[m]
-  const bf16_t* Wm = h->Wm + (long long)l * nm * 4 * C * C;
-  FWCHK(gemm(h, true, true, Z, Wm, h->Y, B, 2 * C, 2 * C, 1, nm, (long long)nm * 2 * C, 1, 2 * C, 1,
-             (long long)nm * 2 * C, 1, 0, 2 * C, 0, (long long)4 * C * C, 0, 2 * C, 1.f, 0.f));
-  hipLaunchKernelGGL(fno::disassemble, dim3(grid_for((long long)B * 2 * nm * C)), dim3(256), 0, h->stream, h->Y,
-                     h->Yst, B, Mm, C, 0);
-  // S4: H[b][r(2N)][q(M O)] = sum_s Gp[r][s] Yst[b][s][q], s < 2 KP
-  const long long QO = (long long)Mm * C;
-  FWCHK(gemm(h, false, false, h->Gp, h->Yst, h->H, 2 * N, (int)QO, 2 * KP, B, 1, 2 * KP, 1, QO, 1, QO, 1, 0, 0,
-             (long long)2 * KP * QO, 0, (long long)2 * N * QO, 0, 1.f, 0.f));
-  // S5: out[b][p][x][o] = sum_ky Gxr[x][ky] Hr[b][p][ky][o] + Gxi[x][ky] Hi[b][p][ky][o]
-  FWCHK(gemm(h, false, false, h->Gxr, h->H, outp, N, C, Mm, B, N, Mm, 1, C, 1, C, 1, 0, 0, (long long)2 * N * QO, QO,
-             NN * C, (long long)N * C, 1.f, beta));
-  FWCHK(gemm(h, false, false, h->Gxi, h->H + (long long)N * QO, outp, N, C, Mm, B, N, Mm, 1, C, 1, C, 1, 0, 0,
-             (long long)2 * N * QO, QO, NN * C, (long long)N * C, 1.f, 1.f));
-  return FW_OK;
+fno::DftArgs dft_common(fw_fno* h) {
+  fno::DftArgs a{};
+  a.Fx = h->Fx;
+  a.Fp = h->Fp;
+  a.xs = 2.f / (float)h->N;
+  a.ps = 2.f / (float)h->N;
+  return a;
 }
 
-// forward over a batch in h->bin / h->bft; prediction in h->out
-int forward(fw_fno* h, int B) {
-  const int N = h->N, C = h->C;
-  const long long npts = (long long)B * N * N, tot = npts * C;
-  const float xs = (float)(2.0 * h->d.x_amplitude / N / h->d.x_amplitude);
-  const float ps = (float)(2.0 * h->d.p_amplitude / N / h->d.p_amplitude);
-  hipLaunchKernelGGL(fno::lift_fwd, dim3(grid_for(tot)), dim3(256), 0, h->stream, h->bin, h->bft, h->P + h->o_Pw,
-                     h->P + h->o_pb, h->v[0], B, N, C, xs, ps);
-  for (int l = 0; l < h->L; ++l) {
-    float* acc = h->pre[l];
-    FWCHK(spectral_fwd(h, l, h->v[l], acc, B, 0.f));
-    // pointwise: acc += v Wp
-    FWCHK(gemm(h, true, true, h->v[l], h->Wpb + (long long)l * C * C, acc, (int)npts, C, C, 1, 1, C, 1, C, 1, C, 1, 0,
-               0, 0, 0, 0, 0, 1.f, 1.f));
-    hipLaunchKernelGGL(fno::bias_act, dim3(grid_for(tot)), dim3(256), 0, h->stream, acc, h->P + h->o_b[l], h->v[l + 1],
-                       tot, C, l < h->L - 1 ? 1 : 0);
-  }
-  hipLaunchKernelGGL(fno::proj_fwd, dim3(grid_for(npts)), dim3(256), 0, h->stream, h->bin, h->v[h->L], h->P + h->o_Q,
-                     h->P + h->o_qb, h->out, npts, C, h->out_scale);
-  if (h->zero_mass)
-    hipLaunchKernelGGL(fno::center_kernel, dim3(B), dim3(1024), 0, h->stream, h->out, h->bin, (long long)N * N);
-  HIPCHK(hipGetLastError());
-  return FW_OK;
-}
-
-// backward from h->dout; gradients into h->G
-int backward(fw_fno* h, int B) {
-  const int N = h->N, C = h->C, Mm = h->Mm, KP = 2 * Mm, nm = h->nm;
-  const long long NN = (long long)N * N, npts = (long long)B * NN, tot = npts * C;
-  const int nr = h->nblk_red;
-  if (npts % nr) return fw::fail(FW_ERR_INVALID_ARG, "batch * grid points must be a multiple of 512");
-  HIPCHK(hipMemsetAsync(h->G, 0, sizeof(float) * h->nparam, h->stream));
-  // projection
-  hipLaunchKernelGGL(fno::proj_bwd, dim3(nr), dim3(256), 0, h->stream, h->v[h->L], h->dout, h->P + h->o_Q, h->dv, h->part,
-                     npts, C, h->out_scale);
-  hipLaunchKernelGGL(fno::reduce_rows, dim3((C + 1 + 255) / 256), dim3(256), 0, h->stream, h->part, h->G + h->o_Q, nr,
-                     C + 1, 1.f, 0);   // Q then qb (adjacent)
-  float* dcur = h->dv;    // gradient w.r.t. v[l+1]
-  float* dnext = h->dv2;  // gradient w.r.t. v[l]
-  const long long Q = (long long)2 * Mm * C, QO = (long long)Mm * C;
-  for (int l = h->L - 1; l >= 0; --l) {
-    // activation and bias: dcur <- dpre
-    hipLaunchKernelGGL(fno::act_bwd, dim3(nr), dim3(256), 0, h->stream, h->pre[l], dcur, h->part, tot, C,
-                       l < h->L - 1 ? 1 : 0);
-    hipLaunchKernelGGL(fno::reduce_rows, dim3(1), dim3(256), 0, h->stream, h->part, h->G + h->o_b[l], nr, C, 1.f, 0);
-    // pointwise: dnext = dpre Wp^T ; dWp = v^T dpre (split over nr chunks of points)
-    FWCHK(gemm(h, true, true, dcur, h->Wpb + (long long)l * C * C, dnext, (int)npts, C, C, 1, 1, C, 1, 1, C, C, 1, 0, 0,
-               0, 0, 0, 0, 1.f, 0.f));
-    const long long chunk = npts / nr;   // nr divides npts (checked on entry)
-    {
-      FWCHK(gemm(h, true, false, h->v[l], dcur, h->part, C, C, (int)chunk, nr, 1, 1, C, C, 1, C, 1, chunk * C, 0,
-                 chunk * C, 0, (long long)C * C, 0, 1.f, 0.f));
-      hipLaunchKernelGGL(fno::reduce_rows, dim3((C * C + 255) / 256), dim3(256), 0, h->stream, h->part, h->G + h->o_Wp[l],
-                         nr, C * C, 1.f, 0);
+// forward over a batch in h->bin / h->bft; prediction in h->out. train: keep what
+// the backward needs (pre-activations of the GELU layers, v_L).
+int forward(fw_fno* h, int B, bool train) {
+  const int N = h->N, L = h->L, C = fno::C;
+  for (int l = 0; l < L; ++l) {
+    fno::DftArgs da = dft_common(h);
+    da.zscale = h->zs_f;
+    da.Z = h->Zs[l];
+    if (l == 0) {
+      da.W = h->bin;
+      da.feat = h->bft;
+      da.Pw = h->P + h->o_Pw;
+      da.pb = h->P + h->o_pb;
+      da.vout = h->v[0];
+    } else {
+      da.v = h->v[l];
     }
-    // spectral backward, gradient w.r.t. the spectral output is dcur
-    // S5^T: dH[b][p][ky][o] = sum_x Gxr[x][ky] d[b][p][x][o] ; dHi likewise with Gxi
-    FWCHK(gemm(h, false, false, h->Gxr, dcur, h->H, Mm, C, N, B, N, 1, Mm, C, 1, C, 1, 0, 0, NN * C, (long long)N * C,
-               (long long)2 * N * QO, QO, 1.f, 0.f));
-    FWCHK(gemm(h, false, false, h->Gxi, dcur, h->H + (long long)N * QO, Mm, C, N, B, N, 1, Mm, C, 1, C, 1, 0, 0, NN * C,
-               (long long)N * C, (long long)2 * N * QO, QO, 1.f, 0.f));
-    // S4^T: dYst[b][s][q] = sum_r Gp[r][s] dH[b][r][q]
-    FWCHK(gemm(h, false, false, h->Gp, h->H, h->Yst, 2 * KP, (int)QO, 2 * N, B, 1, 1, 2 * KP, QO, 1, QO, 1, 0, 0,
-               (long long)2 * N * QO, 0, (long long)2 * KP * QO, 0, 1.f, 0.f));
-    hipLaunchKernelGGL(fno::disassemble, dim3(grid_for((long long)B * 2 * nm * C)), dim3(256), 0, h->stream, h->Y,
-                       h->Yst, B, Mm, C, 1);
-    // S3^T: dZ[b][m] = dY[b][m] Wm[m]^T ; dWm[m] = Z[:, m]^T dY[:, m]
-    const bf16_t* Wm = h->Wm + (long long)l * nm * 4 * C * C;
-    float* dZ = h->T1;   // scratch (B nm 2C <= B N 2M C)
-    FWCHK(gemm(h, true, true, h->Y, Wm, dZ, B, 2 * C, 2 * C, 1, nm, (long long)nm * 2 * C, 1, 1, 2 * C,
-               (long long)nm * 2 * C, 1, 0, 2 * C, 0, (long long)4 * C * C, 0, 2 * C, 1.f, 0.f));
-    FWCHK(gemm(h, true, false, h->Zs[l], h->Y, h->dWm, 2 * C, 2 * C, B, 1, nm, 1, (long long)nm * 2 * C,
-               (long long)nm * 2 * C, 1, 2 * C, 1, 0, 2 * C, 0, 2 * C, 0, (long long)4 * C * C, 1.f, 0.f));
-    hipLaunchKernelGGL(fno::fold_modes, dim3(grid_for((long long)nm * C * C)), dim3(256), 0, h->stream, h->dWm,
-                       h->G + h->o_R[l], nm, C, C);
-    // S3a^T: dG from dZ
-    hipLaunchKernelGGL(fno::assemble, dim3(grid_for((long long)B * KP * Mm * C)), dim3(256), 0, h->stream, h->Gb, dZ, B,
-                       Mm, C, 1);
-    // S2^T: dT1[b][p][q] = sum_r Fp[r][p] dG[b][r][q]
-    float* dT1 = h->T1 + (long long)B * nm * 2 * C;   // after dZ
-    FWCHK(gemm(h, false, false, h->Fp, h->Gb, dT1, N, (int)Q, 2 * KP, B, 1, 1, N, Q, 1, Q, 1, 0, 0,
-               (long long)2 * KP * Q, 0, (long long)N * Q, 0, 1.f, 0.f));
-    // S1^T: dnext[b][p][x][c] += sum_r Fx[r][x] dT1[b][p][r][c]
-    FWCHK(gemm(h, false, false, h->Fx, dT1, dnext, N, C, 2 * Mm, B, N, 1, N, C, 1, C, 1, 0, 0, (long long)N * Q, Q,
-               NN * C, (long long)N * C, 1.f, 1.f));
+    FWCHK(dft(h, l == 0 ? fno::IN_LIFT : fno::IN_V, da, B));
+    const fno::MixArgs ma{h->Zs[l], h->WmT + (size_t)l * h->nm * 4 * C * C, h->BT, B};
+    FWCHK(mix(h, ma, false, nullptr, nullptr));
+    fno::InvArgs ia{};
+    ia.BT = h->BT;
+    ia.GpT = h->GpT;
+    ia.Gx = h->Gxf;
+    ia.vin = h->v[l];
+    ia.Wpt = h->WpT + (size_t)l * C * C;
+    ia.bias = h->P + h->o_b[l];
+    if (l < L - 1) {
+      ia.vout = h->v[l + 1];
+      ia.pre = train ? h->pre[l] : nullptr;
+      FWCHK(inv(h, fno::OUT_HIDDEN, ia, B));
+    } else {
+      ia.vout = train ? h->v[L] : nullptr;
+      ia.W = h->bin;
+      ia.Q = h->P + h->o_Q;
+      ia.qb = h->P + h->o_qb;
+      ia.oscale = h->out_scale;
+      ia.out = h->out;
+      ia.cpart = h->cpart;
+      FWCHK(inv(h, fno::OUT_LAST, ia, B));
+    }
+  }
+  if (h->zero_mass)
+    hipLaunchKernelGGL(fno::center_out, dim3(64, B), dim3(256), 0, h->stream, h->out, h->cpart, N / fno::PB,
+                       (long long)N * N, h->out_scale);
+  HIPCHK(hipGetLastError());
+  return FW_OK;
+}
+
+// backward from h->dout (after a training forward); gradients into h->G
+int backward(fw_fno* h, int B) {
+  const int N = h->N, C = fno::C, L = h->L;
+  const long long NN = (long long)N * N;
+  HIPCHK(hipMemsetAsync(h->G, 0, sizeof(float) * h->nparam, h->stream));
+  // projection: dv_L, gradients of Q and qb (adjacent)
+  hipLaunchKernelGGL(fno::proj_bwd, dim3(B * N), dim3(256), 0, h->stream, h->v[L], h->dout, h->P + h->o_Q, h->dv,
+                     h->part, N, h->out_scale);
+  hipLaunchKernelGGL(fno::reduce_rows, dim3(1), dim3(256), 0, h->stream, h->part, h->G + h->o_Q, B * N, C + 1, 1.f, 0);
+  float* dcur = h->dv;    // gradient w.r.t. the output of layer l
+  float* dnext = h->dv2;  // gradient w.r.t. its input
+  const long long chunk = 1024, nch = NN / chunk;
+  for (int l = L - 1; l >= 0; --l) {
+    // dpre (bf16) and the bias gradient; dY = (c_ky / N^2) Z(dpre)
+    fno::DftArgs da = dft_common(h);
+    da.dv = dcur;
+    da.pre = l < L - 1 ? h->pre[l] : nullptr;
+    da.vout = h->dpre;
+    da.bias_part = h->part;
+    da.zscale = h->zs_b;
+    da.Z = h->dY;
+    FWCHK(dft(h, l < L - 1 ? fno::IN_GRAD : fno::IN_GRAD_LIN, da, B));
+    hipLaunchKernelGGL(fno::reduce_rows, dim3(1), dim3(256), 0, h->stream, h->part, h->G + h->o_b[l], B, C, 1.f, 0);
+    // mode weights' gradient and dZ = dY Wm^T into the inverse's B operand
+    const fno::MixArgs ma{h->dY, h->Wm + (size_t)l * h->nm * 4 * C * C, h->BT, B};
+    FWCHK(mix(h, ma, true, h->Zs[l], h->G + h->o_R[l]));
+    // pointwise weights' gradient: sum over points of v[c] dpre[o], in chunks of 1024 points
+    FWCHK(gemm(h, G_BF_AB16, h->v[l], h->dpre, h->part, C, C, (int)chunk, B, (int)nch, NN, 1, 1, NN, C, 1,
+               (long long)C * NN, chunk, (long long)C * NN, chunk, nch * C * C, (long long)C * C, 1.f, 0.f));
+    hipLaunchKernelGGL(fno::reduce_rows, dim3((C * C + 255) / 256), dim3(256), 0, h->stream, h->part,
+                       h->G + h->o_Wp[l], (int)(B * nch), C * C, 1.f, 0);
+    // dv = Re sum_modes dZ e^{+i theta} + Wp dpre
+    fno::InvArgs ia{};
+    ia.BT = h->BT;
+    ia.GpT = h->GpT;
+    ia.Gx = h->Gxu;
+    ia.vin = h->dpre;
+    ia.Wpt = h->Wpb + (size_t)l * C * C;
+    ia.dvout = dnext;
+    FWCHK(inv(h, fno::OUT_GRAD, ia, B));
     std::swap(dcur, dnext);
   }
-  // lift: dcur is the gradient w.r.t. v0
-  const float xs = (float)(2.0 * h->d.x_amplitude / N / h->d.x_amplitude);
-  const float ps = (float)(2.0 * h->d.p_amplitude / N / h->d.p_amplitude);
-  hipLaunchKernelGGL(fno::lift_bwd_partial, dim3(nr), dim3(256), 0, h->stream, h->bin, h->bft, dcur, h->part, B, N, C,
-                     xs, ps);
-  hipLaunchKernelGGL(fno::reduce_rows, dim3((9 * C + 255) / 256), dim3(256), 0, h->stream, h->part, h->G + h->o_Pw, nr,
-                     9 * C, 1.f, 0);   // Pw (8 C) then pb (C): adjacent in the parameter vector
+  hipLaunchKernelGGL(fno::lift_bwd_img, dim3(B * C), dim3(256), 0, h->stream, dcur, h->bin, h->part, N, 2.f / N,
+                     2.f / N);
+  hipLaunchKernelGGL(fno::lift_bwd_fin, dim3(1), dim3(64), 0, h->stream, h->part, h->bft, h->G + h->o_Pw,
+                     h->G + h->o_pb, B);
   HIPCHK(hipGetLastError());
   return FW_OK;
 }
@@ -724,50 +1132,67 @@ int loss(fw_fno* h, int B, bool want_grad, double* out) {
   return FW_OK;
 }
 
-void dft_matrices(int N, int Mm, std::vector<float>& Fx, std::vector<float>& Fp, std::vector<float>& Gp,
-                  std::vector<float>& Gxr, std::vector<float>& Gxi) {
-  const int KP = 2 * Mm;
-  auto kp_of = [&](int i) { return i < Mm ? i : N - KP + i; };
-  const long double tp = 2.0L * 3.141592653589793238462643383279502884L / N;
-  Fx.assign((size_t)2 * Mm * N, 0.f);
-  for (int r = 0; r < Mm; ++r)
+// bf16 (round to nearest even) of a float on the host, as the kernels round
+bf16_t host_bf16(float f) {
+  unsigned int u;
+  std::memcpy(&u, &f, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (bf16_t)(u >> 16);
+}
+float host_float(bf16_t h) {
+  const unsigned int u = (unsigned int)h << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+// Transform tables. Angles reduced exactly in integers, 2 pi ((k n) mod N) / N,
+// evaluated in double, rounded to float and then to bf16.
+//   Fx[r][x]  (2M x N): cos(ky x) for r = ky < M, -sin(ky x) for r = M + ky
+//   Fp[s][p]  (4M x N): cos(kp p) for s < 2M, sin(kp p) for s = 2M + i; kp_i = i (i < M), N - 2M + i
+//   GpT[p][k] = Fp[k][p]
+//   Gxf[x][k'] = (c_ky / N^2) Fx[k'][x] (irfft weights; exact power-of-two scaling),
+//   Gxu[x][k'] = Fx[k'][x]; both zero for k' >= 2M (padding to 32)
+void fno_tables(int N, int M, std::vector<bf16_t>& Fx, std::vector<bf16_t>& Fp, std::vector<bf16_t>& GpT,
+                std::vector<bf16_t>& Gxf, std::vector<bf16_t>& Gxu, std::vector<float>& cky) {
+  auto ang = [N](long long k, long long n) { return 2.0 * 3.14159265358979323846 * (double)((k * n) % N) / N; };
+  auto kp_of = [N, M](int i) { return i < M ? i : N - 2 * M + i; };
+  Fx.assign((size_t)2 * M * N, 0);
+  for (int r = 0; r < M; ++r)
     for (int x = 0; x < N; ++x) {
-      const long double a = tp * ((long long)r * x % N);
-      Fx[(size_t)r * N + x] = (float)cosl(a);
-      Fx[(size_t)(Mm + r) * N + x] = (float)-sinl(a);
+      Fx[(size_t)r * N + x] = host_bf16((float)std::cos(ang(r, x)));
+      Fx[(size_t)(M + r) * N + x] = host_bf16((float)-std::sin(ang(r, x)));
     }
-  Fp.assign((size_t)2 * KP * N, 0.f);
-  for (int r = 0; r < KP; ++r)
+  Fp.assign((size_t)4 * M * N, 0);
+  for (int i = 0; i < 2 * M; ++i)
     for (int p = 0; p < N; ++p) {
-      const long double a = tp * ((long long)kp_of(r) * p % N);
-      Fp[(size_t)r * N + p] = (float)cosl(a);
-      Fp[(size_t)(KP + r) * N + p] = (float)sinl(a);
+      Fp[(size_t)i * N + p] = host_bf16((float)std::cos(ang(kp_of(i), p)));
+      Fp[(size_t)(2 * M + i) * N + p] = host_bf16((float)std::sin(ang(kp_of(i), p)));
     }
-  Gp.assign((size_t)2 * N * 2 * KP, 0.f);
+  GpT.assign((size_t)N * 4 * M, 0);
   for (int p = 0; p < N; ++p)
-    for (int s = 0; s < KP; ++s) {
-      const long double a = tp * ((long long)kp_of(s) * p % N);
-      const float c = (float)cosl(a), sn = (float)sinl(a);
-      Gp[(size_t)p * 2 * KP + s] = c;             // Hr from Yr
-      Gp[(size_t)p * 2 * KP + KP + s] = -sn;      // Hr from Yi
-      Gp[(size_t)(N + p) * 2 * KP + s] = sn;      // Hi from Yr
-      Gp[(size_t)(N + p) * 2 * KP + KP + s] = c;  // Hi from Yi
-    }
-  Gxr.assign((size_t)N * Mm, 0.f);
-  Gxi.assign((size_t)N * Mm, 0.f);
-  const long double inv = 1.0L / ((long double)N * N);
+    for (int k = 0; k < 4 * M; ++k) GpT[(size_t)p * 4 * M + k] = Fp[(size_t)k * N + p];
+  cky.assign(M, 0.f);
+  for (int ky = 0; ky < M; ++ky) cky[ky] = (float)((ky == 0 ? 1.0 : 2.0) / ((double)N * N));
+  Gxf.assign((size_t)N * fno::XK, 0);
+  Gxu.assign((size_t)N * fno::XK, 0);
   for (int x = 0; x < N; ++x)
-    for (int ky = 0; ky < Mm; ++ky) {
-      const long double a = tp * ((long long)ky * x % N);
-      const long double cw = (ky == 0 ? 1.0L : 2.0L) * inv;
-      Gxr[(size_t)x * Mm + ky] = (float)(cw * cosl(a));
-      Gxi[(size_t)x * Mm + ky] = (float)(-cw * sinl(a));
+    for (int k = 0; k < 2 * M; ++k) {
+      const bf16_t f = Fx[(size_t)k * N + x];
+      Gxu[(size_t)x * fno::XK + k] = f;
+      Gxf[(size_t)x * fno::XK + k] = host_bf16(host_float(f) * cky[k % M]);
     }
 }
 
 template <typename T>
 int dalloc(T** p, size_t n) {
   HIPCHK(hipMalloc((void**)p, sizeof(T) * std::max<size_t>(n, 1)));
+  return FW_OK;
+}
+template <typename T>
+int upload(T** p, const std::vector<T>& host, hipStream_t s) {
+  FWCHK(dalloc(p, host.size()));
+  HIPCHK(hipMemcpyAsync(*p, host.data(), sizeof(T) * host.size(), hipMemcpyHostToDevice, s));
   return FW_OK;
 }
 
@@ -782,14 +1207,13 @@ extern "C" int fw_fno_create(const fw_fno_desc* d, fw_fno** out) {
   fw::clear_error();
   if (!d || !out) return fw::fail(FW_ERR_INVALID_ARG, "NULL argument");
   *out = nullptr;
-  if (d->grid_dim < 64 || d->grid_dim > 1024 || !fw::is_pow2(d->grid_dim))
-    return fw::fail(FW_ERR_INVALID_ARG, "grid_dim must be a power of two in [64, 1024]");
-  if (d->width < 8 || d->width > 128 || d->width % 8) return fw::fail(FW_ERR_INVALID_ARG, "width must be 8..128, a multiple of 8");
-  if (d->modes < 1 || 2 * d->modes > d->grid_dim / 2) return fw::fail(FW_ERR_INVALID_ARG, "modes must be in [1, N/4]");
+  if (d->grid_dim != 64 && d->grid_dim != 128 && d->grid_dim != 256)
+    return fw::fail(FW_ERR_INVALID_ARG, "grid_dim must be 64, 128 or 256");
+  if (d->width != fno::C) return fw::fail(FW_ERR_INVALID_ARG, "width must be 32");
+  if (d->modes != 8 && d->modes != 16) return fw::fail(FW_ERR_INVALID_ARG, "modes must be 8 or 16");
+  if (4 * d->modes > d->grid_dim) return fw::fail(FW_ERR_INVALID_ARG, "modes must be at most grid_dim / 4");
   if (d->layers < 1 || d->layers > 16) return fw::fail(FW_ERR_INVALID_ARG, "layers must be in [1, 16]");
-  if (d->max_batch < 1 || d->max_batch > 64) return fw::fail(FW_ERR_INVALID_ARG, "max_batch must be in [1, 64]");
-  if (256 % d->width != 0 && d->width > 256) return fw::fail(FW_ERR_INVALID_ARG, "width must divide 256");
-  if (256 % d->width != 0) return fw::fail(FW_ERR_INVALID_ARG, "width must divide 256");
+  if (d->max_batch < 1 || d->max_batch > 16) return fw::fail(FW_ERR_INVALID_ARG, "max_batch must be in [1, 16]");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fw::fail(FW_ERR_NO_DEVICE, "no GPU visible");
   if (d->device < 0 || d->device >= ndev) return fw::fail(FW_ERR_INVALID_ARG, "device ordinal out of range");
@@ -797,9 +1221,9 @@ extern "C" int fw_fno_create(const fw_fno_desc* d, fw_fno** out) {
   auto h = new fw_fno();
   std::unique_ptr<fw_fno, int (*)(fw_fno*)> guard(h, fw_fno_destroy);
   h->d = *d;
-  h->N = d->grid_dim; h->C = d->width; h->Mm = d->modes; h->L = d->layers; h->Bmax = d->max_batch;
+  h->N = d->grid_dim; h->Mm = d->modes; h->L = d->layers; h->Bmax = d->max_batch;
   h->nm = 2 * h->Mm * h->Mm;
-  const int N = h->N, C = h->C, Mm = h->Mm, L = h->L, B = h->Bmax, nm = h->nm;
+  const int N = h->N, C = fno::C, Mm = h->Mm, L = h->L, B = h->Bmax, nm = h->nm;
   // parameter layout
   long long o = 0;
   h->o_Pw = o; o += 8LL * C;
@@ -821,42 +1245,43 @@ extern "C" int fw_fno_create(const fw_fno_desc* d, fw_fno** out) {
   HIPCHK(hipMemsetAsync(h->Ma, 0, sizeof(float) * o, h->stream));
   HIPCHK(hipMemsetAsync(h->Va, 0, sizeof(float) * o, h->stream));
   FWCHK(dalloc(&h->Wm, (size_t)L * nm * 4 * C * C));
+  FWCHK(dalloc(&h->WmT, (size_t)L * nm * 4 * C * C));
   FWCHK(dalloc(&h->Wpb, (size_t)L * C * C));
-  std::vector<float> Fx, Fp, Gp, Gxr, Gxi;
-  dft_matrices(N, Mm, Fx, Fp, Gp, Gxr, Gxi);
-  FWCHK(dalloc(&h->Fx, Fx.size()));
-  FWCHK(dalloc(&h->Fp, Fp.size()));
-  FWCHK(dalloc(&h->Gp, Gp.size()));
-  FWCHK(dalloc(&h->Gxr, Gxr.size()));
-  FWCHK(dalloc(&h->Gxi, Gxi.size()));
-  HIPCHK(hipMemcpyAsync(h->Fx, Fx.data(), sizeof(float) * Fx.size(), hipMemcpyHostToDevice, h->stream));
-  HIPCHK(hipMemcpyAsync(h->Fp, Fp.data(), sizeof(float) * Fp.size(), hipMemcpyHostToDevice, h->stream));
-  HIPCHK(hipMemcpyAsync(h->Gp, Gp.data(), sizeof(float) * Gp.size(), hipMemcpyHostToDevice, h->stream));
-  HIPCHK(hipMemcpyAsync(h->Gxr, Gxr.data(), sizeof(float) * Gxr.size(), hipMemcpyHostToDevice, h->stream));
-  HIPCHK(hipMemcpyAsync(h->Gxi, Gxi.data(), sizeof(float) * Gxi.size(), hipMemcpyHostToDevice, h->stream));
-  const size_t act = (size_t)B * N * N * C;
+  FWCHK(dalloc(&h->WpT, (size_t)L * C * C));
+  std::vector<bf16_t> Fx, Fp, GpT, Gxf, Gxu;
+  std::vector<float> cky, ones(Mm, 1.f);
+  fno_tables(N, Mm, Fx, Fp, GpT, Gxf, Gxu, cky);
+  FWCHK(upload(&h->Fx, Fx, h->stream));
+  FWCHK(upload(&h->Fp, Fp, h->stream));
+  FWCHK(upload(&h->GpT, GpT, h->stream));
+  FWCHK(upload(&h->Gxf, Gxf, h->stream));
+  FWCHK(upload(&h->Gxu, Gxu, h->stream));
+  FWCHK(upload(&h->zs_f, ones, h->stream));
+  FWCHK(upload(&h->zs_b, cky, h->stream));
+  const size_t act = (size_t)B * C * N * N;
   for (int l = 0; l <= L; ++l) {
-    float* p;
+    bf16_t* p;
     FWCHK(dalloc(&p, act));
     h->v.push_back(p);
   }
   for (int l = 0; l < L; ++l) {
     float* p;
-    FWCHK(dalloc(&p, act));
-    h->pre.push_back(p);
+    if (l < L - 1) {
+      FWCHK(dalloc(&p, act));
+      h->pre.push_back(p);
+    }
     FWCHK(dalloc(&p, (size_t)B * nm * 2 * C));
     h->Zs.push_back(p);
   }
-  // T1 also holds dZ and dT1 in the backward pass: B N 2M C + B nm 2C
-  FWCHK(dalloc(&h->T1, (size_t)B * N * 2 * Mm * C + (size_t)B * nm * 2 * C));
-  FWCHK(dalloc(&h->Gb, (size_t)B * 4 * Mm * 2 * Mm * C));
-  FWCHK(dalloc(&h->Y, (size_t)B * nm * 2 * C));
-  FWCHK(dalloc(&h->Yst, (size_t)B * nm * 2 * C));
-  FWCHK(dalloc(&h->H, (size_t)B * 2 * N * Mm * C));
+  FWCHK(dalloc(&h->BT, (size_t)B * 2 * Mm * C * 4 * Mm));
+  FWCHK(dalloc(&h->dY, (size_t)B * nm * 2 * C));
+  FWCHK(dalloc(&h->dpre, act));
   FWCHK(dalloc(&h->dv, act));
   FWCHK(dalloc(&h->dv2, act));
-  FWCHK(dalloc(&h->dWm, (size_t)nm * 4 * C * C));
-  FWCHK(dalloc(&h->part, (size_t)h->nblk_red * std::max(9 * C, C * C + 1)));
+  // partial sums: pointwise weight gradient (B * N^2/1024 * C^2), projection (B N (C+1)), lift (B C 4)
+  const size_t npart = std::max({(size_t)B * (N * N / 1024) * C * C, (size_t)B * N * (C + 1), (size_t)B * C * 4});
+  FWCHK(dalloc(&h->part, npart));
+  FWCHK(dalloc(&h->cpart, (size_t)B * N / fno::PB));
   FWCHK(dalloc(&h->out, (size_t)B * N * N));
   FWCHK(dalloc(&h->dout, (size_t)B * N * N));
   FWCHK(dalloc(&h->bin, (size_t)B * N * N));
@@ -874,9 +1299,10 @@ extern "C" int fw_fno_create(const fw_fno_desc* d, fw_fno** out) {
 extern "C" int fw_fno_destroy(fw_fno* h) {
   if (!h) return FW_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
-  std::vector<void*> bufs = {h->P, h->G, h->Ma, h->Va, h->Wm, h->Wpb, h->Fx, h->Fp, h->Gp, h->Gxr, h->Gxi, h->T1,
-                             h->Gb, h->Y, h->Yst, h->H, h->dv, h->dv2, h->dWm, h->part, h->out, h->dout, h->bin,
-                             h->btg, h->bft, h->bidx, h->persample, h->mom, h->din, h->dtg, h->dft};
+  std::vector<void*> bufs = {h->P, h->G, h->Ma, h->Va, h->Wm, h->WmT, h->Wpb, h->WpT, h->Fx, h->Fp, h->GpT, h->Gxf,
+                             h->Gxu, h->zs_f, h->zs_b, h->BT, h->dY, h->dpre, h->dv, h->dv2, h->part, h->cpart,
+                             h->out, h->dout, h->bin, h->btg, h->bft, h->bidx, h->persample, h->mom, h->din, h->dtg,
+                             h->dft};
   for (auto p : h->v) bufs.push_back(p);
   for (auto p : h->pre) bufs.push_back(p);
   for (auto p : h->Zs) bufs.push_back(p);
@@ -935,7 +1361,7 @@ extern "C" int fw_fno_forward(fw_fno* h, const float* host_in, const float* host
   if (!h || !host_in || !host_feat || !host_out) return fw::fail(FW_ERR_INVALID_ARG, "NULL argument");
   FWCHK(ensure_batch(h, batch));
   FWCHK(upload_batch(h, host_in, host_feat, nullptr, batch));
-  FWCHK(forward(h, batch));
+  FWCHK(forward(h, batch, false));
   HIPCHK(hipMemcpyAsync(host_out, h->out, sizeof(float) * batch * h->N * h->N, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
   return FW_OK;
@@ -960,7 +1386,7 @@ extern "C" int fw_fno_loss_grad(fw_fno* h, const float* host_in, const float* ho
   if (!h || !host_in || !host_feat || !host_target || !loss_out) return fw::fail(FW_ERR_INVALID_ARG, "NULL argument");
   FWCHK(ensure_batch(h, batch));
   FWCHK(upload_batch(h, host_in, host_feat, host_target, batch));
-  FWCHK(forward(h, batch));
+  FWCHK(forward(h, batch, host_grad != nullptr));
   FWCHK(loss(h, batch, true, loss_out));
   if (host_grad) {
     FWCHK(backward(h, batch));
@@ -1012,7 +1438,7 @@ extern "C" int fw_fno_train_step(fw_fno* h, const int* host_idx, int batch, floa
   FWCHK(ensure_batch(h, batch));
   if (!h->din) return fw::fail(FW_ERR_INVALID_ARG, "no dataset");
   FWCHK(gather(h, host_idx, batch));
-  FWCHK(forward(h, batch));
+  FWCHK(forward(h, batch, true));
   FWCHK(loss(h, batch, true, loss_out));
   FWCHK(backward(h, batch));
   h->t_adam += 1;
@@ -1034,7 +1460,7 @@ extern "C" int fw_fno_eval_pairs(fw_fno* h, const int* host_idx, int n, double* 
   for (int s = 0; s < n; s += h->Bmax) {
     const int B = std::min(h->Bmax, n - s);
     FWCHK(gather(h, host_idx + s, B));
-    FWCHK(forward(h, B));
+    FWCHK(forward(h, B, false));
     FWCHK(loss(h, B, false, nullptr));
     std::vector<double> ps(B);
     HIPCHK(hipMemcpyAsync(ps.data(), h->persample, sizeof(double) * B, hipMemcpyDeviceToHost, h->stream));
@@ -1051,9 +1477,9 @@ extern "C" int fw_fno_infer_time(fw_fno* h, int batch, int reps, double* us_per_
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  FWCHK(forward(h, batch));
+  FWCHK(forward(h, batch, false));
   HIPCHK(hipEventRecord(e0, h->stream));
-  for (int r = 0; r < reps; ++r) FWCHK(forward(h, batch));
+  for (int r = 0; r < reps; ++r) FWCHK(forward(h, batch, false));
   HIPCHK(hipEventRecord(e1, h->stream));
   HIPCHK(hipEventSynchronize(e1));
   float ms = 0;
@@ -1082,7 +1508,7 @@ extern "C" int fw_fno_gemm_test(int bf16, const float* hA, const float* hB, floa
   // A is stored M x K row-major (transA = 0) or K x M (transA = 1); B is K x N or N x K
   const long long sAm = transA ? 1 : K, sAk = transA ? M : 1;
   const long long sBk = transB ? 1 : N, sBn = transB ? K : 1;
-  int rc = gemm(&tmp, bf16 != 0, false, dA, dB, dC, M, N, K, 1, 1, sAm, sAk, sBk, sBn, N, 1, 0, 0, 0, 0, 0, 0, 1.f, 0.f);
+  int rc = gemm(&tmp, bf16 != 0 ? G_BF : G_F32, dA, dB, dC, M, N, K, 1, 1, sAm, sAk, sBk, sBn, N, 1, 0, 0, 0, 0, 0, 0, 1.f, 0.f);
   if (rc == FW_OK) {
     HIPCHK(hipStreamSynchronize(tmp.stream));
     HIPCHK(hipMemcpy(hC, dC, sizeof(float) * M * N, hipMemcpyDeviceToHost));

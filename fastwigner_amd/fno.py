@@ -10,8 +10,10 @@ random-collision step, wigner_random_collision_model_fftw_1d.py:56-76).
 Network: lift 8 -> C, L Fourier layers (spectral convolution on M x M retained
 modes plus pointwise C x C mixing, GELU), projection C -> 1 added to the input
 W (the untrained network, with a zero projection, is the identity map).
-Channel mixing runs on BF16 MFMA with FP32 accumulation; the truncated
-transforms on FP32 MFMA.
+Each layer is three fused kernels on BF16 MFMA with FP32 accumulation: the
+truncated forward DFT of each channel image, the per-mode channel mixing, and
+the inverse fused with the pointwise mixing, bias and GELU (the projection in
+the last layer). Activations are stored as bf16, channel-first.
 
 Base method and learned correction: with base_steps = b > 0 the input of the
 network is not W(t) but the propagator's own coarse prediction, b split-operator

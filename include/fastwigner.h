@@ -367,11 +367,11 @@ int fw_batch_copy_states(fw_batch* dst, fw_batch* src);
 typedef struct fw_fno fw_fno;
 
 typedef struct fw_fno_desc {
-  int grid_dim;           /* N: inputs are N x N Wigner functions, N a power of two in [64, 1024] */
-  int width;              /* channels C (divides 256) */
-  int modes;              /* retained modes M per axis (ky < M; kp < M or kp >= N - M) */
-  int layers;             /* Fourier layers */
-  int max_batch;          /* samples per forward / training step */
+  int grid_dim;           /* N: inputs are N x N Wigner functions, N = 64, 128 or 256 */
+  int width;              /* channels C: 32 (the fused layer kernels are written for it) */
+  int modes;              /* retained modes M per axis, 8 or 16, 4 M <= N (ky < M; kp < M or kp >= N - M) */
+  int layers;             /* Fourier layers, 1 .. 16 */
+  int max_batch;          /* samples per forward / training step, 1 .. 16 */
   int device;
   double x_amplitude, p_amplitude;
 } fw_fno_desc;
@@ -407,8 +407,9 @@ int fw_fno_train_step(fw_fno* h, const int* host_idx, int batch, float lr, doubl
 int fw_fno_eval_pairs(fw_fno* h, const int* host_idx, int n, double* rel_l2_out);
 /* device time of one forward pass over `batch` samples (us), averaged over reps */
 int fw_fno_infer_time(fw_fno* h, int batch, int reps, double* us_per_batch);
-/* C = A B through the MFMA GEMM kernel (bf16 != 0: BF16 operands, else FP32), host arrays; A is M x K
- * (K x M if transA), B is K x N (N x K if transB), C is M x N row-major. For testing the kernel. */
+/* C = A B through the strided MFMA GEMM kernel that reduces the pointwise weights' gradient over
+ * points (bf16 != 0: BF16 operands, else FP32), host arrays; A is M x K (K x M if transA), B is
+ * K x N (N x K if transB), C is M x N row-major. For testing the kernel. */
 int fw_fno_gemm_test(int bf16, const float* A, const float* B, float* C, int M, int N, int K, int transA, int transB);
 
 #ifdef __cplusplus

@@ -3,14 +3,27 @@ backward) -- TEST INFRASTRUCTURE ONLY: the checker of the HIP kernels in
 fastwigner_amd/csrc/fw_fno.cpp, never called by the product.
 
 The reference has no learned component; this network is defined by the build
-(the surrogate of k split-operator steps). The restatement uses numpy's FFTs
-for the truncated transforms (the HIP code multiplies by cosine/sine matrices)
-and float64 arithmetic. With ``bf16=True`` it rounds to bfloat16 (round to
-nearest even, through float32) exactly the operands the HIP channel-mixing
-GEMMs round: the mode-mixing inputs and weights, and the pointwise-mixing
-inputs and weights, forward and backward. Pinned in tests/test_fno_oracle.py
-by central finite differences (bf16=False) and by the identity-weight case,
-where the spectral convolution reduces to a low-pass filter.
+(the surrogate of k split-operator steps). Activations are channel-first,
+v[b][c][p][x]. The truncated transforms are written as products with explicit
+cosine/sine matrices, exactly as the HIP kernels compute them:
+
+  forward DFT of a channel image (p, x):
+    T[p][r] = sum_x v[p][x] Fx[r][x]          Fx = [cos(2 pi ky x/N) ; -sin(2 pi ky x/N)], ky < M
+    O[s][r] = sum_p Fp[s][p] T[p][r]          Fp = [cos(2 pi kp p/N) ; sin(2 pi kp p/N)], 2M values of kp
+    Z[i][ky] = (O[i][ky] + O[2M+i][M+ky]) + i (O[i][M+ky] - O[2M+i][ky])   (= sum v e^{-i theta})
+  inverse (numpy irfft2 of the truncated spectrum Y):
+    H[p][ky] = sum_i Y[i][ky] e^{+2 pi i kp_i p/N}     (Hr, Hi through the same Fp)
+    S[p][x]  = sum_ky Gx[ky][x] Hr[p][ky] + Gx[M+ky][x] Hi[p][ky],  Gx = (c_ky/N^2) Fx, c_0 = 1, c_ky = 2
+
+With ``bf16=True`` it rounds to bfloat16 (round to nearest even, through
+float32) exactly what the HIP kernels round: the stored activations (v, the
+lift output, dpre), the transform matrices, the intermediates T and H of both
+transforms, the spectra Z / Y entering the mode mixing and the inverse, the
+mixing and pointwise weights, forward and backward. Its gradient treats each
+rounding as the identity. With ``bf16=False`` (no rounding) the network is the
+exact truncated-FFT network and the gradient the exact one: pinned in
+tests/test_fno_oracle.py by central finite differences, by numpy's rfft2 /
+irfft2, and by a direct DFT sum for identity mode weights.
 
 Parameter vector (float32 on the GPU, float64 here), in this order:
   Pw[8][C], pb[C], for each layer {R[2M*M][C][C][2], Wp[C][C], b[C]}, Q[C], qb[1]
@@ -53,6 +66,15 @@ class FNOOracle:
         self.o_qb = o; o += 1
         self.nparam = o
         self.kp = np.concatenate([np.arange(M), np.arange(N - M, N)])
+        # transform matrices; angles reduced exactly in integers, (k n) mod N
+        x = np.arange(N)
+        a = 2.0 * np.pi * ((np.arange(M)[:, None] * x[None, :]) % N) / N
+        self.Fx = self._r(np.concatenate([np.cos(a), -np.sin(a)]))        # (2M, N)
+        b = 2.0 * np.pi * ((self.kp[:, None] * x[None, :]) % N) / N
+        self.Fp = self._r(np.concatenate([np.cos(b), np.sin(b)]))         # (4M, N)
+        # irfft weights c_ky / N^2 (powers of two: scaling a rounded matrix by them is exact)
+        self.cky = np.where(np.arange(M) == 0, 1.0, 2.0) / (N * N)
+        self.Gx = self.Fx * np.concatenate([self.cky, self.cky])[:, None]
 
     # -- helpers -----------------------------------------------------------------------
     def _r(self, x):
@@ -68,6 +90,7 @@ class FNOOracle:
         return d
 
     def features(self, W, feat):
+        """f[b][p][x][j]; x/X_amplitude = 2 (x_index - N/2) / N, likewise p."""
         B, N = W.shape[0], self.N
         g = (np.arange(N) - 0.5 * N) * (2.0 / N)
         f = np.empty((B, N, N, 8))
@@ -79,17 +102,27 @@ class FNOOracle:
         return f
 
     def modes(self, v):
-        """Z[b, m = kpi*M + ky, c] = 2-D DFT of each channel on the retained modes."""
-        F = np.fft.rfft2(v, axes=(1, 2))            # [B, N, N/2+1, C]
-        Z = F[:, self.kp, :self.M, :]               # [B, 2M, M, C]
-        return Z.reshape(v.shape[0], self.nm, self.C)
+        """Z[b, m = i*M + ky, c] (complex) of channel images v[b][c][p][x] on the retained modes."""
+        B, C, M = v.shape[0], v.shape[1], self.M
+        T = self._r(np.einsum("bcpx,rx->bcpr", v, self.Fx, optimize=True))
+        O = np.einsum("sp,bcpr->bcsr", self.Fp, T, optimize=True)
+        Zr = O[:, :, :2 * M, :M] + O[:, :, 2 * M:, M:]
+        Zi = O[:, :, :2 * M, M:] - O[:, :, 2 * M:, :M]
+        return (Zr + 1j * Zi).reshape(B, C, self.nm).transpose(0, 2, 1)
 
-    def inverse(self, Y):
-        """numpy irfft2 of the spectrum Y on the retained modes (zero elsewhere)."""
-        B, N = Y.shape[0], self.N
-        full = np.zeros((B, N, N // 2 + 1, Y.shape[-1]), dtype=complex)
-        full[:, self.kp, :self.M, :] = Y.reshape(B, 2 * self.M, self.M, -1)
-        return np.fft.irfft2(full, s=(N, N), axes=(1, 2))
+    def inverse(self, Y, G=None):
+        """S[b][o][p][x] = sum over the retained modes of Y[b, m, o] (rounded) through the
+        p transform (Fp) and the x table G (default: the irfft weights Gx)."""
+        G = self.Gx if G is None else G
+        B, O, M = Y.shape[0], Y.shape[-1], self.M
+        Y = Y.transpose(0, 2, 1).reshape(B, O, 2 * M, M)
+        Yr, Yi = self._r(Y.real), self._r(Y.imag)
+        cs, sn = self.Fp[:2 * M], self.Fp[2 * M:]
+        Hr = np.einsum("ip,boik->bopk", cs, Yr, optimize=True) - np.einsum("ip,boik->bopk", sn, Yi, optimize=True)
+        Hi = np.einsum("ip,boik->bopk", sn, Yr, optimize=True) + np.einsum("ip,boik->bopk", cs, Yi, optimize=True)
+        Hr, Hi = self._r(Hr), self._r(Hi)
+        return (np.einsum("kx,bopk->bopx", G[:M], Hr, optimize=True)
+                + np.einsum("kx,bopk->bopx", G[M:], Hi, optimize=True))
 
     def mix(self, Z, R):
         """Y[b, m, o] = sum_c Z[b, m, c] R[m, c, o] with the real 2C x 2O block
@@ -104,17 +137,17 @@ class FNOOracle:
     def forward(self, P, W, feat, keep=False):
         p = self.unpack(P)
         f = self.features(W, feat)
-        v = [np.einsum("bpxj,jc->bpxc", f, p["Pw"], optimize=True) + p["pb"]]
+        v = [self._r(np.einsum("bpxj,jc->bcpx", f, p["Pw"], optimize=True) + p["pb"][None, :, None, None])]
         pres, Zs = [], []
         for l in range(self.L):
             Z = self.modes(v[l])
             Zs.append(Z)
             S = self.inverse(self.mix(Z, p["R"][l]))
-            Pt = np.einsum("bpxc,co->bpxo", self._r(v[l]), self._r(p["Wp"][l]), optimize=True)
-            pre = S + Pt + p["b"][l]
+            Pt = np.einsum("bcpx,co->bopx", v[l], self._r(p["Wp"][l]), optimize=True)
+            pre = S + Pt + p["b"][l][None, :, None, None]
             pres.append(pre)
-            v.append(gelu(pre) if l < self.L - 1 else pre)
-        c = np.einsum("bpxc,c->bpx", v[-1], p["Q"], optimize=True) + p["qb"]
+            v.append(self._r(gelu(pre) if l < self.L - 1 else pre))
+        c = np.einsum("bcpx,c->bpx", v[-1], p["Q"], optimize=True) + p["qb"]
         if self.zero_mass:
             c = c - c.mean(axis=(1, 2), keepdims=True)
         out = W + self.out_scale * c
@@ -131,27 +164,27 @@ class FNOOracle:
     def loss_grad(self, P, W, feat, target):
         out, k = self.forward(P, W, feat, keep=True)
         L, st = self.loss(out, target)
-        B, N, C, M = W.shape[0], self.N, self.C, self.M
+        B, C, M = W.shape[0], self.C, self.M
         p = k["p"]
         dout = 2.0 * (out - target) / st[:, None, None] / B
         g = np.zeros(self.nparam)
         dout = self.out_scale * dout     # from here on, the gradient w.r.t. the projection's output c
         if self.zero_mass:
             dout = dout - dout.mean(axis=(1, 2), keepdims=True)
-        g[self.o_Q:self.o_Q + C] = np.einsum("bpxc,bpx->c", k["v"][-1], dout, optimize=True)
+        g[self.o_Q:self.o_Q + C] = np.einsum("bcpx,bpx->c", k["v"][-1], dout, optimize=True)
         g[self.o_qb] = dout.sum()
-        dv = dout[..., None] * p["Q"]
-        cky = np.where(np.arange(M) == 0, 1.0, 2.0)
+        dv = p["Q"][None, :, None, None] * dout[:, None]
+        # dY = (c_ky / N^2) Z(dpre): the adjoint of the inverse is the forward transform
+        # (Gx^T = (c_ky/N^2) Fx, Gp^T = Fp), scaled per ky
+        yscale = np.tile(self.cky, 2 * M)
         for l in range(self.L - 1, -1, -1):
             dpre = dv * gelu_d(k["pre"][l]) if l < self.L - 1 else dv
-            g[self.o_b[l]:self.o_b[l] + C] = dpre.sum(axis=(0, 1, 2))
-            Wp = p["Wp"][l]
-            dvn = np.einsum("bpxo,co->bpxc", self._r(dpre), self._r(Wp), optimize=True)
-            g[self.o_Wp[l]:self.o_Wp[l] + C * C] = np.einsum("bpxc,bpxo->co", self._r(k["v"][l]), self._r(dpre),
-                                                             optimize=True).ravel()
-            # adjoint of the inverse transform: dY = (c_ky / N^2) rfft2(dpre) on the retained modes
-            D = np.fft.rfft2(dpre, axes=(1, 2))[:, self.kp, :M, :] * (cky[None, None, :, None] / (N * N))
-            dY = D.reshape(B, self.nm, C)
+            g[self.o_b[l]:self.o_b[l] + C] = dpre.sum(axis=(0, 2, 3))
+            d16 = self._r(dpre)
+            Wp = self._r(p["Wp"][l])
+            dvn = np.einsum("bopx,co->bcpx", d16, Wp, optimize=True)
+            g[self.o_Wp[l]:self.o_Wp[l] + C * C] = np.einsum("bcpx,bopx->co", k["v"][l], d16, optimize=True).ravel()
+            dY = self.modes(d16) * yscale[None, :, None]
             dYr, dYi = self._r(dY.real), self._r(dY.imag)
             R = p["R"][l]
             Rr, Ri = self._r(R[..., 0]), self._r(R[..., 1])
@@ -168,14 +201,12 @@ class FNOOracle:
                    - np.einsum("bmc,bmo->mco", Zi, dYr, optimize=True))
             gR = np.stack([dRr, dRi], axis=-1)
             g[self.o_R[l]:self.o_R[l] + gR.size] = gR.ravel()
-            # adjoint of the forward transform: dv = Re sum_modes (dZr + i dZi) e^{+i theta}
-            full = np.zeros((B, N, N, C), dtype=complex)
-            full[:, self.kp, :M, :] = (dZr + 1j * dZi).reshape(B, 2 * M, M, C)
-            dvn = dvn + np.real(np.fft.ifft2(full, axes=(1, 2))) * (N * N)
-            dv = dvn
+            # adjoint of the forward transform: the inverse's structure with the table Fx
+            # (unit weights): dv = Re sum_modes (dZr + i dZi) e^{+i theta}
+            dv = dvn + self.inverse(dZr + 1j * dZi, G=self.Fx)
         f = k["f"]
-        g[self.o_Pw:self.o_Pw + 8 * C] = np.einsum("bpxj,bpxc->jc", f, dv, optimize=True).ravel()
-        g[self.o_pb:self.o_pb + C] = dv.sum(axis=(0, 1, 2))
+        g[self.o_Pw:self.o_Pw + 8 * C] = np.einsum("bpxj,bcpx->jc", f, dv, optimize=True).ravel()
+        g[self.o_pb:self.o_pb + C] = dv.sum(axis=(0, 2, 3))
         return L, g
 
 

@@ -1,6 +1,6 @@
 """FNO surrogate kernels against the numpy oracle (oracle/fno_oracle.py, which
-rounds to bf16 where the HIP channel-mixing GEMMs do), training on batched
-propagator trajectories, and the checked jump with fallback."""
+rounds to bf16 where the HIP layer kernels do), training on batched propagator
+trajectories, and the checked jump with fallback."""
 import numpy as np
 import pytest
 
@@ -79,8 +79,11 @@ def test_forward_and_gradients_vs_oracle(gpu, N, C, M, L, scale, zm):
     print("N=%d C=%d M=%d L=%d: network output rel %.2e, loss %.6e vs %.6e, gradient rel %.2e, worst group %s %.2e"
           % (N, C, M, L, e_out, loss, L0, e_g, max(groups, key=groups.get), max(groups.values())))
     # the bf16 roundings can fall on the other side of a tie from FP32 vs FP64
-    # accumulation; each such flip moves one mixed operand by 2^-8 relative
-    assert e_out <= 5e-4
+    # accumulation; each such flip moves one operand by 2^-8 relative. At 256^2
+    # (sums of 256 terms with cancellation at high modes, 4 layers) the oracle
+    # itself moves by 7.0e-4 when its sums are taken in FP32 instead of FP64, so
+    # the bar there is 1.5e-3; wrong indexing or a missing factor is O(1)
+    assert e_out <= (5e-4 if N == 64 else 1.5e-3)
     assert abs(loss - L0) <= 1e-4 * L0
     assert e_g <= 2e-3
     assert max(groups.values()) <= 2e-2
