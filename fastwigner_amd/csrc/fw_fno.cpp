@@ -47,13 +47,10 @@ typedef float float4_t __attribute__((ext_vector_type(4)));
 namespace fno {
 
 // --------------------------------------------------------------------------
-// bf16 (round to nearest even; inputs are finite)
+// bf16, round to nearest even: the IEEE conversion, which gfx950 does in one
+// instruction (v_cvt_pk_bf16_f32)
 // --------------------------------------------------------------------------
-__device__ __forceinline__ bf16_t to_bf16(float f) {
-  unsigned int u = __float_as_uint(f);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (bf16_t)(u >> 16);
-}
+__device__ __forceinline__ bf16_t to_bf16(float f) { return __builtin_bit_cast(bf16_t, (__bf16)f); }
 __device__ __forceinline__ float from_bf16(bf16_t h) { return __uint_as_float((unsigned int)h << 16); }
 
 // --------------------------------------------------------------------------
@@ -193,9 +190,37 @@ __device__ __forceinline__ float4_t mma16(u16x4_t a, u16x4_t b, float4_t c) {
 __device__ __forceinline__ u16x8_t ld8(const bf16_t* p) { return *reinterpret_cast<const u16x8_t*>(p); }
 __device__ __forceinline__ void st8(bf16_t* p, u16x8_t v) { *reinterpret_cast<u16x8_t*>(p) = v; }
 
-__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.f + erff(x * 0.70710678118654752f)); }
+// 1 + erf(z), branch-free: erfc(|z|) by the rational approximation of
+// Abramowitz & Stegun 7.1.26 (Hastings), t P(t) exp(-z^2) with t = 1/(1 + 0.3275911 |z|),
+// |error| <= 1.5e-7; 2 - erfc(|z|) for z >= 0, erfc(|z|) itself for z < 0 (no
+// cancellation in the negative tail). GELU and its derivative through it differ
+// from the erf-exact forms by at most 4.3e-7 (absolute) in FP32.
+__device__ __forceinline__ float one_plus_erf(float z) {
+  const float az = fabsf(z);
+  const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, az, 1.f));
+  float y = fmaf(t, 1.061405429f, -1.453152027f);
+  y = fmaf(t, y, 1.421413741f);
+  y = fmaf(t, y, -0.284496736f);
+  y = fmaf(t, y, 0.254829592f);
+  const float c = y * t * __expf(-az * az);
+  return z >= 0.f ? 2.f - c : c;
+}
+__device__ __forceinline__ float gelu(float x) { return 0.5f * x * one_plus_erf(x * 0.70710678118654752f); }
 __device__ __forceinline__ float gelu_d(float x) {
-  return 0.5f * (1.f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * __expf(-0.5f * x * x);
+  return 0.5f * one_plus_erf(x * 0.70710678118654752f) + x * 0.39894228040143268f * __expf(-0.5f * x * x);
+}
+// sum over the 16 lanes of a row of the wave, in every lane: DPP quad swaps
+// (xor 1, xor 2), then the half-row and row mirrors
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float row16_sum(float v) {
+  v += dpp_f<0xB1>(v);    // quad_perm [1, 0, 3, 2]
+  v += dpp_f<0x4E>(v);    // quad_perm [2, 3, 0, 1]
+  v += dpp_f<0x141>(v);   // row_half_mirror
+  v += dpp_f<0x140>(v);   // row_mirror
+  return v;
 }
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll
@@ -251,11 +276,33 @@ __global__ void __launch_bounds__(DftShape<N>::WAVES * 64) dft_fwd(DftArgs a) {
   const int b = blockIdx.x / C, c = blockIdx.x % C;
   const size_t img = (size_t)blockIdx.x * N * N;
 
+  // every global load of the kernel is independent of the others: IN_V issues
+  // the wave's whole slice of the image first, then the x table and the p
+  // table fragments of all its row groups
+  u16x8_t vpre[MODE == IN_V ? RG : 1][KS];
+  if constexpr (MODE == IN_V) {
+#pragma unroll
+    for (int g = 0; g < RG; ++g)
+#pragma unroll
+      for (int s = 0; s < KS; ++s)
+        vpre[g][s] = ld8(a.v + img + (size_t)(16 * (w * RG + g) + lr) * N + 32 * s + 8 * lg);
+  }
   u16x8_t bfx[RT][KS];
 #pragma unroll
   for (int t = 0; t < RT; ++t)
 #pragma unroll
     for (int s = 0; s < KS; ++s) bfx[t][s] = ld8(a.Fx + (size_t)(16 * t + lr) * N + 32 * s + 8 * lg);
+  // the p table's fragments of every row group: preloaded where the group loop
+  // is unrolled (IN_V), loaded per group otherwise
+  constexpr int FG = MODE == IN_V ? RG : 1;
+  u16x4_t fa[FG][ST];
+  if constexpr (MODE == IN_V) {
+#pragma unroll
+    for (int g = 0; g < RG; ++g)
+#pragma unroll
+      for (int u = 0; u < ST; ++u)
+        fa[g][u] = *reinterpret_cast<const u16x4_t*>(a.Fp + (size_t)(16 * u + lr) * N + 16 * (w * RG + g) + 4 * lg);
+  }
   float pw0 = 0.f, pw1 = 0.f, pw2 = 0.f, kf = 0.f;
   if constexpr (MODE == IN_LIFT) {
     pw0 = a.Pw[c];
@@ -284,7 +331,7 @@ __global__ void __launch_bounds__(DftShape<N>::WAVES * 64) dft_fwd(DftArgs a) {
       const int x = 32 * s + 8 * lg;
       const size_t e = img + (size_t)p * N + x;
       if constexpr (MODE == IN_V) {
-        av[s] = ld8(a.v + e);
+        av[s] = vpre[g][s];
       } else if constexpr (MODE == IN_LIFT) {
         const float4_t* wp = reinterpret_cast<const float4_t*>(a.W + ((size_t)b * N + p) * N + x);
         const float4_t w0 = wp[0], w1 = wp[1];
@@ -331,12 +378,15 @@ __global__ void __launch_bounds__(DftShape<N>::WAVES * 64) dft_fwd(DftArgs a) {
 #pragma unroll
     for (int t = 0; t < RT; ++t)
       tb[t] = (u16x4_t){to_bf16(T[t][0]), to_bf16(T[t][1]), to_bf16(T[t][2]), to_bf16(T[t][3])};
+    if constexpr (MODE != IN_V) {
 #pragma unroll
-    for (int u = 0; u < ST; ++u) {
-      const u16x4_t fa = *reinterpret_cast<const u16x4_t*>(a.Fp + (size_t)(16 * u + lr) * N + p0 + 4 * lg);
-#pragma unroll
-      for (int t = 0; t < RT; ++t) O[u][t] = mma16(fa, tb[t], O[u][t]);
+      for (int u = 0; u < ST; ++u)
+        fa[0][u] = *reinterpret_cast<const u16x4_t*>(a.Fp + (size_t)(16 * u + lr) * N + p0 + 4 * lg);
     }
+#pragma unroll
+    for (int u = 0; u < ST; ++u)
+#pragma unroll
+      for (int t = 0; t < RT; ++t) O[u][t] = mma16(fa[MODE == IN_V ? g : 0][u], tb[t], O[u][t]);
   }
 
 #pragma unroll
@@ -378,12 +428,15 @@ __global__ void __launch_bounds__(DftShape<N>::WAVES * 64) dft_fwd(DftArgs a) {
 }
 
 // ---- mode mixing ---------------------------------------------------------
-// One wave per mode m (4 per block): [Yr | Yi] (B x 2C) = bf16([Zr | Zi]) Wm[m]
-// with the expanded bf16 weights Wx[m][n][k] (n = output column, k contiguous):
-// forward Wm^T of [[Rr, Ri], [-Ri, Rr]], backward (dZ = dY Wm^T) Wm itself. The
-// result goes straight into the B operand of the inverse's p stage,
+// One wave per mode m = i M + ky, 8 waves per block (the 8 modes i = 8 ig .. 8 ig + 7
+// of one ky): [Yr | Yi] (B x 2C) = bf16([Zr | Zi]) Wm[m] with the expanded bf16
+// weights Wx[m][n][k] (n = output column, k contiguous): forward Wm^T of
+// [[Rr, Ri], [-Ri, Rr]], backward (dZ = dY Wm^T) Wm itself. The result is the B
+// operand of the inverse's p stage,
 //   BT[b][n' = (ri M + ky) C + o][k]  (k = i for cos kp, 2M + i for sin kp):
-//   Hr = sum_i cos Yr - sin Yi,  Hi = sum_i sin Yr + cos Yi.
+//   Hr = sum_i cos Yr - sin Yi,  Hi = sum_i sin Yr + cos Yi,
+// staged in LDS so that the block writes 16-byte runs of k (its 8 modes) instead
+// of single elements.
 struct MixArgs {
   const float* Zin;   // [B][2 M M][2][C]
   const bf16_t* Wx;   // [2 M M][2C][2C]
@@ -392,11 +445,12 @@ struct MixArgs {
 };
 
 template <int MM>
-__global__ void __launch_bounds__(256) mix_modes(MixArgs a) {
+__global__ void __launch_bounds__(512) mix_modes(MixArgs a) {
   constexpr int NM = 2 * MM * MM, NP = 2 * MM * C, K4 = 4 * MM;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, lr = lane & 15, lg = lane >> 4;
-  const int m = blockIdx.x * 4 + w;
-  const int i = m / MM, ky = m % MM;
+  __shared__ bf16_t ys[16][2 * C][8];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, lr = lane & 15, lg = lane >> 4;
+  const int ky = blockIdx.x % MM, ig = blockIdx.x / MM;
+  const int m = (8 * ig + w) * MM + ky;
   u16x8_t za[2];
 #pragma unroll
   for (int ks = 0; ks < 2; ++ks) {
@@ -417,22 +471,19 @@ __global__ void __launch_bounds__(256) mix_modes(MixArgs a) {
 #pragma unroll
     for (int ks = 0; ks < 2; ++ks)
       acc = mma32(za[ks], ld8(a.Wx + ((size_t)m * 2 * C + 16 * t + lr) * 2 * C + 32 * ks + 8 * lg), acc);
-    const int n = 16 * t + lr, o = n % C;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int bb = 4 * lg + j;
-      if (bb < a.B) {
-        bf16_t* bt = a.BT + (size_t)bb * NP * K4;
-        const float y = acc[j];
-        if (t < 2) {   // real part
-          bt[((size_t)ky * C + o) * K4 + i] = to_bf16(y);
-          bt[((size_t)(MM + ky) * C + o) * K4 + 2 * MM + i] = to_bf16(y);
-        } else {       // imaginary part
-          bt[((size_t)(MM + ky) * C + o) * K4 + i] = to_bf16(y);
-          bt[((size_t)ky * C + o) * K4 + 2 * MM + i] = to_bf16(-y);
-        }
-      }
-    }
+    for (int j = 0; j < 4; ++j)
+      if (4 * lg + j < a.B) ys[4 * lg + j][16 * t + lr][w] = to_bf16(acc[j]);
+  }
+  __syncthreads();
+  // 16-byte runs: (b, ri_out, o, half h): h = 0 takes k = 8 ig.., h = 1 takes 2M + 8 ig..;
+  // (0, 0) Yr, (0, 1) -Yi, (1, 0) Yi, (1, 1) Yr
+  for (int e = tid; e < a.B * 4 * C; e += 512) {
+    const int h = e & 1, o = (e >> 1) % C, ri = ((e >> 1) / C) & 1, bb = e / (4 * C);
+    u16x8_t v = *reinterpret_cast<const u16x8_t*>(&ys[bb][(ri ^ h) ? C + o : o][0]);
+    if (ri == 0 && h == 1) v ^= (u16x8_t)(unsigned short)0x8000;   // -Yi: the sign bits
+    *reinterpret_cast<u16x8_t*>(a.BT + (size_t)bb * NP * K4 + ((size_t)(ri * MM + ky) * C + o) * K4 + h * 2 * MM +
+                                8 * ig) = v;
   }
 }
 
@@ -469,9 +520,10 @@ __global__ void __launch_bounds__(256) mode_grad(const float* Z, const float* dY
 //     -> bf16 H in LDS as hs[r][o][k' = ri M + ky]
 //   x stage + pointwise, wave w on row p = p0 + w, tiles of 16 x by 16 o:
 //     acc = sum_k' Gx[x][k'] H[o][k']  +  sum_c vin[c][x] Wpt[o][c]
-//   (two 16x16x32 products into one accumulator). vin's row is staged in LDS
-//   (channel rows rotated by 16 (c / 8) elements so that the transposed reads of
-//   8 channels at one x by the 4 lane groups fall in distinct banks).
+//   (two 16x16x32 products into one accumulator). vin's whole row (32 channels
+//   x N) is staged in LDS, channel rows rotated by 16 (c / 8) elements so that
+//   the transposed reads of 8 channels at one x by the 4 lane groups fall in
+//   distinct banks. Every global load is issued before the first product.
 // Epilogues: OUT_HIDDEN pre = acc + bias -> bf16 gelu (and FP32 pre for training);
 // OUT_LAST pre = acc + bias -> bf16 v_L, projection c = Q . v_L + qb over the 32
 // channels (a shuffle sum across the 16 lanes of a row), out = W + scale c and
@@ -499,17 +551,18 @@ struct InvArgs {
 template <int N, int MM, int MODE>
 __global__ void __launch_bounds__(512) inv_point(InvArgs a) {
   constexpr int NP = 2 * MM * C, K4 = 4 * MM, KSB = K4 / 32, NTW = NP / 16 / 8;
-  constexpr int XC = N < 128 ? N : 128, NCH = N / XC, HS = XK + 8, NBR = N / PB;
-  __shared__ bf16_t hs[PB][C][HS];
-  __shared__ bf16_t vs[8][C][XC];
+  constexpr int HS = XK + 8, NBR = N / PB, XT = N / 16, VH = N / 32;
+  __shared__ __attribute__((aligned(16))) bf16_t hs[PB][C][HS];
+  __shared__ __attribute__((aligned(16))) bf16_t vs[8][C][N];
   __shared__ float cred[8];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, lr = lane & 15, lg = lane >> 4;
-  const int b = blockIdx.x / NBR, p0 = (blockIdx.x % NBR) * PB;
+  const int b = blockIdx.x / NBR, p0 = (blockIdx.x % NBR) * PB, p = p0 + w;
   if constexpr (2 * MM < XK) {
     for (int e = tid; e < PB * C * HS; e += 512) (&hs[0][0][0])[e] = 0;
     __syncthreads();
   }
-  // p stage
+  // global loads, all independent: the p stage's operands, then this wave's row
+  // of vin (all channels), then the pointwise weights
   u16x8_t ga[KSB];
 #pragma unroll
   for (int ks = 0; ks < KSB; ++ks) {
@@ -517,108 +570,134 @@ __global__ void __launch_bounds__(512) inv_point(InvArgs a) {
     if (lr < PB) ga[ks] = ld8(a.GpT + (size_t)(p0 + lr) * K4 + 32 * ks + 8 * lg);
   }
   const bf16_t* bt = a.BT + (size_t)b * NP * K4;
+  u16x8_t bb[NTW][KSB];
+#pragma unroll
+  for (int q = 0; q < NTW; ++q)
+#pragma unroll
+    for (int ks = 0; ks < KSB; ++ks) bb[q][ks] = ld8(bt + (size_t)(16 * (w * NTW + q) + lr) * K4 + 32 * ks + 8 * lg);
+  // the row of vin in two halves of x (so that the first half's products start
+  // while the second half is in flight)
+  u16x8_t vr[2][VH];
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int e = 0; e < VH; ++e) {
+      const int f = e * 64 + lane, cc = f / (N / 16), xg = f % (N / 16) + h * (N / 16);
+      vr[h][e] = ld8(a.vin + (((size_t)b * C + cc) * N + p) * N + 8 * xg);
+    }
+  u16x8_t bv[2];
+#pragma unroll
+  for (int ot = 0; ot < 2; ++ot) bv[ot] = ld8(a.Wpt + (16 * ot + lr) * C + 8 * lg);
+  // p stage -> H (bf16) in LDS
 #pragma unroll
   for (int q = 0; q < NTW; ++q) {
     const int np = 16 * (w * NTW + q) + lr;
     float4_t acc = (float4_t){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int ks = 0; ks < KSB; ++ks) acc = mma32(ga[ks], ld8(bt + (size_t)np * K4 + 32 * ks + 8 * lg), acc);
+    for (int ks = 0; ks < KSB; ++ks) acc = mma32(ga[ks], bb[q][ks], acc);
     if (lg < 2) {
       const int o = np % C, kq = np / C;
 #pragma unroll
       for (int j = 0; j < 4; ++j) hs[4 * lg + j][o][kq] = to_bf16(acc[j]);
     }
   }
-  __syncthreads();
+  // the x table's fragments (L2), then the row into LDS
+  u16x8_t gx[XT];
+#pragma unroll
+  for (int xt = 0; xt < XT; ++xt) gx[xt] = ld8(a.Gx + (size_t)(16 * xt + lr) * XK + 8 * lg);
+  __syncthreads();   // hs complete
 
   // x stage and pointwise mixing on row p
-  const int p = p0 + w;
-  u16x8_t bh[2], bv[2];
+  u16x8_t bh[2];
 #pragma unroll
-  for (int ot = 0; ot < 2; ++ot) {
-    bh[ot] = *reinterpret_cast<const u16x8_t*>(&hs[w][16 * ot + lr][8 * lg]);
-    bv[ot] = ld8(a.Wpt + (16 * ot + lr) * C + 8 * lg);
+  for (int ot = 0; ot < 2; ++ot) bh[ot] = *reinterpret_cast<const u16x8_t*>(&hs[w][16 * ot + lr][8 * lg]);
+  // the bias and projection weights of this lane's two output channels (loaded
+  // once: the stores below could alias them for the compiler)
+  float bo[2] = {0.f, 0.f}, qo[2] = {0.f, 0.f}, qb0 = 0.f;
+  if constexpr (MODE != OUT_GRAD) {
+#pragma unroll
+    for (int ot = 0; ot < 2; ++ot) bo[ot] = a.bias[16 * ot + lr];
+  }
+  if constexpr (MODE == OUT_LAST) {
+#pragma unroll
+    for (int ot = 0; ot < 2; ++ot) qo[ot] = a.Q[16 * ot + lr];
+    qb0 = a.qb[0];
   }
   float csum = 0.f;
-  for (int ch = 0; ch < NCH; ++ch) {
-    const int x0 = ch * XC;
 #pragma unroll
-    for (int e = 0; e < XC / 16; ++e) {
-      const int f = e * 64 + lane, cc = f / (XC / 8), xg = f % (XC / 8);
-      const u16x8_t val = ld8(a.vin + (((size_t)b * C + cc) * N + p) * N + x0 + 8 * xg);
-      *reinterpret_cast<u16x8_t*>(&vs[w][cc][(8 * xg + 16 * (cc >> 3)) & (XC - 1)]) = val;
+  for (int xt = 0; xt < XT; ++xt) {
+    if (xt == 0 || xt == XT / 2) {
+      // this half of the row into LDS (each wave its own row; the barrier orders
+      // the lanes' writes before the transposed reads)
+      const int h = xt == 0 ? 0 : 1;
+#pragma unroll
+      for (int e = 0; e < VH; ++e) {
+        const int f = e * 64 + lane, cc = f / (N / 16), xg = f % (N / 16) + h * (N / 16);
+        *reinterpret_cast<u16x8_t*>(&vs[w][cc][(8 * xg + 16 * (cc >> 3)) & (N - 1)]) = vr[h][e];
+      }
+      __syncthreads();
     }
-    __syncthreads();
-#pragma unroll 2
-    for (int xt = 0; xt < XC / 16; ++xt) {
-      const int xl = 16 * xt + lr;
-      const u16x8_t ah = ld8(a.Gx + (size_t)(x0 + xl) * XK + 8 * lg);
-      u16x8_t av;
+    const int xl = 16 * xt + lr;
+    u16x8_t av;
 #pragma unroll
-      for (int j = 0; j < 8; ++j) av[j] = vs[w][8 * lg + j][(xl + 16 * lg) & (XC - 1)];
-      float4_t acc[2];
+    for (int j = 0; j < 8; ++j) av[j] = vs[w][8 * lg + j][(xl + 16 * lg) & (N - 1)];
+    float4_t acc[2];
+#pragma unroll
+    for (int ot = 0; ot < 2; ++ot) {
+      acc[ot] = mma32(gx[xt], bh[ot], (float4_t){0.f, 0.f, 0.f, 0.f});
+      acc[ot] = mma32(av, bv[ot], acc[ot]);
+    }
+    const int xb = 16 * xt + 4 * lg;
+    if constexpr (MODE == OUT_HIDDEN) {
 #pragma unroll
       for (int ot = 0; ot < 2; ++ot) {
-        acc[ot] = mma32(ah, bh[ot], (float4_t){0.f, 0.f, 0.f, 0.f});
-        acc[ot] = mma32(av, bv[ot], acc[ot]);
+        const int o = 16 * ot + lr;
+        const size_t e = (((size_t)b * C + o) * N + p) * N + xb;
+        float4_t pr;
+        u16x4_t vv;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          pr[j] = acc[ot][j] + bo[ot];
+          vv[j] = to_bf16(gelu(pr[j]));
+        }
+        if (a.pre) *reinterpret_cast<float4_t*>(a.pre + e) = pr;
+        *reinterpret_cast<u16x4_t*>(a.vout + e) = vv;
       }
-      const int xb = x0 + 16 * xt + 4 * lg;
-      if constexpr (MODE == OUT_HIDDEN) {
+    } else if constexpr (MODE == OUT_LAST) {
+      float cp[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int ot = 0; ot < 2; ++ot) {
-          const int o = 16 * ot + lr;
-          const float bo = a.bias[o];
-          const size_t e = (((size_t)b * C + o) * N + p) * N + xb;
-          float4_t pr;
-          u16x4_t vv;
+      for (int ot = 0; ot < 2; ++ot) {
+        const int o = 16 * ot + lr;
+        const size_t e = (((size_t)b * C + o) * N + p) * N + xb;
+        u16x4_t vv;
 #pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            pr[j] = acc[ot][j] + bo;
-            vv[j] = to_bf16(gelu(pr[j]));
-          }
-          if (a.pre) *reinterpret_cast<float4_t*>(a.pre + e) = pr;
-          *reinterpret_cast<u16x4_t*>(a.vout + e) = vv;
+        for (int j = 0; j < 4; ++j) {
+          vv[j] = to_bf16(acc[ot][j] + bo[ot]);
+          cp[j] += qo[ot] * from_bf16(vv[j]);
         }
-      } else if constexpr (MODE == OUT_LAST) {
-        float cp[4] = {0.f, 0.f, 0.f, 0.f};
+        if (a.vout) *reinterpret_cast<u16x4_t*>(a.vout + e) = vv;
+      }
 #pragma unroll
-        for (int ot = 0; ot < 2; ++ot) {
-          const int o = 16 * ot + lr;
-          const float bo = a.bias[o], qo = a.Q[o];
-          const size_t e = (((size_t)b * C + o) * N + p) * N + xb;
-          u16x4_t vv;
+      for (int j = 0; j < 4; ++j) cp[j] = row16_sum(cp[j]);
+      if (lr == 0) {
+        const size_t e = ((size_t)b * N + p) * N + xb;
+        const float4_t wv = *reinterpret_cast<const float4_t*>(a.W + e);
+        float4_t ov;
 #pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            vv[j] = to_bf16(acc[ot][j] + bo);
-            cp[j] += qo * from_bf16(vv[j]);
-          }
-          if (a.vout) *reinterpret_cast<u16x4_t*>(a.vout + e) = vv;
+        for (int j = 0; j < 4; ++j) {
+          const float cj = cp[j] + qb0;
+          ov[j] = wv[j] + a.oscale * cj;
+          csum += cj;
         }
+        *reinterpret_cast<float4_t*>(a.out + e) = ov;
+      }
+    } else {
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-#pragma unroll
-          for (int s = 1; s < 16; s <<= 1) cp[j] += __shfl_xor(cp[j], s, 16);
-        if (lr == 0) {
-          const size_t e = ((size_t)b * N + p) * N + xb;
-          const float4_t wv = *reinterpret_cast<const float4_t*>(a.W + e);
-          float4_t ov;
-#pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            const float cj = cp[j] + a.qb[0];
-            ov[j] = wv[j] + a.oscale * cj;
-            csum += cj;
-          }
-          *reinterpret_cast<float4_t*>(a.out + e) = ov;
-        }
-      } else {
-#pragma unroll
-        for (int ot = 0; ot < 2; ++ot) {
-          const int o = 16 * ot + lr;
-          *reinterpret_cast<float4_t*>(a.dvout + (((size_t)b * C + o) * N + p) * N + xb) = acc[ot];
-        }
+      for (int ot = 0; ot < 2; ++ot) {
+        const int o = 16 * ot + lr;
+        *reinterpret_cast<float4_t*>(a.dvout + (((size_t)b * C + o) * N + p) * N + xb) = acc[ot];
       }
     }
-    __syncthreads();
   }
   if constexpr (MODE == OUT_LAST) {
     csum = wave_sum(csum);
@@ -982,10 +1061,10 @@ int dft(fw_fno* h, int mode, const fno::DftArgs& a, int B) { return launch_layer
 int inv(fw_fno* h, int mode, const fno::InvArgs& a, int B) { return launch_layer_kernel(h, false, mode, &a, B); }
 int mix(fw_fno* h, const fno::MixArgs& a, bool grad_R, const float* Z, float* gR) {
   if (h->Mm == 8) {
-    hipLaunchKernelGGL(fno::mix_modes<8>, dim3(h->nm / 4), dim3(256), 0, h->stream, a);
+    hipLaunchKernelGGL(fno::mix_modes<8>, dim3(h->nm / 8), dim3(512), 0, h->stream, a);
     if (grad_R) hipLaunchKernelGGL(fno::mode_grad<8>, dim3(h->nm), dim3(256), 0, h->stream, Z, a.Zin, gR, a.B);
   } else {
-    hipLaunchKernelGGL(fno::mix_modes<16>, dim3(h->nm / 4), dim3(256), 0, h->stream, a);
+    hipLaunchKernelGGL(fno::mix_modes<16>, dim3(h->nm / 8), dim3(512), 0, h->stream, a);
     if (grad_R) hipLaunchKernelGGL(fno::mode_grad<16>, dim3(h->nm), dim3(256), 0, h->stream, Z, a.Zin, gR, a.B);
   }
   HIPCHK(hipGetLastError());

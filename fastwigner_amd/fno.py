@@ -260,20 +260,40 @@ def checked_jump(surrogate, W0, feat, ref_states, dXdP, l2_tol=1e-3, norm_tol=1e
 
 
 def forward_flops(batch, n=256, width=32, modes=16, layers=4):
-    """FLOPs (2 M N K per GEMM) of one forward over `batch` inputs, stage by stage
-    as csrc/fw_fno.cpp runs them: per layer the x-DFT (2M x N times N x C per
-    row), the p-DFT, the per-mode channel mixing, the inverse p- and x-DFTs and
-    the pointwise C x C mixing. Returns (total, FP32-MFMA part): the DFT stages
-    run on FP32 MFMA, the two mixings on BF16 MFMA."""
+    """FLOPs (2 M N K per product) of one forward over `batch` inputs, stage by
+    stage as csrc/fw_fno.cpp runs them, all on BF16 MFMA: per layer the forward
+    DFT's x stage (2M outputs per row of each channel image) and p stage, the
+    per-mode channel mixing, the inverse's p stage (2 M C outputs per row from 4M
+    inputs) and x stage, and the pointwise C x C mixing."""
     C, M, N, B = width, modes, n, batch
-    KP, Q, QO, nm = 2 * M, 2 * M * width, M * width, 2 * M * M
-    s1 = 2 * (2 * M) * C * N * (B * N)
-    s2 = 2 * (2 * KP) * Q * N * B
-    s3 = 2 * B * (2 * C) * (2 * C) * nm
-    s4 = 2 * (2 * N) * QO * (2 * KP) * B
-    s5 = 2 * (2 * N * C * M * (B * N))
-    pw = 2 * (B * N * N) * C * C
-    return layers * (s1 + s2 + s3 + s4 + s5 + pw), layers * (s1 + s2 + s4 + s5)
+    nm = 2 * M * M
+    stage_a = 2 * (2 * M) * N * N * (B * C)
+    stage_b = 2 * (4 * M) * (2 * M) * N * (B * C)
+    mixing = 2 * B * (2 * C) * (2 * C) * nm
+    inv_p = 2 * N * (2 * M * C) * (4 * M) * B
+    inv_x = 2 * N * N * (2 * M) * C * B
+    pointwise = 2 * B * N * N * C * C
+    return layers * (stage_a + stage_b + mixing + inv_p + inv_x + pointwise)
+
+
+def forward_bytes(batch, n=256, width=32, modes=16, layers=4, zero_mass=True):
+    """Algorithmic HBM bytes of one forward (each tensor read or written once per
+    kernel): layer 0's DFT reads W (FP32) and writes v0 (bf16), the others read
+    v (bf16); the DFT writes the spectrum Z (FP32), the mixing reads Z and the
+    bf16 mode weights and writes the inverse's operand BT (bf16); the inverse
+    reads BT and v and writes the next v (bf16), the last layer W and the
+    prediction (FP32) instead; the zero-mass correction reads and writes the
+    prediction once more."""
+    C, M, N, B = width, modes, n, batch
+    nm = 2 * M * M
+    act, img = B * C * N * N * 2, B * N * N * 4
+    z, bt, wm = B * nm * 2 * C * 4, B * (2 * M * C) * (4 * M) * 2, nm * (2 * C) * (2 * C) * 2
+    total = 0
+    for l in range(layers):
+        total += (img + act) if l == 0 else act
+        total += z + z + wm + bt
+        total += bt + act + (act if l < layers - 1 else 2 * img)
+    return total + (2 * img if zero_mass else 0)
 
 
 def hamiltonian_grid(n, amplitude, a2, a3, a4):
@@ -426,12 +446,15 @@ def surrogate_experiment(n_pairs=20000, pairs_per_instance=20, held_out=512, epo
     us = s.infer_time_us(batch, reps=20)
     rate = batch * 256 * 256 * 20 / ((us + base_us) * 1e-6)
     rate_refereed = batch * 256 * 256 * 20 / ((us + base_us + phys_us + audit_us) * 1e-6)
-    flops, flops32 = forward_flops(batch)
-    roof = dict(bound="mfma", kernel="FNO forward (truncated-DFT GEMMs on FP32 MFMA, mixings on BF16 MFMA)",
-                achieved=round(flops / (us * 1e-6) / 1e12, 2), peak=157.3, unit="TFLOP/s",
-                frac=round(flops / (us * 1e-6) / 1e12 / 157.3, 4), traffic=None, flops_per_forward=flops,
-                fp32_mfma_flops_per_forward=flops32,
-                note="peak: FP32 MFMA, which the DFT stages (70% of the FLOPs) use; the mixings' BF16 peak is 2.5 PFLOP/s")
+    flops, nbytes = forward_flops(batch), forward_bytes(batch, zero_mass=s.zero_mass)
+    gbs = nbytes / (us * 1e-6) / 1e9
+    roof = dict(bound="hbm", kernel="FNO forward (13 fused kernels: DFT, mixing, inverse + pointwise per layer; BF16 MFMA)",
+                achieved=round(gbs, 1), peak=8000.0, unit="GB/s", frac=round(gbs / 8000.0, 4), traffic=None,
+                bytes_per_forward=nbytes, flops_per_forward=flops,
+                mfma_bf16_tflops=round(flops / (us * 1e-6) / 1e12, 2),
+                mfma_bf16_frac=round(flops / (us * 1e-6) / 1e12 / 2500.0, 4),
+                note="all products on BF16 MFMA, whose dense peak (2.5 PFLOP/s) is far above what this "
+                     "work needs; the bound is the activations' HBM traffic")
     out = dict(pairs=s.npairs, train_instances=len(kept), adam_steps=steps, batch=batch, train_s=round(t_train, 1),
                data_s=round(t_data, 1), held_out=len(phys), held_out_unphysical=held_out - len(phys),
                rel_l2_median=float(np.median(err)), rel_l2_mean=float(np.mean(err)), rel_l2_max=float(np.max(err)),
@@ -472,7 +495,14 @@ if __name__ == "__main__":
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--base-steps", type=int, default=2)
     ap.add_argument("--out-scale", type=float, default=1e-2)
+    ap.add_argument("--time-inference", type=int, default=0, metavar="REPS",
+                    help="only time REPS forward passes of the 256^2 network over a batch of 8 (random weights)")
     a = ap.parse_args()
+    if a.time_inference:
+        s = FNOSurrogate(grid_dim=256, width=32, modes=16, layers=4, max_batch=8, seed=0, out_scale=a.out_scale,
+                         zero_mass=True)
+        print(json.dumps({"us_per_batch": s.infer_time_us(8, reps=a.time_inference), "batch": 8, "grid": 256}))
+        sys.exit(0)
     print(json.dumps(surrogate_experiment(a.pairs, held_out=a.held_out, epochs=a.epochs, lr=a.lr,
                                           base_steps=a.base_steps, out_scale=a.out_scale,
                                           log=lambda m: print(m, file=sys.stderr, flush=True))))

@@ -20,7 +20,9 @@ float32) exactly what the HIP kernels round: the stored activations (v, the
 lift output, dpre), the transform matrices, the intermediates T and H of both
 transforms, the spectra Z / Y entering the mode mixing and the inverse, the
 mixing and pointwise weights, forward and backward. Its gradient treats each
-rounding as the identity. With ``bf16=False`` (no rounding) the network is the
+rounding as the identity. GELU uses scipy's erf; the kernels evaluate erf by
+Abramowitz & Stegun 7.1.26, within 4.3e-7 of this GELU in FP32 (far below the
+bf16 resolution of the stored activations). With ``bf16=False`` (no rounding) the network is the
 exact truncated-FFT network and the gradient the exact one: pinned in
 tests/test_fno_oracle.py by central finite differences, by numpy's rfft2 /
 irfft2, and by a direct DFT sum for identity mode weights.
