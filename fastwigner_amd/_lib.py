@@ -276,7 +276,6 @@ PROTOTYPES = {
     "fw_fno_train_step": [_P, _PI, _I, ctypes.c_float, _PD],
     "fw_fno_eval_pairs": [_P, _PI, _I, _PD],
     "fw_fno_infer_time": [_P, _I, _I, _PD],
-    "fw_fno_gemm_test": [_I, _PF, _PF, _PF, _I, _I, _I, _I, _I],
 }
 
 _lib = None

@@ -31,7 +31,6 @@
 #include <cstring>
 #include <memory>
 #include <string>
-#include <type_traits>
 #include <vector>
 
 #include "fastwigner.h"
@@ -52,120 +51,6 @@ namespace fno {
 // --------------------------------------------------------------------------
 __device__ __forceinline__ bf16_t to_bf16(float f) { return __builtin_bit_cast(bf16_t, (__bf16)f); }
 __device__ __forceinline__ float from_bf16(bf16_t h) { return __uint_as_float((unsigned int)h << 16); }
-
-// --------------------------------------------------------------------------
-// Strided, batched GEMM  C[z] = alpha A[z] B[z] + beta C[z]  (the pointwise
-// weights' gradient, a reduction over points; and the GEMM kernel test)
-//   A(m, k) at A + z1 sA1 + z2 sA2 + m sAm + k sAk, likewise B(k, n), C(m, n);
-//   z = z1 * nb2 + z2 = blockIdx.z.
-// 64 x 64 tile per 256-thread block, each wave a 32 x 32 quadrant of 2 x 2
-// MFMA 16 x 16 blocks; K in steps of 32 staged through LDS. Out-of-range
-// elements load as zero; stores are masked.
-// --------------------------------------------------------------------------
-struct GemmArgs {
-  const void* A;
-  const void* B;
-  float* C;
-  int M, N, K, nb2;
-  long long sAm, sAk, sBk, sBn, sCm, sCn;
-  long long sA1, sA2, sB1, sB2, sC1, sC2;
-  float alpha, beta;
-};
-
-constexpr int GT = 64, GK = 32, GPAD = 4;
-
-template <typename T> __device__ __forceinline__ float ld(const T* p, long long i);
-template <> __device__ __forceinline__ float ld<float>(const float* p, long long i) { return p[i]; }
-template <> __device__ __forceinline__ float ld<bf16_t>(const bf16_t* p, long long i) { return from_bf16(p[i]); }
-
-// BF16: operands rounded to bf16 in LDS; F32: full FP32 operands.
-template <typename TA, typename TB, bool BF>
-__global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
-  typedef typename std::conditional<BF, bf16_t, float>::type ST;
-  __shared__ ST As[GT][GK + GPAD];
-  __shared__ ST Bs[GT][GK + GPAD];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
-  const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
-  const int z = blockIdx.z, z1 = z / g.nb2, z2 = z % g.nb2;
-  const TA* A = (const TA*)g.A + z1 * g.sA1 + z2 * g.sA2;
-  const TB* B = (const TB*)g.B + z1 * g.sB1 + z2 * g.sB2;
-  float* C = g.C + z1 * g.sC1 + z2 * g.sC2;
-  const bool a_kfast = g.sAk == 1, b_nfast = g.sBn == 1;
-
-  float4_t acc[2][2];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = (float4_t){0.f, 0.f, 0.f, 0.f};
-
-  for (int k0 = 0; k0 < g.K; k0 += GK) {
-    // stage A (64 x 32) and B (32 x 64, stored n-major): 8 elements per thread each
-#pragma unroll
-    for (int r = 0; r < (GT * GK) / 256; ++r) {
-      const int e = tid + 256 * r;
-      int mm, kk;
-      if (a_kfast) { mm = e / GK; kk = e % GK; } else { mm = e % GT; kk = e / GT; }
-      const int gm = m0 + mm, gk = k0 + kk;
-      float v = 0.f;
-      if (gm < g.M && gk < g.K) v = ld<TA>(A, gm * g.sAm + gk * g.sAk);
-      if constexpr (BF) As[mm][kk] = to_bf16(v); else As[mm][kk] = v;
-      int nn;
-      if (b_nfast) { nn = e % GT; kk = e / GT; } else { nn = e / GK; kk = e % GK; }
-      const int gn = n0 + nn, gk2 = k0 + kk;
-      float w = 0.f;
-      if (gn < g.N && gk2 < g.K) w = ld<TB>(B, gk2 * g.sBk + gn * g.sBn);
-      if constexpr (BF) Bs[nn][kk] = to_bf16(w); else Bs[nn][kk] = w;
-    }
-    __syncthreads();
-    if constexpr (BF) {
-#pragma unroll
-      for (int kk = 0; kk < GK; kk += 16) {
-        short4_t af[2], bfr[2];
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-          const ST* pa = &As[wm * 32 + i * 16 + (lane & 15)][kk + 4 * (lane >> 4)];
-          af[i] = (short4_t){(short)pa[0], (short)pa[1], (short)pa[2], (short)pa[3]};
-          const ST* pb = &Bs[wn * 32 + i * 16 + (lane & 15)][kk + 4 * (lane >> 4)];
-          bfr[i] = (short4_t){(short)pb[0], (short)pb[1], (short)pb[2], (short)pb[3]};
-        }
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-          for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[i], bfr[j], acc[i][j], 0, 0, 0);
-      }
-    } else {
-#pragma unroll
-      for (int kk = 0; kk < GK; kk += 4) {
-        float af[2], bfr[2];
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-          af[i] = As[wm * 32 + i * 16 + (lane & 15)][kk + (lane >> 4)];
-          bfr[i] = Bs[wn * 32 + i * 16 + (lane & 15)][kk + (lane >> 4)];
-        }
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-          for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i], bfr[j], acc[i][j], 0, 0, 0);
-      }
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int gm = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + r;
-        const int gn = n0 + wn * 32 + j * 16 + (lane & 15);
-        if (gm < g.M && gn < g.N) {
-          float* pc = C + gm * g.sCm + gn * g.sCn;
-          const float v = g.alpha * acc[i][j][r];
-          *pc = (g.beta != 0.f) ? v + g.beta * *pc : v;
-        }
-      }
-}
 
 // --------------------------------------------------------------------------
 // Fused layer kernels (BF16 MFMA, FP32 accumulation). Activations are
@@ -751,21 +636,63 @@ __global__ void __launch_bounds__(256) proj_bwd(const bf16_t* vL, const float* d
   if (x <= C) part[(size_t)row * (C + 1) + x] = red[0][x] + red[1][x] + red[2][x] + red[3][x];
 }
 
+// pointwise weights' gradient over one chunk of 1024 points of sample b (block
+// (chunk, b), 4 waves of 256 points): part[b][chunk][c][o] = sum_pt v[c][pt] dpre[o][pt]
+// on 16x16x32 (A rows c, B columns o, K = points, both k-contiguous in the
+// channel-first layout); the waves' tiles are added in LDS
+__global__ void __launch_bounds__(256) wp_grad(const bf16_t* v, const bf16_t* d, float* part, long long nn) {
+  __shared__ float red[4][C * C];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, lr = lane & 15, lg = lane >> 4;
+  const int b = blockIdx.y, ch = blockIdx.x;
+  const size_t base = (size_t)b * C * nn + (size_t)ch * 1024 + (size_t)w * 256;
+  float4_t acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = (float4_t){0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+  for (int ks = 0; ks < 8; ++ks) {
+    const size_t k0 = base + 32 * ks + 8 * lg;
+    u16x8_t av[2], bv[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      av[i] = ld8(v + (size_t)(16 * i + lr) * nn + k0);
+      bv[i] = ld8(d + (size_t)(16 * i + lr) * nn + k0);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = mma32(av[i], bv[j], acc[i][j]);
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) red[w][(16 * i + 4 * lg + r) * C + 16 * j + lr] = acc[i][j][r];
+  __syncthreads();
+  for (int e = tid; e < C * C; e += 256)
+    part[((size_t)b * gridDim.x + ch) * C * C + e] = red[0][e] + red[1][e] + red[2][e] + red[3][e];
+}
+
 // lift backward: per image (b, c) the sums of dv0, W dv0, x dv0, p dv0; then
 // gPw[j][c] and gpb[c] over the batch (features constant per sample)
-__global__ void __launch_bounds__(256) lift_bwd_img(const float* dv0, const float* W, float* part, int N, float xs,
-                                                    float ps) {
-  __shared__ float red[4][4];
+__global__ void __launch_bounds__(1024) lift_bwd_img(const float* dv0, const float* W, float* part, int N, float xs,
+                                                     float ps) {
+  __shared__ float red[16][4];
   const int img = blockIdx.x, b = img / C, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const size_t nn = (size_t)N * N;
+  const float4_t* d4 = reinterpret_cast<const float4_t*>(dv0 + (size_t)img * nn);
+  const float4_t* w4 = reinterpret_cast<const float4_t*>(W + (size_t)b * nn);
   float s[4] = {0.f, 0.f, 0.f, 0.f};
-  for (int e = threadIdx.x; e < N * N; e += 256) {
-    const int p = e / N, x = e % N;
-    const float d = dv0[(size_t)img * nn + e];
-    s[0] += d;
-    s[1] += W[(size_t)b * nn + e] * d;
-    s[2] += xs * ((float)x - 0.5f * N) * d;
-    s[3] += ps * ((float)p - 0.5f * N) * d;
+  for (int q = threadIdx.x; q < N * N / 4; q += 1024) {
+    const int p = (4 * q) / N, x0 = (4 * q) % N;
+    const float4_t d = d4[q], wv = w4[q];
+    const float sd = (d[0] + d[1]) + (d[2] + d[3]);
+    s[0] += sd;
+    s[1] += (wv[0] * d[0] + wv[1] * d[1]) + (wv[2] * d[2] + wv[3] * d[3]);
+    s[2] += xs * (((float)x0 - 0.5f * N) * sd + (d[1] + 2.f * d[2] + 3.f * d[3]));
+    s[3] += ps * ((float)p - 0.5f * N) * sd;
   }
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
@@ -775,7 +702,9 @@ __global__ void __launch_bounds__(256) lift_bwd_img(const float* dv0, const floa
   __syncthreads();
   if (threadIdx.x < 4) {
     const int k = threadIdx.x;
-    part[(size_t)img * 4 + k] = red[0][k] + red[1][k] + red[2][k] + red[3][k];
+    float t = 0.f;
+    for (int q = 0; q < 16; ++q) t += red[q][k];
+    part[(size_t)img * 4 + k] = t;
   }
 }
 __global__ void __launch_bounds__(64) lift_bwd_fin(const float* part, const float* feat, float* gPw, float* gpb, int B) {
@@ -824,14 +753,21 @@ __global__ void __launch_bounds__(256) pointwise_weights(const float* Wp, bf16_t
   }
 }
 
-// out[r] = sum over z < nz of part[z][r]  (fixed order)
+// out[r] (+)= scale sum over z < nz of part[z][r]: one block per r, thread t adding
+// z = t, t + 256, ... in order, then a fixed tree
 __global__ void __launch_bounds__(256) reduce_rows(const float* part, float* out, int nz, int nr, float scale,
                                                    int accumulate) {
-  const int r = blockIdx.x * 256 + threadIdx.x;
-  if (r >= nr) return;
+  __shared__ float red[256];
+  const int r = blockIdx.x;
   float s = 0.f;
-  for (int z = 0; z < nz; ++z) s += part[(long long)z * nr + r];
-  out[r] = accumulate ? out[r] + scale * s : scale * s;
+  for (int z = threadIdx.x; z < nz; z += 256) s += part[(size_t)z * nr + r];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[r] = accumulate ? out[r] + scale * red[0] : scale * red[0];
 }
 
 // Moments of predicted Wigner functions for the physical referee, in FP64:
@@ -894,32 +830,44 @@ __global__ void __launch_bounds__(1024) center_kernel(float* x, const float* y, 
   for (long long i = threadIdx.x; i < nn; i += 1024) xb[i] -= m;
 }
 
-// loss L = (1/B) sum_b sum_n (out - t)^2 / sum_n t^2 ; dout = dL/dout. One block per sample.
-__global__ void __launch_bounds__(256) loss_kernel(const float* out, const float* tgt, float* dout, double* per_sample,
-                                                   long long nn, float inv_b) {
+// loss L = (1/B) sum_b sum_n (out - t)^2 / sum_n t^2 and dout = dL/dout, in two
+// passes over blocks (chunk, b), nn divisible by the chunk count: the FP64 sums of
+// (out - t)^2 and t^2 per chunk, then per sample their totals in a fixed order
+__global__ void __launch_bounds__(256) loss_partial(const float* out, const float* tgt, double* lpart, long long nn) {
   __shared__ double r1[256], r2[256];
-  const int b = blockIdx.x;
-  const float* o = out + (long long)b * nn;
-  const float* t = tgt + (long long)b * nn;
-  double se = 0, st = 0;
-  for (long long i = threadIdx.x; i < nn; i += 256) {
-    const double d = (double)o[i] - (double)t[i];
+  const int b = blockIdx.y, nc = gridDim.x;
+  const long long per = nn / nc, i0 = (long long)b * nn + (long long)blockIdx.x * per;
+  double se = 0.0, st = 0.0;
+  for (long long i = threadIdx.x; i < per; i += 256) {
+    const double d = (double)out[i0 + i] - (double)tgt[i0 + i];
     se += d * d;
-    st += (double)t[i] * t[i];
+    st += (double)tgt[i0 + i] * tgt[i0 + i];
   }
   r1[threadIdx.x] = se;
   r2[threadIdx.x] = st;
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) { r1[threadIdx.x] += r1[threadIdx.x + s]; r2[threadIdx.x] += r2[threadIdx.x + s]; }
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) { r1[threadIdx.x] += r1[threadIdx.x + k]; r2[threadIdx.x] += r2[threadIdx.x + k]; }
     __syncthreads();
   }
-  const double SE = r1[0], ST = r2[0];
-  if (threadIdx.x == 0) per_sample[b] = SE / ST;
+  if (threadIdx.x == 0) {
+    lpart[((size_t)b * nc + blockIdx.x) * 2] = r1[0];
+    lpart[((size_t)b * nc + blockIdx.x) * 2 + 1] = r2[0];
+  }
+}
+__global__ void __launch_bounds__(256) loss_finish(const float* out, const float* tgt, float* dout, const double* lpart,
+                                                   double* per_sample, long long nn, float inv_b) {
+  const int b = blockIdx.y, nc = gridDim.x;
+  double se = 0.0, st = 0.0;
+  for (int k = 0; k < nc; ++k) {
+    se += lpart[((size_t)b * nc + k) * 2];
+    st += lpart[((size_t)b * nc + k) * 2 + 1];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) per_sample[b] = se / st;
   if (dout) {
-    const float sc = (float)(2.0 / ST) * inv_b;
-    float* d = dout + (long long)b * nn;
-    for (long long i = threadIdx.x; i < nn; i += 256) d[i] = sc * (o[i] - t[i]);
+    const float sc = (float)(2.0 / st) * inv_b;
+    const long long per = nn / nc, i0 = (long long)b * nn + (long long)blockIdx.x * per;
+    for (long long i = threadIdx.x; i < per; i += 256) dout[i0 + i] = sc * (out[i0 + i] - tgt[i0 + i]);
   }
 }
 
@@ -950,7 +898,6 @@ __global__ void __launch_bounds__(256) gather_pairs(const float* din, const floa
 }  // namespace fno
 
 
-using fno::GemmArgs;
 
 struct fw_fno {
   fw_fno_desc d{};
@@ -987,6 +934,7 @@ struct fw_fno {
   float *bin = nullptr, *btg = nullptr, *bft = nullptr;
   int* bidx = nullptr;
   double* persample = nullptr;
+  double* lpart = nullptr;  // partial sums of the loss [B][64][2]
   double* mom = nullptr;   // moments of the last forward's outputs (max_batch x 3N)
   // dataset
   float *din = nullptr, *dtg = nullptr, *dft = nullptr;
@@ -994,27 +942,6 @@ struct fw_fno {
 };
 
 namespace {
-
-enum { G_F32 = 0, G_BF = 1, G_BF_B16 = 2, G_BF_AB16 = 3 };
-
-int gemm(fw_fno* h, int kind, const void* A, const void* Bm, float* Cm, int M, int N, int K, int nb1, int nb2,
-         long long sAm, long long sAk, long long sBk, long long sBn, long long sCm, long long sCn, long long sA1,
-         long long sA2, long long sB1, long long sB2, long long sC1, long long sC2, float alpha, float beta) {
-  GemmArgs g;
-  g.A = A; g.B = Bm; g.C = Cm; g.M = M; g.N = N; g.K = K; g.nb2 = nb2;
-  g.sAm = sAm; g.sAk = sAk; g.sBk = sBk; g.sBn = sBn; g.sCm = sCm; g.sCn = sCn;
-  g.sA1 = sA1; g.sA2 = sA2; g.sB1 = sB1; g.sB2 = sB2; g.sC1 = sC1; g.sC2 = sC2;
-  g.alpha = alpha; g.beta = beta;
-  dim3 grid((N + fno::GT - 1) / fno::GT, (M + fno::GT - 1) / fno::GT, nb1 * nb2);
-  switch (kind) {
-    case G_F32: hipLaunchKernelGGL((fno::gemm_kernel<float, float, false>), grid, dim3(256), 0, h->stream, g); break;
-    case G_BF: hipLaunchKernelGGL((fno::gemm_kernel<float, float, true>), grid, dim3(256), 0, h->stream, g); break;
-    case G_BF_B16: hipLaunchKernelGGL((fno::gemm_kernel<float, bf16_t, true>), grid, dim3(256), 0, h->stream, g); break;
-    default: hipLaunchKernelGGL((fno::gemm_kernel<bf16_t, bf16_t, true>), grid, dim3(256), 0, h->stream, g); break;
-  }
-  HIPCHK(hipGetLastError());
-  return FW_OK;
-}
 
 int grid_for(long long n) { return (int)std::max(1LL, std::min(4096LL, (n + 255) / 256)); }
 
@@ -1150,10 +1077,10 @@ int backward(fw_fno* h, int B) {
   // projection: dv_L, gradients of Q and qb (adjacent)
   hipLaunchKernelGGL(fno::proj_bwd, dim3(B * N), dim3(256), 0, h->stream, h->v[L], h->dout, h->P + h->o_Q, h->dv,
                      h->part, N, h->out_scale);
-  hipLaunchKernelGGL(fno::reduce_rows, dim3(1), dim3(256), 0, h->stream, h->part, h->G + h->o_Q, B * N, C + 1, 1.f, 0);
+  hipLaunchKernelGGL(fno::reduce_rows, dim3(C + 1), dim3(256), 0, h->stream, h->part, h->G + h->o_Q, B * N, C + 1, 1.f, 0);
   float* dcur = h->dv;    // gradient w.r.t. the output of layer l
   float* dnext = h->dv2;  // gradient w.r.t. its input
-  const long long chunk = 1024, nch = NN / chunk;
+  const long long nch = NN / 1024;   // chunks of the pointwise weights' gradient
   for (int l = L - 1; l >= 0; --l) {
     // dpre (bf16) and the bias gradient; dY = (c_ky / N^2) Z(dpre)
     fno::DftArgs da = dft_common(h);
@@ -1164,14 +1091,13 @@ int backward(fw_fno* h, int B) {
     da.zscale = h->zs_b;
     da.Z = h->dY;
     FWCHK(dft(h, l < L - 1 ? fno::IN_GRAD : fno::IN_GRAD_LIN, da, B));
-    hipLaunchKernelGGL(fno::reduce_rows, dim3(1), dim3(256), 0, h->stream, h->part, h->G + h->o_b[l], B, C, 1.f, 0);
+    hipLaunchKernelGGL(fno::reduce_rows, dim3(C), dim3(256), 0, h->stream, h->part, h->G + h->o_b[l], B, C, 1.f, 0);
     // mode weights' gradient and dZ = dY Wm^T into the inverse's B operand
     const fno::MixArgs ma{h->dY, h->Wm + (size_t)l * h->nm * 4 * C * C, h->BT, B};
     FWCHK(mix(h, ma, true, h->Zs[l], h->G + h->o_R[l]));
-    // pointwise weights' gradient: sum over points of v[c] dpre[o], in chunks of 1024 points
-    FWCHK(gemm(h, G_BF_AB16, h->v[l], h->dpre, h->part, C, C, (int)chunk, B, (int)nch, NN, 1, 1, NN, C, 1,
-               (long long)C * NN, chunk, (long long)C * NN, chunk, nch * C * C, (long long)C * C, 1.f, 0.f));
-    hipLaunchKernelGGL(fno::reduce_rows, dim3((C * C + 255) / 256), dim3(256), 0, h->stream, h->part,
+    // pointwise weights' gradient: sum over points of v[c] dpre[o], per chunk of 1024 points
+    hipLaunchKernelGGL(fno::wp_grad, dim3((int)nch, B), dim3(256), 0, h->stream, h->v[l], h->dpre, h->part, NN);
+    hipLaunchKernelGGL(fno::reduce_rows, dim3(C * C), dim3(256), 0, h->stream, h->part,
                        h->G + h->o_Wp[l], (int)(B * nch), C * C, 1.f, 0);
     // dv = Re sum_modes dZ e^{+i theta} + Wp dpre
     fno::InvArgs ia{};
@@ -1184,7 +1110,7 @@ int backward(fw_fno* h, int B) {
     FWCHK(inv(h, fno::OUT_GRAD, ia, B));
     std::swap(dcur, dnext);
   }
-  hipLaunchKernelGGL(fno::lift_bwd_img, dim3(B * C), dim3(256), 0, h->stream, dcur, h->bin, h->part, N, 2.f / N,
+  hipLaunchKernelGGL(fno::lift_bwd_img, dim3(B * C), dim3(1024), 0, h->stream, dcur, h->bin, h->part, N, 2.f / N,
                      2.f / N);
   hipLaunchKernelGGL(fno::lift_bwd_fin, dim3(1), dim3(64), 0, h->stream, h->part, h->bft, h->G + h->o_Pw,
                      h->G + h->o_pb, B);
@@ -1194,8 +1120,9 @@ int backward(fw_fno* h, int B) {
 
 int loss(fw_fno* h, int B, bool want_grad, double* out) {
   const long long nn = (long long)h->N * h->N;
-  hipLaunchKernelGGL(fno::loss_kernel, dim3(B), dim3(256), 0, h->stream, h->out, h->btg, want_grad ? h->dout : nullptr,
-                     h->persample, nn, 1.f / B);
+  hipLaunchKernelGGL(fno::loss_partial, dim3(64, B), dim3(256), 0, h->stream, h->out, h->btg, h->lpart, nn);
+  hipLaunchKernelGGL(fno::loss_finish, dim3(64, B), dim3(256), 0, h->stream, h->out, h->btg,
+                     want_grad ? h->dout : nullptr, h->lpart, h->persample, nn, 1.f / B);
   // zero-mass correction: dL/dc = s (dout - mean dout) per sample; proj_bwd applies s
   if (want_grad && h->zero_mass)
     hipLaunchKernelGGL(fno::center_kernel, dim3(B), dim3(1024), 0, h->stream, h->dout, nullptr, nn);
@@ -1368,6 +1295,7 @@ extern "C" int fw_fno_create(const fw_fno_desc* d, fw_fno** out) {
   FWCHK(dalloc(&h->bft, (size_t)B * 5));
   FWCHK(dalloc(&h->bidx, (size_t)B));
   FWCHK(dalloc(&h->persample, (size_t)B));
+  FWCHK(dalloc(&h->lpart, (size_t)B * 64 * 2));
   FWCHK(dalloc(&h->mom, (size_t)B * 3 * N));
   FWCHK(refresh_weights(h));
   HIPCHK(hipStreamSynchronize(h->stream));
@@ -1380,7 +1308,7 @@ extern "C" int fw_fno_destroy(fw_fno* h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   std::vector<void*> bufs = {h->P, h->G, h->Ma, h->Va, h->Wm, h->WmT, h->Wpb, h->WpT, h->Fx, h->Fp, h->GpT, h->Gxf,
                              h->Gxu, h->zs_f, h->zs_b, h->BT, h->dY, h->dpre, h->dv, h->dv2, h->part, h->cpart,
-                             h->out, h->dout, h->bin, h->btg, h->bft, h->bidx, h->persample, h->mom, h->din, h->dtg,
+                             h->out, h->dout, h->bin, h->btg, h->bft, h->bidx, h->persample, h->lpart, h->mom, h->din, h->dtg,
                              h->dft};
   for (auto p : h->v) bufs.push_back(p);
   for (auto p : h->pre) bufs.push_back(p);
@@ -1567,34 +1495,4 @@ extern "C" int fw_fno_infer_time(fw_fno* h, int batch, int reps, double* us_per_
   (void)hipEventDestroy(e1);
   *us_per_batch = ms * 1000.0 / reps;
   return FW_OK;
-}
-
-// Strided batched GEMM on MFMA, exported for the parity tests of the kernel itself.
-extern "C" int fw_fno_gemm_test(int bf16, const float* hA, const float* hB, float* hC, int M, int N, int K,
-                                int transA, int transB) {
-  fw::clear_error();
-  if (!hA || !hB || !hC || M < 1 || N < 1 || K < 1) return fw::fail(FW_ERR_INVALID_ARG, "bad arguments");
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fw::fail(FW_ERR_NO_DEVICE, "no GPU visible");
-  fw_fno tmp;
-  HIPCHK(hipStreamCreateWithFlags(&tmp.stream, hipStreamNonBlocking));
-  float *dA, *dB, *dC;
-  HIPCHK(hipMalloc(&dA, sizeof(float) * M * K));
-  HIPCHK(hipMalloc(&dB, sizeof(float) * K * N));
-  HIPCHK(hipMalloc(&dC, sizeof(float) * M * N));
-  HIPCHK(hipMemcpy(dA, hA, sizeof(float) * M * K, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(dB, hB, sizeof(float) * K * N, hipMemcpyHostToDevice));
-  // A is stored M x K row-major (transA = 0) or K x M (transA = 1); B is K x N or N x K
-  const long long sAm = transA ? 1 : K, sAk = transA ? M : 1;
-  const long long sBk = transB ? 1 : N, sBn = transB ? K : 1;
-  int rc = gemm(&tmp, bf16 != 0 ? G_BF : G_F32, dA, dB, dC, M, N, K, 1, 1, sAm, sAk, sBk, sBn, N, 1, 0, 0, 0, 0, 0, 0, 1.f, 0.f);
-  if (rc == FW_OK) {
-    HIPCHK(hipStreamSynchronize(tmp.stream));
-    HIPCHK(hipMemcpy(hC, dC, sizeof(float) * M * N, hipMemcpyDeviceToHost));
-  }
-  (void)hipFree(dA);
-  (void)hipFree(dB);
-  (void)hipFree(dC);
-  (void)hipStreamDestroy(tmp.stream);
-  return rc;
 }

@@ -407,10 +407,6 @@ int fw_fno_train_step(fw_fno* h, const int* host_idx, int batch, float lr, doubl
 int fw_fno_eval_pairs(fw_fno* h, const int* host_idx, int n, double* rel_l2_out);
 /* device time of one forward pass over `batch` samples (us), averaged over reps */
 int fw_fno_infer_time(fw_fno* h, int batch, int reps, double* us_per_batch);
-/* C = A B through the strided MFMA GEMM kernel that reduces the pointwise weights' gradient over
- * points (bf16 != 0: BF16 operands, else FP32), host arrays; A is M x K (K x M if transA), B is
- * K x N (N x K if transB), C is M x N row-major. For testing the kernel. */
-int fw_fno_gemm_test(int bf16, const float* A, const float* B, float* C, int M, int N, int K, int transA, int transB);
 
 #ifdef __cplusplus
 }

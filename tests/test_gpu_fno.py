@@ -13,25 +13,6 @@ def rel(a, b):
     return np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b))
 
 
-@pytest.mark.parametrize("bf16", [0, 1])
-@pytest.mark.parametrize("M,N,K,ta,tb", [(70, 45, 100, 0, 0), (64, 64, 64, 1, 0), (33, 130, 17, 0, 1),
-                                         (128, 8, 300, 1, 1)])
-def test_mfma_gemm(gpu, bf16, M, N, K, ta, tb):
-    from fastwigner_amd import _lib
-    rng = np.random.default_rng(M * N + K)
-    A = rng.standard_normal((K, M) if ta else (M, K)).astype(np.float32)
-    B = rng.standard_normal((N, K) if tb else (K, N)).astype(np.float32)
-    C = np.empty((M, N), dtype=np.float32)
-    _lib.call("fw_fno_gemm_test", bf16, _lib.fptr(A), _lib.fptr(B), _lib.fptr(C), M, N, K, ta, tb)
-    Am = A.T if ta else A
-    Bm = B.T if tb else B
-    if bf16:
-        ref = fo.bf16(Am) @ fo.bf16(Bm)
-    else:
-        ref = Am.astype(np.float64) @ Bm.astype(np.float64)
-    assert rel(C, ref) <= 2e-6
-
-
 def random_problem(o, B, seed):
     rng = np.random.default_rng(seed)
     P = fo.init_params(o, seed=seed, spectral_scale=3.0 / (o.C * o.C))
