@@ -326,39 +326,47 @@ struct MixArgs {
   const float* Zin;   // [B][2 M M][2][C]
   const bf16_t* Wx;   // [2 M M][2C][2C]
   bf16_t* BT;         // [B][2 M C][4M]
-  int B;              // <= 16
+  int B;              // <= 64 (tiles of 16 rows)
 };
 
 template <int MM>
 __global__ void __launch_bounds__(512) mix_modes(MixArgs a) {
   constexpr int NM = 2 * MM * MM, NP = 2 * MM * C, K4 = 4 * MM;
-  __shared__ bf16_t ys[16][2 * C][8];
+  __shared__ bf16_t ys[64][2 * C][8];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, lr = lane & 15, lg = lane >> 4;
   const int ky = blockIdx.x % MM, ig = blockIdx.x / MM;
   const int m = (8 * ig + w) * MM + ky;
-  u16x8_t za[2];
+  u16x8_t wf[4][2];
 #pragma unroll
-  for (int ks = 0; ks < 2; ++ks) {
-    za[ks] = (u16x8_t){0, 0, 0, 0, 0, 0, 0, 0};
-    if (lr < a.B) {
-      const float4_t* zp = reinterpret_cast<const float4_t*>(a.Zin + ((size_t)lr * NM + m) * 2 * C + 32 * ks + 8 * lg);
-      const float4_t z0 = zp[0], z1 = zp[1];
+  for (int t = 0; t < 4; ++t)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        za[ks][j] = to_bf16(z0[j]);
-        za[ks][4 + j] = to_bf16(z1[j]);
+    for (int ks = 0; ks < 2; ++ks) wf[t][ks] = ld8(a.Wx + ((size_t)m * 2 * C + 16 * t + lr) * 2 * C + 32 * ks + 8 * lg);
+  for (int b0 = 0; b0 < a.B; b0 += 16) {   // tiles of 16 samples
+    u16x8_t za[2];
+    const int bl = b0 + lr;
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      za[ks] = (u16x8_t){0, 0, 0, 0, 0, 0, 0, 0};
+      if (bl < a.B) {
+        const float4_t* zp =
+            reinterpret_cast<const float4_t*>(a.Zin + ((size_t)bl * NM + m) * 2 * C + 32 * ks + 8 * lg);
+        const float4_t z0 = zp[0], z1 = zp[1];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          za[ks][j] = to_bf16(z0[j]);
+          za[ks][4 + j] = to_bf16(z1[j]);
+        }
       }
     }
-  }
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    float4_t acc = (float4_t){0.f, 0.f, 0.f, 0.f};
+    for (int t = 0; t < 4; ++t) {
+      float4_t acc = (float4_t){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int ks = 0; ks < 2; ++ks)
-      acc = mma32(za[ks], ld8(a.Wx + ((size_t)m * 2 * C + 16 * t + lr) * 2 * C + 32 * ks + 8 * lg), acc);
+      for (int ks = 0; ks < 2; ++ks) acc = mma32(za[ks], wf[t][ks], acc);
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
-      if (4 * lg + j < a.B) ys[4 * lg + j][16 * t + lr][w] = to_bf16(acc[j]);
+      for (int j = 0; j < 4; ++j)
+        if (b0 + 4 * lg + j < a.B) ys[b0 + 4 * lg + j][16 * t + lr][w] = to_bf16(acc[j]);
+    }
   }
   __syncthreads();
   // 16-byte runs: (b, ri_out, o, half h): h = 0 takes k = 8 ig.., h = 1 takes 2M + 8 ig..;
@@ -377,7 +385,7 @@ __global__ void __launch_bounds__(512) mix_modes(MixArgs a) {
 template <int MM>
 __global__ void __launch_bounds__(256) mode_grad(const float* Z, const float* dY, float* gR, int B) {
   constexpr int NM = 2 * MM * MM;
-  __shared__ float zs[16][2 * C], ys[16][2 * C];
+  __shared__ float zs[64][2 * C], ys[64][2 * C];
   const int m = blockIdx.x, tid = threadIdx.x;
   for (int e = tid; e < B * 2 * C; e += 256) {
     const int bb = e / (2 * C), k = e % (2 * C);
@@ -1219,7 +1227,7 @@ extern "C" int fw_fno_create(const fw_fno_desc* d, fw_fno** out) {
   if (d->modes != 8 && d->modes != 16) return fw::fail(FW_ERR_INVALID_ARG, "modes must be 8 or 16");
   if (4 * d->modes > d->grid_dim) return fw::fail(FW_ERR_INVALID_ARG, "modes must be at most grid_dim / 4");
   if (d->layers < 1 || d->layers > 16) return fw::fail(FW_ERR_INVALID_ARG, "layers must be in [1, 16]");
-  if (d->max_batch < 1 || d->max_batch > 16) return fw::fail(FW_ERR_INVALID_ARG, "max_batch must be in [1, 16]");
+  if (d->max_batch < 1 || d->max_batch > 64) return fw::fail(FW_ERR_INVALID_ARG, "max_batch must be in [1, 64]");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fw::fail(FW_ERR_NO_DEVICE, "no GPU visible");
   if (d->device < 0 || d->device >= ndev) return fw::fail(FW_ERR_INVALID_ARG, "device ordinal out of range");

@@ -361,7 +361,7 @@ def state_energy(W, feat, amplitude):
     return ((mx * V).sum(axis=1) + mp @ (0.5 * x ** 2)) / mx.sum(axis=1)
 
 
-def surrogate_experiment(n_pairs=20000, pairs_per_instance=20, held_out=512, epochs=2.0, batch=8, lr=2e-3,
+def surrogate_experiment(n_pairs=20000, pairs_per_instance=20, held_out=512, epochs=2.0, batch=8, infer_batch=32, lr=2e-3,
                          seed=4, held_seed=5, base_steps=2, out_scale=1e-2, zero_mass=True, log=None):
     """The learned-surrogate workload: a 4-layer, 32-channel, 16 x 16-mode FNO on
     256^2 Wigner functions jumping k = 20 steps of dt = 0.01, trained on about
@@ -373,11 +373,13 @@ def surrogate_experiment(n_pairs=20000, pairs_per_instance=20, held_out=512, epo
     coarse prediction of b propagator steps of k dt / b (FP32), whose own error
     is reported as the method without the network, and its output is scaled by
     out_scale (the correction is small); with zero_mass the correction carries
-    no mass, so the prediction keeps the base method's norm. Returns a dict of
-    measurements."""
+    no mass, so the prediction keeps the base method's norm. Training takes
+    minibatches of `batch`; the rates are for inference over `infer_batch`
+    samples per forward (per-sample results do not depend on the batch). Returns a
+    dict of measurements."""
     import time
     say = log or (lambda *a: None)
-    s = FNOSurrogate(grid_dim=256, width=32, modes=16, layers=4, max_batch=batch, k_steps=20, dt=0.01, seed=0,
+    s = FNOSurrogate(grid_dim=256, width=32, modes=16, layers=4, max_batch=max(batch, infer_batch), k_steps=20, dt=0.01, seed=0,
                      base_steps=base_steps, out_scale=out_scale if base_steps else 1.0, zero_mass=zero_mass)
     n_inst = int(np.ceil(n_pairs / pairs_per_instance / 0.97))
     t0 = time.perf_counter()
@@ -419,7 +421,7 @@ def surrogate_experiment(n_pairs=20000, pairs_per_instance=20, held_out=512, epo
         c.propagate(base_steps, record=False)
         Xin = np.stack([c.get_wignerfunction(i) for i in phys])
         # device time of the base for one network batch: its share of the coarse batch's steps
-        base_us = c.step_time_us(64) * base_steps * batch / held_out
+        base_us = c.step_time_us(64) * base_steps * infer_batch / held_out
         del c
     else:
         Xin = W0p
@@ -434,7 +436,7 @@ def surrogate_experiment(n_pairs=20000, pairs_per_instance=20, held_out=512, epo
     _, mom = s.predict(Xin, feat, moments=True)
     t0 = time.perf_counter()
     pok, pdrift, pedrift, ppur = physical_referee(mom, e_in, feat, e_gibbs, 20, 0.01, 8.)
-    phys_us = (time.perf_counter() - t0) * 1e6 * batch / len(phys) + s.moments_time_us(batch)
+    phys_us = (time.perf_counter() - t0) * 1e6 * infer_batch / len(phys) + s.moments_time_us(infer_batch)
     # the same law on the FP64 answers: the splitting's own energy drift, the floor of the criterion
     _, _, ref_edrift, _ = physical_referee(state_moments(WJ), e_in, feat, e_gibbs, 20, 0.01, 8.)
     # a sampled FP64 audit of one jump in audit_every: its share of the FP64 referee's cost
@@ -442,11 +444,11 @@ def surrogate_experiment(n_pairs=20000, pairs_per_instance=20, held_out=512, epo
     rng = np.random.default_rng(held_seed)
     audited = rng.choice(len(phys), size=max(1, len(phys) // audit_every), replace=False)
     fp64_us_per_jump = t_referee * 1e6 / held_out
-    audit_us = fp64_us_per_jump * batch / audit_every
-    us = s.infer_time_us(batch, reps=20)
-    rate = batch * 256 * 256 * 20 / ((us + base_us) * 1e-6)
-    rate_refereed = batch * 256 * 256 * 20 / ((us + base_us + phys_us + audit_us) * 1e-6)
-    flops, nbytes = forward_flops(batch), forward_bytes(batch, zero_mass=s.zero_mass)
+    audit_us = fp64_us_per_jump * infer_batch / audit_every
+    us = s.infer_time_us(infer_batch, reps=20)
+    rate = infer_batch * 256 * 256 * 20 / ((us + base_us) * 1e-6)
+    rate_refereed = infer_batch * 256 * 256 * 20 / ((us + base_us + phys_us + audit_us) * 1e-6)
+    flops, nbytes = forward_flops(infer_batch), forward_bytes(infer_batch, zero_mass=s.zero_mass)
     gbs = nbytes / (us * 1e-6) / 1e9
     roof = dict(bound="hbm", kernel="FNO forward (13 fused kernels: DFT, mixing, inverse + pointwise per layer; BF16 MFMA)",
                 achieved=round(gbs, 1), peak=8000.0, unit="GB/s", frac=round(gbs / 8000.0, 4), traffic=None,
@@ -455,7 +457,8 @@ def surrogate_experiment(n_pairs=20000, pairs_per_instance=20, held_out=512, epo
                 mfma_bf16_frac=round(flops / (us * 1e-6) / 1e12 / 2500.0, 4),
                 note="all products on BF16 MFMA, whose dense peak (2.5 PFLOP/s) is far above what this "
                      "work needs; the bound is the activations' HBM traffic")
-    out = dict(pairs=s.npairs, train_instances=len(kept), adam_steps=steps, batch=batch, train_s=round(t_train, 1),
+    out = dict(pairs=s.npairs, train_instances=len(kept), adam_steps=steps, batch=batch, infer_batch=infer_batch,
+               train_s=round(t_train, 1),
                data_s=round(t_data, 1), held_out=len(phys), held_out_unphysical=held_out - len(phys),
                rel_l2_median=float(np.median(err)), rel_l2_mean=float(np.mean(err)), rel_l2_max=float(np.max(err)),
                identity_rel_l2_mean=float(np.mean(ident)), base_steps=base_steps, out_scale=s.out_scale,
@@ -495,14 +498,16 @@ if __name__ == "__main__":
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--base-steps", type=int, default=2)
     ap.add_argument("--out-scale", type=float, default=1e-2)
+    ap.add_argument("--infer-batch", type=int, default=32, help="samples per forward for the timed inference")
     ap.add_argument("--time-inference", type=int, default=0, metavar="REPS",
-                    help="only time REPS forward passes of the 256^2 network over a batch of 8 (random weights)")
+                    help="only time REPS forward passes of the 256^2 network over --infer-batch samples (random weights)")
     a = ap.parse_args()
     if a.time_inference:
-        s = FNOSurrogate(grid_dim=256, width=32, modes=16, layers=4, max_batch=8, seed=0, out_scale=a.out_scale,
-                         zero_mass=True)
-        print(json.dumps({"us_per_batch": s.infer_time_us(8, reps=a.time_inference), "batch": 8, "grid": 256}))
+        s = FNOSurrogate(grid_dim=256, width=32, modes=16, layers=4, max_batch=a.infer_batch, seed=0,
+                         out_scale=a.out_scale, zero_mass=True)
+        us = s.infer_time_us(a.infer_batch, reps=a.time_inference)
+        print(json.dumps({"us_per_batch": us, "batch": a.infer_batch, "us_per_sample": us / a.infer_batch, "grid": 256}))
         sys.exit(0)
-    print(json.dumps(surrogate_experiment(a.pairs, held_out=a.held_out, epochs=a.epochs, lr=a.lr,
+    print(json.dumps(surrogate_experiment(a.pairs, held_out=a.held_out, epochs=a.epochs, lr=a.lr, infer_batch=a.infer_batch,
                                           base_steps=a.base_steps, out_scale=a.out_scale,
                                           log=lambda m: print(m, file=sys.stderr, flush=True))))

@@ -371,7 +371,7 @@ typedef struct fw_fno_desc {
   int width;              /* channels C: 32 (the fused layer kernels are written for it) */
   int modes;              /* retained modes M per axis, 8 or 16, 4 M <= N (ky < M; kp < M or kp >= N - M) */
   int layers;             /* Fourier layers, 1 .. 16 */
-  int max_batch;          /* samples per forward / training step, 1 .. 16 */
+  int max_batch;          /* samples per forward / training step, 1 .. 64 */
   int device;
   double x_amplitude, p_amplitude;
 } fw_fno_desc;

@@ -153,3 +153,23 @@ def test_device_moments_of_predictions(gpu):
     assert np.array_equal(a[0], b[0])
     for x, y in zip(a[1:], b[1:]):
         assert np.max(np.abs(x - y)) <= 1e-10 * max(1.0, np.max(np.abs(y)))
+
+
+def test_prediction_does_not_depend_on_the_batch():
+    """Each sample's prediction is the same bits whether it is computed in forwards
+    of 8 or of 40 samples (the mode mixing takes the batch in tiles of 16 rows:
+    two full tiles and a partial one)."""
+    from fastwigner_amd.batch import gaussian_state
+    from fastwigner_amd.fno import FNOSurrogate
+    rng = np.random.default_rng(9)
+    W = np.stack([gaussian_state(256, 8., rng.uniform(-2, 2), rng.uniform(-2, 2)) for _ in range(40)])
+    feat = rng.uniform(0, 1, (40, 5))
+    outs = []
+    for mb in (8, 40):
+        s = FNOSurrogate(grid_dim=256, max_batch=mb, seed=3, out_scale=1e-2, zero_mass=True)
+        P = s.get_params()
+        P[s.layout()["Q"][0]:s.layout()["Q"][0] + 32] = np.linspace(-1, 1, 32)   # a nonzero projection
+        s.set_params(P)
+        outs.append(s.predict(W, feat))
+    assert np.array_equal(outs[0], outs[1])
+    assert np.max(np.abs(outs[0] - W)) > 1e-6
