@@ -216,6 +216,7 @@ PROTOTYPES = {
     "fw_wigner_get_state": [_P, _PD],
     "fw_wigner_state_ptr": [_P, _PP, _PI],
     "fw_wigner_propagate": [_P, _PD, _I, _PD],
+    "fw_wigner_single_step": [_P, _D, _PD],
     "fw_wigner_propagate_noalloc": [_P, _P, _PD, _I, _PD],
     "fw_wigner_moments": [_P, _D, _PD],
     "fw_wigner_get_average": [_P, ctypes.c_char_p, _D, _PD],
@@ -237,6 +238,7 @@ PROTOTYPES = {
     "fw_rho_get_state": [_P, _PD],
     "fw_rho_state_ptr": [_P, _PP, _PI],
     "fw_rho_propagate": [_P, _PD, _I, _I, _PD],
+    "fw_rho_single_step": [_P, _D, _PD],
     "fw_rho_moments": [_P, _D, _PD],
     "fw_rho_get_average": [_P, ctypes.POINTER(ctypes.c_char_p), _I, _D, _PD],
     "fw_rho_bloch_setup": [_P, _D, _PD],

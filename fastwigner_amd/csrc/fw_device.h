@@ -96,6 +96,7 @@ enum {
   F_KTAB = 128,      // K factors from the cached table
   F_FROM_PARTIALS = 256,  // apply_norm: norm from the last V pass's partials (end of a run)
   F_KNORM = 512,     // two-pass steps: the K pass applies the previous step's deferred norm
+  F_NO_SCALE = 1024, // apply_norm: publish the sums only, W stays unnormalised (one bare step)
 };
 
 // two adjacent factors (columns c, c+1) in one load
@@ -977,7 +978,8 @@ FWD void begin_chunk(Args a, double t0, double inc, int n, int reset, int nf) {
 
 // W *= norm: the norm is scal[SC_NORM], or, at the end of a run
 // (F_FROM_PARTIALS), that of the last step, whose sums blocks 0..NSER-1
-// publish to series[step] (the grid has at least NSER blocks).
+// publish to series[step] (the grid has at least NSER blocks). F_NO_SCALE
+// publishes and leaves W as it is.
 FWD void apply_norm(const Args& a) {
   __shared__ double red[16];
   double c;
@@ -992,6 +994,7 @@ FWD void apply_norm(const Args& a) {
   } else {
     c = a.scal[SC_NORM];
   }
+  if (a.flags & F_NO_SCALE) return;
   const real_t cr = (real_t)c;
   const size_t n = (size_t)FW_NX * FW_NP;
   for (size_t i = (size_t)blockIdx.x * HT + threadIdx.x; i < n; i += (size_t)gridDim.x * HT) a.W[i] *= cr;

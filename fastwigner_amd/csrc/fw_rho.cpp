@@ -367,8 +367,10 @@ int enqueue_chunk(fw_rho* h, bool bloch, int n, bool momentum, const RArgs& b0, 
 }
 
 // wtime (real-time steps only): the Wigner time of every step into wtime_out[s]
+// normalise = false: the closing pass publishes the last step's sums and leaves rho
+// unnormalised (the next run takes rho as it is: it starts with RC_FRESH = 1).
 int run(fw_rho* h, bool bloch, const RArgs& b0, const double* times, int steps, bool momentum, double* series,
-        int* done, double* wtime_out = nullptr) {
+        int* done, double* wtime_out = nullptr, bool normalise = true) {
   FWCHK(set_scalar(h, RC_STOP, 0.0));
   FWCHK(set_scalar(h, RC_FRESH, 1.0));
   const bool wtime = wtime_out != nullptr && !bloch;
@@ -444,7 +446,7 @@ int run(fw_rho* h, bool bloch, const RArgs& b0, const double* times, int steps, 
     for (int k = 0; k < 3; ++k) h->ktimes[k] = acc[k] / nacc;
   RArgs a = b0;
   a.step = kChunk;
-  int fp = 1;
+  int fp = normalise ? 1 : 2;
   void* params[] = {&a, &fp};
   FWCHK(launch(h, h->f_norm, 1024, 256, params));
   if (series && applied > 0)
@@ -679,6 +681,13 @@ extern "C" int fw_rho_propagate(fw_rho* h, const double* times, int steps, int m
   if (steps == 0) return FW_OK;
   RArgs a = base(h);
   return run(h, false, a, times, steps, momentum_sums != 0, series, nullptr);
+}
+
+extern "C" int fw_rho_single_step(fw_rho* h, double t, double* series) {
+  if (!h) return fw::fail(FW_ERR_INVALID_ARG, "NULL handle");
+  const double times[1] = {t};
+  RArgs a = base(h);
+  return run(h, false, a, times, 1, false, series, nullptr, nullptr, false);
 }
 
 extern "C" int fw_rho_moments(fw_rho* h, double t, double sums[FW_RSER]) {

@@ -472,7 +472,8 @@ FWD void rho_diag_finish(const RArgs& a) {
 
 // ---------------------------------------------------------------------------
 // rho *= c (complex): c from the partials of the last pass C (end of a run;
-// block 0 also publishes that step's sums), or c = scal[0] + i scal[1].
+// block 0 also publishes that step's sums), or c = scal[0] + i scal[3].
+// from_partials = 2 publishes the sums and leaves rho as it is (one bare step).
 // ---------------------------------------------------------------------------
 FWD void rho_apply_norm(const RArgs& a, int from_partials) {
   __shared__ double red[RSER * 4];
@@ -484,6 +485,7 @@ FWD void rho_apply_norm(const RArgs& a, int from_partials) {
   } else {
     c = mk((real_t)a.scal[0], (real_t)a.scal[3]);
   }
+  if (from_partials == 2) return;
   const size_t n = (size_t)FW_N * FW_N;
   for (size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x; idx < n; idx += (size_t)gridDim.x * 256)
     a.rho[idx] = cmul(a.rho[idx], c);

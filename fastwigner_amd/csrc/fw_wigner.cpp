@@ -672,8 +672,10 @@ int ensure_graph(fw_wigner* h, int kind, const KArgs& base) {
 // s+1 into slot (s+1) mod chunk of the device series buffer, and those of the
 // last step by the closing normalisation into slot kChunk. *done = steps
 // applied before a purity-guard stop.
+// normalise = false: the closing pass publishes the last step's sums and leaves W
+// unnormalised (the next run takes W as it is: it starts with SC_FRESH = 1).
 int run_steps(fw_wigner* h, int kind, const KArgs& base0, const double* times, int steps, double* series,
-              int* done) {
+              int* done, bool normalise = true) {
   KArgs base = base0;
   FWCHK(ensure_tables(h, kind, base));
   const bool use_graph = !h->timing && kind == 0;
@@ -723,7 +725,7 @@ int run_steps(fw_wigner* h, int kind, const KArgs& base0, const double* times, i
     for (int k = 0; k < 3; ++k) h->ktimes[k] = acc[k] / nacc;
   // the deferred normalisation of the last applied step, which also publishes its sums
   KArgs a = base;
-  a.flags |= fw::F_FROM_PARTIALS;
+  a.flags |= fw::F_FROM_PARTIALS | (normalise ? 0 : fw::F_NO_SCALE);
   a.step = kChunk;
   FWCHK(launch(h, h->f_apply_norm, 1024, 256, a));
   FWCHK(sink.final_slot(applied - 1));
@@ -768,7 +770,7 @@ int ensure_graph_fused(fw_wigner* h, const KArgs& base) {
 // kernel arguments name the handle's own buffers, so a run on other buffers
 // launches every kernel itself)
 int run_steps_fused(fw_wigner* h, const KArgs& base0, const double* times, int steps, double* series,
-                    bool graphs = true) {
+                    bool graphs = true, bool normalise = true) {
   KArgs base = base0;
   FWCHK(ensure_tables(h, 0, base));
   const bool timing = h->timing;
@@ -818,7 +820,7 @@ int run_steps_fused(fw_wigner* h, const KArgs& base0, const double* times, int s
     h->ktimes[2] = 0.0;
   }
   KArgs a = base;
-  a.flags |= fw::F_FROM_PARTIALS;
+  a.flags |= fw::F_FROM_PARTIALS | (normalise ? 0 : fw::F_NO_SCALE);
   a.step = kChunk;
   FWCHK(launch(h, h->f_apply_norm, 1024, 256, a));
   FWCHK(sink.final_slot(steps - 1));
@@ -1113,6 +1115,15 @@ extern "C" int fw_wigner_propagate(fw_wigner* h, const double* times, int steps,
   if (!h->dissip && h->d.step_passes != 3) return run_steps_fused(h, a, times, steps, series);
   h->last_passes = 3;
   return run_steps(h, 0, a, times, steps, series, nullptr);
+}
+
+extern "C" int fw_wigner_single_step(fw_wigner* h, double t, double* sums) {
+  if (!h) return fw::fail(FW_ERR_INVALID_ARG, "NULL handle");
+  const double times[1] = {t};
+  KArgs a = base_args(h);
+  if (!h->dissip && h->d.step_passes != 3) return run_steps_fused(h, a, times, 1, sums, false, false);
+  h->last_passes = 3;
+  return run_steps(h, 0, a, times, 1, sums, nullptr, false);
 }
 
 extern "C" int fw_wigner_propagate_noalloc(fw_wigner* h, void* w_dev, const double* times, int steps,

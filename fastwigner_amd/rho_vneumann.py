@@ -246,9 +246,9 @@ class RhoVNeumannHIP1D:
         self._winit_src = w_init
 
     def single_step_propagation(self):
-        """One step at the current t (renormalised, unlike the reference)."""
-        times = np.array([self.t])
-        _lib.call("fw_rho_propagate", self._handle, _lib.dptr(times), 1, 0, None)
+        """One step at the current t, t not advanced (rho_vneumann_cuda_1d.py:340-353).
+        As in the reference the returned rho is not normalised."""
+        _lib.call("fw_rho_single_step", self._handle, float(self.t), None)
         return self.rho
 
     def get_Ehrenfest(self, t):

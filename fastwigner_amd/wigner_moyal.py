@@ -264,11 +264,11 @@ class WignerMoyalHIP1D:
         return self.wignerfunction
 
     def single_step_propagation(self):
-        """One Strang step at the current t (wigner_moyal_cuda_1d.py:496-519).
-        Unlike the reference the returned state is already renormalised."""
-        times = np.array([self.t], dtype=np.float64)
-        series = np.empty((1, _lib.FW_NSER))
-        _lib.call("fw_wigner_propagate", self._handle, _lib.dptr(times), 1, _lib.dptr(series))
+        """One Strang step at the current t, t not advanced, no Ehrenfest record
+        (wigner_moyal_cuda_1d.py:496-519; with the dissipator
+        wigner_random_collision_model_fftw_1d.py:56-75). As in the reference the
+        returned state is not normalised; a later ``propagate`` takes it as it is."""
+        _lib.call("fw_wigner_single_step", self._handle, float(self.t), None)
         return self.wignerfunction
 
     def get_average(self, observable):

@@ -133,6 +133,12 @@ int fw_wigner_state_ptr(fw_wigner* h, void** dev_ptr, int* elem_size);
  * No host synchronisation inside; steps are replayed from a captured hipGraph. */
 int fw_wigner_propagate(fw_wigner* h, const double* times, int steps, double* series);
 
+/* One Moyal step at time t (single_step_propagation, wigner_moyal_cuda_1d.py:496-519;
+ * with the dissipator, wigner_random_collision_model_fftw_1d.py:56-75): W is left
+ * unnormalised, as the reference leaves it; a later fw_wigner_propagate takes it as
+ * it is. sums (optional) gets the FW_NSER sums over the new W. */
+int fw_wigner_single_step(fw_wigner* h, double t, double* sums);
+
 /* No counterpart in the reference. For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc): fw_wigner_propagate's two-pass Moyal steps on the
@@ -242,6 +248,10 @@ int fw_rho_state_ptr(fw_rho* h, void** dev_ptr, int* elem_size);
  * each; series (optional) gets FW_RSER sums per step; momentum_sums != 0 adds the Tr(rho g(P)) family
  * (one extra read of rho per step). */
 int fw_rho_propagate(fw_rho* h, const double* times, int steps, int momentum_sums, double* series);
+/* One von Neumann step at time t (single_step_propagation, rho_vneumann_cuda_1d.py:340-353): rho is
+ * left unnormalised, as the reference leaves it. series (optional) gets the trace family (the first
+ * 11 of FW_RSER sums) over the new rho. */
+int fw_rho_single_step(fw_rho* h, double t, double* series);
 /* FW_RSER sums over the current rho at time t (the momentum family included) */
 int fw_rho_moments(fw_rho* h, double t, double sums[FW_RSER]);
 /* Tr[rho F1(x) g1(p) F2(x) ...] dX, real part (get_average, rho_vneumann_cuda_1d.py:431-472); obs[k] NULL

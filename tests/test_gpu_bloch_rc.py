@@ -133,6 +133,28 @@ def test_random_collision_parity(gpu):
         assert np.max(np.abs(np.array(getattr(g, name)) - ref)) <= 1e-11, name
 
 
+def test_random_collision_single_step_on_an_unnormalised_state(gpu):
+    """single_step_propagation with the dissipator
+    (wigner_random_collision_model_fftw_1d.py:56-75): W <- w W + W0s around the
+    unitary step, W left unnormalised. The dissipator is affine, so a state of
+    norm 2 (set without normalisation) must go through it as it is, in the
+    single step and in the propagate that follows."""
+    from fastwigner_amd import _lib
+    g, o = make_rc(0.05)
+    W0 = np.exp(-(o.X - 1.) ** 2 - (o.P - 0.5) ** 2)
+    W0 = 2.0 * W0 / (W0.sum() * o.dXdP)
+    _lib.call("fw_wigner_set_state", g._handle, _lib.dptr(np.ascontiguousarray(W0)))
+    o.W = W0.copy()
+    Wg = g.single_step_propagation().get()
+    o.single_step_propagation()
+    assert rel_l2(Wg, o.W) <= 1e-12
+    norm = Wg.sum() * o.dXdP
+    assert abs(norm - o.W.sum() * o.dXdP) <= 1e-12 and norm > 1.5
+    Wg = g.propagate(50).get()
+    o.propagate(50)
+    assert rel_l2(Wg, o.W) <= 1e-12
+
+
 def test_random_collision_gamma0_is_moyal(gpu):
     from fastwigner_amd.wigner_moyal import WignerMoyalHIP1D
     g, o = make_rc(0.0)

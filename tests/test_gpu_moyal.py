@@ -128,6 +128,33 @@ def test_rectangular_masks_time_dependent(gpu, nx, npp, xp, passes):
     assert np.max(np.abs(np.array(g.P_average_RHS) - dip)) <= 1e-11
 
 
+@pytest.mark.parametrize("passes", [0, 3])
+def test_single_step_leaves_the_state_unnormalised(gpu, passes):
+    """single_step_propagation (wigner_moyal_cuda_1d.py:496-519): one step at the
+    current t, t not advanced, nothing recorded, W not renormalised. An x-p mask
+    takes about 9% of the norm away in the step, so a renormalised result would
+    miss by that much; the propagate after it takes the unnormalised W as it
+    is, as the reference's loop does."""
+    masks = dict(abs_boundary_x_p="pow(fabs(cos(0.5*M_PI*X/X_amplitude)), 0.5)")
+    g, o = make_pair(128, 8., 128, 8., 0.01, V_Q, K_Q, DV_Q, DK_Q, masks=masks, step_passes=passes)
+    W0 = gaussian(o, x0=3.0)
+    g.set_wignerfunction(W0)
+    o.set_wignerfunction(W0)
+    t0 = g.t
+    Wg = g.single_step_propagation().get()
+    o.single_step_propagation()
+    assert g.t == t0 and len(g.X_average) == 0
+    assert rel_l2(Wg, o.W) <= 1e-12
+    norm = Wg.sum() * o.dXdP
+    assert abs(norm - o.W.sum() * o.dXdP) <= 1e-13
+    assert norm < 0.95
+    Wg = g.propagate(20).get()
+    o.propagate(20)
+    assert rel_l2(Wg, o.W) <= 1e-12
+    ref = np.array([s["X_average"] for s in o.series])
+    assert np.max(np.abs(np.array(g.X_average) - ref)) <= 1e-11
+
+
 @pytest.mark.parametrize("nx,npp,passes", [(64, 32, 0), (64, 32, 3), (16, 16, 0), (32, 64, 0)])
 def test_small_grid_observables(gpu, nx, npp, passes):
     """Grids whose column blocks hold fewer threads than there are sums (4 or 2

@@ -86,6 +86,27 @@ def test_vneumann_masks_and_time_dependent_field(gpu):
     assert rel_l2(g.rho.get(), o.rho) <= 1e-12
 
 
+def test_single_step_leaves_rho_unnormalised(gpu):
+    """single_step_propagation (rho_vneumann_cuda_1d.py:340-353): one step at the
+    current t, t not advanced, rho not renormalised. An absorbing boundary in x
+    and x' takes about a fifth of the trace away in the step."""
+    mask = "sqrt(fabs(cos(0.5*M_PI*X/X_amplitude)*cos(0.5*M_PI*X_prime/X_amplitude)))"
+    g, o = make(n=128, abs_boundary_x=mask)
+    r0 = gaussian_rho(o.Xv, 4.0, 0.0, 0.7)
+    g.set_rho(r0)
+    o.set_rho(r0)
+    t0 = g.t
+    rg = g.single_step_propagation().get()
+    o.single_step_propagation()
+    assert g.t == t0
+    assert rel_l2(rg, o.rho) <= 1e-12
+    tr = np.trace(rg).real * o.dX
+    assert abs(tr - np.trace(o.rho).real * o.dX) <= 1e-13 and tr < 0.9
+    g.propagate(10)
+    o.propagate(10)
+    assert rel_l2(g.rho.get(), o.rho) <= 1e-12
+
+
 def test_get_average_tuples(gpu):
     g, o = make(n=128)
     r0 = gaussian_rho(o.Xv, -0.5, 1.0, 0.8)
