@@ -102,9 +102,73 @@ typedef cpx tw_t;
 FWD cpx twmul(cpx a, const tw_t& w) { return cmul(a, w); }
 #endif
 
+// Real constant times a complex value. In the FP32 form the constant is split
+// into float hi + lo parts (the compensation of cmulk).
+FWD cpx kscale(cpx a, double c) {
+#if FW_FP32
+  const float h = (float)c, l = (float)(c - (double)h);
+  return mk(fmaf(a.x, l, a.x * h), fmaf(a.y, l, a.y * h));
+#else
+  return cscale(a, c);
+#endif
+}
+
+// cos and sin of 2 pi j / n for compile-time constants: the angle reduced
+// exactly in integers to the nearest quarter turn q and a remainder
+// |phi| <= pi/4, the Taylor series of sin to phi^17 and of cos to phi^18
+// (truncation below 1e-19), then the quarter-turn rotation; about 1 ulp.
+struct ct_cs {
+  double c, s;
+};
+constexpr ct_cs ct_unit(long j, long n) {
+  const long u = 4 * (((j % n) + n) % n);
+  const long q = (u + n / 2) / n;
+  const long r = u - q * n;
+  const double phi = (double)r * (3.14159265358979323846 / (2.0 * (double)n));
+  const double p2 = phi * phi;
+  double ps = 1.0 / 355687428096000.0;
+  ps = ps * p2 - 1.0 / 1307674368000.0;
+  ps = ps * p2 + 1.0 / 6227020800.0;
+  ps = ps * p2 - 1.0 / 39916800.0;
+  ps = ps * p2 + 1.0 / 362880.0;
+  ps = ps * p2 - 1.0 / 5040.0;
+  ps = ps * p2 + 1.0 / 120.0;
+  ps = ps * p2 - 1.0 / 6.0;
+  const double sn = phi + phi * p2 * ps;
+  double pc = 1.0 / 6402373705728000.0;
+  pc = pc * p2 - 1.0 / 20922789888000.0;
+  pc = pc * p2 + 1.0 / 87178291200.0;
+  pc = pc * p2 - 1.0 / 479001600.0;
+  pc = pc * p2 + 1.0 / 3628800.0;
+  pc = pc * p2 - 1.0 / 40320.0;
+  pc = pc * p2 + 1.0 / 720.0;
+  pc = pc * p2 - 1.0 / 24.0;
+  pc = pc * p2 + 0.5;
+  const double cs = 1.0 - p2 * pc;
+  const long qq = q & 3;
+  ct_cs w{0.0, 0.0};
+  w.c = qq == 0 ? cs : qq == 1 ? -sn : qq == 2 ? -cs : sn;
+  w.s = qq == 0 ? sn : qq == 1 ? cs : qq == 2 ? -sn : -cs;
+  return w;
+}
+// cos and sin of 2 pi j / R, j < R
+template <int R>
+struct UnitTab {
+  double c[R], s[R];
+  constexpr UnitTab() : c(), s() {
+    for (int j = 0; j < R; ++j) {
+      const ct_cs w = ct_unit(j, R);
+      c[j] = w.c;
+      s[j] = w.s;
+    }
+  }
+};
+
 // ---------------------------------------------------------------------------
-// Forward DFT kernels of radix 2, 4, 8, 16 on registers, natural order in/out.
-// X_q = sum_m a_m exp(-2 pi i m q / R).
+// Forward DFT kernels on registers, natural order in/out:
+// X_q = sum_m a_m exp(-2 pi i m q / R). Radix 2, 4, 8, 16 by the kernels
+// below; an odd radix by its conjugate-symmetric pairs (DftOdd); a radix
+// 2^k * odd (k >= 1) by the prime-factor map over the two (DftPfa).
 // ---------------------------------------------------------------------------
 FWD void dft4(cpx& a0, cpx& a1, cpx& a2, cpx& a3) {
   const cpx t0 = cadd(a0, a2), t1 = csub(a0, a2);
@@ -115,7 +179,88 @@ FWD void dft4(cpx& a0, cpx& a1, cpx& a2, cpx& a3) {
   a3 = csub(t1, t3);
 }
 
+// Odd R >= 3. With s_m = a_m + a_{R-m}, d_m = a_m - a_{R-m} (m = 1..H,
+// H = (R-1)/2) and phi = 2 pi m q / R:
+//   X_0 = a_0 + sum_m s_m,
+//   X_q = c_q - i e_q,  X_{R-q} = c_q + i e_q,  q = 1..H,
+//   c_q = a_0 + sum_m s_m cos(phi),  e_q = sum_m d_m sin(phi).
 template <int R> struct Dft;
+
+template <int R>
+struct DftOdd {
+  static_assert(R >= 3 && R % 2 == 1, "odd radix");
+  FWD static void run(cpx* a) {
+    constexpr int H = (R - 1) / 2;
+    constexpr UnitTab<R> U;
+    cpx s[H + 1], d[H + 1];
+    cpx x0 = a[0];
+#pragma unroll
+    for (int m = 1; m <= H; ++m) {
+      s[m] = cadd(a[m], a[R - m]);
+      d[m] = csub(a[m], a[R - m]);
+      x0 = cadd(x0, s[m]);
+    }
+    // the outputs q, R - q overwrite inputs already folded into s and d; a[0]
+    // is read by every q and written last
+#pragma unroll
+    for (int q = 1; q <= H; ++q) {
+      cpx c = a[0], e = mk(0, 0);
+#pragma unroll
+      for (int m = 1; m <= H; ++m) {
+        c = cadd(c, kscale(s[m], U.c[(m * q) % R]));
+        e = cadd(e, kscale(d[m], U.s[(m * q) % R]));
+      }
+      a[q] = mk(c.x + e.y, c.y - e.x);
+      a[R - q] = mk(c.x - e.y, c.y + e.x);
+    }
+    a[0] = x0;
+  }
+};
+
+constexpr int inv_mod(int x, int m) {
+  for (int y = 0; y < m; ++y)
+    if ((x * y) % m == 1 % m) return y;
+  return 0;
+}
+
+// R = N1 N2 with N1, N2 coprime (Good-Thomas): input n = (N2 n1 + N1 n2) mod R
+// and output k = (N2 s k1 + N1 u k2) mod R, s = N2^-1 mod N1, u = N1^-1 mod N2,
+// turn the DFT of length R into N2 DFTs of length N1 followed by N1 of length
+// N2, with no twiddles (every index is a compile-time constant).
+template <int N1, int N2>
+struct DftPfa {
+  FWD static void run(cpx* a) {
+    constexpr int R = N1 * N2;
+    constexpr int S = inv_mod(N2 % N1, N1), U = inv_mod(N1 % N2, N2);
+    cpx y[N2][N1];
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2)
+#pragma unroll
+      for (int n1 = 0; n1 < N1; ++n1) y[n2][n1] = a[(N2 * n1 + N1 * n2) % R];
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) Dft<N1>::run(y[n2]);
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) {
+      cpx z[N2];
+#pragma unroll
+      for (int n2 = 0; n2 < N2; ++n2) z[n2] = y[n2][k1];
+      Dft<N2>::run(z);
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) a[(N2 * S * k1 + N1 * U * k2) % R] = z[k2];
+    }
+  }
+};
+
+// radices other than the powers of two specialised below
+template <int R>
+struct Dft {
+  static constexpr int A = R & -R, B = R / A;   // power-of-two and odd parts
+  static_assert(B > 1 && A <= 16, "radix: 2^k * odd, 2^k <= 16");
+  FWD static void run(cpx* a) {
+    if constexpr (A == 1) DftOdd<R>::run(a);
+    else DftPfa<A, B>::run(a);
+  }
+};
 
 template <> struct Dft<1> {
   FWD static void run(cpx*) {}
@@ -239,11 +384,20 @@ struct TwTab<TWN, 2> {
   FWD static int src(int e) { return e; }
   FWD static int dst(int e) { return e; }
   FWD static tw_t get(const tw_t*, int idx) {
-    static_assert(TWN >= 4 && (TWN & (TWN - 1)) == 0, "power-of-two twiddle table length");
-    constexpr int QN = TWN / 4;
-    const int q = (idx + QN / 2) / QN;
-    const int r = idx - q * QN;
-    const double phi = (double)r * (2.0 * 3.14159265358979323846 / TWN);
+    static_assert(TWN >= 4 && TWN % 2 == 0, "even twiddle table length");
+    int q, r;
+    double phi;
+    if constexpr ((TWN & (TWN - 1)) == 0) {
+      constexpr int QN = TWN / 4;
+      q = (idx + QN / 2) / QN;
+      r = idx - q * QN;
+      phi = (double)r * (2.0 * 3.14159265358979323846 / TWN);
+    } else {
+      // quarter turns of 2 pi idx / TWN in units of 1/(4 TWN): r in [-TWN/2, TWN/2]
+      q = (4 * idx + TWN / 2) / TWN;
+      r = 4 * idx - q * TWN;
+      phi = (double)r * (3.14159265358979323846 / (2.0 * TWN));
+    }
     const double p2 = phi * phi;
     double sp = 1.0 / 355687428096000.0;                 // 1/17!
     sp = sp * p2 - 1.0 / 1307674368000.0;                 // 1/15!
@@ -317,11 +471,30 @@ struct TwStage {
 // ---------------------------------------------------------------------------
 // TWN: length of the twiddle table passed in (a table for a multiple of N
 // serves N with stride TWN/N; the batched kernels share one table).
+// Radix of the next Stockham stage over the remaining length rem with E
+// elements per thread: for powers of two the largest power of two up to 16
+// dividing both; otherwise the largest radix up to min(E, 32) dividing both
+// (its power-of-two part at most 16). A schedule is valid when such a radix
+// exists at every stage; the host checks this with the same rule.
+constexpr int next_radix(int rem, int e) {
+  if ((rem & (rem - 1)) == 0 && (e & (e - 1)) == 0) {
+    for (int r = 16; r >= 2; r /= 2)
+      if (rem % r == 0 && e % r == 0) return r;
+  }
+  for (int r = e < 32 ? e : 32; r >= 2; --r)
+    if (rem % r == 0 && e % r == 0 && (r & -r) <= 16) return r;
+  return rem;
+}
+
+// Lengths: any even N. Powers of two take radix-2/4/8/16 stages and the
+// pair-fused spectral step; other lengths add odd-radix stages (every radix
+// dividing E) and take the spectral step through LDS.
 template <int N, int E, int P, bool LM, int TWN = N, int TWO = 0>
 struct Line {
   using TW = TwTab<TWN, TWO>;
-  static_assert((N & (N - 1)) == 0 && N >= 2, "line length must be a power of two");
-  static_assert(E <= N && (E & (E - 1)) == 0, "elements per thread must be a power of two <= N");
+  static constexpr bool POW2 = (N & (N - 1)) == 0;
+  static_assert(N >= 2 && N % 2 == 0, "line length must be even");
+  static_assert(E <= N && N % E == 0 && (POW2 || E % 2 == 0), "elements per thread must divide N (and be even)");
   static constexpr int TPL = N / E;
   static constexpr int NT = P * TPL;
   static constexpr int RMAX = E < 16 ? E : 16;
@@ -373,8 +546,18 @@ struct Line {
 #pragma unroll
       for (int m = 0; m < R; ++m) a[m] = v[b + m * B];
       if constexpr (NS > 1) {
-        const int k = j & (NS - 1);
-        if constexpr (TWO == 2) {
+        const int k = j % NS;
+        if constexpr (TWO == 2 && R > 16) {
+          // radices above 16 (lines with odd factors): powers by a running
+          // product, two twiddles live instead of R
+          const cpx w1 = TW::get(tw, k * TWS);
+          cpx wm = w1;
+#pragma unroll
+          for (int m = 1; m < R; ++m) {
+            a[m] = cmul(a[m], wm);
+            if (m + 1 < R) wm = cmul(wm, w1);
+          }
+        } else if constexpr (TWO == 2) {
           cpx w[R];
           w[1] = TW::get(tw, k * TWS);
 #pragma unroll
@@ -391,7 +574,7 @@ struct Line {
 #pragma unroll
         for (int m = 0; m < R; ++m) v[b + m * B] = a[m];
       } else {
-        const int base = (j & ~(NS - 1)) * R + (j & (NS - 1));
+        const int base = (j / NS) * NS * R + j % NS;
 #pragma unroll
         for (int m = 0; m < R; ++m) lds[lidx(q, base + m * NS)] = a[m];
       }
@@ -403,7 +586,8 @@ struct Line {
   template <bool TOLDS, int NS = 1>
   FWD static void fwd(cpx (&v)[E], cpx* lds, int q, int t, const tw_t* __restrict__ tw) {
     constexpr int REM = N / NS;
-    constexpr int R = REM < RMAX ? REM : RMAX;
+    constexpr int R = POW2 ? (REM < RMAX ? REM : RMAX) : next_radix(REM, E);
+    static_assert(E % R == 0, "every stage radix must divide E");
     constexpr bool LAST = (NS * R == N);
     if constexpr (LAST && !TOLDS) {
       stage<R, NS, true>(v, lds, q, t, tw);
@@ -437,7 +621,7 @@ struct Line {
   // inputs. Against transforms with the multiply between them through LDS
   // (spectrum out, pair gather and write back, natural reload) this saves two
   // LDS rounds and three barriers per spectral step.
-  static constexpr bool PAIRED = E >= 4;
+  static constexpr bool PAIRED = POW2 && E >= 4;
   static constexpr int RL = PAIRED ? (E / 2 < N / 2 ? E / 2 : N / 2) : 1;
   static constexpr int S = N / RL;
   static constexpr int BS = PAIRED ? E / (2 * RL) : 0;   // slots per thread
@@ -663,12 +847,35 @@ FWD double wave_sum(double v) {
 // cross-lane read would pick up, so every thread stores its own slot. After a
 // barrier, total() adds the RN slots of quantity i in order (the same bits in
 // every block). red: >= ns * RN doubles.
+// Sum over the first L < 64 lanes of a wave whose other lanes do not exist
+// (the last wave of a block of NT threads, NT % 64 = L): the DPP row moves
+// read lanes outside the wave's execution mask as 0 (bound_ctrl), and only the
+// rows that hold lanes of the wave are read.
+template <int L>
+FWD double wave_sum_prefix(double v) {
+  v += dpp_move<0xB1>(v);
+  v += dpp_move<0x4E>(v);
+  v += dpp_move<0x141>(v);
+  v += dpp_move<0x140>(v);
+  double s = read_lane(v, 0);
+  if constexpr (L > 16) s += read_lane(v, 16);
+  if constexpr (L > 32) s += read_lane(v, 32);
+  if constexpr (L > 48) s += read_lane(v, 48);
+  return s;
+}
+
 template <int NT>
 struct BlockRed {
-  static constexpr int RN = NT >= 64 ? NT / 64 : NT;
+  static constexpr int RN = NT >= 64 ? (NT + 63) / 64 : NT;
   FWD static void put(double v, double* red, int ns, int i) {
     if constexpr (NT >= 64) {
-      v = wave_sum(v);
+      if constexpr (NT % 64 == 0) {
+        v = wave_sum(v);
+      } else {
+        // blocks of lines whose length has an odd factor: a partial last wave
+        if ((int)(threadIdx.x >> 6) < NT / 64) v = wave_sum(v);
+        else v = wave_sum_prefix<NT % 64>(v);
+      }
       if ((threadIdx.x & 63) == 0) red[(threadIdx.x >> 6) * ns + i] = v;
     } else {
       red[threadIdx.x * ns + i] = v;

@@ -803,7 +803,7 @@ FWD void build_sumw(const Args& a, SumW* sw) {
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k <= N / 2; k += gridDim.x * blockDim.x) {
     double acc[4][2] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
     for (int n = 0; n < N; ++n) {
-      const int m = (int)(((long long)n * k) & (N - 1));
+      const int m = (int)(((long long)n * k) % N);
       double sn, cs;
       sincospi(-2.0 * (double)m / (double)N, &sn, &cs);
       const double P = ::dP * (n - 0.5 * ::P_gridDIM);

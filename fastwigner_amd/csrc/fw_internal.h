@@ -19,6 +19,16 @@ std::string fmt_double(double v);     // round-trip exact C double literal
 void twiddles(int n, std::vector<double>& re, std::vector<double>& im);
 int upload_twiddles(int n, bool fp32, void** dptr);
 bool is_pow2(int n);
+// Line FFT schedules (fw_common.h Line): E elements per thread over a line of
+// n points is valid when E divides n, E <= 32 (even unless n is a power of two,
+// then a power of two) and at every stage some radix up to min(E, 32) divides
+// both E and the remaining length (the device's next_radix rule).
+bool line_schedule_ok(int n, int e);
+// The default E of a line of n points: the power-of-two rule of the propagator
+// for powers of two; otherwise the smallest valid E >= 8, else the largest
+// valid one; 0 when none exists (n = 2^a m needs E >= 2 rad(m) <= 32: odd
+// primes of m whose product is at most 15).
+int line_default_e(int n, int pow2_default);
 
 // Host mirror of fwk::Args (fw_device.h); field order and types must match.
 struct KArgs {

@@ -228,6 +228,38 @@ int upload_twiddles(int n, bool fp32, void** dptr) {
 
 bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
+// the device's radix rule (fw_common.h next_radix)
+static int next_radix(int rem, int e) {
+  if (is_pow2(rem) && is_pow2(e)) {
+    for (int r = 16; r >= 2; r /= 2)
+      if (rem % r == 0 && e % r == 0) return r;
+  }
+  for (int r = e < 32 ? e : 32; r >= 2; --r)
+    if (rem % r == 0 && e % r == 0 && (r & -r) <= 16) return r;
+  return 0;
+}
+
+bool line_schedule_ok(int n, int e) {
+  if (n < 2 || e < 2 || e > 32 || e > n || n % e) return false;
+  if (is_pow2(n)) return is_pow2(e);
+  if (e % 2) return false;
+  for (int rem = n; rem > 1;) {
+    const int r = next_radix(rem, e);
+    if (r < 2) return false;
+    rem /= r;
+  }
+  return true;
+}
+
+int line_default_e(int n, int pow2_default) {
+  if (is_pow2(n)) return pow2_default;
+  for (int e = 8; e <= 32; ++e)
+    if (line_schedule_ok(n, e)) return e;
+  for (int e = 6; e >= 2; --e)
+    if (line_schedule_ok(n, e)) return e;
+  return 0;
+}
+
 }  // namespace fw
 
 // ---------------------------------------------------------------------------

@@ -65,9 +65,7 @@ struct Tiles {
 };
 
 int default_e(int n) {
-  if (n >= 256) return 16;
-  if (n >= 8) return 8;
-  return n;
+  return fw::line_default_e(n, n >= 256 ? 16 : n >= 8 ? 8 : n);
 }
 
 // Default tiles: E elements per thread; column pairs per block so a wave reads
@@ -85,8 +83,8 @@ Tiles default_tiles(int nx, int np, bool fp32) {
     p *= 2;
   t.col_p = p;
   const int tpl = nx / t.row_e;
-  int pr = tpl >= 64 ? 1 : 64 / tpl;
-  while (pr > 1 && pairs_row % pr) pr /= 2;
+  int pr = 1;
+  while (2 * pr * tpl <= 64 && pairs_row % (2 * pr) == 0) pr *= 2;
   t.row_p = pr;
   return t;
 }
@@ -163,9 +161,12 @@ int validate(const fw_wigner_desc* d, const fw_wigner_exprs* x) {
   if (!d || !x) return fw::fail(FW_ERR_INVALID_ARG, "descriptor or expressions are NULL");
   if (d->x_grid_dim % 2 || d->p_grid_dim % 2)
     return fw::fail(FW_ERR_INVALID_ARG, "X_gridDIM and P_gridDIM must be even");
-  if (!fw::is_pow2(d->x_grid_dim) || !fw::is_pow2(d->p_grid_dim) || d->x_grid_dim < 8 || d->p_grid_dim < 8 ||
-      d->x_grid_dim > 4096 || d->p_grid_dim > 4096)
-    return fw::fail(FW_ERR_INVALID_ARG, "grid dimensions must be powers of two in [8, 4096]");
+  if (d->x_grid_dim < 8 || d->p_grid_dim < 8 || d->x_grid_dim > 4096 || d->p_grid_dim > 4096)
+    return fw::fail(FW_ERR_INVALID_ARG, "grid dimensions must be in [8, 4096]");
+  if (!default_e(d->x_grid_dim) || !default_e(d->p_grid_dim))
+    return fw::fail(FW_ERR_INVALID_ARG,
+                    "grid dimensions must be 2^a m with m odd and the product of the distinct primes of m "
+                    "at most 15 (e.g. 2^a 3^b 5^c, 2^a 7^b, 2^a 11^b, 2^a 13^b)");
   if (!(d->x_amplitude > 0) || !(d->p_amplitude > 0))
     return fw::fail(FW_ERR_INVALID_ARG, "X_amplitude and P_amplitude must be positive");
   if (!std::isfinite(d->dt)) return fw::fail(FW_ERR_INVALID_ARG, "dt must be finite");
@@ -197,9 +198,10 @@ int resolve_tiles(const fw_wigner_desc* d, Tiles& t) {
   if (d->col_p > 0) t.col_p = d->col_p;
   if (d->row_e > 0) t.row_e = d->row_e;
   if (d->row_p > 0) t.row_p = d->row_p;
-  auto ok_e = [](int e, int n) { return fw::is_pow2(e) && e >= 2 && e <= n && e <= 32; };
-  if (!ok_e(t.col_e, d->p_grid_dim) || !ok_e(t.row_e, d->x_grid_dim))
-    return fw::fail(FW_ERR_INVALID_ARG, "elements per thread must be a power of two in [2, min(N, 32)]");
+  if (!fw::line_schedule_ok(d->p_grid_dim, t.col_e) || !fw::line_schedule_ok(d->x_grid_dim, t.row_e))
+    return fw::fail(FW_ERR_INVALID_ARG,
+                    "elements per thread must divide the line length, be at most 32 and hold every stage radix "
+                    "(a power of two for power-of-two lines)");
   if (!fw::is_pow2(t.col_p) || (d->x_grid_dim / 2) % t.col_p || !fw::is_pow2(t.row_p) ||
       (d->p_grid_dim / 2) % t.row_p)
     return fw::fail(FW_ERR_INVALID_ARG, "lines per block must be a power of two dividing the line count");

@@ -64,8 +64,9 @@ int fw_device_count(int* count);
 typedef struct fw_wigner fw_wigner;
 
 typedef struct fw_wigner_desc {
-  int x_grid_dim;          /* X_gridDIM: even power of two, 8..4096 */
-  int p_grid_dim;          /* P_gridDIM: even power of two, 8..4096 */
+  int x_grid_dim;          /* X_gridDIM: even, 8..4096: 2^a m, m odd with the product of its distinct
+                              primes at most 15 (2^a 3^b 5^c, 2^a 7^b, 2^a 11^b, 2^a 13^b) */
+  int p_grid_dim;          /* P_gridDIM: as X_gridDIM */
   double x_amplitude;      /* X_amplitude: x in [-X_amplitude, X_amplitude) */
   double p_amplitude;      /* P_amplitude */
   double dt;               /* time step */

@@ -40,8 +40,8 @@ def test_version_and_device_count():
     assert n.value >= 0
 
 
-@pytest.mark.parametrize("nx,npp,msg", [(63, 64, "even"), (96, 64, "powers of two"), (4, 64, "powers of two"),
-                                        (8192, 64, "powers of two")])
+@pytest.mark.parametrize("nx,npp,msg", [(63, 64, "even"), (4, 64, "in [8, 4096]"), (8192, 64, "in [8, 4096]"),
+                                        (34, 64, "at most 15"), (2 * 3 * 7 * 4, 64, "at most 15")])
 def test_invalid_grids(nx, npp, msg):
     lib = _lib.load()
     d = _lib.wigner_desc(nx, 8., npp, 8., 0.01)
@@ -61,6 +61,40 @@ def test_invalid_tiles_and_null_args():
     assert lib.fw_wigner_check(None, ctypes.byref(e)) == -1
     assert lib.fw_wigner_destroy(None) == 0
     assert lib.fw_wigner_propagate(None, None, 1, None) == -1
+
+
+@pytest.mark.parametrize("n,tiles,ok", [
+    (1000, (10, 1, 20, 1), True),    # 2^3 5^3: radix-10 and radix-20 stages
+    (1000, (8, 1, 8, 1), False),     # E without the factor 5
+    (768, (12, 1, 24, 2), True),     # 2^8 3
+    (768, (16, 1, 16, 1), False),    # E a power of two on a line with a factor 3
+    (120, (30, 1, 30, 1), True),     # 2^3 3 5: E holds both odd primes
+    (120, (15, 1, 30, 1), False),    # odd E on a line that is not a power of two
+    (64, (6, 1, 8, 1), False),       # power-of-two line: power-of-two E only
+])
+def test_even_grid_tile_rules(n, tiles, ok):
+    """The schedules of lines that are not powers of two: E divides the
+    line, is even and holds a radix for every stage (checked before any
+    compilation); a valid schedule is only checked by the GPU tests."""
+    lib = _lib.load()
+    d = _lib.wigner_desc(n, 8., n, 8., 0.01, tiles=tiles)
+    e = _lib.wigner_exprs(V="0.5*X*X", K="0.5*P*P")
+    plen = ctypes.c_size_t()
+    rc = lib.fw_wigner_prelude(ctypes.byref(d), ctypes.byref(e), None, 0, ctypes.byref(plen))
+    assert (rc == 0) == ok, lib.fw_last_error().decode()
+    if not ok:
+        assert "elements per thread" in lib.fw_last_error().decode()
+
+
+def test_device_code_compiles_for_an_even_grid():
+    """A 30 x 22 grid (one radix-30 and one radix-22 stage per line, prime-
+    factor and odd-radix kernels) compiles for gfx950 in both precisions."""
+    lib = _lib.load()
+    for prec in (0, 1):
+        d = _lib.wigner_desc(30, 8., 22, 8., 0.01, precision=prec)
+        e = _lib.wigner_exprs(V="0.5*X*X+0.1*X*X*X*X", K="0.5*P*P", diff_V="X+0.4*X*X*X", diff_K="P")
+        rc = lib.fw_wigner_check(ctypes.byref(d), ctypes.byref(e))
+        assert rc == 0, lib.fw_last_error().decode()
 
 
 def test_malformed_expression_is_a_compile_error():
