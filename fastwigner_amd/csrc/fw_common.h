@@ -486,6 +486,13 @@ constexpr int next_radix(int rem, int e) {
   return rem;
 }
 
+// A value materialised in registers at this point of the program (the
+// compiler may neither sink its computation below nor recompute it later).
+FWD cpx pin(cpx c) {
+  asm volatile("" : "+v"(c.x), "+v"(c.y));
+  return c;
+}
+
 // Lengths: any even N. Powers of two take radix-2/4/8/16 stages and the
 // pair-fused spectral step; other lengths add odd-radix stages (every radix
 // dividing E) and take the spectral step through LDS.
@@ -498,6 +505,7 @@ struct Line {
   static constexpr int TPL = N / E;
   static constexpr int NT = P * TPL;
   static constexpr int RMAX = E < 16 ? E : 16;
+  struct Plan;   // base twiddles of a spectral step, computed ahead (below)
   static constexpr int RAW = P * N;
 #ifndef FW_LDS_SWZ
 #define FW_LDS_SWZ 0
@@ -535,8 +543,10 @@ struct Line {
   // the twiddles w^{m k}, k = j mod NS, w = exp(-2 pi i/(NS R)), and writes
   // output q to index (j/NS) NS R + k + q NS. TOREG: the last stage, whose
   // outputs land on the thread's own natural positions, stays in registers.
-  template <int R, int NS, bool TOREG>
-  FWD static void stage(cpx (&v)[E], cpx* lds, int q, int t, const tw_t* __restrict__ tw) {
+  // PRE: the base twiddle of butterfly b is wb[b] (computed ahead by plan)
+  template <int R, int NS, bool TOREG, bool PRE = false>
+  FWD static void stage(cpx (&v)[E], cpx* lds, int q, int t, const tw_t* __restrict__ tw,
+                        const cpx* wb = nullptr) {
     constexpr int B = E / R;
     constexpr int TWS = (N / (NS * R)) * (TWN / N);
 #pragma unroll
@@ -550,7 +560,9 @@ struct Line {
         if constexpr (TWO == 2 && R > 16) {
           // radices above 16 (lines with odd factors): powers by a running
           // product, two twiddles live instead of R
-          const cpx w1 = TW::get(tw, k * TWS);
+          cpx w1;
+          if constexpr (PRE) w1 = wb[b];
+          else w1 = TW::get(tw, k * TWS);
           cpx wm = w1;
 #pragma unroll
           for (int m = 1; m < R; ++m) {
@@ -559,7 +571,8 @@ struct Line {
           }
         } else if constexpr (TWO == 2) {
           cpx w[R];
-          w[1] = TW::get(tw, k * TWS);
+          if constexpr (PRE) w[1] = wb[b];
+          else w[1] = TW::get(tw, k * TWS);
 #pragma unroll
           for (int m = 2; m < R; ++m) w[m] = cmul(w[m / 2], w[m - m / 2]);
 #pragma unroll
@@ -583,21 +596,27 @@ struct Line {
 
   // Forward FFT. TOLDS: leave the spectrum in LDS in natural order (synced);
   // otherwise leave it in registers at the natural positions t + TPL e.
-  template <bool TOLDS, int NS = 1>
-  FWD static void fwd(cpx (&v)[E], cpx* lds, int q, int t, const tw_t* __restrict__ tw) {
+  // PL: the base twiddles of stage I (counting from NS) come from pl->inv
+  // (the inverse of a planned spectral step, which starts at NS = RL > 1)
+  template <bool TOLDS, int NS = 1, bool PL = false, int I = 0>
+  FWD static void fwd(cpx (&v)[E], cpx* lds, int q, int t, const tw_t* __restrict__ tw,
+                      const Plan* pl = nullptr) {
     constexpr int REM = N / NS;
     constexpr int R = POW2 ? (REM < RMAX ? REM : RMAX) : next_radix(REM, E);
     static_assert(E % R == 0, "every stage radix must divide E");
+    static_assert(!PL || NS > 1, "planned stages are twiddled");
     constexpr bool LAST = (NS * R == N);
+    const cpx* wb = nullptr;
+    if constexpr (PL) wb = pl->inv[I];
     if constexpr (LAST && !TOLDS) {
-      stage<R, NS, true>(v, lds, q, t, tw);
+      stage<R, NS, true, PL>(v, lds, q, t, tw, wb);
     } else {
-      stage<R, NS, false>(v, lds, q, t, tw);
+      stage<R, NS, false, PL>(v, lds, q, t, tw, wb);
       sync();
       if constexpr (!LAST) {
         load_natural(v, lds, q, t);
         sync();
-        fwd<TOLDS, NS * R>(v, lds, q, t, tw);
+        fwd<TOLDS, NS * R, PL, I + 1>(v, lds, q, t, tw, pl);
       }
     }
   }
@@ -635,27 +654,33 @@ struct Line {
     return m < RL / 2 ? p : N - p;
   }
   // forward stages before the last one; result in LDS (natural order, synced)
-  template <int NS>
-  FWD static void fwd_pre(cpx (&v)[E], cpx* lds, int q, int t, const tw_t* __restrict__ tw) {
+  // PL: the base twiddles of the I-th twiddled stage come from pl->pre
+  template <int NS, bool PL = false, int I = 0>
+  FWD static void fwd_pre(cpx (&v)[E], cpx* lds, int q, int t, const tw_t* __restrict__ tw,
+                          const Plan* pl = nullptr) {
     constexpr int REM = S / NS;
     constexpr int R = REM < RMAX ? REM : RMAX;
-    stage<R, NS, false>(v, lds, q, t, tw);
+    constexpr bool TWD = PL && NS > 1;
+    const cpx* wb = nullptr;
+    if constexpr (TWD) wb = pl->pre[I];
+    stage<R, NS, false, TWD>(v, lds, q, t, tw, wb);
     sync();
     if constexpr (NS * R < S) {
       load_natural(v, lds, q, t);
       sync();
-      fwd_pre<NS * R>(v, lds, q, t, tw);
+      fwd_pre<NS * R, PL, (NS > 1 ? I + 1 : I)>(v, lds, q, t, tw, pl);
     }
   }
   // butterfly h (0, 1) of slot sl
   FWD static int slot_bfly(int sl, int h) { return sl == 0 ? (h == 0 ? 0 : S / 2) : (h == 0 ? sl : S - sl); }
   // MUL: pair(i, zk, zm) for item i (zk at k = item_k, zm at N - k) and
   // self(z0, zn) for DC and Nyquist (thread 0, with item 0).
-  template <class MUL>
-  FWD static void spectral(cpx (&v)[E], cpx* lds, int q, int t, const tw_t* __restrict__ tw, MUL& mul) {
+  template <class MUL, bool PL = false>
+  FWD static void spectral(cpx (&v)[E], cpx* lds, int q, int t, const tw_t* __restrict__ tw, MUL& mul,
+                           const Plan* pl = nullptr) {
     static_assert(PAIRED, "the pair-fused step needs E >= 4");
     constexpr int TWS = TWN / N;
-    fwd_pre<1>(v, lds, q, t, tw);
+    fwd_pre<1, PL>(v, lds, q, t, tw, pl);
     // last forward stage, slot map; v[(2 b + h) RL + m] = output m of butterfly h of slot b
 #pragma unroll
     for (int b = 0; b < BS; ++b) {
@@ -668,7 +693,8 @@ struct Line {
         for (int m = 0; m < RL; ++m) a[m] = lds[lidx(q, j + m * S)];
         if constexpr (TWO == 2) {
           cpx w[RL];
-          w[1] = TW::get(tw, j * TWS);
+          if constexpr (PL) w[1] = pl->pr[2 * b + h];
+          else w[1] = TW::get(tw, j * TWS);
 #pragma unroll
           for (int m = 2; m < RL; ++m) w[m] = cmul(w[m / 2], w[m - m / 2]);
 #pragma unroll
@@ -720,9 +746,55 @@ struct Line {
     sync();
     load_natural(v, lds, q, t);
     sync();
-    fwd<false, RL>(v, lds, q, t, tw);
+    fwd<false, RL, PL>(v, lds, q, t, tw, pl);
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e].y = -v[e].y;
+  }
+
+  // ---- Base twiddles of a pair-fused spectral step, computed ahead (FP64) ----
+  // Every twiddled stage needs, per butterfly, one base twiddle that depends
+  // on the thread's position in the line only. A pass computes them while its
+  // loads of W are in flight (before the barrier that waits for them), so
+  // their sine/cosine series overlap the load latency instead of following it.
+  static constexpr int PSTG = 4;   // twiddled stages per direction, at most
+  static constexpr bool PLANNED = PAIRED && TWO == 2;
+  struct Plan {
+    cpx pre[PSTG][E];                 // forward stages before the pair stage
+    cpx pr[2 * (BS > 0 ? BS : 1)];    // the pair stage's butterflies
+    cpx inv[PSTG][E];                 // the inverse's stages after its first
+  };
+  template <int NS, int I>
+  FWD static void plan_pre(Plan& p, int t, const tw_t* __restrict__ tw) {
+    constexpr int REM = S / NS;
+    constexpr int R = REM < RMAX ? REM : RMAX;
+    constexpr int B = E / R;
+    constexpr int TWS = (N / (NS * R)) * (TWN / N);
+    if constexpr (NS > 1) {
+      static_assert(I < PSTG, "twiddled stages");
+#pragma unroll
+      for (int b = 0; b < B; ++b) p.pre[I][b] = pin(TW::get(tw, ((t + TPL * b) % NS) * TWS));
+    }
+    if constexpr (NS * R < S) plan_pre<NS * R, (NS > 1 ? I + 1 : I)>(p, t, tw);
+  }
+  template <int NS, int I>
+  FWD static void plan_inv(Plan& p, int t, const tw_t* __restrict__ tw) {
+    constexpr int REM = N / NS;
+    constexpr int R = REM < RMAX ? REM : RMAX;
+    constexpr int B = E / R;
+    constexpr int TWS = (N / (NS * R)) * (TWN / N);
+    static_assert(I < PSTG, "twiddled stages");
+#pragma unroll
+    for (int b = 0; b < B; ++b) p.inv[I][b] = pin(TW::get(tw, ((t + TPL * b) % NS) * TWS));
+    if constexpr (NS * R < N) plan_inv<NS * R, I + 1>(p, t, tw);
+  }
+  FWD static void plan(Plan& p, int t, const tw_t* __restrict__ tw) {
+    static_assert(PAIRED && TWO == 2, "planned twiddles: FP64 series, pair-fused step");
+    plan_pre<1, 0>(p, t, tw);
+#pragma unroll
+    for (int b = 0; b < BS; ++b)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) p.pr[2 * b + h] = pin(TW::get(tw, slot_bfly(t + TPL * b, h) * (TWN / N)));
+    plan_inv<RL, 0>(p, t, tw);
   }
 };
 
@@ -740,6 +812,13 @@ FWD int spec_k(int t, int i) {
 // (k, N - k) and mul.self on DC and Nyquist, inverse FFT; registers to
 // registers at the natural positions. E >= 4: the pair-fused step; E = 2:
 // separate transforms with the multiply through LDS.
+// The pair-fused spectral step with base twiddles computed ahead (L::plan).
+template <class L, int E, class MUL>
+FWD void spectral_step_planned(cpx (&v)[E], cpx* lds, int q, int t, const tw_t* __restrict__ twl, MUL& mul,
+                               const typename L::Plan& pl) {
+  L::template spectral<MUL, true>(v, lds, q, t, twl, mul, &pl);
+}
+
 template <class L, int E, class MUL>
 FWD void spectral_step(cpx (&v)[E], cpx* lds, int q, int t, const tw_t* __restrict__ twl, MUL& mul) {
   if constexpr (L::PAIRED) {

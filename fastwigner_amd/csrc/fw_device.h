@@ -274,6 +274,14 @@ FWD bool previous_norm(const Args& a, int blk, double* red, double& nrm) {
 // block at N = 2048), no table loads, twice the blocks per CU; measured 20-27%
 // faster per pass than the staged full table (mode 0) or the two-level table
 // (mode 1) on gfx950. The FP32 form keeps the full compensated table.
+#ifndef FW_SW_AHEAD
+// the theta-sum weights of a thread's items loaded with the pass's other loads
+// (registers: on where the column tiles leave one wave per SIMD)
+#define FW_SW_AHEAD (FW_NP <= 1024)
+#endif
+#ifndef FW_TW_PLAN
+#define FW_TW_PLAN 1   // base twiddles of the spectral steps computed ahead (FP64)
+#endif
 #ifndef FW_TW2
 #define FW_TW2 (FW_FP32 ? 0 : 2)
 #endif
@@ -444,7 +452,9 @@ struct VVMul {
   const cpx2 (&f0)[FW_COL_E / 2 + 1];   // closing V/2 (step s)
   const cpx2 (&f1)[FW_COL_E / 2 + 1];   // opening V/2 (step s+1)
   real_t ma, mb;                        // x-only x-p mask of columns a, b (1 without a mask)
-  const SumW* __restrict__ sw;          // weights, read at the multiply (registers)
+  // weights: FW_SW_AHEAD, this thread's items (then the Nyquist entry), loaded
+  // with the pass's other loads; otherwise the table, read at the multiply
+  const SumW* __restrict__ sw;
   int t;                                // lane of the line
   double acc[4];                        // sum c_k Re(sigma_k conj F_k), sigma = alpha_a + alpha_b,
                                         // for P, P^2, K', K
@@ -473,7 +483,11 @@ struct VVMul {
     const cpx2 F0 = fac(f0, i, g0);
     const cpx A1 = cscale(cmul(A, F0.a), ma), B1 = cscale(cmul(B, F0.b), mb);
     const double sx = (double)A1.x + (double)B1.x, sy = (double)A1.y + (double)B1.y;
+#if FW_SW_AHEAD
+    const SumW w = sw[i];
+#else
     const SumW w = sw[spec_k<VL>(t, i)];
+#endif
     acc[0] += sy * w.a - sx * (::dP * FW_NP);
     acc[1] += sx * w.b;
     acc[2] += sx * w.dk[0] + sy * w.dk[1];
@@ -501,7 +515,11 @@ struct VVMul {
     a0 = ra;
     b0 = rb;
     const double s0 = (double)ra + (double)rb, sn = (double)na + (double)nb;
+#if FW_SW_AHEAD
+    const SumW w0 = sw[0], wn = sw[FW_COL_E / 2];   // thread 0: item 0 is k = 0
+#else
     const SumW w0 = sw[0], wn = sw[FW_NP / 2];
+#endif
     acc[0] += s0 * w0.a + sn * wn.a;
     acc[1] += s0 * w0.b + sn * wn.b;
     acc[2] += s0 * w0.dk[0] + sn * wn.dk[0];
@@ -754,6 +772,18 @@ FWD void vvpass_theta(const Args& a) {
     v_factors_at(c, t, t0 + FW_PHASE_SHIFT * ::dt, t0, f0);
     v_factors_at(c, t, t1 + FW_PHASE_SHIFT * ::dt, t1, f1);
   }
+#if FW_SW_AHEAD
+  SumW swa[E / 2 + 1];
+#pragma unroll
+  for (int i = 0; i < E / 2; ++i) swa[i] = a.sw[spec_k<L>(t, i)];
+  swa[E / 2] = a.sw[N / 2];
+  const SumW* swp = swa;
+#else
+  const SumW* swp = a.sw;
+#endif
+  // FP64 pair-fused steps: the base twiddles while the loads are in flight
+  typename L::Plan plan;
+  if constexpr (L::PLANNED && FW_TW_PLAN) L::plan(plan, t, twl);
   tws.store(twl, tid);
   // the K pass of this step has read `fresh`; the next K pass uses the partials
   if (blk == 0 && tid == 0 && a.scal[SC_FRESH] != 0.0) a.scal[SC_FRESH] = 0.0;
@@ -769,8 +799,9 @@ FWD void vvpass_theta(const Args& a) {
   g0 = 0.5 * ::dt * field_tau(a, a.step) * ::dTheta;
   g1 = 0.5 * ::dt * field_tau(a, a.step + 1) * ::dTheta;
 #endif
-  VVMul mul{f0, f1, mask.ma, mask.mb, a.sw, t, {0.0, 0.0, 0.0, 0.0}, 0.0, 0.0, 0.0, g0, g1};
-  spectral_step<L>(v, lds, q, t, twl, mul);
+  VVMul mul{f0, f1, mask.ma, mask.mb, swp, t, {0.0, 0.0, 0.0, 0.0}, 0.0, 0.0, 0.0, g0, g1};
+  if constexpr (L::PLANNED && FW_TW_PLAN) spectral_step_planned<L>(v, lds, q, t, twl, mul, plan);
+  else spectral_step<L>(v, lds, q, t, twl, mul);
 #pragma unroll
   for (int e = 0; e < E; ++e) *reinterpret_cast<cpx*>(W + (size_t)(t + TPL * e) * FW_NX + c) = v[e];
 
@@ -890,6 +921,9 @@ FWD void kpass(const Args& a) {
       ftb[i] = kfactor(Lm, Pb, tau, tnow);
     }
   }
+  // FP64 pair-fused steps: the base twiddles while the loads are in flight
+  typename L::Plan plan;
+  if constexpr (L::PLANNED && FW_TW_PLAN) L::plan(plan, t, twl);
   tws.store(twl, tid);
   if (deferred) norm_row.reduce_waves(red);
   __syncthreads();
@@ -901,7 +935,8 @@ FWD void kpass(const Args& a) {
     for (int e = 0; e < E; ++e) { v[e].x *= rn; v[e].y *= rn; }
   }
   KMul mul{fta, ftb};
-  spectral_step<L>(v, lds, q, t, twl, mul);
+  if constexpr (L::PLANNED && FW_TW_PLAN) spectral_step_planned<L>(v, lds, q, t, twl, mul, plan);
+  else spectral_step<L>(v, lds, q, t, twl, mul);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     Wa[t + TPL * e] = v[e].x;
