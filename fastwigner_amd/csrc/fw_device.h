@@ -266,6 +266,21 @@ FWD bool previous_norm(const Args& a, int blk, double* red, double& nrm) {
   return true;
 }
 
+// The loads issued before this point stay ahead of those after it (a compiler
+// memory barrier; the hardware completes loads in order, so the first stage of
+// the FFT then waits for the state's loads only and the factor tables, sum
+// weights and partials still in flight overlap it).
+FWD void issue_order() { asm volatile("" ::: "memory"); }
+
+// The pass that follows the first reader of the `fresh` flag in a step (the
+// three-pass K pass, the fused VV pass) clears it, so the next step takes the
+// partials. At the end of the kernel: a flag load and store between the W
+// loads and their first use would make every wave wait for all of its loads
+// (the wait counter is not known after the branch).
+FWD void clear_fresh(const Args& a, int blk, int tid) {
+  if (blk == 0 && tid == 0 && a.scal[SC_FRESH] != 0.0) a.scal[SC_FRESH] = 0.0;
+}
+
 // ---------------------------------------------------------------------------
 // V pass: columns (along p), lines = pairs of adjacent columns.
 // ---------------------------------------------------------------------------
@@ -556,6 +571,7 @@ FWD void vpass(const Args& a) {
   cpx v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = *reinterpret_cast<const cpx*>(W + (size_t)(t + TPL * e) * FW_NX + c);
+  issue_order();   // W's loads first: the FFT waits for them only, not for the tables
   // the previous step's S0 row (deferred norm), issued before the table loads
   const bool deferred = (FL & F_FIRST) && a.scal[SC_FRESH] == 0.0;
   NormRow<NT> norm_row;
@@ -670,6 +686,7 @@ FWD void vvpass(const Args& a) {
   cpx v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = *reinterpret_cast<const cpx*>(W + (size_t)(t + TPL * e) * FW_NX + c);
+  issue_order();   // W's loads first: the FFT waits for them only, not for the tables
   cpx2 ftab[E / 2 + 1];
   if constexpr (use_tab) {
     v_factors(a, c, t, ftab);
@@ -678,8 +695,6 @@ FWD void vvpass(const Args& a) {
     v_factors_at(c, t, t0 + FW_PHASE_SHIFT * ::dt, t0, ftab);
   }
   tws.store(twl, tid);
-  // the K pass of this step has read `fresh`; the next K pass uses the partials
-  if (blk == 0 && tid == 0 && a.scal[SC_FRESH] != 0.0) a.scal[SC_FRESH] = 0.0;
   __syncthreads();
 
   const double Xa = ::dX * (c - 0.5 * ::X_gridDIM);
@@ -730,6 +745,7 @@ FWD void vvpass(const Args& a) {
   spectral_step<L>(v, lds, q, t2, twl, mul1);
 #pragma unroll
   for (int e = 0; e < E; ++e) *reinterpret_cast<cpx*>(W + (size_t)(t + TPL * e) * FW_NX + c) = v[e];
+  clear_fresh(a, blk, tid);
 }
 
 // The fused column pass with the sums on the theta spectrum (VVMul): no x-p
@@ -757,6 +773,7 @@ FWD void vvpass_theta(const Args& a) {
   cpx v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = *reinterpret_cast<const cpx*>(W + (size_t)(t + TPL * e) * FW_NX + c);
+  issue_order();   // W's loads first: the FFT waits for them only, not for the tables
   static_assert(!FW_V_SPLIT || FW_VTAB, "split potentials use the table of V0");
   cpx2 f0[E / 2 + 1], f1[E / 2 + 1];
   if constexpr (use_tab) {
@@ -785,8 +802,6 @@ FWD void vvpass_theta(const Args& a) {
   typename L::Plan plan;
   if constexpr (L::PLANNED && FW_TW_PLAN) L::plan(plan, t, twl);
   tws.store(twl, tid);
-  // the K pass of this step has read `fresh`; the next K pass uses the partials
-  if (blk == 0 && tid == 0 && a.scal[SC_FRESH] != 0.0) a.scal[SC_FRESH] = 0.0;
   __syncthreads();
 
   const double Xa = ::dX * (c - 0.5 * ::X_gridDIM);
@@ -822,6 +837,7 @@ FWD void vvpass_theta(const Args& a) {
   }
   sums[8] = (double)N * mul.pur;
   write_partial_rows<NT>(sums, red, a.partials, blk, nb);
+  clear_fresh(a, blk, tid);
 }
 
 // Theta-spectrum sum weights (SumW), one entry per k <= N_p/2, by direct
@@ -888,8 +904,6 @@ FWD void kpass(const Args& a) {
   constexpr bool use_tab = (FL & F_BLOCH) || FW_KTAB;
   real_t* __restrict__ Wa = a.W + (size_t)r * FW_NX;
   real_t* __restrict__ Wb = Wa + FW_NX;
-  if constexpr (!(FL & F_KNORM))
-    if (blk == 0 && tid == 0 && a.scal[SC_FRESH] != 0.0) a.scal[SC_FRESH] = 0.0;
 
   TwStage<typename L::TW, NT> tws;
   tws.load(a.tw_row, tid);
@@ -945,6 +959,7 @@ FWD void kpass(const Args& a) {
   // the previous step's sums to the series, after this block's own work
   if constexpr (FL & F_KNORM)
     if (deferred) publish_sums<NT>(a, blk, gridDim.x, red);
+  if constexpr (!(FL & F_KNORM)) clear_fresh(a, blk, tid);
 }
 
 // ---------------------------------------------------------------------------
