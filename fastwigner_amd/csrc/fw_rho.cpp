@@ -45,14 +45,19 @@ struct Mod {
   hipFunction_t f1 = nullptr, f2 = nullptr;
 };
 
-// Diagonal partial kernel tiling: DC diagonals per block (lanes), 256 / DC
-// row subgroups, G row groups (the rows the one-block finish adds).
+// Diagonal partial kernel tiling: DC diagonals per block (lanes; a power of
+// two dividing N, at most 64), 256 / DC row subgroups, G row groups (the rows
+// the one-block finish adds; a divisor of N, at most 16 and N / (256 / DC)).
 void diag_geom(int n, int* dc, int* g) {
-  *dc = n < 64 ? n : 64;
-  const int w = 256 / *dc;
-  int gg = n / w;
-  if (gg > 16) gg = 16;
-  if (gg < 1) gg = 1;
+  int c = 1;
+  while (c < 64 && n % (2 * c) == 0) c *= 2;
+  *dc = c;
+  const int w = 256 / c;
+  int cap = n / w;
+  if (cap > 16) cap = 16;
+  if (cap < 1) cap = 1;
+  int gg = cap;
+  while (gg > 1 && n % gg) --gg;
   *g = gg;
 }
 
@@ -62,12 +67,16 @@ int diag_blocks(int n) {
   return (n / dc) * g;
 }
 
+// E of the one-block momentum finish (a single line of N / E threads)
 int diag_e(int n) {
   int e = n / 256;
   if (e < 2) e = 2;
   if (e > 32) e = 32;
   if (e > n) e = n;
-  return e;
+  if (fw::is_pow2(n)) return e;
+  for (int f = e; f <= 32; ++f)
+    if (fw::line_schedule_ok(n, f) && n / f <= 1024) return f;
+  return 0;
 }
 
 }  // namespace
@@ -127,8 +136,12 @@ std::string or_default(const char* s, const char* d) { return (s && *s) ? std::s
 
 int validate(const fw_rho_desc* d, const fw_rho_exprs* x) {
   if (!d || !x) return fw::fail(FW_ERR_INVALID_ARG, "descriptor or expressions are NULL");
-  if (!fw::is_pow2(d->x_grid_dim) || d->x_grid_dim < 8 || d->x_grid_dim > 4096)
-    return fw::fail(FW_ERR_INVALID_ARG, "X_gridDIM must be a power of two in [8, 4096]");
+  if (d->x_grid_dim < 8 || d->x_grid_dim > 4096 || d->x_grid_dim % 2)
+    return fw::fail(FW_ERR_INVALID_ARG, "X_gridDIM must be even, in [8, 4096]");
+  if (!fw::line_default_e(d->x_grid_dim, 8) || !diag_e(d->x_grid_dim))
+    return fw::fail(FW_ERR_INVALID_ARG,
+                    "X_gridDIM must be 2^a m with m odd and the product of the distinct primes of m at most 15 "
+                    "(e.g. 2^a 3^b 5^c, 2^a 7^b, 2^a 11^b, 2^a 13^b)");
   if (!(d->x_amplitude > 0)) return fw::fail(FW_ERR_INVALID_ARG, "X_amplitude must be positive");
   if (!std::isfinite(d->dt)) return fw::fail(FW_ERR_INVALID_ARG, "dt must be finite");
   if (d->precision != FW_FP64 && d->precision != FW_FP32)
@@ -148,11 +161,14 @@ int tiles(const fw_rho_desc* d, int& re, int& rp, int& ce, int& cp) {
   const size_t esz = d->precision == FW_FP32 ? 8 : 16;
   // defaults: 256-thread row tiles of 8 elements per thread, column tiles of
   // 16 elements per thread and 4 columns (fastest at 1024^2 FP64 on gfx950)
-  re = d->row_e > 0 ? d->row_e : (n >= 8 ? 8 : n);
-  ce = d->col_e > 0 ? d->col_e : (n >= 256 ? 16 : (n >= 8 ? 8 : n));
+  re = d->row_e > 0 ? d->row_e : fw::line_default_e(n, n >= 8 ? 8 : n);
+  ce = d->col_e > 0 ? d->col_e : fw::line_default_e(n, n >= 256 ? 16 : (n >= 8 ? 8 : n));
   const int tpl = n / re;
-  rp = d->row_p > 0 ? d->row_p : (tpl >= 256 ? 1 : 256 / tpl);
-  while (rp > 1 && n % rp) rp /= 2;
+  rp = d->row_p;
+  if (rp <= 0) {
+    rp = 1;
+    while (2 * rp * tpl <= 256 && n % (2 * rp) == 0) rp *= 2;
+  }
   // FP32 stages its twiddle table in LDS; FP64 computes its twiddles (no table)
   const size_t tws = d->precision == FW_FP32 ? ((size_t)n * 17 / 16 + 1) * 16 : 16;
   auto lds = [&](int lines) {
@@ -163,8 +179,10 @@ int tiles(const fw_rho_desc* d, int& re, int& rp, int& ce, int& cp) {
   while (cp > 1 && n % cp) cp /= 2;
   // the default column tile shrinks to what fits the 160 KiB of LDS of a CU
   while (d->col_p <= 0 && cp > 1 && lds(cp) > 160 * 1024) cp /= 2;
-  auto ok_e = [&](int e) { return fw::is_pow2(e) && e >= 2 && e <= n && e <= 32; };
-  if (!ok_e(re) || !ok_e(ce)) return fw::fail(FW_ERR_INVALID_ARG, "elements per thread must be a power of two in [2, 32]");
+  if (!fw::line_schedule_ok(n, re) || !fw::line_schedule_ok(n, ce))
+    return fw::fail(FW_ERR_INVALID_ARG,
+                    "elements per thread must divide N, be at most 32 and hold every stage radix "
+                    "(a power of two for power-of-two N)");
   if (!fw::is_pow2(rp) || !fw::is_pow2(cp) || n % rp || n % cp)
     return fw::fail(FW_ERR_INVALID_ARG, "lines per block must be a power of two dividing N");
   if (rp * (n / re) > 1024 || cp * (n / ce) > 1024) return fw::fail(FW_ERR_INVALID_ARG, "tile needs more than 1024 threads");

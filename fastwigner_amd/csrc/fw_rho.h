@@ -148,7 +148,7 @@ FWD void rho_row(const RArgs& a) {
   tw_t twr[TWN];
   if constexpr (STAGE) {
 #pragma unroll
-    for (int r = 0; r < TWN; ++r) twr[r] = a.tw[(tid + r * NT) & (N - 1)];
+    for (int r = 0; r < TWN; ++r) twr[r] = a.tw[(tid + r * NT) % N];
   }
   cpx v[E];
 #pragma unroll
@@ -250,7 +250,7 @@ FWD void rho_col(const RArgs& a) {
   tw_t twr[TWN];
   if constexpr (STAGE) {
 #pragma unroll
-    for (int r = 0; r < TWN; ++r) twr[r] = a.tw[(tid + r * NT) & (N - 1)];
+    for (int r = 0; r < TWN; ++r) twr[r] = a.tw[(tid + r * NT) % N];
   }
   cpx v[E];
 #pragma unroll
@@ -338,7 +338,7 @@ FWD void rho_diag_partial(const RArgs& a) {
     for (int r = 0; r < RT; ++r) {
       const int i = g * DRG + w + r * DW;
       if ((DRG % DW) == 0 || w + r * DW < DRG) {
-        const cpx z = a.rho[(size_t)i * FW_N + ((i - d) & (FW_N - 1))];
+        const cpx z = a.rho[(size_t)i * FW_N + (i - d + FW_N) % FW_N];
         cr += (double)z.x;
         ci += (double)z.y;
       }
@@ -405,7 +405,7 @@ FWD void rho_diag_weights(const RArgs& a, double* dw) {
   for (int d = blockIdx.x * blockDim.x + threadIdx.x; d < FW_N; d += gridDim.x * blockDim.x) {
     double acc[4][2] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
     for (int k = 0; k < FW_N; ++k) {
-      const int m = (int)(((long long)k * d) & (FW_N - 1));
+      const int m = (int)(((long long)k * d) % FW_N);
       double sn, cs;
       sincospi(2.0 * (double)m / (double)FW_N, &sn, &cs);
       const double P = pcoord(k);

@@ -212,7 +212,8 @@ int fw_wigner_step_passes(fw_wigner* h, int* passes);
 typedef struct fw_rho fw_rho;
 
 typedef struct fw_rho_desc {
-  int x_grid_dim;          /* X_gridDIM: power of two, 8..4096 */
+  int x_grid_dim;          /* X_gridDIM: even, 8..4096, as fw_wigner_desc's (2^a m, distinct primes of m
+                              with a product of at most 15) */
   int precision;           /* FW_FP64 (complex128) or FW_FP32 (complex64, FP64 phases and sums) */
   double x_amplitude;      /* X_amplitude */
   double dt;

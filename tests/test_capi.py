@@ -86,6 +86,21 @@ def test_even_grid_tile_rules(n, tiles, ok):
         assert "elements per thread" in lib.fw_last_error().decode()
 
 
+@pytest.mark.parametrize("n,tiles,msg", [
+    (34, None, "at most 15"),            # 2 * 17
+    (63, None, "even"),
+    (1000, (8, 1, 16, 1), "elements per thread"),   # E without the factor 5
+])
+def test_density_matrix_size_rules(n, tiles, msg):
+    """The density matrix takes the even sizes the Wigner classes take; refused
+    sizes and schedules fail before any compilation."""
+    lib = _lib.load()
+    d = _lib.rho_desc(n, 10., 0.01, tiles=tiles)
+    e = _lib.rho_exprs(V="0.5*X*X", K="0.5*P*P")
+    assert lib.fw_rho_check(ctypes.byref(d), ctypes.byref(e)) == -1
+    assert msg in lib.fw_last_error().decode()
+
+
 def test_device_code_compiles_for_an_even_grid():
     """A 30 x 22 grid (one radix-30 and one radix-22 stage per line, prime-
     factor and odd-radix kernels) compiles for gfx950 in both precisions."""

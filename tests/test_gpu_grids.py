@@ -1,7 +1,9 @@
-"""GPU parity on even grids that are not powers of two (the reference only
-asserts evenness, wigner_moyal_cuda_1d.py:57-62). Such lines take odd-radix
-and prime-factor stages in the FFT engine and the spectral step through LDS;
-lines whose thread count has an odd factor run blocks with a partial last wave.
+"""GPU parity on even grids that are not powers of two (the reference's Wigner
+classes only assert evenness, wigner_moyal_cuda_1d.py:57-62; its density-matrix
+classes only check that X_gridDIM is given, rho_vneumann_cuda_1d.py:53-56).
+Such lines take odd-radix and prime-factor stages in the FFT engine and the
+spectral step through LDS; lines whose thread count has an odd factor run
+blocks with a partial last wave.
 
 Bars as for the power-of-two grids: FP64 relative L2 <= 1e-12 against the
 oracle, series to 1e-11 (relative to their scale); FP32 form <= 1e-4 against
@@ -117,3 +119,64 @@ def test_bloch_and_random_collision_on_an_even_grid(gpu):
     Wg = g.propagate(200).get()
     o.propagate(200)
     assert rel_l2(Wg, o.W) <= 1e-12
+
+
+# ---------------------------------------------------------------------------
+# Density matrix on even grids that are not powers of two
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n,K,dK,steps", [
+    (200, "0.5*P*P", "P", 100),                                        # 2^3 5^2: momentum sums by weights
+    (200, "0.5*P*P + 0.05*P*sin(0.3*t)", "P + 0.05*sin(0.3*t)", 60),  # the one-block inverse FFT form
+    (1000, "0.5*P*P", "P", 30),                                        # 100-thread lines: partial waves
+])
+def test_density_matrix_parity_even_grids(gpu, n, K, dK, steps):
+    from tests.test_gpu_rho import gaussian_rho, make
+    g, o = make(n=n, K=K, dK=dK)
+    r0 = gaussian_rho(o.Xv)
+    g.set_rho(r0)
+    o.set_rho(r0)
+    g.propagate(steps)
+    xs, ps, hs = [], [], []
+    for _ in range(steps):
+        o.propagate(1)
+        xs.append(o.get_average((lambda X, t: X,)))
+        ps.append(o.get_average((None, lambda P, t: P)))
+        hs.append(o.get_average((None, o.K)) + o.get_average((o.V,)))
+    err = rel_l2(g.rho.get(), o.rho)
+    print("von Neumann %d^2, %d steps: rel L2 %.2e" % (n, steps, err))
+    assert err <= 1e-12
+    for name, ref in (("X_average", xs), ("P_average", ps), ("hamiltonian_average", hs)):
+        assert np.max(np.abs(np.array(getattr(g, name)) - np.array(ref))) <= 1e-11, name
+
+
+def test_density_matrix_wigner_fp32_and_bloch_even_grids(gpu):
+    """The Wigner transform (three shears, sign-flip FFT) at 240 = 2^4 3 5
+    against the oracle and the Gaussian's closed form; the FP32 form at 240
+    against the FP64 oracle; the harmonic Gibbs state at 120 against the
+    oracle."""
+    from fastwigner_amd.rho_bloch import RhoBlochHIP1D
+    from tests.test_gpu_rho import gaussian_rho, make
+    g, o = make(n=240)
+    r0 = gaussian_rho(o.Xv, 1.0, 0.5, 1.0)
+    g.set_rho(r0)
+    o.set_rho(r0)
+    W = g.get_wignerfunction().get()
+    assert rel_l2(W, o.get_wignerfunction()) <= 1e-12
+    Wa = np.exp(-(g.X_wigner - 1.0) ** 2 - (g.P_wigner - 0.5) ** 2) / np.pi
+    Wa /= Wa.sum() * g.wigner_dxdp
+    assert rel_l2(W.real, Wa) <= 1e-12
+
+    g32, o32 = make(n=240, precision="fp32")
+    g32.set_rho(r0)
+    o32.set_rho(r0)
+    g32.propagate(200)
+    o32.propagate(200)
+    err = rel_l2(g32.rho.get(), o32.rho)
+    print("von Neumann 240^2 fp32, 200 steps: rel L2 vs fp64 oracle %.2e" % err)
+    assert err <= 1e-4
+
+    V, K = "0.5*X*X + 0.05*X*X*X*X", "0.5*P*P"
+    b = RhoBlochHIP1D(X_gridDIM=120, X_amplitude=10., kT=0.5, dbeta=0.005, t=0., V=V, K=K)
+    rho = b.get_gibbs_state().get()
+    ob = fo.RhoBlochOracle(120, 10., 0.005, kT=0.5, dbeta=0.005, V=cexpr.make(V), K=cexpr.make(K, args=("P", "t")))
+    assert rel_l2(rho, ob.get_gibbs_state()) <= 1e-12
