@@ -756,7 +756,7 @@ struct Line {
   // on the thread's position in the line only. A pass computes them while its
   // loads of W are in flight (before the barrier that waits for them), so
   // their sine/cosine series overlap the load latency instead of following it.
-  static constexpr int PSTG = 4;   // twiddled stages per direction, at most
+  static constexpr int PSTG = 6;   // twiddled stages per direction, at most (N <= 4096, E >= 4)
   static constexpr bool PLANNED = PAIRED && TWO == 2;
   struct Plan {
     cpx pre[PSTG][E];                 // forward stages before the pair stage

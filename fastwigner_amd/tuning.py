@@ -59,5 +59,12 @@ def store_entry(name, value, path=PATH):
 
 
 def store(kind, nx, npp, precision, tiles, info=None, path=PATH):
-    """Add or replace one entry of the tuning file at ``path`` (merging with what it holds)."""
-    store_entry(key(kind, nx, npp, precision), {"tiles": list(tiles), **(info or {})}, path=path)
+    """Set the tiles (and timings) of one entry of the tuning file at ``path``; the
+    entry's other keys (a tuned V-factor policy, say) are kept."""
+    name = key(kind, nx, npp, precision)
+    try:
+        with open(path) as f:
+            old = json.load(f).get(name, {})
+    except (OSError, ValueError):
+        old = {}
+    store_entry(name, {**old, "tiles": list(tiles), **(info or {})}, path=path)

@@ -101,6 +101,16 @@ def test_density_matrix_size_rules(n, tiles, msg):
     assert msg in lib.fw_last_error().decode()
 
 
+def test_device_code_compiles_for_the_longest_schedules():
+    """4096-point lines at four elements per thread: the most stages per
+    direction (six, with their base twiddles computed ahead)."""
+    lib = _lib.load()
+    d = _lib.wigner_desc(4096, 8., 4096, 8., 0.01, tiles=(4, 1, 4, 1))
+    e = _lib.wigner_exprs(V="0.5*X*X", K="0.5*P*P")
+    rc = lib.fw_wigner_check(ctypes.byref(d), ctypes.byref(e))
+    assert rc == 0, lib.fw_last_error().decode()
+
+
 def test_device_code_compiles_for_an_even_grid():
     """A 30 x 22 grid (one radix-30 and one radix-22 stage per line, prime-
     factor and odd-radix kernels) compiles for gfx950 in both precisions."""
