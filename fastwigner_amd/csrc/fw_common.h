@@ -556,7 +556,7 @@ struct Line {
 #pragma unroll
       for (int m = 0; m < R; ++m) a[m] = v[b + m * B];
       if constexpr (NS > 1) {
-        const int k = j % NS;
+        const int k = ((NS & (NS - 1)) == 0) ? (j & (NS - 1)) : j % NS;
         if constexpr (TWO == 2 && R > 16) {
           // radices above 16 (lines with odd factors): powers by a running
           // product, two twiddles live instead of R
@@ -587,7 +587,8 @@ struct Line {
 #pragma unroll
         for (int m = 0; m < R; ++m) v[b + m * B] = a[m];
       } else {
-        const int base = (j / NS) * NS * R + j % NS;
+        // (j / NS) NS R + j mod NS; a mask for a power-of-two NS
+        const int base = ((NS & (NS - 1)) == 0) ? (j & ~(NS - 1)) * R + (j & (NS - 1)) : (j / NS) * NS * R + j % NS;
 #pragma unroll
         for (int m = 0; m < R; ++m) lds[lidx(q, base + m * NS)] = a[m];
       }
@@ -772,7 +773,7 @@ struct Line {
     if constexpr (NS > 1) {
       static_assert(I < PSTG, "twiddled stages");
 #pragma unroll
-      for (int b = 0; b < B; ++b) p.pre[I][b] = pin(TW::get(tw, ((t + TPL * b) % NS) * TWS));
+      for (int b = 0; b < B; ++b) p.pre[I][b] = pin(TW::get(tw, ((t + TPL * b) & (NS - 1)) * TWS));
     }
     if constexpr (NS * R < S) plan_pre<NS * R, (NS > 1 ? I + 1 : I)>(p, t, tw);
   }
@@ -784,7 +785,7 @@ struct Line {
     constexpr int TWS = (N / (NS * R)) * (TWN / N);
     static_assert(I < PSTG, "twiddled stages");
 #pragma unroll
-    for (int b = 0; b < B; ++b) p.inv[I][b] = pin(TW::get(tw, ((t + TPL * b) % NS) * TWS));
+    for (int b = 0; b < B; ++b) p.inv[I][b] = pin(TW::get(tw, ((t + TPL * b) & (NS - 1)) * TWS));
     if constexpr (NS * R < N) plan_inv<NS * R, I + 1>(p, t, tw);
   }
   FWD static void plan(Plan& p, int t, const tw_t* __restrict__ tw) {

@@ -266,12 +266,6 @@ FWD bool previous_norm(const Args& a, int blk, double* red, double& nrm) {
   return true;
 }
 
-// The loads issued before this point stay ahead of those after it (a compiler
-// memory barrier; the hardware completes loads in order, so the first stage of
-// the FFT then waits for the state's loads only and the factor tables, sum
-// weights and partials still in flight overlap it).
-FWD void issue_order() { asm volatile("" ::: "memory"); }
-
 // The pass that follows the first reader of the `fresh` flag in a step (the
 // three-pass K pass, the fused VV pass) clears it, so the next step takes the
 // partials. At the end of the kernel: a flag load and store between the W
@@ -291,11 +285,16 @@ FWD void clear_fresh(const Args& a, int blk, int tid) {
 // (mode 1) on gfx950. The FP32 form keeps the full compensated table.
 #ifndef FW_SW_AHEAD
 // the theta-sum weights of a thread's items loaded with the pass's other loads
-// (registers: on where the column tiles leave one wave per SIMD)
-#define FW_SW_AHEAD (FW_NP <= 1024)
+// (FP64, where the column tiles leave one wave per SIMD; the FP32 pass kept
+// them in scratch memory)
+#define FW_SW_AHEAD (!FW_FP32 && FW_NP <= 1024)
 #endif
 #ifndef FW_TW_PLAN
 #define FW_TW_PLAN 1   // base twiddles of the spectral steps computed ahead (FP64)
+#endif
+#ifndef FW_TW_PLAN_COL
+// the column pass too, unless its tile already fills the registers (E >= 16)
+#define FW_TW_PLAN_COL (FW_TW_PLAN && FW_COL_E < 16)
 #endif
 #ifndef FW_TW2
 #define FW_TW2 (FW_FP32 ? 0 : 2)
@@ -571,7 +570,6 @@ FWD void vpass(const Args& a) {
   cpx v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = *reinterpret_cast<const cpx*>(W + (size_t)(t + TPL * e) * FW_NX + c);
-  issue_order();   // W's loads first: the FFT waits for them only, not for the tables
   // the previous step's S0 row (deferred norm), issued before the table loads
   const bool deferred = (FL & F_FIRST) && a.scal[SC_FRESH] == 0.0;
   NormRow<NT> norm_row;
@@ -686,7 +684,6 @@ FWD void vvpass(const Args& a) {
   cpx v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = *reinterpret_cast<const cpx*>(W + (size_t)(t + TPL * e) * FW_NX + c);
-  issue_order();   // W's loads first: the FFT waits for them only, not for the tables
   cpx2 ftab[E / 2 + 1];
   if constexpr (use_tab) {
     v_factors(a, c, t, ftab);
@@ -773,7 +770,6 @@ FWD void vvpass_theta(const Args& a) {
   cpx v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = *reinterpret_cast<const cpx*>(W + (size_t)(t + TPL * e) * FW_NX + c);
-  issue_order();   // W's loads first: the FFT waits for them only, not for the tables
   static_assert(!FW_V_SPLIT || FW_VTAB, "split potentials use the table of V0");
   cpx2 f0[E / 2 + 1], f1[E / 2 + 1];
   if constexpr (use_tab) {
@@ -800,7 +796,7 @@ FWD void vvpass_theta(const Args& a) {
 #endif
   // FP64 pair-fused steps: the base twiddles while the loads are in flight
   typename L::Plan plan;
-  if constexpr (L::PLANNED && FW_TW_PLAN) L::plan(plan, t, twl);
+  if constexpr (L::PLANNED && FW_TW_PLAN_COL) L::plan(plan, t, twl);
   tws.store(twl, tid);
   __syncthreads();
 
@@ -815,7 +811,7 @@ FWD void vvpass_theta(const Args& a) {
   g1 = 0.5 * ::dt * field_tau(a, a.step + 1) * ::dTheta;
 #endif
   VVMul mul{f0, f1, mask.ma, mask.mb, swp, t, {0.0, 0.0, 0.0, 0.0}, 0.0, 0.0, 0.0, g0, g1};
-  if constexpr (L::PLANNED && FW_TW_PLAN) spectral_step_planned<L>(v, lds, q, t, twl, mul, plan);
+  if constexpr (L::PLANNED && FW_TW_PLAN_COL) spectral_step_planned<L>(v, lds, q, t, twl, mul, plan);
   else spectral_step<L>(v, lds, q, t, twl, mul);
 #pragma unroll
   for (int e = 0; e < E; ++e) *reinterpret_cast<cpx*>(W + (size_t)(t + TPL * e) * FW_NX + c) = v[e];

@@ -230,10 +230,14 @@ class WignerMoyalHIP1D:
         return self
 
     def _times(self, steps):
-        times = np.empty(steps)
-        for s in range(steps):
-            self.t += self.dt       # the reference accumulates t += dt (wigner_moyal_cuda_1d.py:529)
-            times[s] = self.t
+        """t after each of the steps, accumulated as the reference does, t += dt
+        (wigner_moyal_cuda_1d.py:529): np.add.accumulate adds in order, so the
+        values are those of the sequential loop, bit for bit."""
+        a = np.empty(steps + 1)
+        a[0] = self.t
+        a[1:] = self.dt
+        times = np.add.accumulate(a)[1:]
+        self.t = float(times[-1])
         return times
 
     def _record(self, series):
@@ -242,12 +246,13 @@ class WignerMoyalHIP1D:
         if not self.isEhrenfest:
             return
         S0 = series[:, 0]
-        self.X_average.extend(series[:, 1] / S0)
-        self.X_average_RHS.extend(series[:, 5] / S0)
-        self.P_average.extend(series[:, 2] / S0)
-        self.P_average_RHS.extend(-series[:, 6] / S0)
-        self.hamiltonian_average.extend(series[:, 7] / S0)
-        self.purity_series.extend(2. * np.pi * series[:, 8] / (S0 * S0 * self.dXdP))
+        q = series[:, (1, 5, 2, 7)] / S0[:, np.newaxis]
+        self.X_average.extend(q[:, 0].tolist())
+        self.X_average_RHS.extend(q[:, 1].tolist())
+        self.P_average.extend(q[:, 2].tolist())
+        self.P_average_RHS.extend((-series[:, 6] / S0).tolist())
+        self.hamiltonian_average.extend(q[:, 3].tolist())
+        self.purity_series.extend((2. * np.pi * series[:, 8] / (S0 * S0 * self.dXdP)).tolist())
 
     def propagate(self, steps=1):
         """wigner_moyal_cuda_1d.py:521-540: ``steps`` x (t += dt, Strang step,
