@@ -228,6 +228,7 @@ PROTOTYPES = {
     "fw_wigner_kernel_times": [_P, _PD],
     "fw_wigner_kernel_bench": [_P, ctypes.c_double, _I, _PD],
     "fw_wigner_step_passes": [_P, _PI],
+    "fw_wigner_k_factors": [_P, _PI, _PD],
     "fw_rho_grid_constants": [ctypes.POINTER(RhoDesc), _PD],
     "fw_rho_check": [ctypes.POINTER(RhoDesc), ctypes.POINTER(RhoExprs)],
     "fw_rho_create": [ctypes.POINTER(RhoDesc), ctypes.POINTER(RhoExprs), _PP],

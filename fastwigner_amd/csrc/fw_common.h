@@ -502,6 +502,7 @@ struct Line {
   static constexpr bool POW2 = (N & (N - 1)) == 0;
   static_assert(N >= 2 && N % 2 == 0, "line length must be even");
   static_assert(E <= N && N % E == 0 && (POW2 || E % 2 == 0), "elements per thread must divide N (and be even)");
+  static constexpr int LN = N, LE = E;   // line length, elements per thread
   static constexpr int TPL = N / E;
   static constexpr int NT = P * TPL;
   static constexpr int RMAX = E < 16 ? E : 16;

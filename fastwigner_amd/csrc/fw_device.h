@@ -97,6 +97,7 @@ enum {
   F_FROM_PARTIALS = 256,  // apply_norm: norm from the last V pass's partials (end of a run)
   F_KNORM = 512,     // two-pass steps: the K pass applies the previous step's deferred norm
   F_NO_SCALE = 1024, // apply_norm: publish the sums only, W stays unnormalised (one bare step)
+  F_KGEN = 2048,     // K passes: factors generated as powers (kgen_factors) instead of the table
 };
 
 // two adjacent factors (columns c, c+1) in one load
@@ -427,6 +428,51 @@ struct VMul {
     zn = self_apply(zn, ftab[FW_COL_E / 2].a, ftab[FW_COL_E / 2].b, s);
   }
 };
+
+// Generated K factors. When K(P + L/2) - K(P - L/2) is linear in L (K
+// quadratic in P, as P^2/2 without a lambda-p mask) the factor of row P at
+// spectral index k is z^k, z = exp(i beta) with beta = phase(N/2) / (N/2).
+// The items of thread t of a pair-fused row line are k = t + m S (m < RL/2)
+// and k = (RL - m) S - t (m >= RL/2), and thread 0 holds S/2 + (m - RL/2) S
+// instead, so with a = z^t, h = z^(S/2) and w = h^2 = z^S every factor is a
+// or conj(a) or h times a power w^j, j <= RL/2 (two sincos per row, the rest
+// products; the Nyquist factor is w^(RL/2)). Used when these values match the
+// table on every grid point (kgen_check), in place of reading it.
+template <class L>
+FWD void kgen_factors(double P, double tau, int t, cpx (&f)[L::LE / 2 + 1]) {
+  static_assert(L::PAIRED && L::BS == 1, "one slot per thread of a pair-fused line");
+  constexpr int N = L::LN, RL = L::RL, S = L::S;
+  const double Lh = ::dLambda * (N / 2);
+  const double beta = -::dt * (::K(P + 0.5 * Lh, tau) - ::K(P - 0.5 * Lh, tau)) * (2.0 / N);
+  double hs, hc, as, ac;
+  sincos_fp64(beta * (0.5 * S), hs, hc);
+  sincos_fp64(beta * t, as, ac);
+  double wr[RL / 2 + 1], wi[RL / 2 + 1];
+  wr[0] = 1.0;
+  wi[0] = 0.0;
+  wr[1] = hc * hc - hs * hs;
+  wi[1] = 2.0 * hc * hs;
+#pragma unroll
+  for (int j = 2; j <= RL / 2; ++j) {
+    wr[j] = wr[j - 1] * wr[1] - wi[j - 1] * wi[1];
+    wi[j] = wr[j - 1] * wi[1] + wi[j - 1] * wr[1];
+  }
+  const bool t0 = (t == 0);
+#pragma unroll
+  for (int m = 0; m < RL; ++m) {
+    double br, bi;   // the factor multiplying w^j
+    int j;
+    if (m < RL / 2) {
+      br = ac; bi = as; j = m;
+    } else {
+      br = t0 ? hc : ac;
+      bi = t0 ? hs : -as;
+      j = t0 ? m - RL / 2 : RL - m;
+    }
+    f[m] = mk((real_t)(br * wr[j] - bi * wi[j]), (real_t)(br * wi[j] + bi * wr[j]));
+  }
+  f[L::LE / 2] = mk((real_t)wr[RL / 2], (real_t)wi[RL / 2]);
+}
 
 // Pair multiply of the row passes: rows a, b and their K factors.
 struct KMul {
@@ -911,7 +957,14 @@ FWD void kpass(const Args& a) {
   NormRow<NT> norm_row;
   if (deferred) norm_row.load(a.partials);
   cpx fta[E / 2 + 1], ftb[E / 2 + 1];
-  if constexpr (use_tab) {
+  constexpr bool can_gen = !(FL & F_BLOCH) && L::PAIRED && L::BS == 1;
+  if (can_gen && (a.flags & F_KGEN)) {
+    if constexpr (can_gen) {
+      const double Pa = ::dP * (r - 0.5 * ::P_gridDIM);
+      kgen_factors<L>(Pa, tau, t, fta);
+      kgen_factors<L>(::dP * (r + 1 - 0.5 * ::P_gridDIM), tau, t, ftb);
+    }
+  } else if constexpr (use_tab) {
 #pragma unroll
     for (int i = 0; i < E / 2; ++i) {
       const int k = spec_k<L>(t, i);
@@ -1069,6 +1122,42 @@ FWD void build_ktab(const Args& a, cpx* tab) {
   }
 }
 
+// Largest |generated - table| K factor over the grid (the rows of the K
+// pass's blocks, each thread its items), as the bits of a non-negative double
+// maxed into scal[SC_OUT] (cleared by the host); NaN counts as infinite. Lines
+// that are not pair-fused report infinity: their passes read the table.
+FWD void kgen_check(const Args& a) {
+  using L = KL;
+  constexpr int N = FW_NX, E = FW_ROW_E, P = FW_ROW_P, TPL = L::TPL, KW = FW_NX / 2 + 1;
+  double d = __builtin_inf();
+  if constexpr (L::PAIRED && L::BS == 1) {
+    const int t = threadIdx.x % TPL, q = threadIdx.x / TPL;
+    const int r = (blockIdx.x * P + q) * 2;
+    const double tnow = a.times[0], tau = tnow + FW_PHASE_SHIFT * ::dt;
+    d = 0.0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      cpx f[E / 2 + 1];
+      kgen_factors<L>(::dP * (r + h - 0.5 * ::P_gridDIM), tau, t, f);
+#pragma unroll
+      for (int i = 0; i <= E / 2; ++i) {
+        const int k = i < E / 2 ? spec_k<L>(t, i) : N / 2;
+        const cpx g = a.ktab[(size_t)(r + h) * KW + k];
+        const double e = fabs((double)f[i].x - (double)g.x) + fabs((double)f[i].y - (double)g.y);
+        d = (e <= d) ? d : (e == e ? e : __builtin_inf());
+      }
+    }
+  }
+  unsigned long long* out = reinterpret_cast<unsigned long long*>(a.scal + SC_OUT);
+  if constexpr (L::NT % 64 == 0) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) d = fmax(d, __shfl_xor(d, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(d));
+  } else {
+    atomicMax(out, (unsigned long long)__double_as_longlong(d));
+  }
+}
+
 // Minima of V(X-Th/2)+V(X+Th/2) over the theta-x grid and of K(P+L/2)+K(P-L/2)
 // over the p-lambda grid at t_initial (fill_V_K + gpuarray.min,
 // wigner_bloch_cuda_1d.py:72-102,362-445). One block; result in scal[SC_OUT..+1].
@@ -1135,6 +1224,7 @@ extern "C" __global__ void fw_set_scalar(double* p, double v) { *p = v; }
 extern "C" __global__ void __launch_bounds__(1024) fw_vk_min(fwk::Args a) { fwk::vk_min(a); }
 extern "C" __global__ void __launch_bounds__(256) fw_build_vtab(fwk::Args a, fwk::cpx* tab) { fwk::build_vtab(a, tab); }
 extern "C" __global__ void __launch_bounds__(256) fw_build_ktab(fwk::Args a, fwk::cpx* tab) { fwk::build_ktab(a, tab); }
+extern "C" __global__ void __launch_bounds__(fwk::KL::NT) fw_kgen_check(fwk::Args a) { fwk::kgen_check(a); }
 #elif FW_KSET == 2
 FW_VK(fw_bv_first, fwk::F_FIRST | fwk::F_BLOCH)
 FW_VK(fw_bv_last, fwk::F_LAST | fwk::F_BLOCH)

@@ -55,7 +55,7 @@ struct KArgs {
 
 enum { SC_NORM = 0, SC_STOP = 1, SC_FRESH = 2, SC_OUT = 8, NSCAL = 32 };
 enum { F_FIRST = 1, F_LAST = 2, F_BLOCH = 4, F_DISSIP = 8, F_MASK_XP = 16, F_BLOCH_MASK = 32, F_VTAB = 64,
-       F_KTAB = 128, F_FROM_PARTIALS = 256, F_KNORM = 512, F_NO_SCALE = 1024 };
+       F_KTAB = 128, F_FROM_PARTIALS = 256, F_KNORM = 512, F_NO_SCALE = 1024, F_KGEN = 2048 };
 
 }  // namespace fw
 

@@ -139,7 +139,10 @@ struct fw_wigner {
   void* ktab = nullptr;
   int tab_kind = -1;          // what the tables hold: 0 Moyal, 1 Bloch
   double tab_dbeta = 0.0;
-  hipFunction_t f_build_vtab = nullptr, f_build_ktab = nullptr, f_build_sumw = nullptr, f_begin = nullptr;
+  int kgen = -1;              // Moyal K factors generated in the K passes: -1 not decided, 0 no, 1 yes
+  double kgen_err = -1.0;     // largest |generated - table| factor over the grid (-1: not checked)
+  hipFunction_t f_build_vtab = nullptr, f_build_ktab = nullptr, f_build_sumw = nullptr, f_begin = nullptr,
+                f_kgen_check = nullptr;
   double* h_stage = nullptr;   // pinned: staged per-step sums (kStageChunks chunks + the closing slot)
   double* h_stop = nullptr;    // pinned: the purity guard's stop flag
 
@@ -147,6 +150,13 @@ struct fw_wigner {
   int gkind = -1;
   bool gdissip = false;   // the three-pass graph was captured with the dissipator kernels
   hipGraphExec_t gexec2 = nullptr;   // two-pass Moyal steps
+  // two-pass runs' partial chunks (key: steps | first << 8 | last << 9 | unnormalised << 10),
+  // captured on a key's second use
+  struct ShortGraph {
+    int uses = 0;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::map<int, ShortGraph> sgraphs;
   int last_passes = 3;
 
   std::map<std::string, ExprMod> init_mods, avg_mods;
@@ -593,6 +603,27 @@ struct SeriesSink {
   }
 };
 
+// With phase_tables 0 the FP64 K passes generate the Moyal K factors as
+// powers (kgen_factors) when those agree with the freshly built table to 1e-11
+// on every grid point (K(P + L/2) - K(P - L/2) linear in L, no lambda-p mask;
+// for P^2/2 the two differ by the rounding of the phases, below 1e-13 up to
+// 2048^2). The pass then reads W alone. Otherwise, with phase_tables 1, and in
+// FP32 (whose row pass the two FP64 sincos per row slow down: 2048^2 17.1 ->
+// 21.9 us) they read the table.
+int decide_kgen(fw_wigner* h, KArgs b) {
+  const double tol = 1e-11;
+  h->kgen = 0;
+  if (h->d.phase_tables != 0 || h->fp32) return FW_OK;
+  HIPCHK(hipMemsetAsync(h->scal + fw::SC_OUT, 0, sizeof(double), h->stream));
+  FWCHK(launch(h, h->f_kgen_check, h->row_nb, h->row_nt, b));
+  double err = -1.0;
+  HIPCHK(hipMemcpyAsync(&err, h->scal + fw::SC_OUT, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->kgen_err = err;
+  h->kgen = (err >= 0.0 && err <= tol) ? 1 : 0;
+  return FW_OK;
+}
+
 // Fill the factor tables for a run of `kind` (0 Moyal, 1 Bloch with a.dbeta)
 // and set the table flags in a. Tables are allocated once; contents follow kind.
 int ensure_tables(fw_wigner* h, int kind, KArgs& a) {
@@ -620,9 +651,11 @@ int ensure_tables(fw_wigner* h, int kind, KArgs& a) {
     }
     h->tab_kind = kind;
     h->tab_dbeta = a.dbeta;
+    if (kind == 0 && need_k && h->kgen < 0) FWCHK(decide_kgen(h, b));
   }
   if (need_v) a.flags |= fw::F_VTAB;
   if (need_k) a.flags |= fw::F_KTAB;
+  if (kind == 0 && need_k && h->kgen == 1) a.flags |= fw::F_KGEN;
   return FW_OK;
 }
 
@@ -768,9 +801,55 @@ int ensure_graph_fused(fw_wigner* h, const KArgs& base) {
   return FW_OK;
 }
 
-// graphs: replay the handle's captured 64-step graph for whole chunks (its
-// kernel arguments name the handle's own buffers, so a run on other buffers
-// launches every kernel itself)
+// The closing normalisation of a two-pass run: the norm and the last step's
+// sums from the partials of the closing V/2 (into series slot kChunk).
+int enqueue_close(fw_wigner* h, const KArgs& base, bool normalise) {
+  KArgs a = base;
+  a.flags |= fw::F_FROM_PARTIALS | (normalise ? 0 : fw::F_NO_SCALE);
+  a.step = kChunk;
+  return launch(h, h->f_apply_norm, 1024, 256, a);
+}
+
+// The kernels of a partial chunk of n < kChunk steps, or of a run's last
+// chunk: [opening V/2 when first] n x (K pass, VV pass or the closing V/2 at
+// the run's end) [closing normalisation when last], captured as one graph on
+// the second use of the same shape, so that repeated short runs
+// (propagate(50) per frame, wigner_moyal_cuda_1d.py:854) issue one launch
+// instead of 2n + 2. *out stays null until then.
+int short_graph(fw_wigner* h, const KArgs& base, int n, bool first, bool last, bool normalise,
+                hipGraphExec_t* out) {
+  *out = nullptr;
+  const int key = n | (first ? 1 << 8 : 0) | (last ? 1 << 9 : 0) | (normalise ? 0 : 1 << 10);
+  fw_wigner::ShortGraph& sg = h->sgraphs[key];
+  if (!sg.exec && ++sg.uses >= 2) {
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    int rc = FW_OK;
+    if (first) {
+      KArgs a = base;
+      a.step = 0;
+      a.flags |= fw::F_FIRST;
+      rc = launch(h, h->f_v_first, h->col_nb, h->col_nt, a);
+    }
+    for (int s = 0; s < n && rc == FW_OK; ++s) rc = enqueue_fused(h, s, last && s == n - 1, base, nullptr);
+    if (last && rc == FW_OK) rc = enqueue_close(h, base, normalise);
+    hipError_t e = hipStreamEndCapture(h->stream, &g);
+    if (rc != FW_OK) return rc;
+    if (e != hipSuccess) return fw::fail(FW_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&sg.exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      sg.exec = nullptr;
+      return fw::fail(FW_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+  }
+  *out = sg.exec;
+  return FW_OK;
+}
+
+// graphs: replay the handle's captured 64-step graph for whole chunks and the
+// short-chunk graphs for the rest (their kernel arguments name the handle's
+// own buffers, so a run on other buffers launches every kernel itself)
 int run_steps_fused(fw_wigner* h, const KArgs& base0, const double* times, int steps, double* series,
                     bool graphs = true, bool normalise = true) {
   KArgs base = base0;
@@ -782,18 +861,24 @@ int run_steps_fused(fw_wigner* h, const KArgs& base0, const double* times, int s
   std::vector<hipEvent_t> evs;
   double acc[2] = {0, 0};
   int nacc = 0;
+  bool closed = false;   // the closing normalisation ran inside a short-chunk graph
   for (int off = 0; off < steps; off += kChunk) {
     const int n = std::min(kChunk, steps - off);
     const int nt = std::min(kChunk + 1, steps - off);   // t of steps off .. off+n (the next chunk's first)
+    const bool has_last = off + n == steps;
     FWCHK(begin_chunk(h, base, times + off, nt, off == 0));
-    if (off == 0) {
+    hipGraphExec_t sg = nullptr;
+    if (graphs && (has_last || n < kChunk)) FWCHK(short_graph(h, base, n, off == 0, has_last, normalise, &sg));
+    if (off == 0 && !sg) {
       KArgs a = base;
       a.step = 0;
       a.flags |= fw::F_FIRST;
       FWCHK(launch(h, h->f_v_first, h->col_nb, h->col_nt, a));
     }
-    const bool has_last = off + n == steps;
-    if (graphs && !has_last && n == kChunk) {
+    if (sg) {
+      HIPCHK(hipGraphLaunch(sg, h->stream));
+      closed = has_last;
+    } else if (graphs && !has_last && n == kChunk) {
       HIPCHK(hipGraphLaunch(h->gexec2, h->stream));
     } else if (timing) {
       evs.resize(3 * n);
@@ -821,10 +906,7 @@ int run_steps_fused(fw_wigner* h, const KArgs& base0, const double* times, int s
     h->ktimes[1] = acc[1] / nacc;
     h->ktimes[2] = 0.0;
   }
-  KArgs a = base;
-  a.flags |= fw::F_FROM_PARTIALS | (normalise ? 0 : fw::F_NO_SCALE);
-  a.step = kChunk;
-  FWCHK(launch(h, h->f_apply_norm, 1024, 256, a));
+  if (!closed) FWCHK(enqueue_close(h, base, normalise));
   FWCHK(sink.final_slot(steps - 1));
   HIPCHK(hipStreamSynchronize(h->stream));
   sink.finish();
@@ -947,7 +1029,7 @@ extern "C" int fw_wigner_create(const fw_wigner_desc* desc, const fw_wigner_expr
              {"fw_vk_min", &h->f_vk_min},   {"fw_build_vtab", &h->f_build_vtab},
              {"fw_build_ktab", &h->f_build_ktab}, {"fw_vv", &h->f_vv},
              {"fw_k_norm", &h->f_k_norm},     {"fw_build_sumw", &h->f_build_sumw},
-             {"fw_begin", &h->f_begin}};
+             {"fw_begin", &h->f_begin},       {"fw_kgen_check", &h->f_kgen_check}};
   for (auto& f : fns)
     if ((rc = getfn(h->mod, f.n, f.f)) != FW_OK) return bail(rc);
 
@@ -1000,6 +1082,8 @@ extern "C" int fw_wigner_destroy(fw_wigner* h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
   if (h->gexec2) (void)hipGraphExecDestroy(h->gexec2);
+  for (auto& kv : h->sgraphs)
+    if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
   for (auto* m : {&h->init_mods, &h->avg_mods})
     for (auto& kv : *m)
       if (kv.second.mod) (void)hipModuleUnload(kv.second.mod);
@@ -1297,6 +1381,13 @@ extern "C" int fw_wigner_set_timing(fw_wigner* h, int enable) {
 extern "C" int fw_wigner_step_passes(fw_wigner* h, int* passes) {
   if (!h || !passes) return fw::fail(FW_ERR_INVALID_ARG, "NULL argument");
   *passes = h->last_passes;
+  return FW_OK;
+}
+
+extern "C" int fw_wigner_k_factors(fw_wigner* h, int* generated, double* max_dev) {
+  if (!h || !generated || !max_dev) return fw::fail(FW_ERR_INVALID_ARG, "NULL argument");
+  *generated = h->kgen == 1 ? 1 : 0;
+  *max_dev = h->kgen_err;
   return FW_OK;
 }
 

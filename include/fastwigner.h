@@ -197,6 +197,14 @@ int fw_wigner_kernel_times(fw_wigner* h, double us_out[3]);
 int fw_wigner_kernel_bench(fw_wigner* h, double t, int reps, double us_out[2]);
 /* Passes per step of the last Moyal propagate call (2 or 3). */
 int fw_wigner_step_passes(fw_wigner* h, int* passes);
+/* How the Moyal K passes obtain their factors exp(-i dt (K(P + L/2) - K(P - L/2))) (the
+ * expK kernels, wigner_moyal_cuda_1d.py:592-622): *generated = 1 when they form them as
+ * powers z^k of one phase per row (FP64, phase_tables 0, power-of-two rows of at least
+ * four elements per thread, and K(P + L/2) - K(P - L/2) linear in L as for P^2/2: the
+ * powers match the table to 1e-11 on every grid point, checked at creation), 0 when they
+ * read the table or evaluate each factor; *max_dev = that check's largest
+ * |generated - table| (-1 if not run). No counterpart in the reference (introspection). */
+int fw_wigner_k_factors(fw_wigner* h, int* generated, double* max_dev);
 
 /* ------------------------------------------------------------------------ */
 /* Density-matrix propagators (von Neumann, Bloch)                           */

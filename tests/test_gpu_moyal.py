@@ -50,6 +50,77 @@ def gaussian(o, x0=1.0, p0=0.5):
     return np.exp(-(o.X - x0) ** 2 - (o.P - p0) ** 2) / np.pi
 
 
+def k_factors(g):
+    """(generated, max |generated - table|) of fw_wigner_k_factors."""
+    import ctypes
+    from fastwigner_amd import _lib
+    gen, dev = ctypes.c_int(), ctypes.c_double()
+    _lib.call("fw_wigner_k_factors", g._handle, ctypes.byref(gen), ctypes.byref(dev))
+    return gen.value, dev.value
+
+
+def test_generated_k_factors(gpu):
+    """K = P^2/2 in FP64: the K passes form their factors as powers of one phase per row
+    (checked against the table at creation) and 500 steps match the oracle at the parity
+    bar and the table-reading run (phase_tables = 1) to 1e-13, both passes per step
+    forms; a K with a P^4 term, a lambda-p mask and the FP32 form read the table."""
+    for passes in (0, 3):
+        g, o = make_pair(512, 8., 512, 8., 0.01, V_Q, K_Q, DV_Q, DK_Q, step_passes=passes)
+        gt, _ = make_pair(512, 8., 512, 8., 0.01, V_Q, K_Q, DV_Q, DK_Q, step_passes=passes, phase_tables=1)
+        gen, dev = k_factors(g)
+        print("passes %d: generated %d, max |generated - table| %.2e" % (passes or 2, gen, dev))
+        assert gen == 1 and 0 <= dev <= 1e-13
+        assert k_factors(gt) == (0, -1.0)
+        W0 = gaussian(o)
+        for s in (g, gt, o):
+            s.set_wignerfunction(W0)
+        Wg = g.propagate(500).get()
+        Wt = gt.propagate(500).get()
+        o.propagate(500)
+        e_o, e_t = rel_l2(Wg, o.W), rel_l2(Wg, Wt)
+        print("  rel L2 against the oracle %.2e, against the table run %.2e" % (e_o, e_t))
+        assert e_o <= 1e-12 and e_t <= 1e-13
+        assert np.allclose(g.P_average, gt.P_average, rtol=0, atol=1e-13)
+    gq, _ = make_pair(64, 8., 64, 8., 0.01, V_Q, "0.5*P*P+0.01*P*P*P*P")
+    gen, dev = k_factors(gq)
+    assert gen == 0 and dev > 1e-11
+    gm, _ = make_pair(64, 8., 64, 8., 0.01, V_Q, K_Q,
+                      masks={"abs_boundary_lambda_p": "exp(-0.001*Lambda*Lambda)"})
+    gen, dev = k_factors(gm)
+    assert gen == 0 and dev > 1e-11
+    g32, _ = make_pair(256, 8., 256, 8., 0.01, V_Q, K_Q, precision="fp32")
+    assert k_factors(g32) == (0, -1.0)
+
+
+def test_short_run_graphs_match_launches(gpu):
+    """Repeated runs of one shape replay a captured graph from the second call on
+    (opening V/2, the steps, the closing normalisation); W and every series value equal
+    those of the same calls launched kernel by kernel (timing on: no graphs), bit for
+    bit, over short, multi-chunk and one-step runs, and the whole sequence matches the
+    oracle stepped through it at the parity bar."""
+    from fastwigner_amd import _lib
+    g, o = make_pair(256, 8., 256, 8., 0.01, V_Q, K_Q, DV_Q, DK_Q)
+    h, _ = make_pair(256, 8., 256, 8., 0.01, V_Q, K_Q, DV_Q, DK_Q)
+    _lib.call("fw_wigner_set_timing", h._handle, 1)
+    W0 = gaussian(o)
+    for s in (g, h, o):
+        s.set_wignerfunction(W0)
+    for n in (20, 20, 20, 7, 100, 100, 64, 64, 1, 1, 20):
+        Wg = g.propagate(n).get()
+        Wh = h.propagate(n).get()
+        assert np.array_equal(Wg, Wh), n
+        o.propagate(n)
+    for name in ("X_average", "X_average_RHS", "P_average", "P_average_RHS", "hamiltonian_average",
+                 "purity_series"):
+        assert getattr(g, name) == getattr(h, name), name
+    err = rel_l2(Wg, o.W)
+    print("after 417 steps in 11 calls: rel L2 against the oracle %.2e" % err)
+    assert err <= 1e-12
+    op = np.array([x["P_average"] for x in o.series])
+    assert len(g.P_average) == len(op) == 417
+    assert np.max(np.abs(np.array(g.P_average) - op)) <= 1e-11
+
+
 @pytest.mark.parametrize("tables,passes", [(0, 0), (-1, 0), (0, 3), (-1, 3)])
 def test_quartic_256_parity_fp64(gpu, tables, passes):
     """Quartic Moyal workload: 256^2, x,p in [-8,8), V=0.5x^2+0.1x^4, dt=0.01, 1000 steps,
