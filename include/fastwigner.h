@@ -77,7 +77,9 @@ typedef struct fw_wigner_desc {
   int row_e, row_p;        /* K pass: elements per thread, row pairs per block (0 = default) */
   int phase_tables;        /* cache the V/K factors in tables (as the FFTW class caches _expV/_expK,
                               wigner_moyal_fftw_1d.py:164-213): 0 = when the V (K) string and its
-                              mask do not contain the token t, 1 = always, -1 = never */
+                              mask do not contain the token t (the FP64 K passes then generate
+                              linear-in-Lambda K factors instead of reading the table, see
+                              fw_wigner_k_factors), 1 = always, -1 = never */
   int step_passes;         /* Moyal steps without the dissipator: 0 or 2 = two passes per step (the
                               closing V/2 of a step fused with the opening V/2 of the next), 3 = three */
   int v_table;             /* 0 = the V factors follow phase_tables; -1 = evaluate them in the column
