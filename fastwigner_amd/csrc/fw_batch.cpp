@@ -195,6 +195,10 @@ std::string make_prelude(const fw_batch_desc* d, const fw_batch_instance* in, co
   s << "#define FW_REAL " << (fp32 ? "float" : "double") << "\n";
   s << "#define FW_FP32 " << (fp32 ? 1 : 0) << "\n";
   s << "#define FW_PHASE_SHIFT (" << fw::fmt_double(d->phase_time_shift) << ")\n";
+  // plain (write-back) stores of W: the batch's grids exceed the caches, its
+  // passes are long against their boundaries, and device-scope stores made the
+  // 256-instance FP32 step slower (842 -> 1105 us)
+  s << "#define FW_W_STORE 0\n";
   s << "#define FW_B_NMAX " << nmax << "\n";
   int emax = 2;
   for (int k = 0; k < kNSizes; ++k) {

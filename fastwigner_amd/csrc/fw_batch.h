@@ -235,7 +235,7 @@ FWD void b_vpass(const BArgs& a, int i, int tile, int tid, cpx* lds, const tw_t*
       sums[7] += ::K(Pn, tnow, prm) * ws;
       sums[8] += wa * wa + wb * wb;
     }
-    *reinterpret_cast<cpx*>(W + idx) = w;
+    st_w(reinterpret_cast<cpx*>(W + idx), w);
   }
   if constexpr (LAST) {
     sums[0] = ca + cb;
@@ -305,8 +305,8 @@ FWD void b_kpass(const BArgs& a, int i, int tile, int tid, cpx* lds, const tw_t*
   spectral_step<L>(v, lds, q, t, twl, mul);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    Wa[t + TPL * e] = v[e].x;
-    Wb[t + TPL * e] = v[e].y;
+    st_w(&Wa[t + TPL * e], v[e].x);
+    st_w(&Wb[t + TPL * e], v[e].y);
   }
   // the previous step's sums to series[step]: tile k adds rows k, k + ntile, ...
   // (the last V pass, which overwrites the partials, runs after this pass)
@@ -397,7 +397,7 @@ FWD void b_vvpass(const BArgs& a, int i, int tile, int tid, cpx* lds, const tw_t
   BMul<N, E> mul1{fa, fb};
   spectral_step<L>(v, lds, q, t2, twl, mul1);
 #pragma unroll
-  for (int e = 0; e < E; ++e) *reinterpret_cast<cpx*>(W + (size_t)(t + TPL * e) * N + c) = v[e];
+  for (int e = 0; e < E; ++e) st_w(reinterpret_cast<cpx*>(W + (size_t)(t + TPL * e) * N + c), v[e]);
 }
 
 // ---------------------------------------------------------------------------
@@ -497,7 +497,7 @@ FWD void b_vvpass_theta(const BArgs& a, int i, int tile, int tid, cpx* lds, cons
                       0.0, 0.0, 0.0};
   spectral_step<L>(v, lds, q, t, twl, mul);
 #pragma unroll
-  for (int e = 0; e < E; ++e) *reinterpret_cast<cpx*>(W + (size_t)(t + TPL * e) * N + c) = v[e];
+  for (int e = 0; e < E; ++e) st_w(reinterpret_cast<cpx*>(W + (size_t)(t + TPL * e) * N + c), v[e]);
   // the nine sums of step s (column sums N alpha_0 on thread 0 of each line)
   const double Xa = dX * (c - 0.5 * N), Xb = dX * (c + 1 - 0.5 * N);
   const double ca = (double)N * mul.a0, cb = (double)N * mul.b0;

@@ -679,7 +679,7 @@ FWD void vpass(const Args& a) {
       sums[7] += ::K(Pn, tnow) * ws;
       sums[8] += wa * wa + wb * wb;
     }
-    *reinterpret_cast<cpx*>(W + idx) = w;
+    st_w(reinterpret_cast<cpx*>(W + idx), w);
   }
   if constexpr (FL & F_LAST) {
     sums[0] = ca + cb;
@@ -787,7 +787,7 @@ FWD void vvpass(const Args& a) {
   VMul mul1{ftab};
   spectral_step<L>(v, lds, q, t2, twl, mul1);
 #pragma unroll
-  for (int e = 0; e < E; ++e) *reinterpret_cast<cpx*>(W + (size_t)(t + TPL * e) * FW_NX + c) = v[e];
+  for (int e = 0; e < E; ++e) st_w(reinterpret_cast<cpx*>(W + (size_t)(t + TPL * e) * FW_NX + c), v[e]);
   clear_fresh(a, blk, tid);
 }
 
@@ -860,7 +860,7 @@ FWD void vvpass_theta(const Args& a) {
   if constexpr (L::PLANNED && FW_TW_PLAN_COL) spectral_step_planned<L>(v, lds, q, t, twl, mul, plan);
   else spectral_step<L>(v, lds, q, t, twl, mul);
 #pragma unroll
-  for (int e = 0; e < E; ++e) *reinterpret_cast<cpx*>(W + (size_t)(t + TPL * e) * FW_NX + c) = v[e];
+  for (int e = 0; e < E; ++e) st_w(reinterpret_cast<cpx*>(W + (size_t)(t + TPL * e) * FW_NX + c), v[e]);
 
   // the nine sums of step s (column sums N alpha_0 on thread 0 of each line)
   const double ca = (double)N * mul.a0, cb = (double)N * mul.b0;
@@ -1002,8 +1002,8 @@ FWD void kpass(const Args& a) {
   else spectral_step<L>(v, lds, q, t, twl, mul);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    Wa[t + TPL * e] = v[e].x;
-    Wb[t + TPL * e] = v[e].y;
+    st_w(&Wa[t + TPL * e], v[e].x);
+    st_w(&Wb[t + TPL * e], v[e].y);
   }
   // the previous step's sums to the series, after this block's own work
   if constexpr (FL & F_KNORM)
@@ -1096,7 +1096,7 @@ FWD void apply_norm(const Args& a) {
   if (a.flags & F_NO_SCALE) return;
   const real_t cr = (real_t)c;
   const size_t n = (size_t)FW_NX * FW_NP;
-  for (size_t i = (size_t)blockIdx.x * HT + threadIdx.x; i < n; i += (size_t)gridDim.x * HT) a.W[i] *= cr;
+  for (size_t i = (size_t)blockIdx.x * HT + threadIdx.x; i < n; i += (size_t)gridDim.x * HT) st_w(&a.W[i], a.W[i] * cr);
 }
 
 // Cached factor tables for time-independent V/K/masks (the reference caches

@@ -193,7 +193,7 @@ FWD void rho_row(const RArgs& a) {
     }
     L::template fwd<false>(v, lds, q, t, twl);
 #pragma unroll
-    for (int e = 0; e < E; ++e) row[t + TPL * e] = v[e];
+    for (int e = 0; e < E; ++e) st_w(&row[t + TPL * e], v[e]);
     // the previous step's sums, after this block's own work
     if (deferred) rho_publish<NT>(a, blk, nb, red, false);
   } else {
@@ -211,7 +211,7 @@ FWD void rho_row(const RArgs& a) {
       if (!(FL & RF_BLOCH)) f = cscale(f, (real_t)::fw_mask_x(xcoord(j), Xi, tnow));
 #endif
       cpx w = cmul(cscale(v[e], sN), f);
-      row[j] = w;
+      st_w(&row[j], w);
       const double wr = (double)w.x, wi = (double)w.y;
       s[R_PUR] += wr * wr + wi * wi;
       if (j == i) {
@@ -283,7 +283,7 @@ FWD void rho_col(const RArgs& a) {
   asm volatile("" : "+v"(t2));
   L::inv(v, lds, q, t2, twl);
 #pragma unroll
-  for (int e = 0; e < E; ++e) a.rho[(size_t)(t + TPL * e) * N + j] = v[e];
+  for (int e = 0; e < E; ++e) st_w(&a.rho[(size_t)(t + TPL * e) * N + j], v[e]);
 }
 
 // ---------------------------------------------------------------------------
@@ -488,7 +488,7 @@ FWD void rho_apply_norm(const RArgs& a, int from_partials) {
   if (from_partials == 2) return;
   const size_t n = (size_t)FW_N * FW_N;
   for (size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x; idx < n; idx += (size_t)gridDim.x * 256)
-    a.rho[idx] = cmul(a.rho[idx], c);
+    st_w(&a.rho[idx], cmul(a.rho[idx], c));
 }
 
 // Sums over the current rho (trace family + purity) into scal[8..8+R_PUR];
