@@ -658,6 +658,13 @@ FWD void vpass(const Args& a) {
   for (int i = 0; i < NSER; ++i) sums[i] = 0.0;
   double ca = 0.0, cb = 0.0;
   const MaskXP mask((FL & F_LAST) && (a.flags & F_MASK_XP), Xa, Xb, tnow);
+  // the dissipator's source terms, loaded ahead of the stores of W (a load
+  // after a store through asm cannot be moved above it)
+  cpx g0s[(FL & F_LAST) && (FL & F_DISSIP) ? E : 1];
+  if constexpr ((FL & F_LAST) && (FL & F_DISSIP)) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) g0s[e] = *reinterpret_cast<const cpx*>(a.W0s + (size_t)(t + TPL * e) * FW_NX + c);
+  }
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int n = t + TPL * e;
@@ -667,7 +674,7 @@ FWD void vpass(const Args& a) {
       const double Pn = ::dP * (n - 0.5 * ::P_gridDIM);
       mask.apply(w, Pn);
       if constexpr (FL & F_DISSIP) {
-        const cpx g = *reinterpret_cast<const cpx*>(a.W0s + idx);
+        const cpx g = g0s[e];
         w.x = w.x * (real_t)a.w_coeff + g.x;
         w.y = w.y * (real_t)a.w_coeff + g.y;
       }
@@ -1095,8 +1102,17 @@ FWD void apply_norm(const Args& a) {
   }
   if (a.flags & F_NO_SCALE) return;
   const real_t cr = (real_t)c;
-  const size_t n = (size_t)FW_NX * FW_NP;
-  for (size_t i = (size_t)blockIdx.x * HT + threadIdx.x; i < n; i += (size_t)gridDim.x * HT) st_w(&a.W[i], a.W[i] * cr);
+  const size_t n = (size_t)FW_NX * FW_NP, stride = (size_t)gridDim.x * HT;
+  // eight loads, then eight stores: a load after a store through asm cannot
+  // be moved above it, so interleaved they would wait one latency each
+  for (size_t i0 = (size_t)blockIdx.x * HT + threadIdx.x; i0 < n; i0 += 8 * stride) {
+    real_t w[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) w[j] = (i0 + j * stride < n) ? a.W[i0 + j * stride] : (real_t)0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (i0 + j * stride < n) st_w(&a.W[i0 + j * stride], w[j] * cr);
+  }
 }
 
 // Cached factor tables for time-independent V/K/masks (the reference caches

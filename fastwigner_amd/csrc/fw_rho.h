@@ -486,9 +486,17 @@ FWD void rho_apply_norm(const RArgs& a, int from_partials) {
     c = mk((real_t)a.scal[0], (real_t)a.scal[3]);
   }
   if (from_partials == 2) return;
-  const size_t n = (size_t)FW_N * FW_N;
-  for (size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x; idx < n; idx += (size_t)gridDim.x * 256)
-    st_w(&a.rho[idx], cmul(a.rho[idx], c));
+  const size_t n = (size_t)FW_N * FW_N, stride = (size_t)gridDim.x * 256;
+  // four loads, then four stores (a load after a store through asm cannot be
+  // moved above it)
+  for (size_t i0 = (size_t)blockIdx.x * 256 + threadIdx.x; i0 < n; i0 += 4 * stride) {
+    cpx w[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) w[j] = (i0 + j * stride < n) ? a.rho[i0 + j * stride] : mk((real_t)0, (real_t)0);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (i0 + j * stride < n) st_w(&a.rho[i0 + j * stride], cmul(w[j], c));
+  }
 }
 
 // Sums over the current rho (trace family + purity) into scal[8..8+R_PUR];

@@ -558,6 +558,7 @@ struct SeriesSink {
   Rec recs[kStageChunks];
   int nrec = 0;
   int last = -1;
+  int last_rec = -1;   // the record whose slot kChunk holds the last step's sums (final_with_chunk)
   SeriesSink(fw_wigner* h_, const double* dser_, double* out_) : h(h_), dser(dser_), out(out_) {}
   int chunk(int off, int n) {
     if (!out) return FW_OK;
@@ -566,6 +567,19 @@ struct SeriesSink {
     HIPCHK(hipMemcpyAsync(h->h_stage + (size_t)k * (kChunk + 1) * FW_NSER, dser, sizeof(double) * FW_NSER * n,
                           hipMemcpyDeviceToHost, h->stream));
     recs[nrec++] = {off, n, k};
+    return FW_OK;
+  }
+  // the run's last chunk and the closing slot kChunk in one copy, after the
+  // closing normalisation has published the last step's sums
+  int final_with_chunk(int off, int n, int step) {
+    if (!out) return FW_OK;
+    if (nrec == kStageChunks) FWCHK(flush());
+    const int k = nrec;
+    HIPCHK(hipMemcpyAsync(h->h_stage + (size_t)k * (kChunk + 1) * FW_NSER, dser, sizeof(double) * FW_NSER * (kChunk + 1),
+                          hipMemcpyDeviceToHost, h->stream));
+    recs[nrec++] = {off, n, k};
+    last = step;
+    last_rec = k;
     return FW_OK;
   }
   int final_slot(int step) {
@@ -596,10 +610,11 @@ struct SeriesSink {
   // after the stream is synchronised
   void finish() {
     if (!out) return;
+    // the closing slot: in the last chunk's record (final_with_chunk) or in the area after the records
+    const double* fin = last_rec >= 0 ? h->h_stage + ((size_t)last_rec * (kChunk + 1) + kChunk) * FW_NSER
+                                      : h->h_stage + (size_t)kStageChunks * (kChunk + 1) * FW_NSER;
     scatter();
-    if (last >= 0)
-      std::memcpy(out + (size_t)last * FW_NSER, h->h_stage + (size_t)kStageChunks * (kChunk + 1) * FW_NSER,
-                  sizeof(double) * FW_NSER);
+    if (last >= 0) std::memcpy(out + (size_t)last * FW_NSER, fin, sizeof(double) * FW_NSER);
   }
 };
 
@@ -899,15 +914,18 @@ int run_steps_fused(fw_wigner* h, const KArgs& base0, const double* times, int s
     } else {
       for (int s = 0; s < n; ++s) FWCHK(enqueue_fused(h, s, has_last && s == n - 1, base, nullptr));
     }
-    FWCHK(sink.chunk(off, n));
+    if (has_last) {
+      if (!closed) FWCHK(enqueue_close(h, base, normalise));
+      FWCHK(sink.final_with_chunk(off, n, steps - 1));
+    } else {
+      FWCHK(sink.chunk(off, n));
+    }
   }
   if (nacc) {
     h->ktimes[0] = acc[0] / nacc;
     h->ktimes[1] = acc[1] / nacc;
     h->ktimes[2] = 0.0;
   }
-  if (!closed) FWCHK(enqueue_close(h, base, normalise));
-  FWCHK(sink.final_slot(steps - 1));
   HIPCHK(hipStreamSynchronize(h->stream));
   sink.finish();
   h->last_passes = 2;
