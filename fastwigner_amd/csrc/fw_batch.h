@@ -820,7 +820,17 @@ FWD void b_vkmin(const BArgs& a, double* out) {
 
 }  // namespace fwk
 
-extern "C" __global__ void __launch_bounds__(fwk::BT) fw_batch_step(fwk::BArgs a) {
+// FW_B_WAVES: resident waves per SIMD the step kernel's registers must allow
+// (0: the compiler's choice)
+#ifndef FW_B_WAVES
+#define FW_B_WAVES 4
+#endif
+#if FW_B_WAVES
+#define FW_B_OCC __attribute__((amdgpu_waves_per_eu(FW_B_WAVES)))
+#else
+#define FW_B_OCC
+#endif
+extern "C" __global__ void __launch_bounds__(fwk::BT) FW_B_OCC fw_batch_step(fwk::BArgs a) {
   fwk::b_step<fwk::BK_MOYAL>(a);
 }
 extern "C" __global__ void __launch_bounds__(fwk::BT) fw_batch_bloch_step(fwk::BArgs a) {
