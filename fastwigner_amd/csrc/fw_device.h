@@ -322,35 +322,72 @@ FWD void v_factors(const Args& a, int c, int t, cpx2 (&ftab)[FW_COL_E / 2 + 1]) 
   ftab[E / 2] = *reinterpret_cast<const cpx2*>(a.vtab + vtab_index(c, N / 2));
 }
 
+// The field phases exp(+i g k) of split potentials on a pair-fused column
+// line in FP64, as powers: item m of thread t sits at k = t + m S (m < RL/2),
+// (RL - m) S - t (m >= RL/2), or for thread 0 at S/2 + (m - RL/2) S, so with
+// u = exp(i g t), h = exp(i g S/2) and w = h^2 every item is u, conj(u) or h
+// times w^j, j <= RL/2, and the Nyquist entry is w^(RL/2): two sincos per
+// half-step instead of one per item.
+constexpr bool FIELD_POW = FW_V_SPLIT && !FW_FP32 && VL::PAIRED && VL::BS == 1;
+struct FieldPow {
+  double ur = 1.0, ui = 0.0, hr = 1.0, hi = 0.0, wr = 1.0, wi = 0.0;
+  FWD void init(double g, int t) {
+    sincos_fp64(g * t, ui, ur);
+    sincos_fp64(g * (0.5 * VL::S), hi, hr);
+    wr = hr * hr - hi * hi;
+    wi = 2.0 * hr * hi;
+  }
+  FWD cpx item(int t, int m) const {
+    constexpr int RL = VL::RL;
+    double pr, pi;
+    int j;
+    if (m < RL / 2) {
+      pr = ur; pi = ui; j = m;
+    } else {
+      const bool z = (t == 0);
+      pr = z ? hr : ur;
+      pi = z ? hi : -ui;
+      j = z ? m - RL / 2 : RL - m;
+    }
+#pragma unroll
+    for (int q = 0; q < RL / 2; ++q)
+      if (q < j) {
+        const double x = pr * wr - pi * wi;
+        pi = pr * wi + pi * wr;
+        pr = x;
+      }
+    return mk((real_t)pr, (real_t)pi);
+  }
+  FWD cpx nyquist() const {
+    double pr = 1.0, pi = 0.0;
+#pragma unroll
+    for (int q = 0; q < VL::RL / 2; ++q) {
+      const double x = pr * wr - pi * wi;
+      pi = pr * wi + pi * wr;
+      pr = x;
+    }
+    return mk((real_t)pr, (real_t)pi);
+  }
+};
+
 // The field factor exp(+i dt/2 Theta_k F(tau)) of split potentials on a
 // thread's slots (both columns of the pair share it); nothing otherwise.
 FWD void v_field_phase(int t, double F, cpx2 (&ftab)[FW_COL_E / 2 + 1]) {
 #if FW_V_SPLIT
   constexpr int N = FW_NP, E = FW_COL_E;
   const double g = 0.5 * ::dt * F * ::dTheta;
-  if constexpr (VL::PAIRED && VL::RL == E / 2 && !FW_FP32) {
-    // FP64, one slot per thread: items m < RL/2 sit at k = t + m S, the rest at
-    // N - t - m S (thread 0: S/2 + (m - RL/2) S), so two runs of phases with
-    // step exp(+-i g S) from two directly evaluated starts (at most RL/2 - 1
-    // unit products each, a few ulp). The FP32 form evaluates every phase
-    // (its series is cheap and a float recurrence would drift).
-    constexpr int RL = VL::RL, S = VL::S;
-    const cpx st = unit_phase(g * S, 1.0);
-    const cpx stb = t == 0 ? st : mk(st.x, -st.y);
-    cpx ea = unit_phase(g * t, 1.0);
-    cpx eb = unit_phase(g * (t == 0 ? S / 2 : N - t - (RL / 2) * S), 1.0);
+  if constexpr (FIELD_POW) {
+    // FP64 pair-fused lines: the phases as powers (FieldPow); the FP32 form
+    // evaluates every phase (its series is cheap)
+    FieldPow fp;
+    fp.init(g, t);
 #pragma unroll
-    for (int m = 0; m < RL / 2; ++m) {
-      ftab[m].a = cmul(ftab[m].a, ea);
-      ftab[m].b = cmul(ftab[m].b, ea);
-      ftab[m + RL / 2].a = cmul(ftab[m + RL / 2].a, eb);
-      ftab[m + RL / 2].b = cmul(ftab[m + RL / 2].b, eb);
-      if (m + 1 < RL / 2) {
-        ea = cmul(ea, st);
-        eb = cmul(eb, stb);
-      }
+    for (int i = 0; i < E / 2; ++i) {
+      const cpx e = fp.item(t, i);
+      ftab[i].a = cmul(ftab[i].a, e);
+      ftab[i].b = cmul(ftab[i].b, e);
     }
-    const cpx en = unit_phase(g * (N / 2), 1.0);
+    const cpx en = fp.nyquist();
     ftab[E / 2].a = cmul(ftab[E / 2].a, en);
     ftab[E / 2].b = cmul(ftab[E / 2].b, en);
   } else {
@@ -521,18 +558,22 @@ struct VVMul {
   double pur;                           // sum c_k (|alpha_a|^2 + |alpha_b|^2)
   double a0, b0;                        // alpha_0 of columns a, b (thread 0 of a line)
   double g0, g1;                        // split V: field phase per unit k at the two half-steps
+  FieldPow p0{}, p1{};                  // FIELD_POW: the phases of g0, g1 as powers
   // the factors of item i (k = spec_k): split potentials take the table of V0
   // (f0 = f1) times exp(+i g k), evaluated here so that only one factor set
   // stays in registers
-  FWD cpx2 fac(const cpx2 (&f)[FW_COL_E / 2 + 1], int i, double g) const {
+  FWD cpx2 fac(const cpx2 (&f)[FW_COL_E / 2 + 1], int i, double g, const FieldPow& fp) const {
 #if FW_V_SPLIT
-    const cpx e = unit_phase(g * (double)spec_k<VL>(t, i), 1.0);
+    cpx e;
+    if constexpr (FIELD_POW) e = fp.item(t, i);
+    else e = unit_phase(g * (double)spec_k<VL>(t, i), 1.0);
     cpx2 r;
     r.a = cmul(f[i].a, e);
     r.b = cmul(f[i].b, e);
     return r;
 #else
     (void)g;
+    (void)fp;
     return f[i];
 #endif
   }
@@ -540,7 +581,7 @@ struct VVMul {
     const real_t h = (real_t)0.5 / (real_t)FW_NP;
     const cpx A = mk(h * (zk.x + zm.x), h * (zk.y - zm.y));
     const cpx B = mk(h * (zk.y + zm.y), h * (zm.x - zk.x));
-    const cpx2 F0 = fac(f0, i, g0);
+    const cpx2 F0 = fac(f0, i, g0, p0);
     const cpx A1 = cscale(cmul(A, F0.a), ma), B1 = cscale(cmul(B, F0.b), mb);
     const double sx = (double)A1.x + (double)B1.x, sy = (double)A1.y + (double)B1.y;
 #if FW_SW_AHEAD
@@ -554,7 +595,7 @@ struct VVMul {
     acc[3] += sx * w.kk[0] + sy * w.kk[1];
     const double ax = A1.x, ay = A1.y, bx = B1.x, by = B1.y;
     pur += 2.0 * (ax * ax + ay * ay + bx * bx + by * by);
-    const cpx2 F1 = fac(f1, i, g1);
+    const cpx2 F1 = fac(f1, i, g1, p1);
     const cpx A2 = cmul(A1, F1.a), B2 = cmul(B1, F1.b);
     zk = mk(A2.x - B2.y, A2.y + B2.x);
     zm = mk(A2.x + B2.y, B2.x - A2.y);
@@ -565,7 +606,8 @@ struct VVMul {
     const real_t s = (real_t)1 / (real_t)FW_NP;
 #if FW_V_SPLIT
     // Nyquist: the real part of the phased factor (the DC phase is 1)
-    const cpx en0 = unit_phase(g0 * (FW_NP / 2), 1.0), en1 = unit_phase(g1 * (FW_NP / 2), 1.0);
+    const cpx en0 = FIELD_POW ? p0.nyquist() : unit_phase(g0 * (FW_NP / 2), 1.0);
+    const cpx en1 = FIELD_POW ? p1.nyquist() : unit_phase(g1 * (FW_NP / 2), 1.0);
     const cpx2 n0 = {cmul(f0[E2].a, en0), cmul(f0[E2].b, en0)}, n1 = {cmul(f1[E2].a, en1), cmul(f1[E2].b, en1)};
 #else
     const cpx2 n0 = f0[E2], n1 = f1[E2];
@@ -864,6 +906,10 @@ FWD void vvpass_theta(const Args& a) {
   g1 = 0.5 * ::dt * field_tau(a, a.step + 1) * ::dTheta;
 #endif
   VVMul mul{f0, f1, mask.ma, mask.mb, swp, t, {0.0, 0.0, 0.0, 0.0}, 0.0, 0.0, 0.0, g0, g1};
+  if constexpr (FIELD_POW) {
+    mul.p0.init(g0, t);
+    mul.p1.init(g1, t);
+  }
   if constexpr (L::PLANNED && FW_TW_PLAN_COL) spectral_step_planned<L>(v, lds, q, t, twl, mul, plan);
   else spectral_step<L>(v, lds, q, t, twl, mul);
 #pragma unroll
