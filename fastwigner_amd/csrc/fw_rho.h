@@ -106,22 +106,22 @@ FWD cpx rowfac(const cpx* __restrict__ u, int s, int j) { return u[(size_t)s * F
 using RL = Line<FW_N, FW_ROW_E, FW_ROW_P, true, FW_N, FW_RHO_TW>;
 using CL = Line<FW_N, FW_COL_E, FW_COL_P, false, FW_N, FW_RHO_TW>;
 
-// The 8 momentum sums of step s from the diagonal pass's shares (8 rows of nb
-// in aux), the work of fw_rho_diag_publish, done by block 0 of the next step's
-// pass A so that the step needs no launch for it.
+// The 8 momentum sums of step s from the diagonal pass's shares (8 rows of
+// nshare in aux), the work of fw_rho_diag_publish, done by the last blocks of
+// the next step's pass A after their own rows (sum o by block nb - 1 - o, so
+// that these blocks are not the ones that publish the x-family sums), so that
+// the step needs no launch for it and no block delays its own loads.
 template <int NT>
-FWD void publish_shares(const RArgs& a, double* red, int nb, int s) {
+FWD void publish_shares(const RArgs& a, double* red, int nshare, int s, int nb) {
   const double* part = reinterpret_cast<const double*>(a.aux);
-  double v[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  for (int b = threadIdx.x; b < nb; b += NT) {
-#pragma unroll
-    for (int o = 0; o < 8; ++o) v[o] += part[(size_t)o * nb + b];
+  for (int o = nb - 1 - (int)blockIdx.x; o >= 0 && o < 8; o += nb) {
+    double v = 0.0;
+    for (int b = threadIdx.x; b < nshare; b += NT) v += part[(size_t)o * nshare + b];
+    BlockRed<NT>::put(v, red, 1, 0);
+    __syncthreads();
+    if (threadIdx.x == 0) a.series[(size_t)s * RSER + R_P + o] = BlockRed<NT>::total(red, 1, 0);
+    __syncthreads();   // red is reused
   }
-#pragma unroll
-  for (int o = 0; o < 8; ++o) BlockRed<NT>::put(v[o], red, 8, o);
-  __syncthreads();
-  if (threadIdx.x < 8) a.series[(size_t)s * RSER + R_P + threadIdx.x] = BlockRed<NT>::total(red, 8, threadIdx.x);
-  __syncthreads();   // red is reused by the pass
 }
 
 template <bool PASS_A, int FL>
@@ -135,8 +135,6 @@ FWD void rho_row(const RArgs& a) {
   __shared__ double red[RSER * 32];
   static_assert(8 * BlockRed<NT>::RN <= RSER * 32, "reduction buffer");
   if (a.scal[RC_STOP] != 0.0) return;
-  if constexpr (PASS_A)
-    if (a.pubnb > 0 && blockIdx.x == 0) publish_shares<NT>(a, red, a.pubnb, a.step - 1);
   const int tid = threadIdx.x;
   const int t = tid % TPL, q = tid / TPL;
   const int nb = gridDim.x;
@@ -174,6 +172,7 @@ FWD void rho_row(const RArgs& a) {
         rho_previous<NT>(a, red, nrm, stop);
         if (stop) {
           rho_publish<NT>(a, blk, nb, red, true);
+          if (a.pubnb > 0) publish_shares<NT>(a, red, a.pubnb, a.step - 1, nb);
           return;
         }
       } else {
@@ -196,6 +195,7 @@ FWD void rho_row(const RArgs& a) {
     for (int e = 0; e < E; ++e) st_w(&row[t + TPL * e], v[e]);
     // the previous step's sums, after this block's own work
     if (deferred) rho_publish<NT>(a, blk, nb, red, false);
+    if (a.pubnb > 0) publish_shares<NT>(a, red, a.pubnb, a.step - 1, nb);
   } else {
     __syncthreads();
     L::inv(v, lds, q, t, twl);
