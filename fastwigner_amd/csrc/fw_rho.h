@@ -61,13 +61,58 @@ FWD double pcoord(int k) { return ::dP * ((k + FW_N / 2) % FW_N - 0.5 * FW_N); }
 // The partial sums of pass C are stored by sum (write_partial_rows): sum r of
 // block b at partials[r * nbp + b], so each row the next pass adds is nbp
 // contiguous doubles.
+// The trace rows of the previous step's partials (nbp = N / FW_ROW_P blocks of
+// pass C), loaded with the pass's other loads and added in sum_partials'
+// order, so that the norm is the same bits without a dependent load in the
+// middle of the pass.
+template <int NT>
+struct TraceRows {
+  static constexpr int NBP = FW_N / FW_ROW_P;
+  static constexpr int PL = (NBP + NT - 1) / NT;
+  double re[PL], im[PL];
+  FWD void load(const double* __restrict__ partials) {
+#pragma unroll
+    for (int j = 0; j < PL; ++j) {
+      const int b = threadIdx.x + j * NT;
+      re[j] = b < NBP ? partials[(size_t)R_TR * NBP + b] : 0.0;
+      im[j] = b < NBP ? partials[(size_t)(R_TR + 1) * NBP + b] : 0.0;
+    }
+  }
+  FWD static double comp_sum(const double (&x)[PL]) {
+    double sum = 0.0, comp = 0.0;
+#pragma unroll
+    for (int j = 0; j < PL; ++j) {
+      if (threadIdx.x + j * NT < NBP) {
+        const double tt = sum + x[j];
+        comp += (fabs(sum) >= fabs(x[j])) ? (sum - tt) + x[j] : (x[j] - tt) + sum;
+        sum = tt;
+      }
+    }
+    return sum + comp;
+  }
+  // the two traces (every block the same bits); red: >= 2 BlockRed<NT>::RN
+  FWD void totals(double* red, double& tr, double& ti) const {
+    __syncthreads();
+    BlockRed<NT>::put(comp_sum(re), red, 2, 0);
+    BlockRed<NT>::put(comp_sum(im), red, 2, 1);
+    __syncthreads();
+    tr = BlockRed<NT>::total(red, 2, 0);
+    ti = BlockRed<NT>::total(red, 2, 1);
+  }
+};
+
 // Complex norm of the previous step from the trace rows (the same bits in
 // every block). With the purity guard armed, `stop` tells whether the step
 // must not run; the caller then publishes the sums (rho_publish) and returns.
 template <int NT>
-FWD void rho_previous(const RArgs& a, double* red, cpx& nrm, bool& stop) {
-  const double tr_re = sum_partials<NT, 1>(a.partials + (size_t)R_TR * a.nbp, a.nbp, 0, red);
-  const double tr_im = sum_partials<NT, 1>(a.partials + (size_t)(R_TR + 1) * a.nbp, a.nbp, 0, red);
+FWD void rho_previous(const RArgs& a, double* red, cpx& nrm, bool& stop, const TraceRows<NT>* pre = nullptr) {
+  double tr_re, tr_im;
+  if (pre) {
+    pre->totals(red, tr_re, tr_im);
+  } else {
+    tr_re = sum_partials<NT, 1>(a.partials + (size_t)R_TR * a.nbp, a.nbp, 0, red);
+    tr_im = sum_partials<NT, 1>(a.partials + (size_t)(R_TR + 1) * a.nbp, a.nbp, 0, red);
+  }
   // c = 1 / (Tr rho * dX), complex, as rho /= trace(rho) * dX (rho_vneumann_cuda_1d.py:369)
   const double d = (tr_re * tr_re + tr_im * tr_im) * ::dX;
   nrm = mk((real_t)(tr_re / d), (real_t)(-tr_im / d));
@@ -155,6 +200,9 @@ FWD void rho_row(const RArgs& a) {
 #pragma unroll
   for (int e = 0; e < E; ++e) fj[e] = rowfac(a.uv, a.step, t + TPL * e);
   const cpx fi = cconj(rowfac(a.uv, a.step, i));
+  // pass A after a step: the previous step's trace rows with the row's loads
+  TraceRows<NT> trows;
+  if constexpr (PASS_A && (FL & RF_FIRST)) trows.load(a.partials);
   if constexpr (STAGE) {
 #pragma unroll
     for (int r = 0; r < TWN; ++r)
@@ -169,7 +217,7 @@ FWD void rho_row(const RArgs& a) {
       if (a.scal[RC_FRESH] == 0.0) {
         deferred = true;
         bool stop;
-        rho_previous<NT>(a, red, nrm, stop);
+        rho_previous<NT>(a, red, nrm, stop, &trows);
         if (stop) {
           rho_publish<NT>(a, blk, nb, red, true);
           if (a.pubnb > 0) publish_shares<NT>(a, red, a.pubnb, a.step - 1, nb);
