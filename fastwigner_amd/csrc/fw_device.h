@@ -662,6 +662,10 @@ FWD void vpass(const Args& a) {
   const bool deferred = (FL & F_FIRST) && a.scal[SC_FRESH] == 0.0;
   NormRow<NT> norm_row;
   if (deferred) norm_row.load(a.partials);
+  // the row of the previous step's sums this block publishes at its end
+  const bool pub_pre = deferred && gridDim.x >= NSER && blk < NSER;
+  NormRow<NT> pub_row;
+  if (pub_pre && blk > 0) pub_row.load(a.partials + (size_t)blk * NormRow<NT>::NBP);
   cpx2 ftab[E / 2 + 1];
   if constexpr (use_tab) {
     v_factors(a, c, t, ftab);
@@ -746,8 +750,18 @@ FWD void vpass(const Args& a) {
     // open with this pass only after a fresh start or a K pass)
     write_partial_rows<NT>(sums, red, a.partials, blk, nb);
   }
-  if constexpr (FL & F_FIRST)
-    if (deferred) publish_sums<NT>(a, blk, gridDim.x, red);
+  if constexpr (FL & F_FIRST) {
+    if (pub_pre) {
+      // the preloaded row added in sum_partials' order: the same bits (the
+      // barrier: red may still be read by write_partial_rows above)
+      __syncthreads();
+      (blk == 0 ? norm_row : pub_row).reduce_waves(red);
+      __syncthreads();
+      if (tid == 0) a.series[(size_t)a.step * NSER + blk] = NormRow<NT>::total(red);
+    } else if (deferred && gridDim.x < NSER) {
+      publish_sums<NT>(a, blk, gridDim.x, red);
+    }
+  }
 }
 
 
@@ -1009,6 +1023,11 @@ FWD void kpass(const Args& a) {
   const bool deferred = (FL & F_KNORM) && a.scal[SC_FRESH] == 0.0;
   NormRow<NT> norm_row;
   if (deferred) norm_row.load(a.partials);
+  // the row of the previous step's sums this block publishes at its end (row
+  // blk < NSER; row 0 is S0, already in norm_row), loaded with the rest
+  const bool pub_pre = deferred && gridDim.x >= NSER && blk < NSER;
+  NormRow<NT> pub_row;
+  if (pub_pre && blk > 0) pub_row.load(a.partials + (size_t)blk * NormRow<NT>::NBP);
   cpx fta[E / 2 + 1], ftb[E / 2 + 1];
   constexpr bool can_gen = !(FL & F_BLOCH) && L::PAIRED && L::BS == 1;
   if (can_gen && (a.flags & F_KGEN)) {
@@ -1058,9 +1077,17 @@ FWD void kpass(const Args& a) {
     st_w(&Wa[t + TPL * e], v[e].x);
     st_w(&Wb[t + TPL * e], v[e].y);
   }
-  // the previous step's sums to the series, after this block's own work
-  if constexpr (FL & F_KNORM)
-    if (deferred) publish_sums<NT>(a, blk, gridDim.x, red);
+  // the previous step's sums to the series, after this block's own work (the
+  // preloaded row added in sum_partials' order: the same bits)
+  if constexpr (FL & F_KNORM) {
+    if (pub_pre) {
+      (blk == 0 ? norm_row : pub_row).reduce_waves(red);
+      __syncthreads();
+      if (tid == 0) a.series[(size_t)a.step * NSER + blk] = NormRow<NT>::total(red);
+    } else if (deferred && gridDim.x < NSER) {
+      publish_sums<NT>(a, blk, gridDim.x, red);
+    }
+  }
   if constexpr (!(FL & F_KNORM)) clear_fresh(a, blk, tid);
 }
 
