@@ -302,6 +302,11 @@ FWD void clear_fresh(const Args& a, int blk, int tid) {
 #endif
 using VL = Line<FW_NP, FW_COL_E, FW_COL_P, false, FW_NP, FW_TW2>;
 using KL = Line<FW_NX, FW_ROW_E, FW_ROW_P, true, FW_NX, FW_TW2>;
+// Passes whose whole grid is resident at once, at most one wave per SIMD of
+// the MI355X's 256 CUs x 4 SIMDs: extra registers held through such a pass
+// cost no occupancy.
+constexpr bool PUB_AHEAD_K = (FW_NP / 2 / FW_ROW_P) * ((KL::NT + 63) / 64) <= 1024;
+constexpr bool PUB_AHEAD_V = (FW_NX / 2 / FW_COL_P) * ((VL::NT + 63) / 64) <= 1024;
 
 // The V factor table is stored by column pair: entry (k, x) at
 // ((x/2)(N_p/2+1) + k) 2 + x%2, so a column pass reads each pair's N_p/2+1
@@ -662,8 +667,9 @@ FWD void vpass(const Args& a) {
   const bool deferred = (FL & F_FIRST) && a.scal[SC_FRESH] == 0.0;
   NormRow<NT> norm_row;
   if (deferred) norm_row.load(a.partials);
-  // the row of the previous step's sums this block publishes at its end
-  const bool pub_pre = deferred && gridDim.x >= NSER && blk < NSER;
+  // the row of the previous step's sums this block publishes at its end,
+  // loaded with the rest when the grid is resident at once (PUB_AHEAD_V)
+  const bool pub_pre = PUB_AHEAD_V && deferred && gridDim.x >= NSER && blk < NSER;
   NormRow<NT> pub_row;
   if (pub_pre && blk > 0) pub_row.load(a.partials + (size_t)blk * NormRow<NT>::NBP);
   cpx2 ftab[E / 2 + 1];
@@ -758,7 +764,7 @@ FWD void vpass(const Args& a) {
       (blk == 0 ? norm_row : pub_row).reduce_waves(red);
       __syncthreads();
       if (tid == 0) a.series[(size_t)a.step * NSER + blk] = NormRow<NT>::total(red);
-    } else if (deferred && gridDim.x < NSER) {
+    } else if (deferred) {
       publish_sums<NT>(a, blk, gridDim.x, red);
     }
   }
@@ -1024,8 +1030,10 @@ FWD void kpass(const Args& a) {
   NormRow<NT> norm_row;
   if (deferred) norm_row.load(a.partials);
   // the row of the previous step's sums this block publishes at its end (row
-  // blk < NSER; row 0 is S0, already in norm_row), loaded with the rest
-  const bool pub_pre = deferred && gridDim.x >= NSER && blk < NSER;
+  // blk < NSER; row 0 is S0, already in norm_row), loaded with the rest when
+  // the whole grid is resident at once (PUB_AHEAD_K: one wave per SIMD, where
+  // the registers it holds cost no occupancy; at 2048^2 they cost a wave)
+  const bool pub_pre = PUB_AHEAD_K && deferred && gridDim.x >= NSER && blk < NSER;
   NormRow<NT> pub_row;
   if (pub_pre && blk > 0) pub_row.load(a.partials + (size_t)blk * NormRow<NT>::NBP);
   cpx fta[E / 2 + 1], ftb[E / 2 + 1];
@@ -1084,7 +1092,7 @@ FWD void kpass(const Args& a) {
       (blk == 0 ? norm_row : pub_row).reduce_waves(red);
       __syncthreads();
       if (tid == 0) a.series[(size_t)a.step * NSER + blk] = NormRow<NT>::total(red);
-    } else if (deferred && gridDim.x < NSER) {
+    } else if (deferred) {
       publish_sums<NT>(a, blk, gridDim.x, red);
     }
   }
