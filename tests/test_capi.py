@@ -61,6 +61,8 @@ def test_invalid_tiles_and_null_args():
     assert lib.fw_wigner_check(None, ctypes.byref(e)) == -1
     assert lib.fw_wigner_destroy(None) == 0
     assert lib.fw_wigner_propagate(None, None, 1, None) == -1
+    gen, dev = ctypes.c_int(), ctypes.c_double()
+    assert lib.fw_wigner_k_factors(None, ctypes.byref(gen), ctypes.byref(dev)) == -1
 
 
 @pytest.mark.parametrize("n,tiles,ok", [
