@@ -1169,6 +1169,15 @@ FWD void begin_chunk(Args a, double t0, double inc, int n, int reset, int nf) {
 // publishes and leaves W as it is.
 FWD void apply_norm(const Args& a) {
   __shared__ double red[16];
+  const bool scale = !(a.flags & F_NO_SCALE);
+  const size_t n = (size_t)FW_NX * FW_NP, stride = (size_t)gridDim.x * HT;
+  size_t i0 = (size_t)blockIdx.x * HT + threadIdx.x;
+  // the first eight elements' loads ahead of the norm's reduction
+  real_t w[8];
+  if (scale) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) w[j] = (i0 + j * stride < n) ? a.W[i0 + j * stride] : (real_t)0;
+  }
   double c;
   if (a.flags & F_FROM_PARTIALS) {
     const double S0 = sum_partials<HT, 1>(a.partials, a.nbp, 0, red);
@@ -1181,15 +1190,15 @@ FWD void apply_norm(const Args& a) {
   } else {
     c = a.scal[SC_NORM];
   }
-  if (a.flags & F_NO_SCALE) return;
+  if (!scale) return;
   const real_t cr = (real_t)c;
-  const size_t n = (size_t)FW_NX * FW_NP, stride = (size_t)gridDim.x * HT;
   // eight loads, then eight stores: a load after a store through asm cannot
   // be moved above it, so interleaved they would wait one latency each
-  for (size_t i0 = (size_t)blockIdx.x * HT + threadIdx.x; i0 < n; i0 += 8 * stride) {
-    real_t w[8];
+  for (bool first = true; i0 < n; i0 += 8 * stride, first = false) {
+    if (!first) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) w[j] = (i0 + j * stride < n) ? a.W[i0 + j * stride] : (real_t)0;
+      for (int j = 0; j < 8; ++j) w[j] = (i0 + j * stride < n) ? a.W[i0 + j * stride] : (real_t)0;
+    }
 #pragma unroll
     for (int j = 0; j < 8; ++j)
       if (i0 + j * stride < n) st_w(&a.W[i0 + j * stride], w[j] * cr);
